@@ -1,0 +1,179 @@
+"""Benchmark of the PC-stable skeleton hot path on one MI355X.
+
+A step is one full skeleton of the plain p=2000 workload on device-resident
+observations: correlation (moments + pairwise sums), then levels 0, 1, ...
+(compaction, CI tests, removal) until no edge has enough neighbours.
+
+value = CI tests/s = sum over levels of the canonical test count T_l divided
+by the level wall time (the reference's "Execution duration" scope,
+skeleton.cpp:167-180).  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+for _p in (REPO, os.path.join(REPO, "masterthesis-heterogeneous-pc_amd"), os.path.join(REPO, "tests")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "CI tests/sec + skeleton wall time at p=2000; served p50/p99 vs plain path"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def bytes_per_test(level: int) -> int:
+    """Algorithmic bytes of one CI test: level 0 reads one correlation (8 B)
+    and one adjacency entry (4 B); level l >= 1 reads the (l+2)(l+1)/2
+    off-diagonal correlations of its submatrix and l neighbour ids."""
+    if level == 0:
+        return 12
+    return 8 * (level + 2) * (level + 1) // 2 + 4 * level
+
+
+def run_step(dev):
+    t0 = time.perf_counter()
+    dev.reset()
+    dev.corr()
+    ms_mom, ms_cor = dev.corr_ms()
+    t1 = time.perf_counter()
+    stats = dev.run(-2)
+    dev.sync()
+    t2 = time.perf_counter()
+    return {
+        "e2e_ms": (t2 - t0) * 1e3,
+        "levels_ms": (t2 - t1) * 1e3,
+        "ms_moments": ms_mom,
+        "ms_corr": ms_cor,
+        "levels": stats,
+    }
+
+
+def cpu_baseline(x, n, alpha, max_level, tests_total):
+    """The oracle's reference-faithful CPU path (per-test pseudoinverse,
+    immediate deletion with early exit, OpenMP over rows sorted by length)
+    on all host threads, on the same workload."""
+    import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    t0 = time.perf_counter()
+    cor = oracle.corr(x, threads=threads)
+    t1 = time.perf_counter()
+    sk = oracle.Skeleton(cor, n, alpha, max_level=max_level, mode=oracle.MODE_PINV,
+                         flags=oracle.FLAG_REF_EARLY_EXIT, threads=threads)
+    lv = sk.run(max_level)
+    t2 = time.perf_counter()
+    return {
+        "value": tests_total / (t2 - t1),
+        "unit": "CI tests/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "the whole p=2000 skeleton once (correlation %.1f s, levels %.2f s, %d levels)"
+                  % (t1 - t0, t2 - t1, len(lv)),
+        "skeleton_ms_levels": (t2 - t1) * 1e3,
+        "skeleton_ms_e2e": (t2 - t0) * 1e3,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c2k")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(every instance runs on a single MI355X).", file=sys.stderr)
+        return 2
+
+    import heterogpc
+    import workloads
+
+    x, n, alpha, max_level = workloads.sem(args.workload)
+    p = x.shape[0]
+    dev = heterogpc.DeviceSkeleton(p, n, alpha, max_level=max_level, device=0)
+    dev.set_observations(x)  # inputs resident in HBM before timing
+
+    for _ in range(args.warmup):
+        run_step(dev)
+    heterogpc.load().hpc_device_synchronize(0)
+    t0 = time.perf_counter()
+    steps = [run_step(dev) for _ in range(args.steps)]
+    heterogpc.load().hpc_device_synchronize(0)
+    wall = time.perf_counter() - t0
+
+    last = steps[-1]["levels"]
+    tests_total = sum(lv["tests"] for lv in last)
+    levels_ms = sum(s["levels_ms"] for s in steps) / len(steps)
+    e2e_ms = wall * 1e3 / len(steps)
+    value = tests_total / (levels_ms * 1e-3)
+
+    # per-kernel device time, averaged over steps
+    def avg(fn):
+        return sum(fn(s) for s in steps) / len(steps)
+
+    k_corr = avg(lambda s: s["ms_corr"])
+    k_mom = avg(lambda s: s["ms_moments"])
+    k_test = avg(lambda s: sum(lv["ms_test"] for lv in s["levels"]))
+    k_compact = avg(lambda s: sum(lv["ms_compact"] for lv in s["levels"]))
+    k_commit = avg(lambda s: sum(lv["ms_commit"] for lv in s["levels"]))
+    test_bytes = sum(lv["tests"] * bytes_per_test(lv["level"]) for lv in last)
+    ci_gbs = test_bytes / (k_test * 1e-3) / 1e9 if k_test > 0 else 0.0
+    corr_flops = 2.0 * n * p * (p + 1) / 2.0  # distinct pairs x 2n
+    corr_tflops = corr_flops / (k_corr * 1e-3) / 1e12 if k_corr > 0 else 0.0
+
+    if k_corr >= k_test:
+        roof = {"kernel": "corr_canon_kernel (pairwise sums, fp64)", "bound": "mfma",
+                "achieved": corr_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": corr_tflops / FP64_PEAK_TFLOPS, "traffic": None,
+                "algorithmic": "%.3e flops per launch (n * p * (p+1))" % corr_flops}
+    else:
+        roof = {"kernel": "CI-test kernels (all levels)", "bound": "hbm", "achieved": ci_gbs,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ci_gbs / HBM_PEAK_GBS,
+                "traffic": None}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "CI tests/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": e2e_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic linear-Gaussian SEM (seeded), FP32-rounded samples",
+        "config": {"workload": "plain PC-stable at p=2000: SEM p=2000, n=5000, expected "
+                               "degree 3, alpha=0.05, levels 0..4 (max level 4)"
+                               if args.workload == "c2k" else args.workload,
+                   "p": p, "n": n, "alpha": alpha, "max_level": max_level},
+        "skeleton_ms_levels": levels_ms,
+        "skeleton_ms_e2e": e2e_ms,
+        "tests_total": tests_total,
+        "levels": [{k: lv[k] for k in ("level", "max_deg", "tests", "tests_max", "edges",
+                                       "removed", "edges_left", "ms_compact", "ms_test",
+                                       "ms_commit", "ms_wall")} for lv in last],
+        "kernel_ms": {"moments": k_mom, "corr": k_corr, "ci_tests": k_test,
+                      "compact": k_compact, "commit": k_commit},
+        "ci_tests_hbm": {"achieved_GBs": ci_gbs, "frac": ci_gbs / HBM_PEAK_GBS,
+                         "algorithmic_bytes": test_bytes},
+        "corr_tflops": corr_tflops,
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(x, n, alpha, max_level, tests_total)
+    dev.close()
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

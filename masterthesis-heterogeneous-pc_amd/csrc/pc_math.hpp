@@ -1,0 +1,185 @@
+// Device arithmetic of the CI tests.
+//
+// Every function performs a fixed sequence of IEEE-754 double operations
+// (+, -, *, /, sqrt, and explicit fma), compiled with -ffp-contract=off, so a
+// CPU build of the same sequence yields the same bits.  Removal decisions
+// compare |r| against r_crit, the largest double whose Fisher-z p-value is
+// still >= alpha (computed once on the host), so no transcendental function
+// runs on the device and p >= alpha (gpuRowTests.cu:18) is decided exactly.
+#pragma once
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hpc {
+
+constexpr double kCutThr = 0.9999999;  // gpuInd.cu:3 / cpuInd.cpp:7
+constexpr double kPivTol = 1e-12;
+constexpr int kJacobiSweeps = 60;
+
+// calcPValue (gpuInd.cu:4-9, cpuInd.cpp:8-14) followed by the p >= alpha
+// test: NaN r counts as r = 0 (p = 1, removed); |r| is clamped at 0.9999999.
+__device__ __forceinline__ bool removes(double r, double r_crit) {
+  double re = isnan(r) ? 0.0 : fmin(kCutThr, fabs(r));
+  return re <= r_crit;
+}
+
+// pValL1 (cpuInd.cpp:16-27): x1 = r_ij, x2 = r_ik, x3 = r_jk.
+__device__ __forceinline__ double pcor_l1(double x1, double x2, double x3) {
+  return (x1 - x2 * x3) / sqrt((1.0 - x3 * x3) * (1.0 - x2 * x2));
+}
+
+// Partial correlation -P01 / sqrt(P00 P11) of the m x m unit-diagonal matrix
+// A (row-major, overwritten) through a symmetric cyclic Jacobi
+// eigen-decomposition and the pseudoinverse with Armadillo's default
+// tolerance m * max|lambda| * eps (cpuInd.cpp:35-36).  Taken only when the
+// Cholesky form below meets a non-positive pivot.
+template <int M>
+__device__ __noinline__ double pcor_pinv(double (&A)[M][M]) {
+  double V[M][M];
+  for (int a = 0; a < M; ++a)
+    for (int b = 0; b < M; ++b) V[a][b] = (a == b) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < kJacobiSweeps; ++sweep) {
+    double off = 0.0, dia = 0.0;
+    for (int a = 0; a < M; ++a) {
+      dia = __builtin_fma(A[a][a], A[a][a], dia);
+      for (int b = a + 1; b < M; ++b) off = __builtin_fma(A[a][b], A[a][b], off);
+    }
+    if (!(off > 1e-34 * dia)) break;
+    for (int pp = 0; pp < M - 1; ++pp) {
+      for (int qq = pp + 1; qq < M; ++qq) {
+        double apq = A[pp][qq];
+        if (apq == 0.0) continue;
+        double app = A[pp][pp], aqq = A[qq][qq];
+        double theta = (aqq - app) / (2.0 * apq);
+        double t = 1.0 / (fabs(theta) + sqrt(__builtin_fma(theta, theta, 1.0)));
+        if (theta < 0.0) t = -t;
+        double c = 1.0 / sqrt(__builtin_fma(t, t, 1.0));
+        double s = t * c;
+        for (int k = 0; k < M; ++k) {
+          if (k == pp || k == qq) continue;
+          double akp = A[k][pp], akq = A[k][qq];
+          double nkp = __builtin_fma(c, akp, -(s * akq));
+          double nkq = __builtin_fma(s, akp, c * akq);
+          A[k][pp] = nkp; A[pp][k] = nkp;
+          A[k][qq] = nkq; A[qq][k] = nkq;
+        }
+        A[pp][pp] = __builtin_fma(-t, apq, app);
+        A[qq][qq] = __builtin_fma(t, apq, aqq);
+        A[pp][qq] = 0.0; A[qq][pp] = 0.0;
+        for (int k = 0; k < M; ++k) {
+          double vkp = V[k][pp], vkq = V[k][qq];
+          V[k][pp] = __builtin_fma(c, vkp, -(s * vkq));
+          V[k][qq] = __builtin_fma(s, vkp, c * vkq);
+        }
+      }
+    }
+  }
+  double maxabs = 0.0;
+  for (int a = 0; a < M; ++a) maxabs = fmax(maxabs, fabs(A[a][a]));
+  double tol = (double)M * maxabs * 2.220446049250313e-16;
+  double p00 = 0.0, p11 = 0.0, p01 = 0.0;
+  for (int a = 0; a < M; ++a) {
+    double lam = A[a][a];
+    if (!(fabs(lam) > tol)) continue;
+    double inv = 1.0 / lam;
+    p00 = __builtin_fma(V[0][a] * V[0][a], inv, p00);
+    p11 = __builtin_fma(V[1][a] * V[1][a], inv, p11);
+    p01 = __builtin_fma(V[0][a] * V[1][a], inv, p01);
+  }
+  return -p01 / sqrt(p00 * p11);
+}
+
+// Partial correlation of i and j given S (|S| = L) from a = r_ij,
+// x[c] = r_i,S[c], y[c] = r_j,S[c] and the strictly lower triangle
+// C[r][c] = r_S[c],S[r] (r > c).  With C = L L^T, u = L^-1 x, v = L^-1 y:
+// r = (a - u.v) / sqrt((1 - u.u)(1 - v.v)), the same value as -P01/sqrt(P00
+// P11) for P = inverse of the (L+2)-square submatrix [i, j, S]
+// (gpuRowTests.cu:143-170).  A pivot at or below 1e-12 or a numerically
+// singular 2x2 Schur block switches to the pinv form.
+template <int L>
+__device__ __forceinline__ double pcor_canon(double a, const double (&x)[L],
+                                             const double (&y)[L],
+                                             const double (&C)[L][L],
+                                             bool &used_pinv) {
+  double Lm[L][L];
+  double u[L], v[L];
+  used_pinv = false;
+  bool sing = false;
+#pragma unroll
+  for (int c = 0; c < L; ++c) {
+    double d = 1.0;
+#pragma unroll
+    for (int k = 0; k < c; ++k) d = __builtin_fma(-Lm[c][k], Lm[c][k], d);
+    sing |= !(d > kPivTol);
+    double s = sqrt(d);
+    Lm[c][c] = s;
+#pragma unroll
+    for (int r = c + 1; r < L; ++r) {
+      double acc = C[r][c];
+#pragma unroll
+      for (int k = 0; k < c; ++k) acc = __builtin_fma(-Lm[r][k], Lm[c][k], acc);
+      Lm[r][c] = acc / s;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < L; ++c) {
+    double su = x[c], sv = y[c];
+#pragma unroll
+    for (int k = 0; k < c; ++k) {
+      su = __builtin_fma(-Lm[c][k], u[k], su);
+      sv = __builtin_fma(-Lm[c][k], v[k], sv);
+    }
+    u[c] = su / Lm[c][c];
+    v[c] = sv / Lm[c][c];
+  }
+  double s00 = 1.0, s11 = 1.0, s01 = a;
+#pragma unroll
+  for (int k = 0; k < L; ++k) {
+    s00 = __builtin_fma(-u[k], u[k], s00);
+    s11 = __builtin_fma(-v[k], v[k], s11);
+    s01 = __builtin_fma(-u[k], v[k], s01);
+  }
+  double q = s00 * s11;
+  sing |= !(s00 > kPivTol) || !(s11 > kPivTol) ||
+          !(__builtin_fma(-s01, s01, q) > kPivTol * q);
+  if (!sing) return s01 / sqrt(q);
+  used_pinv = true;
+  double A[L + 2][L + 2];
+  for (int r = 0; r < L + 2; ++r) A[r][r] = 1.0;
+  A[0][1] = A[1][0] = a;
+  for (int c = 0; c < L; ++c) {
+    A[0][c + 2] = A[c + 2][0] = x[c];
+    A[1][c + 2] = A[c + 2][1] = y[c];
+    for (int r = c + 1; r < L; ++r) A[r + 2][c + 2] = A[c + 2][r + 2] = C[r][c];
+  }
+  return pcor_pinv<L + 2>(A);
+}
+
+// Number of l-subsets of an nn-set, saturating (cuda_util.cu:20-32).
+__device__ __forceinline__ unsigned long long binom(const unsigned long long *tab,
+                                                    int kcols, int nn, int l) {
+  if (nn < l || l < 0) return 0ull;
+  return tab[(size_t)nn * kcols + l];
+}
+
+// ithCombination (cuda_util.cu:34-50), 0-based: positions q[0] < ... < q[L-1]
+// of the rank-th L-subset of {0..nn-1} in lexicographic order.
+template <int L>
+__device__ __forceinline__ void unrank(const unsigned long long *tab, int kcols,
+                                       unsigned long long rank, int nn, int (&q)[L]) {
+  int v = 0;
+#pragma unroll
+  for (int i = 0; i < L; ++i) {
+    for (;;) {
+      unsigned long long c = binom(tab, kcols, nn - 1 - v, L - 1 - i);
+      if (rank < c) break;
+      rank -= c;
+      ++v;
+    }
+    q[i] = v++;
+  }
+}
+
+}  // namespace hpc

@@ -1,0 +1,97 @@
+"""ctypes binding of libheterogpc_hip.so (include/heterogpc_hip.h).
+
+The library is loaded from the package directory; there is no fallback: if it
+is missing, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "libheterogpc_hip.so")
+HEADER = os.path.join(os.path.dirname(PKG_ROOT), "include", "heterogpc_hip.h")
+
+HPC_OK = 0
+HPC_CORR_CANON = 0
+
+
+class LevelStats(C.Structure):
+    _fields_ = [
+        ("level", C.c_int32),
+        ("max_deg", C.c_int32),
+        ("tests", C.c_int64),
+        ("tests_max", C.c_int64),
+        ("edges", C.c_int64),
+        ("removed", C.c_int64),
+        ("edges_left", C.c_int64),
+        ("pinv_fallbacks", C.c_int64),
+        ("ms_compact", C.c_double),
+        ("ms_test", C.c_double),
+        ("ms_commit", C.c_double),
+        ("ms_wall", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class HpcError(RuntimeError):
+    pass
+
+
+_vp, _i, _i64, _u64, _d = C.c_void_p, C.c_int, C.c_int64, C.c_uint64, C.c_double
+_ps = C.POINTER(LevelStats)
+
+SIGNATURES = {
+    "hpc_last_error": (C.c_char_p, []),
+    "hpc_device_count": (_i, [C.POINTER(C.c_int)]),
+    "hpc_device_synchronize": (_i, [_i]),
+    "hpc_fisher_r_crit": (_d, [_i, _d]),
+    "hpc_sem_generate": (_i, [_i, _i, _d, _d, _d, _u64, _i, _vp, _vp]),
+    "hpc_state_create": (_i, [_u64, _i, _d, _i, _i, C.POINTER(_vp)]),
+    "hpc_state_destroy": (_i, [_vp]),
+    "hpc_reset": (_i, [_vp]),
+    "hpc_sync": (_i, [_vp]),
+    "hpc_max_cond_size": (_i, [_vp]),
+    "hpc_pmcc": (_i, [_vp, _vp, _i]),
+    "hpc_set_observations": (_i, [_vp, _vp]),
+    "hpc_corr": (_i, [_vp, _i]),
+    "hpc_set_corr": (_i, [_vp, _vp]),
+    "hpc_corr_ms": (_i, [_vp, C.POINTER(_d), C.POINTER(_d)]),
+    "hpc_compact": (_i, [_vp, C.POINTER(C.c_int)]),
+    "hpc_test_level": (_i, [_vp, _i, _vp, _i, _ps]),
+    "hpc_level": (_i, [_vp, _i, _ps]),
+    "hpc_run": (_i, [_vp, _i, _ps, _i, C.POINTER(C.c_int)]),
+    "hpc_get_cor": (_i, [_vp, _vp]),
+    "hpc_get_adj": (_i, [_vp, _vp]),
+    "hpc_get_compact": (_i, [_vp, _vp]),
+    "hpc_get_sepsets": (_i, [_vp, _vp, _i]),
+    "hpc_get_first_pass": (_i, [_vp, _vp, _vp, C.POINTER(_i64)]),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise HpcError(
+            f"{path} is missing: build it with masterthesis-heterogeneous-pc_amd/build.py"
+        )
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    if rc != HPC_OK:
+        msg = load().hpc_last_error()
+        raise HpcError(f"rc={rc}: {msg.decode() if msg else ''}")

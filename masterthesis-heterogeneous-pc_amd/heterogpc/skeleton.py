@@ -1,0 +1,148 @@
+"""numpy-facing wrapper of one hpc_state (one skeleton instance on one GPU)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import HPC_CORR_CANON, LevelStats, check, load
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def r_crit(n: int, alpha: float) -> float:
+    """Largest |r| whose Fisher-z p-value (cpuInd.cpp:8-14) is >= alpha."""
+    return load().hpc_fisher_r_crit(int(n), float(alpha))
+
+
+def sem_generate(p: int, n: int, degree: float, wmin: float = 0.1,
+                 wmax: float = 1.0, seed: int = 0, fp32: bool = True,
+                 with_dag: bool = False):
+    """Linear-Gaussian SEM sample, variable-major [p][n] float64."""
+    x = np.empty((p, n), dtype=np.float64)
+    dag = np.zeros((p, p), dtype=np.uint8) if with_dag else None
+    check(load().hpc_sem_generate(p, n, float(degree), float(wmin), float(wmax),
+                                  C.c_uint64(seed), 1 if fp32 else 0, _ptr(x),
+                                  _ptr(dag) if dag is not None else None))
+    return (x, dag) if with_dag else x
+
+
+class DeviceSkeleton:
+    """One skeleton instance resident on a GPU (MMState + executors)."""
+
+    def __init__(self, p: int, n: int, alpha: float, max_level: int = 4,
+                 device: int = 0):
+        self.lib = load()
+        self.p, self.n, self.alpha, self.max_level = int(p), int(n), float(alpha), int(max_level)
+        h = C.c_void_p()
+        check(self.lib.hpc_state_create(self.p, self.n, self.alpha, self.max_level,
+                                        int(device), C.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            check(self.lib.hpc_state_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- inputs ----------------------------------------------------------
+    def pmcc(self, x: np.ndarray, mode: int = HPC_CORR_CANON) -> None:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        assert x.shape == (self.p, self.n)
+        check(self.lib.hpc_pmcc(self.h, _ptr(x), int(mode)))
+
+    def set_observations(self, x: np.ndarray) -> None:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        assert x.shape == (self.p, self.n)
+        check(self.lib.hpc_set_observations(self.h, _ptr(x)))
+
+    def corr(self, mode: int = HPC_CORR_CANON) -> None:
+        check(self.lib.hpc_corr(self.h, int(mode)))
+
+    def corr_ms(self) -> tuple[float, float]:
+        a, b = C.c_double(), C.c_double()
+        check(self.lib.hpc_corr_ms(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def set_corr(self, cor: np.ndarray) -> None:
+        cor = np.ascontiguousarray(cor, dtype=np.float64)
+        assert cor.shape == (self.p, self.p)
+        check(self.lib.hpc_set_corr(self.h, _ptr(cor)))
+
+    # ---- levels ----------------------------------------------------------
+    def reset(self) -> None:
+        check(self.lib.hpc_reset(self.h))
+
+    def sync(self) -> None:
+        check(self.lib.hpc_sync(self.h))
+
+    def compact(self) -> int:
+        md = C.c_int()
+        check(self.lib.hpc_compact(self.h, C.byref(md)))
+        return md.value
+
+    def level(self, level: int) -> dict:
+        s = LevelStats()
+        check(self.lib.hpc_level(self.h, int(level), C.byref(s)))
+        return s.as_dict()
+
+    def test_level(self, level: int, rows=None) -> dict:
+        s = LevelStats()
+        if rows is None:
+            check(self.lib.hpc_test_level(self.h, int(level), None, 0, C.byref(s)))
+        else:
+            r = np.ascontiguousarray(rows, dtype=np.int32)
+            check(self.lib.hpc_test_level(self.h, int(level), _ptr(r), len(r), C.byref(s)))
+        return s.as_dict()
+
+    def run(self, max_level: int = -2, cap: int = 64) -> list[dict]:
+        arr = (LevelStats * cap)()
+        ran = C.c_int()
+        check(self.lib.hpc_run(self.h, int(max_level), arr, cap, C.byref(ran)))
+        return [arr[k].as_dict() for k in range(min(ran.value, cap))]
+
+    # ---- outputs ---------------------------------------------------------
+    def cor(self) -> np.ndarray:
+        out = np.empty((self.p, self.p), dtype=np.float64)
+        check(self.lib.hpc_get_cor(self.h, _ptr(out)))
+        return out
+
+    def adj(self) -> np.ndarray:
+        out = np.empty((self.p, self.p), dtype=np.int32)
+        check(self.lib.hpc_get_adj(self.h, _ptr(out)))
+        return out
+
+    def adj_compact(self) -> np.ndarray:
+        out = np.empty((self.p, self.p), dtype=np.int32)
+        check(self.lib.hpc_get_compact(self.h, _ptr(out)))
+        return out
+
+    def max_cond_size(self) -> int:
+        return self.lib.hpc_max_cond_size(self.h)
+
+    def sepsets(self, mc: int | None = None) -> np.ndarray:
+        mc = self.max_cond_size() if mc is None else int(mc)
+        out = np.empty((self.p, self.p, mc), dtype=np.int32)
+        check(self.lib.hpc_get_sepsets(self.h, _ptr(out), mc))
+        return out
+
+    def first_pass(self):
+        ne = C.c_int64()
+        check(self.lib.hpc_get_first_pass(self.h, None, None, C.byref(ne)))
+        fp = np.empty(max(ne.value, 1), dtype=np.int64)
+        rp = np.empty(self.p + 1, dtype=np.int32)
+        check(self.lib.hpc_get_first_pass(self.h, _ptr(fp), _ptr(rp), C.byref(ne)))
+        return fp[: ne.value], rp

@@ -1,0 +1,143 @@
+"""HIP path vs the CPU oracle through the C ABI, on the same seeded inputs.
+
+Bar: correlation bit-identical (same summation order and operations on both
+sides), adjacency and neighbour lists bit-identical after every level, first
+passing rank of every ordered edge identical, canonical test counts equal,
+sepsets identical and each one accepted by the reference's test.
+"""
+import numpy as np
+import pytest
+
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare_run(hpc, orc, x, n, alpha, max_level, cor=None, label=""):
+    p = x.shape[0] if x is not None else cor.shape[0]
+    print(f"[parity] {label}: p={p} n={n} alpha={alpha} max_level={max_level}", flush=True)
+    dev = hpc.DeviceSkeleton(p, n, alpha, max_level=max_level)
+    if cor is None:
+        dev.pmcc(x)
+        ref_cor = orc.corr(x)
+        got = dev.cor()
+        assert np.array_equal(got, ref_cor, equal_nan=True), \
+            f"correlation differs: max |d| = {np.nanmax(np.abs(got - ref_cor))}"
+    else:
+        dev.set_corr(cor)
+        ref_cor = cor
+    ref = orc.Skeleton(ref_cor, n, alpha, max_level=max_level)
+    level = 0
+    while True:
+        if max_level >= 0 and level > max_level:
+            break
+        d = dev.level(level)
+        o = ref.level(level)
+        if level >= 1:
+            assert np.array_equal(dev.adj_compact(), ref.adj_compact()), f"compact differs at level {level}"
+            if max_level < 0 and o["max_deg"] - 1 < level:
+                break
+            fp_d, rp_d = dev.first_pass()
+            fp_o, rp_o = ref.first_pass()
+            assert np.array_equal(rp_d, rp_o)
+            assert np.array_equal(fp_d, fp_o), f"first passing ranks differ at level {level}"
+        assert np.array_equal(dev.adj(), ref.adj()), f"adjacency differs after level {level}"
+        assert d["tests"] == o["tests"], (level, d, o)
+        assert d["tests_max"] == o["tests_max"], (level, d, o)
+        assert d["removed"] == o["removed"], (level, d, o)
+        level += 1
+        if level > 12:
+            break
+    mc = ref.mc()
+    sd = dev.sepsets(mc)
+    so = ref.sepsets()
+    assert np.array_equal(sd, so), "sepsets differ"
+    # every recorded sepset is one the reference's test accepts
+    adj = ref.adj()
+    checked = 0
+    for i in range(p):
+        for j in range(i + 1, p):
+            if adj[i, j]:
+                continue
+            s = so[i, j]
+            if s[0] == -2:
+                continue
+            S = [int(v) for v in s if v >= 0]
+            pv = orc.pval_ids(ref_cor, n, i, j, S)
+            pv2 = orc.pval_ids(ref_cor, n, j, i, S)
+            assert pv >= alpha or pv2 >= alpha, (i, j, S, pv, pv2)
+            checked += 1
+            if checked > 3000:
+                break
+    dev.close()
+    return level
+
+
+@pytest.mark.parametrize("name", ["c1", "small", "dense", "deep"])
+def test_sem_parity(hpc, orc, name):
+    x, n, alpha, ml = workloads.sem(name)
+    _compare_run(hpc, orc, x, n, alpha, ml, label=name)
+
+
+def test_sem_parity_level_cap(hpc, orc):
+    x, n, alpha, _ = workloads.sem("small")
+    _compare_run(hpc, orc, x, n, alpha, 2, label="small/max_level=2")
+
+
+def test_cooling_house_parity(hpc, orc):
+    x = workloads.cooling_house()
+    _compare_run(hpc, orc, x, x.shape[1], 0.05, 4, label="cooling_house")
+
+
+def test_nan_and_constant_columns(hpc, orc):
+    x, n, alpha, _ = workloads.sem("small")
+    x = x[:80].copy()
+    rng = np.random.default_rng(3)
+    mask = rng.random(x.shape) < 0.01
+    x[mask] = np.nan
+    x[5] = 1.25          # zero variance
+    x[9, :] = np.nan     # all NaN
+    _compare_run(hpc, orc, x, n, alpha, 3, label="nan")
+
+
+def test_duplicate_columns_pinv(hpc, orc):
+    x, n, alpha, _ = workloads.sem("small")
+    x = x[:60].copy()
+    x[7] = x[3]
+    x[11] = 2.0 * x[3] - 1.0
+    x[20] = x[21] + x[22]
+    _compare_run(hpc, orc, x, n, alpha, -1, label="collinear")
+
+
+def test_given_correlation_matrix(hpc, orc):
+    x, n, alpha, _ = workloads.sem("small")
+    cor = np.corrcoef(x[:150])
+    _compare_run(hpc, orc, None, n, alpha, -1, cor=cor, label="--corr")
+
+
+def test_c2_full_parity(hpc, orc):
+    x, n, alpha, ml = workloads.sem("c2")
+    _compare_run(hpc, orc, x, n, alpha, ml, label="c2")
+
+
+def test_row_subset_then_rest(hpc, orc):
+    """hpc_test_level on two disjoint row sets gives the same first passes."""
+    x, n, alpha, _ = workloads.sem("small")
+    dev = hpc.DeviceSkeleton(x.shape[0], n, alpha, max_level=-1)
+    dev.pmcc(x)
+    dev.level(0)
+    dev.compact()
+    full = hpc.DeviceSkeleton(x.shape[0], n, alpha, max_level=-1)
+    full.pmcc(x)
+    full.level(0)
+    full.level(1)
+    rows = np.arange(0, x.shape[0], 2)
+    dev.test_level(1, rows)
+    fp_half, rp = dev.first_pass()
+    fp_full, _ = full.first_pass()
+    for i in range(x.shape[0]):
+        seg = slice(rp[i], rp[i + 1])
+        if i % 2 == 0:
+            assert np.array_equal(fp_half[seg], fp_full[seg])
+        else:
+            assert np.all(fp_half[seg] == -1)
