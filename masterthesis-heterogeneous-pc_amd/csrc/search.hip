@@ -173,8 +173,11 @@ __global__ __launch_bounds__(256) void commit_kernel(
   if (j < i) return;
   const int bi = v.rowptr[i], bj = v.rowptr[j];
   const int yi = (int)(e - bi);
-  const int yj = find_pos(v.nbr + bj, v.rowptr[j + 1] - bj, i);
-  const long long fi = v.fp[e], fj = v.fp[bj + yj];
+  const int lenj = v.rowptr[j + 1] - bj;
+  const int yj = find_pos(v.nbr + bj, lenj, i);
+  const long long fi = v.fp[e];
+  // the adjacency is symmetric, so i is in N(j); guard anyway
+  const long long fj = (yj < lenj && v.nbr[bj + yj] == i) ? v.fp[bj + yj] : -1;
   int row, y, base, nn;
   unsigned long long rank;
   if (fi >= 0) { row = i; y = yi; base = bi; rank = (unsigned long long)fi; }
