@@ -18,6 +18,7 @@ CSRC = os.path.join(PKG_DIR, "csrc")
 BUILD = os.path.join(PKG_DIR, "_build")
 LIB = os.path.join(PKG_DIR, "libheterogpc_hip.so")
 ORACLE_DIR = os.path.join(REPO, "oracle")
+COPY_PROBE = os.path.join(PKG_DIR, "copy_probe")
 
 SOURCES = ["abi.hip", "corr.hip", "graph.hip", "search.hip"]
 
@@ -56,6 +57,7 @@ def build_hip(jobs: int = 4) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp])
     os.replace(tmp, LIB)
+    _run([HIPCC, *HIP_FLAGS, os.path.join(CSRC, "copy_probe.hip"), "-o", COPY_PROBE])
     return LIB
 
 
