@@ -167,7 +167,7 @@ struct hpc_state {
   uint8_t *d_nanm = nullptr;
   int32_t *d_hasnan = nullptr;
   double *d_cor = nullptr;
-  bool have_x = false, have_cor = false;
+  bool have_x = false, have_cor = false, x_has_nan = false;
   // graph
   unsigned long long *d_adj = nullptr;
   int32_t *d_deg = nullptr, *d_rowptr = nullptr, *d_nbr = nullptr, *d_erow = nullptr;
@@ -341,6 +341,11 @@ extern "C" int hpc_set_observations(hpc_state *st, const double *h_mat) {
   }
   HIPCHK(hipMemcpyAsync(st->d_x, h_mat, p * st->n * sizeof(double),
                         hipMemcpyHostToDevice, st->stream));
+  // NaN cells need the masked pairwise kernel (corHelper.cu:101 skips them)
+  bool any_nan = false;
+  for (size_t k = 0; k < p * st->n; ++k)
+    if (h_mat[k] != h_mat[k]) { any_nan = true; break; }
+  st->x_has_nan = any_nan;
   HIPCHK(hipStreamSynchronize(st->stream));
   st->have_x = true;
   return HPC_OK;
@@ -356,8 +361,11 @@ extern "C" int hpc_corr(hpc_state *st, int mode) {
                  st->d_sd, st->d_z, st->d_nanm, st->d_hasnan);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(st->ev[5], st->stream));
-  launch_corr_canon(st->stream, st->d_z, st->d_nanm, st->d_hasnan, st->p, st->n,
-                    st->npad, st->d_cor);
+  if (st->x_has_nan)
+    launch_corr_canon(st->stream, st->d_z, st->d_nanm, st->d_hasnan, st->p, st->n,
+                      st->npad, st->d_cor);
+  else
+    launch_corr_tiled(st->stream, st->d_z, st->p, st->n, st->npad, st->d_cor);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(st->ev[6], st->stream));
   st->have_cor = true;
@@ -413,6 +421,7 @@ static Views views(hpc_state *st) {
   v.fp = st->d_fp;
   v.ctr = st->d_ctr;
   v.p = st->p;
+  v.max_deg = st->max_deg;
   v.r_crit = st->r_crit;
   return v;
 }

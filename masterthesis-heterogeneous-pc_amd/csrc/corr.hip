@@ -152,7 +152,110 @@ __global__ __launch_bounds__(256) void corr_canon_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Tiled form for NaN-free data: a 256-thread workgroup owns a 16 x 16 tile of
+// variable pairs; each wavefront an 8 x 8 sub-tile (64 accumulators per lane,
+// lane t = residue class t).  Samples stream through LDS 64 at a time, double
+// buffered, so each z element is read from L2 once per workgroup.  Tiles are
+// visited in 8 x 8 super-tiles of the upper triangle and every XCD (workgroup
+// id mod 8) walks a contiguous range of super-tiles, keeping the rows it
+// re-reads in its own L2.
+// ---------------------------------------------------------------------------
+constexpr int CT = 16;  // variables per workgroup tile side
+constexpr int ST = 8;   // tiles per super-tile side
+constexpr int kXcds = 8;
+
+__global__ __launch_bounds__(256) void corr_tile_kernel(
+    const double *__restrict__ z, int p, int n, int npad, int nt, int ns,
+    long long per_xcd, double *__restrict__ cor) {
+  __shared__ double sa[2][CT][64];
+  __shared__ double sb[2][CT][64];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wa = wave >> 1, wb = wave & 1;
+  // XCD-aware logical tile index
+  const long long g = blockIdx.x;
+  const long long L = (g % kXcds) * per_xcd + g / kXcds;
+  const long long sb_idx = L / (ST * ST);
+  const int within = (int)(L % (ST * ST));
+  const long long nsb = (long long)ns * (ns + 1) / 2;
+  if (sb_idx >= nsb) return;
+  // upper-triangle super-tile (SA <= SB), row-major over SA
+  int SA = (int)((2.0 * ns + 1.0 - sqrt((2.0 * ns + 1.0) * (2.0 * ns + 1.0) - 8.0 * (double)sb_idx)) * 0.5);
+  if (SA < 0) SA = 0;
+  while (SA > 0 && (long long)SA * ns - (long long)SA * (SA - 1) / 2 > sb_idx) --SA;
+  while ((long long)(SA + 1) * ns - (long long)(SA + 1) * SA / 2 <= sb_idx) ++SA;
+  const int SB = SA + (int)(sb_idx - ((long long)SA * ns - (long long)SA * (SA - 1) / 2));
+  const int TA = SA * ST + within / ST, TB = SB * ST + within % ST;
+  if (TA >= nt || TB >= nt || TA > TB) return;
+  const int a0 = TA * CT, b0 = TB * CT;
+
+  // loader mapping: 16 rows x 64 samples per operand = 256 threads x 4 doubles
+  const int lr = tid >> 4, lc = (tid & 15) * 4;
+  const double *ga = z + (size_t)min(a0 + lr, p - 1) * npad + lc;
+  const double *gb = z + (size_t)min(b0 + lr, p - 1) * npad + lc;
+  const int nsteps = npad >> 6;
+
+  double acc[8][8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u)
+#pragma unroll
+    for (int v = 0; v < 8; ++v) acc[u][v] = 0.0;
+
+  double4 pa = *reinterpret_cast<const double4 *>(ga);
+  double4 pb = *reinterpret_cast<const double4 *>(gb);
+  *reinterpret_cast<double4 *>(&sa[0][lr][lc]) = pa;
+  *reinterpret_cast<double4 *>(&sb[0][lr][lc]) = pb;
+  __syncthreads();
+  for (int m = 0; m < nsteps; ++m) {
+    const int cur = m & 1;
+    if (m + 1 < nsteps) {
+      pa = *reinterpret_cast<const double4 *>(ga + (size_t)(m + 1) * 64);
+      pb = *reinterpret_cast<const double4 *>(gb + (size_t)(m + 1) * 64);
+    }
+    double za[8], zb[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) za[u] = sa[cur][wa * 8 + u][lane];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) zb[v] = sb[cur][wb * 8 + v][lane];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int v = 0; v < 8; ++v) acc[u][v] = __builtin_fma(za[u], zb[v], acc[u][v]);
+    if (m + 1 < nsteps) {
+      *reinterpret_cast<double4 *>(&sa[cur ^ 1][lr][lc]) = pa;
+      *reinterpret_cast<double4 *>(&sb[cur ^ 1][lr][lc]) = pb;
+    }
+    __syncthreads();
+  }
+  const double div = (double)((float)n - 1.f);
+#pragma unroll
+  for (int u = 0; u < 8; ++u)
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      double r = tree64(acc[u][v]);
+      const int a = a0 + wa * 8 + u, b = b0 + wb * 8 + v;
+      if (lane == 0 && a < p && b < p) {
+        r = r / div;
+        cor[(size_t)a * p + b] = r;
+        cor[(size_t)b * p + a] = r;
+      }
+    }
+}
+
 }  // namespace
+
+void launch_corr_tiled(hipStream_t s, const double *z, int p, int n, int npad,
+                       double *cor) {
+  const int nt = (p + CT - 1) / CT;
+  const int ns = (nt + ST - 1) / ST;
+  const long long nsb = (long long)ns * (ns + 1) / 2;
+  const long long total = nsb * ST * ST;
+  const long long per_xcd = (total + kXcds - 1) / kXcds;
+  dim3 grid((unsigned)(per_xcd * kXcds)), block(256);
+  hipLaunchKernelGGL(corr_tile_kernel, grid, block, 0, s, z, p, n, npad, nt, ns,
+                     per_xcd, cor);
+}
 
 void launch_moments(hipStream_t s, const double *x, int p, int n, int npad,
                     double *mean, double *sd, double *z, uint8_t *nanm,

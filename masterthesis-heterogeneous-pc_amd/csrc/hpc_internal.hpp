@@ -38,6 +38,7 @@ struct Views {
   int64_t *fp;              // first passing rank per ordered edge, -1 = none
   Counters *ctr;
   int p;
+  int max_deg;              // largest row length of the snapshot
   double r_crit;
 };
 
@@ -47,6 +48,8 @@ void launch_moments(hipStream_t s, const double *x, int p, int n, int npad,
                     int32_t *hasnan);
 void launch_corr_canon(hipStream_t s, const double *z, const uint8_t *nanm,
                        const int32_t *hasnan, int p, int n, int npad,
+                       double *cor);
+void launch_corr_tiled(hipStream_t s, const double *z, int p, int n, int npad,
                        double *cor);
 void launch_adj_init(hipStream_t s, unsigned long long *adj, int p, int W);
 void launch_level0(hipStream_t s, const double *cor, unsigned long long *adj,

@@ -166,19 +166,30 @@ __device__ __forceinline__ unsigned long long binom(const unsigned long long *ta
 
 // ithCombination (cuda_util.cu:34-50), 0-based: positions q[0] < ... < q[L-1]
 // of the rank-th L-subset of {0..nn-1} in lexicographic order.
+//
+// With v0 the smallest free position and k = L - i members still to place,
+// the sets whose member i lies in [v0, v] number C(nn-v0, k) - C(nn-1-v, k)
+// (hockey-stick identity), so member i is nn-1-m for the largest m with
+// C(m, k) < C(nn-v0, k) - rank: a binary search of log2(nn) table reads per
+// member instead of a walk over every candidate position.
 template <int L>
 __device__ __forceinline__ void unrank(const unsigned long long *tab, int kcols,
                                        unsigned long long rank, int nn, int (&q)[L]) {
-  int v = 0;
+  int v0 = 0;
 #pragma unroll
   for (int i = 0; i < L; ++i) {
-    for (;;) {
-      unsigned long long c = binom(tab, kcols, nn - 1 - v, L - 1 - i);
-      if (rank < c) break;
-      rank -= c;
-      ++v;
+    const int k = L - i;
+    const unsigned long long top = tab[(size_t)(nn - v0) * kcols + k];
+    const unsigned long long X = top - rank;
+    int lo = k - 1, hi = nn - 1 - v0;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tab[(size_t)mid * kcols + k] < X) lo = mid; else hi = mid - 1;
     }
-    q[i] = v++;
+    const int v = nn - 1 - lo;
+    q[i] = v;
+    rank = rank + tab[(size_t)(lo + 1) * kcols + k] - top;
+    v0 = v + 1;
   }
 }
 

@@ -87,7 +87,21 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_kernel(
 // cpuWorkstealingTests.cpp:39-112).
 template <int L>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_kernel(
-    Views v, long long n_edges, const int32_t *__restrict__ row_mask) {
+    Views v, long long n_edges, const int32_t *__restrict__ row_mask, int lds_binom) {
+  // C(m, k) for m <= max_deg, k <= L staged in LDS (columns 0..L)
+  extern __shared__ unsigned long long s_binom[];
+  const unsigned long long *tab = v.binom;
+  int kc = kBinomK;
+  if (lds_binom) {
+    const int rows = v.max_deg + 1;
+    for (int t = threadIdx.x; t < rows * (L + 1); t += blockDim.x) {
+      const int m = t / (L + 1), k = t % (L + 1);
+      s_binom[t] = v.binom[(size_t)m * kBinomK + k];
+    }
+    __syncthreads();
+    tab = s_binom;
+    kc = L + 1;
+  }
   const int lane = threadIdx.x & 63;
   const long long wave0 = (long long)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   const long long nw = (long long)gridDim.x * kWavesPerBlock;
@@ -99,7 +113,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_kernel(
     const int base = v.rowptr[i];
     const int nn = v.rowptr[i + 1] - base - 1;
     const int y = (int)(e - base);
-    const unsigned long long total = binom(v.binom, kBinomK, nn, L);
+    const unsigned long long total = binom(tab, kc, nn, L);
     if (total == 0) { if (lane == 0) v.fp[e] = -1; continue; }
     const int j = v.nbr[e];
     const double a = v.cor[(size_t)i * p + j];
@@ -109,7 +123,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_kernel(
       bool pass = false;
       if (rank < total) {
         int q[L], S[L];
-        unrank<L>(v.binom, kBinomK, rank, nn, q);
+        unrank<L>(tab, kc, rank, nn, q);
 #pragma unroll
         for (int c = 0; c < L; ++c) S[c] = v.nbr[base + (q[c] < y ? q[c] : q[c] + 1)];
         double x[L], yv[L], C[L][L];
@@ -206,7 +220,9 @@ __global__ __launch_bounds__(256) void commit_kernel(
 template <int L>
 void launch_ln(hipStream_t s, dim3 g, dim3 b, const Views &v, long long ne,
                const int32_t *mask) {
-  hipLaunchKernelGGL(search_ln_kernel<L>, g, b, 0, s, v, ne, mask);
+  const size_t lds = (size_t)(v.max_deg + 1) * (L + 1) * sizeof(unsigned long long);
+  const int use = lds <= 48 * 1024;
+  hipLaunchKernelGGL(search_ln_kernel<L>, g, b, use ? lds : 0, s, v, ne, mask, use);
 }
 
 }  // namespace
