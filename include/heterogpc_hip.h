@@ -119,6 +119,22 @@ int hpc_compact(hpc_state *st, int *max_deg);
  * tests all adjacent pairs.  Requires hpc_compact first for level >= 1. */
 int hpc_test_level(hpc_state *st, int level, const int32_t *rows, int nrows,
                    hpc_level_stats *stats);
+/* The two halves of hpc_test_level, for a level split between executors
+ * (Balancer::execute, src/loadbalance/balancer.cpp:138-184):
+ * hpc_search_level runs the device search of rows[0..nrows) (all rows when
+ * rows == NULL) and records each ordered edge's first passing rank; level 0
+ * also removes its edges here.  hpc_commit_level then removes the edges and
+ * records the sepsets of every ordered edge searched since hpc_compact. */
+int hpc_search_level(hpc_state *st, int level, const int32_t *rows, int nrows,
+                     hpc_level_stats *stats);
+int hpc_commit_level(hpc_state *st, int level, hpc_level_stats *stats);
+/* CPUExecutor::executeLevel (src/executor/cpuExecutor.cpp:80-119): the
+ * host's share of a level >= 1 -- rows[0..nrows) searched on `threads` host
+ * threads (all cores when <= 0) with the device's arithmetic, ranks written
+ * to the device.  May run concurrently with hpc_search_level on the same
+ * state when the row sets are disjoint; hpc_commit_level after both. */
+int hpc_search_level_host(hpc_state *st, int level, const int32_t *rows,
+                          int nrows, int threads, hpc_level_stats *stats);
 /* SkeletonCalculator::calcLevel (src/independence/skeleton.cpp:104-150):
  * compaction (level >= 1) followed by hpc_test_level on all rows. */
 int hpc_level(hpc_state *st, int level, hpc_level_stats *stats);

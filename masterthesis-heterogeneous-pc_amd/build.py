@@ -19,8 +19,11 @@ BUILD = os.path.join(PKG_DIR, "_build")
 LIB = os.path.join(PKG_DIR, "libheterogpc_hip.so")
 ORACLE_DIR = os.path.join(REPO, "oracle")
 COPY_PROBE = os.path.join(PKG_DIR, "copy_probe")
+HOST_DIR = os.path.join(PKG_DIR, "host")
+CLI = os.path.join(PKG_DIR, "bin", "heterogpc")
+HOST_SOURCES = ["state.cpp", "io.cpp", "executors.cpp", "balancer.cpp", "skeleton.cpp", "main.cpp"]
 
-SOURCES = ["abi.hip", "corr.hip", "graph.hip", "search.hip"]
+SOURCES = ["abi.hip", "corr.hip", "graph.hip", "search.hip", "hostshare.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
@@ -61,6 +64,16 @@ def build_hip(jobs: int = 4) -> str:
     return LIB
 
 
+def build_cli() -> str:
+    """The heterogpc command line: C++ host code above the C ABI."""
+    os.makedirs(os.path.dirname(CLI), exist_ok=True)
+    cxx = shutil.which("g++") or "g++"
+    _run([cxx, "-O2", "-std=c++17", "-Wall", "-Wno-unused-variable", "-pthread",
+          *[os.path.join(HOST_DIR, f) for f in HOST_SOURCES],
+          "-L" + PKG_DIR, "-lheterogpc_hip", "-Wl,-rpath,$ORIGIN/..", "-o", CLI])
+    return CLI
+
+
 def build_oracle() -> str:
     make = shutil.which("make")
     if make:
@@ -75,6 +88,7 @@ def build_oracle() -> str:
 def build_all() -> None:
     build_oracle()
     build_hip()
+    build_cli()
 
 
 if __name__ == "__main__":

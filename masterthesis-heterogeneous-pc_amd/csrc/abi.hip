@@ -21,7 +21,7 @@ using namespace hpc;
 // ---------------------------------------------------------------------------
 static thread_local std::string g_err;
 
-static int fail(int code, const char *fmt, ...) {
+int hpc_fail(int code, const char *fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -30,6 +30,8 @@ static int fail(int code, const char *fmt, ...) {
   g_err = buf;
   return code;
 }
+
+#define fail hpc_fail
 
 #define HIPCHK(call)                                                        \
   do {                                                                      \
@@ -156,37 +158,6 @@ extern "C" int hpc_sem_generate(int p, int n, double degree, double wmin,
 // ---------------------------------------------------------------------------
 // state
 // ---------------------------------------------------------------------------
-struct hpc_state {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  int p = 0, n = 0, npad = 0, W = 0;
-  int max_level = 0, mc = 1;
-  double alpha = 0.0, r_crit = 0.0;
-  // observations and correlation
-  double *d_x = nullptr, *d_z = nullptr, *d_mean = nullptr, *d_sd = nullptr;
-  uint8_t *d_nanm = nullptr;
-  int32_t *d_hasnan = nullptr;
-  double *d_cor = nullptr;
-  bool have_x = false, have_cor = false, x_has_nan = false;
-  // graph
-  unsigned long long *d_adj = nullptr;
-  int32_t *d_deg = nullptr, *d_rowptr = nullptr, *d_nbr = nullptr, *d_erow = nullptr;
-  int32_t *d_rowmask = nullptr;
-  int64_t *d_fp = nullptr;
-  int32_t *d_sep = nullptr;  // [mc][p][p]
-  int8_t *d_rmlev = nullptr;
-  unsigned long long *d_binom = nullptr;
-  Counters *d_ctr = nullptr;
-  Counters *h_ctr = nullptr;  // pinned
-  // snapshot bookkeeping
-  bool compacted = false;     // neighbour lists match the adjacency
-  bool ever_compacted = false;
-  int64_t n_edges = 0;        // ordered edges in the snapshot
-  int max_deg = 0;
-  int64_t undirected = 0;     // edges currently present
-  hipEvent_t ev[7] = {};
-  bool corr_timed = false;
-};
 
 static int alloc(void **ptr, size_t bytes) {
   hipError_t e = hipMalloc(ptr, bytes ? bytes : 8);
@@ -212,6 +183,12 @@ extern "C" int hpc_state_destroy(hpc_state *st) {
   for (void *b : bufs)
     if (b) hipFree(b);
   if (st->h_ctr) hipHostFree(st->h_ctr);
+  if (st->d_scatter_idx) hipFree(st->d_scatter_idx);
+  if (st->d_scatter_val) hipFree(st->d_scatter_val);
+  if (st->copy_stream) {
+    hipStreamSynchronize(st->copy_stream);
+    hipStreamDestroy(st->copy_stream);
+  }
   for (auto &e : st->ev)
     if (e) hipEventDestroy(e);
   if (st->stream) hipStreamDestroy(st->stream);
@@ -233,6 +210,7 @@ static int upload_binom(hpc_state *st) {
   }
   HIPCHK(hipMemcpy(st->d_binom, tab.data(), tab.size() * sizeof(tab[0]),
                    hipMemcpyHostToDevice));
+  st->h_binom = std::move(tab);
   return HPC_OK;
 }
 
@@ -274,6 +252,10 @@ extern "C" int hpc_state_create(uint64_t p64, int observations, double alpha,
   st->r_crit = hpc_fisher_r_crit(observations, alpha);
   if (hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking) != hipSuccess) {
     delete st;
+    return fail(HPC_EHIP, "hipStreamCreate failed");
+  }
+  if (hipStreamCreateWithFlags(&st->copy_stream, hipStreamNonBlocking) != hipSuccess) {
+    hpc_state_destroy(st);
     return fail(HPC_EHIP, "hipStreamCreate failed");
   }
   for (auto &e : st->ev) {
@@ -369,6 +351,7 @@ extern "C" int hpc_corr(hpc_state *st, int mode) {
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(st->ev[6], st->stream));
   st->have_cor = true;
+  st->h_cor_valid = false;
   st->corr_timed = true;
   return HPC_OK;
 }
@@ -397,6 +380,7 @@ extern "C" int hpc_set_corr(hpc_state *st, const double *h_cor) {
                         hipMemcpyHostToDevice, st->stream));
   HIPCHK(hipStreamSynchronize(st->stream));
   st->have_cor = true;
+  st->h_cor_valid = false;
   return HPC_OK;
 }
 
@@ -437,6 +421,9 @@ static int compact_impl(hpc_state *st, float *ms) {
   if (rc) return rc;
   st->n_edges = (int64_t)st->h_ctr->n_edges;
   st->max_deg = st->h_ctr->max_deg;
+  // every ordered edge of the new snapshot starts with no passing set
+  if (st->n_edges > 0)
+    HIPCHK(hipMemsetAsync(st->d_fp, 0xFF, (size_t)st->n_edges * sizeof(int64_t), st->stream));
   st->compacted = true;
   st->ever_compacted = true;
   if (ms) *ms = elapsed(st->ev[0], st->ev[1]);
@@ -466,8 +453,7 @@ static int grow_sepsets(hpc_state *st, int mc) {
   return HPC_OK;
 }
 
-extern "C" int hpc_test_level(hpc_state *st, int level, const int32_t *rows,
-                              int nrows, hpc_level_stats *stats) {
+static int check_level(hpc_state *st, int level) {
   if (!st) return fail(HPC_EINVAL, "state is NULL");
   if (!st->have_cor) return fail(HPC_ESTATE, "no correlation matrix");
   if (level < 0) return fail(HPC_EINVAL, "level %d < 0", level);
@@ -475,24 +461,35 @@ extern "C" int hpc_test_level(hpc_state *st, int level, const int32_t *rows,
     return fail(HPC_ELEVEL, "level %d exceeds the device maximum %d", level, kMaxLevel);
   if (level >= 1 && !st->compacted)
     return fail(HPC_ESTATE, "level %d needs hpc_compact first", level);
+  return HPC_OK;
+}
+
+static void counters_to_stats(const hpc_state *st, hpc_level_stats &s) {
+  const Counters &c = *st->h_ctr;
+  s.tests = (int64_t)c.tests;
+  s.tests_max = (int64_t)c.tests_max;
+  s.edges = (int64_t)c.edges;
+  s.pinv_fallbacks = (int64_t)c.pinv_fallbacks;
+}
+
+extern "C" int hpc_search_level(hpc_state *st, int level, const int32_t *rows,
+                                int nrows, hpc_level_stats *stats) {
+  int rc = check_level(st, level);
+  if (rc) return rc;
   HIPCHK(hipSetDevice(st->device));
   auto t0 = std::chrono::steady_clock::now();
   hpc_level_stats s;
   memset(&s, 0, sizeof(s));
   s.level = level;
   s.max_deg = level == 0 ? st->p : st->max_deg;
+  HIPCHK(hipMemsetAsync(st->d_ctr, 0, sizeof(Counters), st->stream));
+  HIPCHK(hipEventRecord(st->ev[2], st->stream));
   if (level == 0) {
-    HIPCHK(hipMemsetAsync(st->d_ctr, 0, sizeof(Counters), st->stream));
-    HIPCHK(hipEventRecord(st->ev[2], st->stream));
+    // level 0 tests and removes in one pass (testRowL0, gpuRowTests.cu:4-27)
     launch_level0(st->stream, st->d_cor, st->d_adj, st->p, st->W, st->r_crit, st->d_ctr);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(st->ev[3], st->stream));
-    int rc = read_counters(st);
-    if (rc) return rc;
-    s.ms_test = elapsed(st->ev[2], st->ev[3]);
   } else {
     if (level > st->mc) {
-      int rc = grow_sepsets(st, level);
+      rc = grow_sepsets(st, level);
       if (rc) return rc;
     }
     const int32_t *mask = nullptr;
@@ -504,39 +501,71 @@ extern "C" int hpc_test_level(hpc_state *st, int level, const int32_t *rows,
       }
       HIPCHK(hipMemcpyAsync(st->d_rowmask, m.data(), st->p * sizeof(int32_t),
                             hipMemcpyHostToDevice, st->stream));
-      HIPCHK(hipMemsetAsync(st->d_fp, 0xFF, (size_t)st->n_edges * sizeof(int64_t), st->stream));
       mask = st->d_rowmask;
     }
-    HIPCHK(hipMemsetAsync(st->d_ctr, 0, sizeof(Counters), st->stream));
-    Views v = views(st);
-    HIPCHK(hipEventRecord(st->ev[2], st->stream));
-    if (st->max_deg - 1 >= level)
-      launch_search(st->stream, level, v, st->n_edges, mask);
-    else if (st->n_edges > 0)  // no row has enough neighbours: nothing passes
-      HIPCHK(hipMemsetAsync(st->d_fp, 0xFF, (size_t)st->n_edges * sizeof(int64_t), st->stream));
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(st->ev[3], st->stream));
-    if (st->max_deg - 1 >= level)
-      launch_commit(st->stream, level, v, st->n_edges, st->d_adj, st->W, st->d_sep,
-                    st->mc, st->d_rmlev);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(st->ev[4], st->stream));
-    int rc = read_counters(st);
-    if (rc) return rc;
-    s.ms_test = elapsed(st->ev[2], st->ev[3]);
-    s.ms_commit = elapsed(st->ev[3], st->ev[4]);
+    if (st->max_deg - 1 >= level && (!rows || nrows > 0))
+      launch_search(st->stream, level, views(st), st->n_edges, mask);
   }
-  const Counters &c = *st->h_ctr;
-  s.tests = (int64_t)c.tests;
-  s.tests_max = (int64_t)c.tests_max;
-  s.edges = (int64_t)c.edges;
-  s.removed = (int64_t)c.removed;
-  s.pinv_fallbacks = (int64_t)c.pinv_fallbacks;
-  st->undirected -= s.removed;
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(st->ev[3], st->stream));
+  rc = read_counters(st);
+  if (rc) return rc;
+  s.ms_test = elapsed(st->ev[2], st->ev[3]);
+  counters_to_stats(st, s);
+  if (level == 0) {
+    s.removed = (int64_t)st->h_ctr->removed;
+    st->undirected -= s.removed;
+    if (s.removed) st->compacted = false;
+  }
   s.edges_left = st->undirected;
-  if (s.removed) st->compacted = false;
   s.ms_wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (stats) *stats = s;
+  return HPC_OK;
+}
+
+extern "C" int hpc_commit_level(hpc_state *st, int level, hpc_level_stats *stats) {
+  int rc = check_level(st, level);
+  if (rc) return rc;
+  auto t0 = std::chrono::steady_clock::now();
+  hpc_level_stats s;
+  memset(&s, 0, sizeof(s));
+  s.level = level;
+  s.max_deg = level == 0 ? st->p : st->max_deg;
+  if (level >= 1 && st->max_deg - 1 >= level) {
+    HIPCHK(hipSetDevice(st->device));
+    HIPCHK(hipMemsetAsync(st->d_ctr, 0, sizeof(Counters), st->stream));
+    HIPCHK(hipEventRecord(st->ev[3], st->stream));
+    launch_commit(st->stream, level, views(st), st->n_edges, st->d_adj, st->W, st->d_sep,
+                  st->mc, st->d_rmlev);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(st->ev[4], st->stream));
+    rc = read_counters(st);
+    if (rc) return rc;
+    s.ms_commit = elapsed(st->ev[3], st->ev[4]);
+    s.removed = (int64_t)st->h_ctr->removed;
+    st->undirected -= s.removed;
+    if (s.removed) st->compacted = false;
+  }
+  s.edges_left = st->undirected;
+  s.ms_wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) *stats = s;
+  return HPC_OK;
+}
+
+extern "C" int hpc_test_level(hpc_state *st, int level, const int32_t *rows,
+                              int nrows, hpc_level_stats *stats) {
+  hpc_level_stats a, b;
+  int rc = hpc_search_level(st, level, rows, nrows, &a);
+  if (rc) return rc;
+  rc = hpc_commit_level(st, level, &b);
+  if (rc) return rc;
+  if (level >= 1) {
+    a.removed = b.removed;
+    a.edges_left = b.edges_left;
+    a.ms_commit = b.ms_commit;
+  }
+  a.ms_wall += b.ms_wall;
+  if (stats) *stats = a;
   return HPC_OK;
 }
 

@@ -1,0 +1,243 @@
+// Host share of a level: the CPU executor's rows (CPUExecutor::executeLevel,
+// cpuExecutor.cpp:80-119) searched on the host beside the device.
+//
+// The search runs the arithmetic of pc_math.hpp compiled for the host, so a
+// row gives the same first passing rank on either side; the ranks are then
+// scattered into the device's per-edge array and removed by the same commit.
+// Threads take (row, neighbour) work items longest row first, ten at a time,
+// as the reference's `collapse(2) schedule(dynamic, 10)` loop does.
+#pragma clang fp contract(off)
+
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <atomic>
+#include <chrono>
+#include <thread>
+#include <vector>
+
+#include "hpc_internal.hpp"
+#include "pc_math.hpp"
+
+using namespace hpc;
+using std::isnan;
+
+#define HIPCHK(call)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      return hpc_fail(HPC_EHIP, "%s failed: %s (%s:%d)", #call,             \
+                      hipGetErrorString(e_), __FILE__, __LINE__);           \
+  } while (0)
+
+namespace {
+
+struct HostSnap {
+  const double *cor;
+  const int32_t *rowptr, *nbr;
+  const unsigned long long *binom;
+  int p;
+  double r_crit;
+};
+
+inline int slot_node(const HostSnap &s, int base, int y, int q) {
+  return s.nbr[base + (q < y ? q : q + 1)];
+}
+
+// lexicographic successor of q over {0..nn-1}; false after the last set
+template <int L>
+inline bool next_comb(int (&q)[L], int nn) {
+  int i = L - 1;
+  while (i >= 0 && q[i] == nn - L + i) --i;
+  if (i < 0) return false;
+  ++q[i];
+  for (int k = i + 1; k < L; ++k) q[k] = q[k - 1] + 1;
+  return true;
+}
+
+// First passing rank of ordered edge (i, position y), ranks in order.
+template <int L>
+long long search_edge(const HostSnap &s, int i, int y, unsigned long long &tests,
+                      unsigned long long &total_out, unsigned long long &fallbacks) {
+  const int base = s.rowptr[i];
+  const int nn = s.rowptr[i + 1] - base - 1;
+  const int j = s.nbr[base + y];
+  const double *cor = s.cor;
+  const size_t p = s.p;
+  const double a = cor[(size_t)i * p + j];
+  if (L == 1) {
+    total_out = nn > 0 ? (unsigned long long)nn : 0ull;
+    for (int q = 0; q < nn; ++q) {
+      const int k = slot_node(s, base, y, q);
+      const double x2 = cor[(size_t)i * p + k], x3 = cor[(size_t)j * p + k];
+      ++tests;
+      if (!(isnan(a) || isnan(x2) || isnan(x3)) && removes(pcor_l1(a, x2, x3), s.r_crit))
+        return q;
+    }
+    return -1;
+  }
+  const unsigned long long total = binom(s.binom, kBinomK, nn, L);
+  total_out = total;
+  if (total == 0) return -1;
+  int q[L];
+  for (int c = 0; c < L; ++c) q[c] = c;
+  long long rank = 0;
+  do {
+    int S[L];
+    for (int c = 0; c < L; ++c) S[c] = slot_node(s, base, y, q[c]);
+    double x[L], yv[L], C[L][L];
+    bool anynan = isnan(a);
+    for (int c = 0; c < L; ++c) {
+      x[c] = cor[(size_t)i * p + S[c]];
+      yv[c] = cor[(size_t)j * p + S[c]];
+      anynan |= isnan(x[c]) || isnan(yv[c]);
+    }
+    for (int c = 0; c < L; ++c)
+      for (int r = c + 1; r < L; ++r) {
+        C[r][c] = cor[(size_t)S[c] * p + S[r]];
+        anynan |= isnan(C[r][c]);
+      }
+    ++tests;
+    if (!anynan) {
+      bool fb;
+      const double r = pcor_canon<L>(a, x, yv, C, fb);
+      fallbacks += fb ? 1 : 0;
+      if (removes(r, s.r_crit)) return rank;
+    }
+    ++rank;
+  } while (next_comb<L>(q, nn));
+  return -1;
+}
+
+using SearchFn = long long (*)(const HostSnap &, int, int, unsigned long long &,
+                               unsigned long long &, unsigned long long &);
+
+template <int L>
+SearchFn pick() { return &search_edge<L>; }
+
+SearchFn search_for(int level) {
+  switch (level) {
+    case 1: return pick<1>();   case 2: return pick<2>();   case 3: return pick<3>();
+    case 4: return pick<4>();   case 5: return pick<5>();   case 6: return pick<6>();
+    case 7: return pick<7>();   case 8: return pick<8>();   case 9: return pick<9>();
+    case 10: return pick<10>(); case 11: return pick<11>(); case 12: return pick<12>();
+    case 13: return pick<13>(); case 14: return pick<14>(); case 15: return pick<15>();
+    case 16: return pick<16>();
+    default: return nullptr;
+  }
+}
+
+__global__ void scatter_kernel(const int64_t *__restrict__ idx, const int64_t *__restrict__ val,
+                               long long count, int64_t *__restrict__ fp) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) fp[idx[t]] = val[t];
+}
+
+}  // namespace
+
+extern "C" int hpc_search_level_host(hpc_state *st, int level, const int32_t *rows,
+                                     int nrows, int threads, hpc_level_stats *stats) {
+  if (!st) return hpc_fail(HPC_EINVAL, "state is NULL");
+  if (level < 1 || level > kMaxLevel)
+    return hpc_fail(HPC_ELEVEL, "host share supports levels 1..%d, not %d", kMaxLevel, level);
+  if (!st->compacted) return hpc_fail(HPC_ESTATE, "level %d needs hpc_compact first", level);
+  if (!rows && nrows > 0) return hpc_fail(HPC_EINVAL, "rows is NULL");
+  auto t0 = std::chrono::steady_clock::now();
+  HIPCHK(hipSetDevice(st->device));
+  const int p = st->p;
+  // host copies of the snapshot (and the correlation, once per matrix)
+  st->h_rowptr.resize(p + 1);
+  st->h_nbr.resize(st->n_edges > 0 ? st->n_edges : 1);
+  HIPCHK(hipMemcpyAsync(st->h_rowptr.data(), st->d_rowptr, (p + 1) * sizeof(int32_t),
+                        hipMemcpyDeviceToHost, st->copy_stream));
+  if (st->n_edges > 0)
+    HIPCHK(hipMemcpyAsync(st->h_nbr.data(), st->d_nbr, st->n_edges * sizeof(int32_t),
+                          hipMemcpyDeviceToHost, st->copy_stream));
+  if (!st->h_cor_valid) {
+    st->h_cor.resize((size_t)p * p);
+    HIPCHK(hipMemcpyAsync(st->h_cor.data(), st->d_cor, (size_t)p * p * sizeof(double),
+                          hipMemcpyDeviceToHost, st->copy_stream));
+  }
+  HIPCHK(hipStreamSynchronize(st->copy_stream));
+  st->h_cor_valid = true;
+  HostSnap snap{st->h_cor.data(), st->h_rowptr.data(), st->h_nbr.data(), st->h_binom.data(),
+                p, st->r_crit};
+  // work items: (row, neighbour position), rows longest first
+  std::vector<std::pair<int, int>> order;
+  for (int k = 0; k < nrows; ++k) {
+    const int i = rows[k];
+    if (i < 0 || i >= p) return hpc_fail(HPC_EINVAL, "row %d out of range", i);
+    order.push_back({snap.rowptr[i + 1] - snap.rowptr[i], i});
+  }
+  std::stable_sort(order.begin(), order.end(),
+                   [](const std::pair<int, int> &a, const std::pair<int, int> &b) { return a.first > b.first; });
+  std::vector<std::pair<int, int>> items;
+  for (auto &o : order)
+    for (int y = 0; y < o.first; ++y) items.push_back({o.second, y});
+  std::vector<int64_t> idx(items.size()), val(items.size());
+  SearchFn fn = search_for(level);
+  std::atomic<size_t> next{0};
+  std::atomic<unsigned long long> T{0}, U{0}, E{0}, F{0};
+  if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+  auto worker = [&]() {
+    unsigned long long t = 0, u = 0, e = 0, f = 0;
+    for (;;) {
+      const size_t b = next.fetch_add(10);
+      if (b >= items.size()) break;
+      const size_t end = std::min(items.size(), b + 10);
+      for (size_t w = b; w < end; ++w) {
+        const int i = items[w].first, y = items[w].second;
+        unsigned long long tests = 0, total = 0;
+        const long long f0 = fn(snap, i, y, tests, total, f);
+        idx[w] = snap.rowptr[i] + y;
+        val[w] = f0;
+        if (total > 0) {
+          t += f0 >= 0 ? (unsigned long long)f0 + 1 : total;
+          u += total;
+          e += 1;
+        }
+      }
+    }
+    T += t; U += u; E += e; F += f;
+  };
+  std::vector<std::thread> pool;
+  for (int k = 1; k < threads; ++k) pool.emplace_back(worker);
+  worker();
+  for (auto &th : pool) th.join();
+  // scatter the ranks into the device array
+  const size_t cnt = items.size();
+  if (cnt > 0) {
+    if (cnt > st->scatter_cap) {
+      if (st->d_scatter_idx) HIPCHK(hipFree(st->d_scatter_idx));
+      if (st->d_scatter_val) HIPCHK(hipFree(st->d_scatter_val));
+      st->d_scatter_idx = st->d_scatter_val = nullptr;
+      HIPCHK(hipMalloc(&st->d_scatter_idx, cnt * sizeof(int64_t)));
+      HIPCHK(hipMalloc(&st->d_scatter_val, cnt * sizeof(int64_t)));
+      st->scatter_cap = cnt;
+    }
+    HIPCHK(hipMemcpyAsync(st->d_scatter_idx, idx.data(), cnt * sizeof(int64_t),
+                          hipMemcpyHostToDevice, st->copy_stream));
+    HIPCHK(hipMemcpyAsync(st->d_scatter_val, val.data(), cnt * sizeof(int64_t),
+                          hipMemcpyHostToDevice, st->copy_stream));
+    hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0,
+                       st->copy_stream, st->d_scatter_idx, st->d_scatter_val, (long long)cnt,
+                       st->d_fp);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st->copy_stream));
+  }
+  if (stats) {
+    hpc_level_stats s;
+    memset(&s, 0, sizeof(s));
+    s.level = level;
+    s.max_deg = st->max_deg;
+    s.tests = (int64_t)T.load();
+    s.tests_max = (int64_t)U.load();
+    s.edges = (int64_t)E.load();
+    s.pinv_fallbacks = (int64_t)F.load();
+    s.edges_left = st->undirected;
+    s.ms_wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    *stats = s;
+  }
+  return HPC_OK;
+}

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/heterogpc_hip.h"
 
@@ -64,3 +65,49 @@ void launch_commit(hipStream_t s, int level, const Views &v, int64_t n_edges,
                    int8_t *rmlev);
 
 }  // namespace hpc
+
+// One skeleton instance (include/heterogpc_hip.h).
+struct hpc_state {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int p = 0, n = 0, npad = 0, W = 0;
+  int max_level = 0, mc = 1;
+  double alpha = 0.0, r_crit = 0.0;
+  // observations and correlation
+  double *d_x = nullptr, *d_z = nullptr, *d_mean = nullptr, *d_sd = nullptr;
+  uint8_t *d_nanm = nullptr;
+  int32_t *d_hasnan = nullptr;
+  double *d_cor = nullptr;
+  bool have_x = false, have_cor = false, x_has_nan = false;
+  // graph
+  unsigned long long *d_adj = nullptr;
+  int32_t *d_deg = nullptr, *d_rowptr = nullptr, *d_nbr = nullptr, *d_erow = nullptr;
+  int32_t *d_rowmask = nullptr;
+  int64_t *d_fp = nullptr;
+  int32_t *d_sep = nullptr;  // [mc][p][p]
+  int8_t *d_rmlev = nullptr;
+  unsigned long long *d_binom = nullptr;
+  hpc::Counters *d_ctr = nullptr;
+  hpc::Counters *h_ctr = nullptr;  // pinned
+  // snapshot bookkeeping
+  bool compacted = false;     // neighbour lists match the adjacency
+  bool ever_compacted = false;
+  int64_t n_edges = 0;        // ordered edges in the snapshot
+  int max_deg = 0;
+  int64_t undirected = 0;     // edges currently present
+  hipEvent_t ev[7] = {};
+  bool corr_timed = false;
+  // host share of a level (CPU executor): a second stream for its copies and
+  // host copies of the snapshot and correlation
+  hipStream_t copy_stream = nullptr;
+  std::vector<double> h_cor;
+  bool h_cor_valid = false;
+  std::vector<int32_t> h_rowptr, h_nbr;
+  std::vector<unsigned long long> h_binom;
+  int64_t *d_scatter_idx = nullptr;
+  int64_t *d_scatter_val = nullptr;
+  size_t scatter_cap = 0;
+};
+
+// Error reporting shared by the ABI translation units.
+int hpc_fail(int code, const char *fmt, ...);

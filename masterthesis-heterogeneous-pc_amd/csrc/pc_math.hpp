@@ -1,4 +1,5 @@
-// Device arithmetic of the CI tests.
+// Arithmetic of the CI tests, shared by the device kernels and the host share
+// of a level (hostshare.hip).
 //
 // Every function performs a fixed sequence of IEEE-754 double operations
 // (+, -, *, /, sqrt, and explicit fma), compiled with -ffp-contract=off, so a
@@ -20,23 +21,23 @@ constexpr int kJacobiSweeps = 60;
 
 // calcPValue (gpuInd.cu:4-9, cpuInd.cpp:8-14) followed by the p >= alpha
 // test: NaN r counts as r = 0 (p = 1, removed); |r| is clamped at 0.9999999.
-__device__ __forceinline__ bool removes(double r, double r_crit) {
-  double re = isnan(r) ? 0.0 : fmin(kCutThr, fabs(r));
+__host__ __device__ __forceinline__ bool removes(double r, double r_crit) {
+  double re = __builtin_isnan(r) ? 0.0 : __builtin_fmin(kCutThr, __builtin_fabs(r));
   return re <= r_crit;
 }
 
 // pValL1 (cpuInd.cpp:16-27): x1 = r_ij, x2 = r_ik, x3 = r_jk.
-__device__ __forceinline__ double pcor_l1(double x1, double x2, double x3) {
-  return (x1 - x2 * x3) / sqrt((1.0 - x3 * x3) * (1.0 - x2 * x2));
+__host__ __device__ __forceinline__ double pcor_l1(double x1, double x2, double x3) {
+  return (x1 - x2 * x3) / __builtin_sqrt((1.0 - x3 * x3) * (1.0 - x2 * x2));
 }
 
-// Partial correlation -P01 / sqrt(P00 P11) of the m x m unit-diagonal matrix
+// Partial correlation -P01 / __builtin_sqrt(P00 P11) of the m x m unit-diagonal matrix
 // A (row-major, overwritten) through a symmetric cyclic Jacobi
 // eigen-decomposition and the pseudoinverse with Armadillo's default
 // tolerance m * max|lambda| * eps (cpuInd.cpp:35-36).  Taken only when the
 // Cholesky form below meets a non-positive pivot.
 template <int M>
-__device__ __noinline__ double pcor_pinv(double (&A)[M][M]) {
+__host__ __device__ __noinline__ double pcor_pinv(double (&A)[M][M]) {
   double V[M][M];
   for (int a = 0; a < M; ++a)
     for (int b = 0; b < M; ++b) V[a][b] = (a == b) ? 1.0 : 0.0;
@@ -53,9 +54,9 @@ __device__ __noinline__ double pcor_pinv(double (&A)[M][M]) {
         if (apq == 0.0) continue;
         double app = A[pp][pp], aqq = A[qq][qq];
         double theta = (aqq - app) / (2.0 * apq);
-        double t = 1.0 / (fabs(theta) + sqrt(__builtin_fma(theta, theta, 1.0)));
+        double t = 1.0 / (__builtin_fabs(theta) + __builtin_sqrt(__builtin_fma(theta, theta, 1.0)));
         if (theta < 0.0) t = -t;
-        double c = 1.0 / sqrt(__builtin_fma(t, t, 1.0));
+        double c = 1.0 / __builtin_sqrt(__builtin_fma(t, t, 1.0));
         double s = t * c;
         for (int k = 0; k < M; ++k) {
           if (k == pp || k == qq) continue;
@@ -77,29 +78,29 @@ __device__ __noinline__ double pcor_pinv(double (&A)[M][M]) {
     }
   }
   double maxabs = 0.0;
-  for (int a = 0; a < M; ++a) maxabs = fmax(maxabs, fabs(A[a][a]));
+  for (int a = 0; a < M; ++a) maxabs = __builtin_fmax(maxabs, __builtin_fabs(A[a][a]));
   double tol = (double)M * maxabs * 2.220446049250313e-16;
   double p00 = 0.0, p11 = 0.0, p01 = 0.0;
   for (int a = 0; a < M; ++a) {
     double lam = A[a][a];
-    if (!(fabs(lam) > tol)) continue;
+    if (!(__builtin_fabs(lam) > tol)) continue;
     double inv = 1.0 / lam;
     p00 = __builtin_fma(V[0][a] * V[0][a], inv, p00);
     p11 = __builtin_fma(V[1][a] * V[1][a], inv, p11);
     p01 = __builtin_fma(V[0][a] * V[1][a], inv, p01);
   }
-  return -p01 / sqrt(p00 * p11);
+  return -p01 / __builtin_sqrt(p00 * p11);
 }
 
 // Partial correlation of i and j given S (|S| = L) from a = r_ij,
 // x[c] = r_i,S[c], y[c] = r_j,S[c] and the strictly lower triangle
 // C[r][c] = r_S[c],S[r] (r > c).  With C = L L^T, u = L^-1 x, v = L^-1 y:
-// r = (a - u.v) / sqrt((1 - u.u)(1 - v.v)), the same value as -P01/sqrt(P00
+// r = (a - u.v) / __builtin_sqrt((1 - u.u)(1 - v.v)), the same value as -P01/__builtin_sqrt(P00
 // P11) for P = inverse of the (L+2)-square submatrix [i, j, S]
 // (gpuRowTests.cu:143-170).  A pivot at or below 1e-12 or a numerically
 // singular 2x2 Schur block switches to the pinv form.
 template <int L>
-__device__ __forceinline__ double pcor_canon(double a, const double (&x)[L],
+__host__ __device__ __forceinline__ double pcor_canon(double a, const double (&x)[L],
                                              const double (&y)[L],
                                              const double (&C)[L][L],
                                              bool &used_pinv) {
@@ -113,7 +114,7 @@ __device__ __forceinline__ double pcor_canon(double a, const double (&x)[L],
 #pragma unroll
     for (int k = 0; k < c; ++k) d = __builtin_fma(-Lm[c][k], Lm[c][k], d);
     sing |= !(d > kPivTol);
-    double s = sqrt(d);
+    double s = __builtin_sqrt(d);
     Lm[c][c] = s;
 #pragma unroll
     for (int r = c + 1; r < L; ++r) {
@@ -144,7 +145,7 @@ __device__ __forceinline__ double pcor_canon(double a, const double (&x)[L],
   double q = s00 * s11;
   sing |= !(s00 > kPivTol) || !(s11 > kPivTol) ||
           !(__builtin_fma(-s01, s01, q) > kPivTol * q);
-  if (!sing) return s01 / sqrt(q);
+  if (!sing) return s01 / __builtin_sqrt(q);
   used_pinv = true;
   double A[L + 2][L + 2];
   for (int r = 0; r < L + 2; ++r) A[r][r] = 1.0;
@@ -158,7 +159,7 @@ __device__ __forceinline__ double pcor_canon(double a, const double (&x)[L],
 }
 
 // Number of l-subsets of an nn-set, saturating (cuda_util.cu:20-32).
-__device__ __forceinline__ unsigned long long binom(const unsigned long long *tab,
+__host__ __device__ __forceinline__ unsigned long long binom(const unsigned long long *tab,
                                                     int kcols, int nn, int l) {
   if (nn < l || l < 0) return 0ull;
   return tab[(size_t)nn * kcols + l];
@@ -173,7 +174,7 @@ __device__ __forceinline__ unsigned long long binom(const unsigned long long *ta
 // C(m, k) < C(nn-v0, k) - rank: a binary search of log2(nn) table reads per
 // member instead of a walk over every candidate position.
 template <int L>
-__device__ __forceinline__ void unrank(const unsigned long long *tab, int kcols,
+__host__ __device__ __forceinline__ void unrank(const unsigned long long *tab, int kcols,
                                        unsigned long long rank, int nn, int (&q)[L]) {
   int v0 = 0;
 #pragma unroll

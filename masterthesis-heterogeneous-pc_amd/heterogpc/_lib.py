@@ -61,6 +61,9 @@ SIGNATURES = {
     "hpc_corr_ms": (_i, [_vp, C.POINTER(_d), C.POINTER(_d)]),
     "hpc_compact": (_i, [_vp, C.POINTER(C.c_int)]),
     "hpc_test_level": (_i, [_vp, _i, _vp, _i, _ps]),
+    "hpc_search_level": (_i, [_vp, _i, _vp, _i, _ps]),
+    "hpc_commit_level": (_i, [_vp, _i, _ps]),
+    "hpc_search_level_host": (_i, [_vp, _i, _vp, _i, _i, _ps]),
     "hpc_level": (_i, [_vp, _i, _ps]),
     "hpc_run": (_i, [_vp, _i, _ps, _i, C.POINTER(C.c_int)]),
     "hpc_get_cor": (_i, [_vp, _vp]),

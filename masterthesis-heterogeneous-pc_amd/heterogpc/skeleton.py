@@ -108,6 +108,27 @@ class DeviceSkeleton:
             check(self.lib.hpc_test_level(self.h, int(level), _ptr(r), len(r), C.byref(s)))
         return s.as_dict()
 
+    def search_level(self, level: int, rows=None, host: bool = False, threads: int = 0) -> dict:
+        """Search rows (all when None) on the device, or on the host when
+        host=True; hpc_commit_level then removes the edges."""
+        s = LevelStats()
+        r = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
+        ptr, cnt = (None, 0) if r is None else (_ptr(r), len(r))
+        if host:
+            if r is None:
+                r = np.arange(self.p, dtype=np.int32)
+                ptr, cnt = _ptr(r), len(r)
+            check(self.lib.hpc_search_level_host(self.h, int(level), ptr, cnt, int(threads),
+                                                 C.byref(s)))
+        else:
+            check(self.lib.hpc_search_level(self.h, int(level), ptr, cnt, C.byref(s)))
+        return s.as_dict()
+
+    def commit_level(self, level: int) -> dict:
+        s = LevelStats()
+        check(self.lib.hpc_commit_level(self.h, int(level), C.byref(s)))
+        return s.as_dict()
+
     def run(self, max_level: int = -2, cap: int = 64) -> list[dict]:
         arr = (LevelStats * cap)()
         ran = C.c_int()

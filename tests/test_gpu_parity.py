@@ -120,6 +120,31 @@ def test_c2_full_parity(hpc, orc):
     _compare_run(hpc, orc, x, n, alpha, ml, label="c2")
 
 
+def test_host_share_matches_device(hpc, orc):
+    """Rows split between the device and host threads (the heterogeneous
+    executor) give the device-only first passes, adjacency and sepsets."""
+    x, n, alpha, _ = workloads.sem("small")
+    p = x.shape[0]
+    full = hpc.DeviceSkeleton(p, n, alpha, max_level=5)
+    mixed = hpc.DeviceSkeleton(p, n, alpha, max_level=5)
+    full.pmcc(x)
+    mixed.pmcc(x)
+    full.level(0)
+    mixed.level(0)
+    for level in range(1, 6):
+        full.level(level)
+        mixed.compact()
+        rows = np.arange(p)
+        mixed.search_level(level, rows[rows % 3 != 0])
+        mixed.search_level(level, rows[rows % 3 == 0], host=True, threads=4)
+        mixed.commit_level(level)
+        fa, _ = full.first_pass()
+        fb, _ = mixed.first_pass()
+        assert np.array_equal(fa, fb), level
+        assert np.array_equal(full.adj(), mixed.adj()), level
+    assert np.array_equal(full.sepsets(5), mixed.sepsets(5))
+
+
 def test_row_subset_then_rest(hpc, orc):
     """hpc_test_level on two disjoint row sets gives the same first passes."""
     x, n, alpha, _ = workloads.sem("small")
