@@ -94,6 +94,13 @@ int hpc_sync(hpc_state *st);
 /* Current maxCondSize (sepset width). */
 int hpc_max_cond_size(const hpc_state *st);
 
+/* Launch-shape knobs (set by the autotuner; results do not depend on them). */
+enum {
+  HPC_TUNE_RANKS_PER_TASK = 1 /* conditioning sets per search task at levels
+                                 >= 2: a multiple of 64, default 512 */
+};
+int hpc_set_tuning(hpc_state *st, int param, int value);
+
 /* ---- correlation (src/correlation/corOwn.cu:10-48) ---------------------- */
 /* gpuPMCC(h_mat, p, n, state.cor, device): uploads the variable-major
  * observations and computes the correlation matrix on the device. */

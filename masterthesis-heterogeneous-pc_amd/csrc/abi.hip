@@ -179,7 +179,8 @@ extern "C" int hpc_state_destroy(hpc_state *st) {
   void *bufs[] = {st->d_x, st->d_z, st->d_mean, st->d_sd, st->d_nanm,
                   st->d_hasnan, st->d_cor, st->d_adj, st->d_deg, st->d_rowptr,
                   st->d_nbr, st->d_erow, st->d_rowmask, st->d_fp, st->d_sep,
-                  st->d_rmlev, st->d_binom, st->d_ctr};
+                  st->d_rmlev, st->d_binom, st->d_ctr, st->d_row_tasks,
+                  st->d_task_off, st->d_tasks};
   for (void *b : bufs)
     if (b) hipFree(b);
   if (st->h_ctr) hipHostFree(st->h_ctr);
@@ -275,6 +276,8 @@ extern "C" int hpc_state_create(uint64_t p64, int observations, double alpha,
   ALLOC(st->d_fp, pp * sizeof(int64_t));
   ALLOC(st->d_sep, (size_t)st->mc * pp * sizeof(int32_t));
   ALLOC(st->d_rmlev, pp);
+  ALLOC(st->d_row_tasks, p * sizeof(long long));
+  ALLOC(st->d_task_off, p * sizeof(long long));
   ALLOC(st->d_binom, (p + 1) * kBinomK * sizeof(unsigned long long));
   ALLOC(st->d_ctr, sizeof(Counters));
   if (hipHostMalloc((void **)&st->h_ctr, sizeof(Counters), hipHostMallocDefault) != hipSuccess) {
@@ -297,6 +300,19 @@ extern "C" int hpc_sync(hpc_state *st) {
 }
 
 extern "C" int hpc_max_cond_size(const hpc_state *st) { return st ? st->mc : 0; }
+
+extern "C" int hpc_set_tuning(hpc_state *st, int param, int value) {
+  if (!st) return fail(HPC_EINVAL, "state is NULL");
+  switch (param) {
+    case HPC_TUNE_RANKS_PER_TASK:
+      if (value < 64 || value % 64 != 0 || value > (1 << 24))
+        return fail(HPC_EINVAL, "ranks per task must be a multiple of 64 in [64, 2^24]");
+      st->ranks_per_task = value;
+      return HPC_OK;
+    default:
+      return fail(HPC_EINVAL, "unknown tuning parameter %d", param);
+  }
+}
 
 static float elapsed(hipEvent_t a, hipEvent_t b) {
   float ms = 0.f;
@@ -503,8 +519,31 @@ extern "C" int hpc_search_level(hpc_state *st, int level, const int32_t *rows,
                             hipMemcpyHostToDevice, st->stream));
       mask = st->d_rowmask;
     }
-    if (st->max_deg - 1 >= level && (!rows || nrows > 0))
-      launch_search(st->stream, level, views(st), st->n_edges, mask);
+    if (st->max_deg - 1 >= level && (!rows || nrows > 0)) {
+      const Views v = views(st);
+      if (level == 1) {
+        launch_search_l1(st->stream, v, st->n_edges, mask);
+      } else {
+        const int rpt = st->ranks_per_task;
+        launch_task_plan(st->stream, level, v, rpt, mask, st->d_row_tasks, st->d_task_off);
+        HIPCHK(hipGetLastError());
+        rc = read_counters(st);
+        if (rc) return rc;
+        const size_t n_tasks = (size_t)st->h_ctr->n_tasks;
+        if (n_tasks > st->task_cap) {
+          if (st->d_tasks) HIPCHK(hipFree(st->d_tasks));
+          st->d_tasks = nullptr;
+          st->task_cap = 0;
+          const size_t cap = n_tasks + n_tasks / 4 + 1024;
+          rc = alloc((void **)&st->d_tasks, cap * sizeof(unsigned long long));
+          if (rc) return rc;
+          st->task_cap = cap;
+        }
+        launch_task_fill(st->stream, level, v, rpt, st->d_row_tasks, st->d_task_off, st->d_tasks);
+        HIPCHK(hipGetLastError());
+        launch_search_ln(st->stream, level, v, st->d_tasks, (long long)n_tasks, rpt);
+      }
+    }
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(st->ev[3], st->stream));
@@ -536,12 +575,14 @@ extern "C" int hpc_commit_level(hpc_state *st, int level, hpc_level_stats *stats
     HIPCHK(hipMemsetAsync(st->d_ctr, 0, sizeof(Counters), st->stream));
     HIPCHK(hipEventRecord(st->ev[3], st->stream));
     launch_commit(st->stream, level, views(st), st->n_edges, st->d_adj, st->W, st->d_sep,
-                  st->mc, st->d_rmlev);
+                  st->d_rmlev);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(st->ev[4], st->stream));
     rc = read_counters(st);
     if (rc) return rc;
     s.ms_commit = elapsed(st->ev[3], st->ev[4]);
+    counters_to_stats(st, s);
+    s.pinv_fallbacks = 0;
     s.removed = (int64_t)st->h_ctr->removed;
     st->undirected -= s.removed;
     if (s.removed) st->compacted = false;
@@ -560,6 +601,9 @@ extern "C" int hpc_test_level(hpc_state *st, int level, const int32_t *rows,
   rc = hpc_commit_level(st, level, &b);
   if (rc) return rc;
   if (level >= 1) {
+    a.tests = b.tests;
+    a.tests_max = b.tests_max;
+    a.edges = b.edges;
     a.removed = b.removed;
     a.edges_left = b.edges_left;
     a.ms_commit = b.ms_commit;

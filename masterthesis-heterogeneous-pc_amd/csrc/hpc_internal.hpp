@@ -22,6 +22,7 @@ struct Counters {
   int max_deg;                   // largest row length at level start
   int pad_;
   unsigned long long pinv_fallbacks;  // tests that took the pinv form
+  unsigned long long n_tasks;         // search tasks of a level >= 2
 };
 
 // Largest level the device kernels accept (conditioning-set size).
@@ -58,11 +59,19 @@ void launch_level0(hipStream_t s, const double *cor, unsigned long long *adj,
 void launch_compact(hipStream_t s, const unsigned long long *adj, int p, int W,
                     int32_t *deg, int32_t *rowptr, int32_t *nbr, int32_t *erow,
                     Counters *ctr);
-void launch_search(hipStream_t s, int level, const Views &v, int64_t n_edges,
-                   const int32_t *row_mask);
+void launch_search_l1(hipStream_t s, const Views &v, int64_t n_edges,
+                      const int32_t *row_mask);
+// Levels >= 2: per-row task counts and their scan (total to ctr->n_tasks),
+// then the task list, then the search over it.
+void launch_task_plan(hipStream_t s, int level, const Views &v, int rpt,
+                      const int32_t *row_mask, long long *row_tasks, long long *task_off);
+void launch_task_fill(hipStream_t s, int level, const Views &v, int rpt,
+                      const long long *row_tasks, const long long *task_off,
+                      unsigned long long *tasks);
+void launch_search_ln(hipStream_t s, int level, const Views &v,
+                      const unsigned long long *tasks, long long n_tasks, int rpt);
 void launch_commit(hipStream_t s, int level, const Views &v, int64_t n_edges,
-                   unsigned long long *adj, int W, int32_t *sep, int mc,
-                   int8_t *rmlev);
+                   unsigned long long *adj, int W, int32_t *sep, int8_t *rmlev);
 
 }  // namespace hpc
 
@@ -104,6 +113,11 @@ struct hpc_state {
   bool h_cor_valid = false;
   std::vector<int32_t> h_rowptr, h_nbr;
   std::vector<unsigned long long> h_binom;
+  // level >= 2 search tasks
+  long long *d_row_tasks = nullptr, *d_task_off = nullptr;
+  unsigned long long *d_tasks = nullptr;
+  size_t task_cap = 0;
+  int ranks_per_task = 512;
   int64_t *d_scatter_idx = nullptr;
   int64_t *d_scatter_val = nullptr;
   size_t scatter_cap = 0;

@@ -14,6 +14,7 @@ HEADER = os.path.join(os.path.dirname(PKG_ROOT), "include", "heterogpc_hip.h")
 
 HPC_OK = 0
 HPC_CORR_CANON = 0
+HPC_TUNE_RANKS_PER_TASK = 1
 
 
 class LevelStats(C.Structure):
@@ -54,6 +55,7 @@ SIGNATURES = {
     "hpc_reset": (_i, [_vp]),
     "hpc_sync": (_i, [_vp]),
     "hpc_max_cond_size": (_i, [_vp]),
+    "hpc_set_tuning": (_i, [_vp, _i, _i]),
     "hpc_pmcc": (_i, [_vp, _vp, _i]),
     "hpc_set_observations": (_i, [_vp, _vp]),
     "hpc_corr": (_i, [_vp, _i]),

@@ -82,6 +82,9 @@ class DeviceSkeleton:
         assert cor.shape == (self.p, self.p)
         check(self.lib.hpc_set_corr(self.h, _ptr(cor)))
 
+    def set_ranks_per_task(self, value: int) -> None:
+        check(self.lib.hpc_set_tuning(self.h, 1, int(value)))
+
     # ---- levels ----------------------------------------------------------
     def reset(self) -> None:
         check(self.lib.hpc_reset(self.h))

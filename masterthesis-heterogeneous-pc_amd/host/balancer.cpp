@@ -108,8 +108,10 @@ void Balancer::commit(int level, std::tuple<TestResult, TestResult> &res) {
   hpcCheck(hpc_commit_level(state->dev, level, &s), "level commit");
   const auto &cpu = std::get<0>(res);
   const auto &gpu = std::get<1>(res);
+  // the canonical count of the level, known once both shares are in
+  const int64_t tests = level == 0 ? cpu.tests + gpu.tests : s.tests;
   if (verbose)
-    std::cout << "Order " << level << " finished with " << cpu.tests + gpu.tests << " tests in "
+    std::cout << "Order " << level << " finished with " << tests << " tests in "
               << std::max(cpu.duration, gpu.duration) << " ms.\n"
               << std::endl;
 }
