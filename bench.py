@@ -79,6 +79,124 @@ def cpu_baseline(x, n, alpha, max_level, tests_total):
     }
 
 
+def _pct(vals, q):
+    import numpy as np
+
+    return float(np.percentile(np.asarray(vals, dtype=np.float64), q)) if vals else 0.0
+
+
+def _plain_skeleton(heterogpc, x, alpha, max_level):
+    """One instance through the plain single-instance path: state, upload,
+    correlation, levels, results to the host."""
+    p, n = x.shape
+    with heterogpc.DeviceSkeleton(p, n, alpha, max_level=max_level) as dev:
+        dev.pmcc(x)
+        stats = dev.run(-2)
+        return dev.adj(), dev.sepsets(), stats
+
+
+def bench_batched(heterogpc, insts):
+    """Batched form: the same instances one per launch sequence (plain) and all
+    in one batch; skeletons/hour of both and whether every result agrees."""
+    for x, a, ml in insts[:4]:
+        _plain_skeleton(heterogpc, x, a, ml)
+    with heterogpc.Batch() as b:
+        for x, a, ml in insts[:8]:
+            b.add(x, a, ml)
+        b.run()
+    heterogpc.load().hpc_device_synchronize(0)
+    t0 = time.perf_counter()
+    plain = [_plain_skeleton(heterogpc, x, a, ml) for x, a, ml in insts]
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    with heterogpc.Batch() as b:
+        ids = [b.add(x, a, ml) for x, a, ml in insts]
+        steps = b.run()
+        got = [(b.adj(i), b.sepsets(i, pl[1].shape[2])) for i, pl in zip(ids, plain)]
+    t_batch = time.perf_counter() - t0
+    same = all((ga == pa).all() and (gs == ps).all() for (ga, gs), (pa, ps, _) in zip(got, plain))
+    tests = sum(lv["tests"] for _, _, st in plain for lv in st)
+    k = len(insts)
+    return {
+        "workload": "%d SEM instances, p U[20,400], n U[200,5000], degree U[2,6], alpha "
+                    "{0.01,0.05}, FP32 data, levels until no edge has enough neighbours" % k,
+        "instances": k,
+        "plain_s": t_plain,
+        "batched_s": t_batch,
+        "batched_steps": steps,
+        "plain_skeletons_per_hour": k / t_plain * 3600.0,
+        "batched_skeletons_per_hour": k / t_batch * 3600.0,
+        "speedup": t_plain / t_batch,
+        "tests_total": tests,
+        "batched_ci_tests_per_s": tests / t_batch,
+        "results_identical": bool(same),
+        "scope": "host observations -> device, correlation, all levels, adjacency and "
+                 "sepsets back on the host",
+    }
+
+
+def bench_served(heterogpc, trace, rate):
+    """Served form: the arrival trace replayed in real time through the
+    continuous-batching server and through the plain path (one request at a
+    time, in arrival order); latency = arrival -> result ready."""
+    import threading
+
+    x0 = trace[0]
+    _plain_skeleton(heterogpc, x0[1], x0[2], x0[3])
+    n_req = len(trace)
+    sub = [None] * n_req
+    with heterogpc.Server() as srv:
+        t0 = time.perf_counter()
+
+        def feeder():
+            for k, (ta, x, a, ml) in enumerate(trace):
+                d = ta - (time.perf_counter() - t0)
+                if d > 0:
+                    time.sleep(d)
+                ts = time.perf_counter() - t0
+                sub[k] = (srv.submit(x, a, ml), ts)
+
+        th = threading.Thread(target=feeder)
+        th.start()
+        th.join()
+        served = []
+        for k in range(n_req):
+            req, ts = sub[k]
+            srv.wait(req)
+            adj, sep, info = srv.result(req)
+            served.append((adj, sep, info, ts))
+        stats = srv.stats()
+    s_lat = [info["latency_ms"] for _, _, info, _ in served]
+    s_end = max(ts + info["latency_ms"] * 1e-3 for _, _, info, ts in served)
+    t0 = time.perf_counter()
+    plain_lat, plain = [], []
+    for ta, x, a, ml in trace:
+        d = ta - (time.perf_counter() - t0)
+        if d > 0:
+            time.sleep(d)
+        plain.append(_plain_skeleton(heterogpc, x, a, ml))
+        plain_lat.append((time.perf_counter() - t0 - ta) * 1e3)
+    p_end = time.perf_counter() - t0
+    same = all((sa == pa).all() and (ss[:, :, :ps.shape[2]] == ps).all()
+               for (sa, ss, _, _), (pa, ps, _) in zip(served, plain))
+    return {
+        "workload": "%d requests, Poisson arrivals at %.0f/s, p log-uniform [50,2000], n U[500,"
+                    "10000], degree U[2,5], alpha {0.01,0.05}, FP32 data, max level 4"
+                    % (n_req, rate),
+        "requests": n_req,
+        "served_p50_ms": _pct(s_lat, 50),
+        "served_p99_ms": _pct(s_lat, 99),
+        "plain_p50_ms": _pct(plain_lat, 50),
+        "plain_p99_ms": _pct(plain_lat, 99),
+        "served_skeletons_per_hour": n_req / s_end * 3600.0,
+        "plain_skeletons_per_hour": n_req / p_end * 3600.0,
+        "served_makespan_s": s_end,
+        "plain_makespan_s": p_end,
+        "server": {k: stats[k] for k in ("steps", "peak_active", "peak_bytes", "budget", "busy_ms")},
+        "results_identical": bool(same),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +204,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2k")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=256,
+                    help="instances of the batched form (0 = skip)")
+    ap.add_argument("--served-count", type=int, default=200,
+                    help="requests of the served form (0 = skip)")
+    ap.add_argument("--served-rate", type=float, default=100.0,
+                    help="Poisson arrival rate of the served form, requests/s")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -168,9 +292,17 @@ def main() -> int:
         "corr_tflops": corr_tflops,
         "roofline": roof,
     }
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(x, n, alpha, max_level, tests_total)
     dev.close()
+    if args.batch_count > 0:
+        print("[bench] batched form", file=sys.stderr, flush=True)
+        out["batched"] = bench_batched(heterogpc, workloads.batch_mix(args.batch_count))
+    if args.served_count > 0:
+        print("[bench] served form", file=sys.stderr, flush=True)
+        out["served"] = bench_served(heterogpc, workloads.served_trace(
+            args.served_count, args.served_rate), args.served_rate)
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(x, n, alpha, max_level, tests_total)
     print(json.dumps(out))
     return 0
 

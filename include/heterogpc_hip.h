@@ -164,6 +164,88 @@ int hpc_get_sepsets(hpc_state *st, int32_t *sep, int mc);
 int hpc_get_first_pass(hpc_state *st, int64_t *fp, int32_t *rowptr,
                        int64_t *n_edges);
 
+/* ---- batched skeletons --------------------------------------------------
+ * No counterpart in the reference, which runs one dataset per process
+ * (src/main.cpp:57-151).  A batch holds many independent instances on one
+ * device; each step runs the next level of every unfinished instance in one
+ * set of launches (level 0, compaction, the CI-test kernels of every level
+ * present and the removal kernel), whatever their p, n, alpha and level.
+ * Per instance the results equal a single hpc_state run on the same input:
+ * the arithmetic of a test and the sepset choice do not depend on the other
+ * instances.  Instance ids are small non-negative ints, reused after
+ * hpc_batch_remove. */
+typedef struct hpc_batch hpc_batch;
+
+/* Device bytes one instance of (p, n, max_level) needs at most. */
+int hpc_footprint(uint64_t p, int n, int max_level, uint64_t *bytes);
+int hpc_batch_create(int device, hpc_batch **out);
+int hpc_batch_destroy(hpc_batch *b);
+/* New instance on observations x ([p][n], as hpc_set_observations); max_level
+ * as the state's (-1 = until no row has enough neighbours). */
+int hpc_batch_add(hpc_batch *b, uint64_t p, int n, double alpha, int max_level,
+                  const double *x, int *id);
+int hpc_batch_remove(hpc_batch *b, int id);
+/* One level of every unfinished instance (correlation first for those that
+ * have none); *unfinished receives how many remain. */
+int hpc_batch_step(hpc_batch *b, int *unfinished);
+/* Steps until every instance has finished; *steps receives the count. */
+int hpc_batch_run(hpc_batch *b, int *steps);
+int hpc_batch_status(hpc_batch *b, int id, int *finished, int *levels, int *error);
+int hpc_batch_get_stats(hpc_batch *b, int id, hpc_level_stats *stats, int cap,
+                        int *levels);
+int hpc_batch_max_cond_size(hpc_batch *b, int id);
+int hpc_batch_get_cor(hpc_batch *b, int id, double *cor);
+int hpc_batch_get_adj(hpc_batch *b, int id, int32_t *adj);
+int hpc_batch_get_sepsets(hpc_batch *b, int id, int32_t *sep, int mc);
+int hpc_batch_set_tuning(hpc_batch *b, int param, int value);
+
+/* ---- continuous-batching server -------------------------------------------
+ * No counterpart in the reference.  A worker thread admits submitted requests
+ * in arrival order while the sum of their hpc_footprint stays within the HBM
+ * budget (and at most max_active run), steps every admitted request together
+ * and retires each as soon as it finishes; results are copied to host memory
+ * and the request's device blocks freed.  A request's result is the same as
+ * a single hpc_state run on its input. */
+typedef struct hpc_server hpc_server;
+
+typedef struct hpc_request_info {
+  int32_t p;
+  int32_t levels;         /* levels run */
+  int32_t max_cond_size;  /* sepset width held */
+  int32_t pad_;
+  int64_t tests;          /* canonical CI tests over all levels */
+  double queue_ms;        /* submit -> admission */
+  double latency_ms;      /* submit -> result ready */
+} hpc_request_info;
+
+typedef struct hpc_server_info {
+  int64_t steps;          /* batch steps run */
+  int64_t peak_active;    /* most requests admitted at once */
+  uint64_t peak_bytes;    /* largest admitted footprint */
+  uint64_t budget;        /* HBM budget in bytes */
+  double busy_ms;         /* wall time spent in steps */
+  int32_t queued, running;
+} hpc_server_info;
+
+/* hbm_budget 0 = 80% of the device's free memory; max_active <= 0 = 1024. */
+int hpc_server_create(int device, uint64_t hbm_budget, int max_active,
+                      hpc_server **out);
+/* Stops the worker; queued requests are dropped, running ones finish first. */
+int hpc_server_destroy(hpc_server *s);
+/* Copies x; *req receives the request id.  HPC_ENOMEM when the request alone
+ * exceeds the budget. */
+int hpc_server_submit(hpc_server *s, uint64_t p, int n, double alpha,
+                      int max_level, const double *x, int64_t *req);
+/* Waits up to timeout_s seconds (< 0: no limit); *done = 1 when finished. */
+int hpc_server_wait(hpc_server *s, int64_t req, double timeout_s, int *done);
+/* Adjacency (p*p) and sepsets (width mc) of a finished request; either may be
+ * NULL.  Returns the request's error when it failed. */
+int hpc_server_result(hpc_server *s, int64_t req, int32_t *adj, int32_t *sep,
+                      int mc, hpc_request_info *info);
+/* Drops a finished request's results. */
+int hpc_server_release(hpc_server *s, int64_t req);
+int hpc_server_stats(hpc_server *s, hpc_server_info *info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,7 +23,8 @@ HOST_DIR = os.path.join(PKG_DIR, "host")
 CLI = os.path.join(PKG_DIR, "bin", "heterogpc")
 HOST_SOURCES = ["state.cpp", "io.cpp", "executors.cpp", "balancer.cpp", "skeleton.cpp", "main.cpp"]
 
-SOURCES = ["abi.hip", "corr.hip", "graph.hip", "search.hip", "hostshare.hip"]
+SOURCES = ["abi.hip", "engine.hip", "corr.hip", "graph.hip", "search.hip", "hostshare.hip",
+           "batch.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [

@@ -1,0 +1,427 @@
+// Batched and served forms of the skeleton (no counterpart in the reference):
+//
+//   hpc_batch  -- many independent instances advanced together: every step
+//                 runs one level of every unfinished instance through one set
+//                 of launches (engine_step), whatever their p, n, alpha and
+//                 level.
+//   hpc_server -- continuous batching: a worker thread admits submitted
+//                 requests while their device footprint fits a budget, steps
+//                 every admitted instance, retires finished ones and copies
+//                 their results out.  A request's result is the same alone and
+//                 in any mix: the per-test arithmetic and the sepset choice do
+//                 not depend on what else is in a step.
+#pragma clang fp contract(off)
+
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "hpc_internal.hpp"
+
+using namespace hpc;
+
+// ---------------------------------------------------------------------------
+// batch
+// ---------------------------------------------------------------------------
+struct hpc_batch {
+  Engine eng;
+  int steps = 0;
+};
+
+extern "C" int hpc_footprint(uint64_t p, int n, int max_level, uint64_t *bytes) {
+  if (!bytes || p < 2 || n < 1) return hpc_fail(HPC_EINVAL, "bad argument");
+  *bytes = engine_footprint((int)p, n, max_level);
+  return HPC_OK;
+}
+
+extern "C" int hpc_batch_create(int device, hpc_batch **out) {
+  if (!out) return hpc_fail(HPC_EINVAL, "out is NULL");
+  *out = nullptr;
+  hpc_batch *b = new hpc_batch();
+  int rc = engine_init(b->eng, device);
+  if (rc) { engine_free(b->eng); delete b; return rc; }
+  *out = b;
+  return HPC_OK;
+}
+
+extern "C" int hpc_batch_destroy(hpc_batch *b) {
+  if (!b) return HPC_OK;
+  engine_free(b->eng);
+  delete b;
+  return HPC_OK;
+}
+
+static int check_id(hpc_batch *b, int id) {
+  if (!b) return hpc_fail(HPC_EINVAL, "batch is NULL");
+  if (id < 0 || id >= (int)b->eng.inst.size() || !b->eng.inst[id].used)
+    return hpc_fail(HPC_EINVAL, "no instance %d", id);
+  return HPC_OK;
+}
+
+extern "C" int hpc_batch_add(hpc_batch *b, uint64_t p, int n, double alpha, int max_level,
+                             const double *x, int *id) {
+  if (!b || !x || !id) return hpc_fail(HPC_EINVAL, "NULL argument");
+  int rc = engine_add(b->eng, (int)p, n, alpha, max_level, id);
+  if (rc) return rc;
+  rc = engine_set_observations(b->eng, *id, x);
+  if (rc) { engine_remove(b->eng, *id); return rc; }
+  return HPC_OK;
+}
+
+extern "C" int hpc_batch_remove(hpc_batch *b, int id) {
+  int rc = check_id(b, id);
+  if (rc) return rc;
+  engine_remove(b->eng, id);
+  return HPC_OK;
+}
+
+static std::vector<int> live(Engine &e) {
+  std::vector<int> ids;
+  for (size_t k = 0; k < e.inst.size(); ++k)
+    if (e.inst[k].used && !e.inst[k].finished) ids.push_back((int)k);
+  return ids;
+}
+
+extern "C" int hpc_batch_step(hpc_batch *b, int *unfinished) {
+  if (!b) return hpc_fail(HPC_EINVAL, "batch is NULL");
+  std::vector<int> ids = live(b->eng);
+  int rc = engine_step(b->eng, ids);
+  if (rc) return rc;
+  ++b->steps;
+  if (unfinished) *unfinished = (int)live(b->eng).size();
+  return HPC_OK;
+}
+
+extern "C" int hpc_batch_run(hpc_batch *b, int *steps) {
+  if (!b) return hpc_fail(HPC_EINVAL, "batch is NULL");
+  int n = 0, left = 0;
+  do {
+    int rc = hpc_batch_step(b, &left);
+    if (rc) return rc;
+    ++n;
+  } while (left > 0);
+  if (steps) *steps = n;
+  return HPC_OK;
+}
+
+extern "C" int hpc_batch_status(hpc_batch *b, int id, int *finished, int *levels, int *error) {
+  int rc = check_id(b, id);
+  if (rc) return rc;
+  const InstHost &h = b->eng.inst[id];
+  if (finished) *finished = h.finished ? 1 : 0;
+  if (levels) *levels = (int)h.hist.size();
+  if (error) *error = h.error;
+  return HPC_OK;
+}
+
+extern "C" int hpc_batch_get_stats(hpc_batch *b, int id, hpc_level_stats *stats, int cap,
+                                   int *levels) {
+  int rc = check_id(b, id);
+  if (rc) return rc;
+  const InstHost &h = b->eng.inst[id];
+  for (int k = 0; k < (int)h.hist.size() && k < cap; ++k) stats[k] = h.hist[k];
+  if (levels) *levels = (int)h.hist.size();
+  return HPC_OK;
+}
+
+extern "C" int hpc_batch_max_cond_size(hpc_batch *b, int id) {
+  return check_id(b, id) ? 0 : b->eng.inst[id].mc;
+}
+
+extern "C" int hpc_batch_get_adj(hpc_batch *b, int id, int32_t *adj) {
+  int rc = check_id(b, id);
+  if (rc) return rc;
+  if (!adj) return hpc_fail(HPC_EINVAL, "adj is NULL");
+  return export_adj(b->eng, id, adj);
+}
+
+extern "C" int hpc_batch_get_sepsets(hpc_batch *b, int id, int32_t *sep, int mc) {
+  int rc = check_id(b, id);
+  if (rc) return rc;
+  if (!sep || mc < 1) return hpc_fail(HPC_EINVAL, "bad argument");
+  return export_sepsets(b->eng, id, sep, mc);
+}
+
+extern "C" int hpc_batch_get_cor(hpc_batch *b, int id, double *cor) {
+  int rc = check_id(b, id);
+  if (rc) return rc;
+  if (!cor) return hpc_fail(HPC_EINVAL, "cor is NULL");
+  return export_cor(b->eng, id, cor);
+}
+
+extern "C" int hpc_batch_set_tuning(hpc_batch *b, int param, int value) {
+  if (!b) return hpc_fail(HPC_EINVAL, "batch is NULL");
+  if (param != HPC_TUNE_RANKS_PER_TASK || value < 64 || value % 64 != 0 || value > (1 << 24))
+    return hpc_fail(HPC_EINVAL, "bad tuning parameter");
+  b->eng.ranks_per_task = value;
+  return HPC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// server
+// ---------------------------------------------------------------------------
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+  int64_t id = 0;
+  int p = 0, n = 0, max_level = 0;
+  double alpha = 0.0;
+  std::vector<double> x;
+  size_t bytes = 0;
+  int slot = -1;       // engine instance while admitted
+  int state = 0;       // 0 queued, 1 running, 2 done, 3 failed
+  int error = 0;
+  std::string message;
+  Clock::time_point t_submit, t_admit, t_done;
+  std::vector<int32_t> adj, sep;
+  int mc = 0;
+  int levels = 0;
+  int64_t tests = 0;
+};
+
+}  // namespace
+
+struct hpc_server {
+  Engine eng;
+  size_t budget = 0, in_use = 0;
+  int max_active = 0;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<std::shared_ptr<Request>> queue;
+  std::vector<std::shared_ptr<Request>> running;
+  std::map<int64_t, std::shared_ptr<Request>> all;
+  int64_t next_id = 1;
+  bool stop = false;
+  std::thread worker;
+  // statistics
+  int64_t steps = 0;
+  int peak_active = 0;
+  size_t peak_bytes = 0;
+  double busy_ms = 0.0;
+};
+
+static void server_finish(hpc_server *s, std::shared_ptr<Request> &r) {
+  InstHost &h = s->eng.inst[r->slot];
+  r->levels = (int)h.hist.size();
+  for (auto &l : h.hist) r->tests += l.tests;
+  r->mc = h.mc;
+  r->adj.resize((size_t)r->p * r->p);
+  r->sep.resize((size_t)r->p * r->p * r->mc);
+  int rc = h.error;
+  if (!rc) rc = export_adj(s->eng, r->slot, r->adj.data());
+  if (!rc) rc = export_sepsets(s->eng, r->slot, r->sep.data(), r->mc);
+  engine_remove(s->eng, r->slot);
+  r->slot = -1;
+  r->error = rc;
+  r->t_done = Clock::now();
+  r->state = rc ? 3 : 2;
+}
+
+static void server_loop(hpc_server *s) {
+  (void)hipSetDevice(s->eng.device);
+  for (;;) {
+    // admission: queued requests in arrival order while they fit
+    std::vector<std::shared_ptr<Request>> admit;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty() || !s->running.empty(); });
+      if (s->stop && s->running.empty()) return;
+      size_t used = s->in_use;
+      int active = (int)s->running.size();
+      while (!s->queue.empty()) {
+        auto &r = s->queue.front();
+        if (active >= s->max_active) break;
+        if (used + r->bytes > s->budget && active > 0) break;
+        used += r->bytes;
+        ++active;
+        admit.push_back(r);
+        s->queue.pop_front();
+      }
+    }
+    for (auto &r : admit) {
+      int slot = -1;
+      int rc = engine_add(s->eng, r->p, r->n, r->alpha, r->max_level, &slot);
+      if (!rc) {
+        rc = engine_set_observations(s->eng, slot, r->x.data());
+        if (rc) engine_remove(s->eng, slot);
+      }
+      std::lock_guard<std::mutex> lk(s->mu);
+      r->t_admit = Clock::now();
+      if (rc) {
+        r->error = rc;
+        r->message = hpc_last_error();
+        r->state = 3;
+        r->t_done = r->t_admit;
+        s->cv_done.notify_all();
+        continue;
+      }
+      r->slot = slot;
+      r->state = 1;
+      r->x.clear();
+      r->x.shrink_to_fit();
+      s->in_use += r->bytes;
+      s->running.push_back(r);
+    }
+    std::vector<int> ids;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      for (auto &r : s->running) ids.push_back(r->slot);
+      s->peak_active = std::max(s->peak_active, (int)ids.size());
+      s->peak_bytes = std::max(s->peak_bytes, s->in_use);
+    }
+    if (ids.empty()) continue;
+    auto t0 = Clock::now();
+    const int rc = engine_step(s->eng, ids);
+    const double ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->busy_ms += ms;
+    ++s->steps;
+    std::vector<std::shared_ptr<Request>> keep;
+    for (auto &r : s->running) {
+      InstHost &h = s->eng.inst[r->slot];
+      if (rc) h.error = rc;
+      if (rc || h.finished) {
+        s->in_use -= r->bytes;
+        if (rc) r->message = hpc_last_error();
+        server_finish(s, r);
+      } else {
+        keep.push_back(r);
+      }
+    }
+    s->running.swap(keep);
+    s->cv_done.notify_all();
+  }
+}
+
+extern "C" int hpc_server_create(int device, uint64_t hbm_budget, int max_active,
+                                 hpc_server **out) {
+  if (!out) return hpc_fail(HPC_EINVAL, "out is NULL");
+  *out = nullptr;
+  hpc_server *s = new hpc_server();
+  int rc = engine_init(s->eng, device);
+  if (rc) { engine_free(s->eng); delete s; return rc; }
+  if (hbm_budget == 0) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)16 << 30;
+    hbm_budget = (uint64_t)(free_b * 0.8);
+  }
+  s->budget = hbm_budget;
+  s->max_active = max_active > 0 ? max_active : 1024;
+  s->worker = std::thread(server_loop, s);
+  *out = s;
+  return HPC_OK;
+}
+
+extern "C" int hpc_server_destroy(hpc_server *s) {
+  if (!s) return HPC_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+    s->queue.clear();
+  }
+  s->cv_work.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  engine_free(s->eng);
+  delete s;
+  return HPC_OK;
+}
+
+extern "C" int hpc_server_submit(hpc_server *s, uint64_t p, int n, double alpha, int max_level,
+                                 const double *x, int64_t *req) {
+  if (!s || !x || !req) return hpc_fail(HPC_EINVAL, "NULL argument");
+  if (p < 2 || p > 46340 || n < 1) return hpc_fail(HPC_EINVAL, "bad request shape");
+  auto r = std::make_shared<Request>();
+  r->p = (int)p;
+  r->n = n;
+  r->alpha = alpha;
+  r->max_level = max_level;
+  r->x.assign(x, x + (size_t)p * n);
+  r->bytes = engine_footprint((int)p, n, max_level);
+  r->t_submit = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (r->bytes > s->budget)
+      return hpc_fail(HPC_ENOMEM, "request needs %zu bytes, budget is %zu", r->bytes, s->budget);
+    r->id = s->next_id++;
+    s->all[r->id] = r;
+    s->queue.push_back(r);
+    *req = r->id;
+  }
+  s->cv_work.notify_all();
+  return HPC_OK;
+}
+
+extern "C" int hpc_server_wait(hpc_server *s, int64_t req, double timeout_s, int *done) {
+  if (!s) return hpc_fail(HPC_EINVAL, "server is NULL");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->all.find(req);
+  if (it == s->all.end()) return hpc_fail(HPC_EINVAL, "no request %lld", (long long)req);
+  auto r = it->second;
+  auto pred = [&] { return r->state >= 2; };
+  if (timeout_s < 0) s->cv_done.wait(lk, pred);
+  else s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred);
+  if (done) *done = r->state >= 2 ? 1 : 0;
+  return HPC_OK;
+}
+
+extern "C" int hpc_server_result(hpc_server *s, int64_t req, int32_t *adj, int32_t *sep, int mc,
+                                 hpc_request_info *info) {
+  if (!s) return hpc_fail(HPC_EINVAL, "server is NULL");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->all.find(req);
+  if (it == s->all.end()) return hpc_fail(HPC_EINVAL, "no request %lld", (long long)req);
+  auto &r = it->second;
+  if (r->state < 2) return hpc_fail(HPC_ESTATE, "request %lld not finished", (long long)req);
+  if (r->state == 3)
+    return hpc_fail(r->error ? r->error : HPC_EHIP, "request %lld failed: %s", (long long)req,
+                    r->message.c_str());
+  if (adj) memcpy(adj, r->adj.data(), r->adj.size() * sizeof(int32_t));
+  if (sep) {
+    const size_t pp = (size_t)r->p * r->p;
+    for (size_t e = 0; e < pp; ++e)
+      for (int k = 0; k < mc; ++k) sep[e * mc + k] = k < r->mc ? r->sep[e * r->mc + k] : -1;
+  }
+  if (info) {
+    info->p = r->p;
+    info->levels = r->levels;
+    info->max_cond_size = r->mc;
+    info->tests = r->tests;
+    info->queue_ms = std::chrono::duration<double, std::milli>(r->t_admit - r->t_submit).count();
+    info->latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+  }
+  return HPC_OK;
+}
+
+extern "C" int hpc_server_release(hpc_server *s, int64_t req) {
+  if (!s) return hpc_fail(HPC_EINVAL, "server is NULL");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->all.find(req);
+  if (it == s->all.end()) return hpc_fail(HPC_EINVAL, "no request %lld", (long long)req);
+  if (it->second->state < 2) return hpc_fail(HPC_ESTATE, "request still running");
+  s->all.erase(it);
+  return HPC_OK;
+}
+
+extern "C" int hpc_server_stats(hpc_server *s, hpc_server_info *info) {
+  if (!s || !info) return hpc_fail(HPC_EINVAL, "NULL argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  info->steps = s->steps;
+  info->peak_active = s->peak_active;
+  info->peak_bytes = s->peak_bytes;
+  info->busy_ms = s->busy_ms;
+  info->budget = s->budget;
+  info->queued = (int)s->queue.size();
+  info->running = (int)s->running.size();
+  return HPC_OK;
+}

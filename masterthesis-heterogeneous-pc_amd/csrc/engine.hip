@@ -1,0 +1,694 @@
+// Host side of the device state: instances, the packed layout of a phase and
+// the phases themselves (correlation, level 0, compaction, search, commit).
+// The single-instance ABI (abi.hip), the batched ABI and the server
+// (batch.hip) all drive these functions.
+#pragma clang fp contract(off)
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+
+#include "hpc_internal.hpp"
+
+namespace hpc {
+
+// calcPValue (cpuInd.cpp:8-14): 2 * Q(z) = erfc(z / sqrt 2).
+static double fisher_p(double r_in, int n) {
+  double r = isnan(r_in) ? 0.0 : fmin(0.9999999, fabs(r_in));
+  double absz = sqrt(n - 3.0) * 0.5 * log1p(2 * r / (1 - r));
+  return erfc(absz / 1.4142135623730951);
+}
+
+double fisher_r_crit(int n, double alpha) {
+  double hi = 0.9999999, lo = 0.0;
+  if (fisher_p(hi, n) >= alpha) return INFINITY;
+  if (!(fisher_p(lo, n) >= alpha)) return -1.0;
+  // positive doubles are ordered like their bit patterns: bisect on bits
+  uint64_t a, b;
+  memcpy(&a, &lo, 8);
+  memcpy(&b, &hi, 8);
+  while (b - a > 1) {
+    uint64_t m = a + (b - a) / 2;
+    double r;
+    memcpy(&r, &m, 8);
+    if (fisher_p(r, n) >= alpha) a = m; else b = m;
+  }
+  double out;
+  memcpy(&out, &a, 8);
+  return out;
+}
+
+static int dalloc(void **ptr, size_t bytes) {
+  *ptr = nullptr;
+  hipError_t e = hipMalloc(ptr, bytes ? bytes : 8);
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    return hpc_fail(HPC_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  return HPC_OK;
+}
+
+template <class T>
+static void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+// grows a device array to hold n elements (contents dropped)
+template <class T>
+static int grow(T *&p, size_t &cap, size_t n) {
+  if (n <= cap && p) return HPC_OK;
+  dfree(p);
+  cap = 0;
+  const size_t want = std::max<size_t>(n + n / 4, 64);
+  int rc = dalloc((void **)&p, want * sizeof(T));
+  if (rc) return rc;
+  cap = want;
+  return HPC_OK;
+}
+
+int engine_init(Engine &e, int device) {
+  int ndev = 0;
+  HPC_HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev)
+    return hpc_fail(HPC_EINVAL, "Only %d GPUs available for kernel execution", ndev);
+  HPC_HIPCHK(hipSetDevice(device));
+  e.device = device;
+  HPC_HIPCHK(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
+  HPC_HIPCHK(hipStreamCreateWithFlags(&e.copy_stream, hipStreamNonBlocking));
+  for (auto &ev : e.ev) HPC_HIPCHK(hipEventCreate(&ev));
+  HPC_HIPCHK(hipEventCreateWithFlags(&e.fp_ready, hipEventDisableTiming));
+  int rc = dalloc((void **)&e.d_ctr, sizeof(Counters));
+  if (rc) return rc;
+  HPC_HIPCHK(hipHostMalloc((void **)&e.h_ctr, sizeof(Counters), hipHostMallocDefault));
+  return HPC_OK;
+}
+
+static void free_blocks(InstHost &h) {
+  dfree(h.x); dfree(h.z); dfree(h.nanm); dfree(h.mean); dfree(h.sd); dfree(h.cor);
+  dfree(h.adj); dfree(h.rmlev); dfree(h.sep);
+}
+
+void engine_free(Engine &e) {
+  (void)hipSetDevice(e.device);
+  if (e.stream) (void)hipStreamSynchronize(e.stream);
+  if (e.copy_stream) (void)hipStreamSynchronize(e.copy_stream);
+  for (auto &h : e.inst) free_blocks(h);
+  e.inst.clear();
+  dfree(e.d_inst); dfree(e.d_ictr); dfree(e.d_row_inst); dfree(e.d_deg); dfree(e.d_rowptr);
+  dfree(e.d_nbr); dfree(e.d_erow); dfree(e.d_rowmask); dfree(e.d_fp); dfree(e.d_row_tasks);
+  dfree(e.d_task_off); dfree(e.d_tasks); dfree(e.d_binom); dfree(e.d_prefix); dfree(e.d_ctr);
+  dfree(e.d_scatter_idx); dfree(e.d_scatter_val);
+  if (e.h_ctr) (void)hipHostFree(e.h_ctr);
+  if (e.h_ictr) (void)hipHostFree(e.h_ictr);
+  e.h_ctr = nullptr;
+  e.h_ictr = nullptr;
+  for (auto &ev : e.ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (e.fp_ready) (void)hipEventDestroy(e.fp_ready);
+  e.fp_ready = nullptr;
+  if (e.stream) (void)hipStreamDestroy(e.stream);
+  if (e.copy_stream) (void)hipStreamDestroy(e.copy_stream);
+  e.stream = e.copy_stream = nullptr;
+}
+
+// saturating C(m, k), k <= kMaxLevel, for m < rows
+static int ensure_binom(Engine &e, int rows) {
+  if ((size_t)rows <= e.cap_binom) return HPC_OK;
+  const size_t R = std::max<size_t>(rows, 64);
+  std::vector<unsigned long long> tab(R * kBinomK, 0ull);
+  for (size_t nn = 0; nn < R; ++nn) {
+    tab[nn * kBinomK] = 1ull;
+    for (int k = 1; k < kBinomK && (size_t)k <= nn; ++k) {
+      unsigned long long a = tab[(nn - 1) * kBinomK + k - 1];
+      unsigned long long b = ((size_t)k <= nn - 1) ? tab[(nn - 1) * kBinomK + k] : 0ull;
+      unsigned long long s = a + b;
+      tab[nn * kBinomK + k] = (s < a) ? ~0ull : s;
+    }
+  }
+  dfree(e.d_binom);
+  int rc = dalloc((void **)&e.d_binom, tab.size() * sizeof(tab[0]));
+  if (rc) return rc;
+  HPC_HIPCHK(hipMemcpy(e.d_binom, tab.data(), tab.size() * sizeof(tab[0]), hipMemcpyHostToDevice));
+  e.h_binom = std::move(tab);
+  e.cap_binom = R;
+  return HPC_OK;
+}
+
+int engine_add(Engine &e, int p, int n, double alpha, int max_level, int *id) {
+  if (p < 2 || p > 46340) return hpc_fail(HPC_EINVAL, "p = %d outside [2, 46340]", p);
+  if (n < 1) return hpc_fail(HPC_EINVAL, "observations must be >= 1");
+  HPC_HIPCHK(hipSetDevice(e.device));
+  int slot = -1;
+  for (size_t k = 0; k < e.inst.size(); ++k)
+    if (!e.inst[k].used) { slot = (int)k; break; }
+  if (slot < 0) { slot = (int)e.inst.size(); e.inst.emplace_back(); }
+  InstHost &h = e.inst[slot];
+  h = InstHost();
+  h.used = true;
+  h.p = p;
+  h.n = n;
+  h.npad = (n + 63) & ~63;
+  h.W = (p + 63) / 64;
+  h.alpha = alpha;
+  h.max_level = max_level;
+  h.mc = max_level > 1 ? std::min(max_level, kMaxLevel) : (max_level < 0 ? 4 : 1);
+  h.r_crit = fisher_r_crit(n, alpha);
+  const size_t pp = (size_t)p * p;
+  int rc = 0;
+  if (!rc) rc = dalloc((void **)&h.cor, pp * sizeof(double));
+  if (!rc) rc = dalloc((void **)&h.adj, (size_t)p * h.W * sizeof(unsigned long long));
+  if (!rc) rc = dalloc((void **)&h.rmlev, pp);
+  if (!rc) rc = dalloc((void **)&h.sep, (size_t)h.mc * pp * sizeof(int32_t));
+  if (!rc) rc = dalloc((void **)&h.mean, p * sizeof(double));
+  if (!rc) rc = dalloc((void **)&h.sd, p * sizeof(double));
+  if (!rc) rc = ensure_binom(e, p + 1);
+  if (rc) { free_blocks(h); h.used = false; return rc; }
+  h.bytes = pp * (8 + 1 + 4 * h.mc) + (size_t)p * h.W * 8 + 16 * (size_t)p;
+  rc = engine_reset(e, slot);
+  if (rc) { free_blocks(h); h.used = false; return rc; }
+  *id = slot;
+  return HPC_OK;
+}
+
+void engine_remove(Engine &e, int id) {
+  if (id < 0 || id >= (int)e.inst.size() || !e.inst[id].used) return;
+  (void)hipSetDevice(e.device);
+  (void)hipStreamSynchronize(e.stream);
+  free_blocks(e.inst[id]);
+  e.inst[id] = InstHost();
+}
+
+int engine_reset(Engine &e, int id) {
+  InstHost &h = e.inst[id];
+  HPC_HIPCHK(hipSetDevice(e.device));
+  launch_adj_init(e.stream, h.adj, h.p, h.W);
+  HPC_HIPCHK(hipGetLastError());
+  HPC_HIPCHK(hipMemsetAsync(h.rmlev, 0xFF, (size_t)h.p * h.p, e.stream));
+  h.next_level = 0;
+  h.finished = false;
+  h.compacted = false;
+  h.ever_compacted = false;
+  h.max_deg = h.p;
+  h.undirected = (int64_t)h.p * (h.p - 1) / 2;
+  return HPC_OK;
+}
+
+int engine_set_observations(Engine &e, int id, const double *h_x) {
+  InstHost &h = e.inst[id];
+  HPC_HIPCHK(hipSetDevice(e.device));
+  int rc = 0;
+  if (!h.x) rc = dalloc((void **)&h.x, (size_t)h.p * h.n * sizeof(double));
+  if (!rc && !h.z) rc = dalloc((void **)&h.z, (size_t)h.p * h.npad * sizeof(double));
+  if (!rc && !h.nanm) rc = dalloc((void **)&h.nanm, (size_t)h.p * h.npad);
+  if (rc) return rc;
+  const size_t cnt = (size_t)h.p * h.n;
+  HPC_HIPCHK(hipMemcpyAsync(h.x, h_x, cnt * sizeof(double), hipMemcpyHostToDevice, e.stream));
+  // NaN cells need the masked pairwise kernel (corHelper.cu:101 skips them)
+  bool any_nan = false;
+  for (size_t k = 0; k < cnt; ++k)
+    if (h_x[k] != h_x[k]) { any_nan = true; break; }
+  h.x_has_nan = any_nan;
+  HPC_HIPCHK(hipStreamSynchronize(e.stream));
+  h.have_x = true;
+  h.bytes += cnt * 8 + (size_t)h.p * h.npad * 9;
+  return HPC_OK;
+}
+
+int engine_set_corr(Engine &e, int id, const double *h_cor) {
+  InstHost &h = e.inst[id];
+  HPC_HIPCHK(hipSetDevice(e.device));
+  HPC_HIPCHK(hipMemcpyAsync(h.cor, h_cor, (size_t)h.p * h.p * sizeof(double),
+                            hipMemcpyHostToDevice, e.stream));
+  HPC_HIPCHK(hipStreamSynchronize(e.stream));
+  h.have_cor = true;
+  e.h_cor_valid = false;
+  return HPC_OK;
+}
+
+int engine_grow_sep(Engine &e, int id, int mc) {
+  InstHost &h = e.inst[id];
+  if (mc <= h.mc) return HPC_OK;
+  const size_t plane = (size_t)h.p * h.p;
+  int32_t *ns = nullptr;
+  int rc = dalloc((void **)&ns, (size_t)mc * plane * sizeof(int32_t));
+  if (rc) return rc;
+  HPC_HIPCHK(hipMemcpyAsync(ns, h.sep, (size_t)h.mc * plane * sizeof(int32_t),
+                            hipMemcpyDeviceToDevice, e.stream));
+  HPC_HIPCHK(hipStreamSynchronize(e.stream));
+  dfree(h.sep);
+  h.sep = ns;
+  h.bytes += (size_t)(mc - h.mc) * plane * 4;
+  h.mc = mc;
+  return HPC_OK;
+}
+
+// Capacity groups: arrays of one group are reallocated together (contents
+// dropped; every phase rewrites them).
+static int ensure_inst_cap(Engine &e, size_t K) {
+  if (K <= e.cap_inst && e.d_inst) return HPC_OK;
+  const size_t c = std::max<size_t>(K + K / 4, 8);
+  dfree(e.d_inst); dfree(e.d_ictr); dfree(e.d_prefix);
+  if (e.h_ictr) (void)hipHostFree(e.h_ictr);
+  e.h_ictr = nullptr;
+  e.cap_inst = 0;
+  int rc = dalloc((void **)&e.d_inst, c * sizeof(Inst));
+  if (!rc) rc = dalloc((void **)&e.d_ictr, c * sizeof(InstCounters));
+  if (!rc) rc = dalloc((void **)&e.d_prefix, 4 * (c + 1) * sizeof(long long));
+  if (!rc && hipHostMalloc((void **)&e.h_ictr, c * sizeof(InstCounters), hipHostMallocDefault) != hipSuccess)
+    rc = hpc_fail(HPC_ENOMEM, "hipHostMalloc failed");
+  if (!rc) e.cap_inst = c;
+  return rc;
+}
+
+static int ensure_row_cap(Engine &e, size_t rows) {
+  if (rows <= e.cap_rows && e.d_row_inst) return HPC_OK;
+  const size_t c = std::max<size_t>(rows + rows / 4, 64);
+  dfree(e.d_row_inst); dfree(e.d_deg); dfree(e.d_rowptr); dfree(e.d_rowmask);
+  dfree(e.d_row_tasks); dfree(e.d_task_off);
+  e.cap_rows = 0;
+  int rc = dalloc((void **)&e.d_row_inst, c * sizeof(int32_t));
+  if (!rc) rc = dalloc((void **)&e.d_deg, c * sizeof(int32_t));
+  if (!rc) rc = dalloc((void **)&e.d_rowptr, (c + 1) * sizeof(int32_t));
+  if (!rc) rc = dalloc((void **)&e.d_rowmask, c * sizeof(int32_t));
+  if (!rc) rc = dalloc((void **)&e.d_row_tasks, c * sizeof(long long));
+  if (!rc) rc = dalloc((void **)&e.d_task_off, c * sizeof(long long));
+  if (!rc) e.cap_rows = c;
+  return rc;
+}
+
+static int ensure_edge_cap(Engine &e, size_t edges) {
+  if (edges <= e.cap_edges && e.d_nbr) return HPC_OK;
+  const size_t c = std::max<size_t>(edges + edges / 4, 1024);
+  dfree(e.d_nbr); dfree(e.d_erow); dfree(e.d_fp);
+  e.cap_edges = 0;
+  int rc = dalloc((void **)&e.d_nbr, c * sizeof(int32_t));
+  if (!rc) rc = dalloc((void **)&e.d_erow, c * sizeof(int32_t));
+  if (!rc) rc = dalloc((void **)&e.d_fp, c * sizeof(int64_t));
+  if (!rc) e.cap_edges = c;
+  return rc;
+}
+
+int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &levels) {
+  HPC_HIPCHK(hipSetDevice(e.device));
+  // the previous phase's uploads read these host vectors
+  HPC_HIPCHK(hipStreamSynchronize(e.stream));
+  const int K = (int)ids.size();
+  int p_total = 0, pmax = 0;
+  size_t edges = 0;
+  for (int id : ids) {
+    p_total += e.inst[id].p;
+    pmax = std::max(pmax, e.inst[id].p);
+    edges += (size_t)e.inst[id].p * (e.inst[id].p - 1);
+  }
+  int rc = ensure_inst_cap(e, (size_t)K);
+  if (!rc) rc = ensure_row_cap(e, (size_t)p_total + 1);
+  if (!rc) rc = ensure_edge_cap(e, edges + 1);
+  if (!rc) rc = ensure_binom(e, pmax + 1);
+  if (rc) return rc;
+  e.h_inst.assign(K, Inst());
+  e.h_row_inst.resize(p_total);
+  int start = 0;
+  for (int k = 0; k < K; ++k) {
+    InstHost &h = e.inst[ids[k]];
+    Inst &I = e.h_inst[k];
+    I.x = h.x; I.z = h.z; I.nanm = h.nanm; I.mean = h.mean; I.sd = h.sd; I.cor = h.cor;
+    I.adj = h.adj; I.rmlev = h.rmlev; I.sep = h.sep;
+    I.r_crit = h.r_crit;
+    I.start = start;
+    I.p = h.p; I.n = h.n; I.npad = h.npad; I.W = h.W; I.mc = h.mc;
+    I.level = levels[k];
+    I.flags = kActive | (h.x_has_nan ? kHasNan : 0);
+    h.start = start;
+    std::fill(e.h_row_inst.begin() + start, e.h_row_inst.begin() + start + h.p, k);
+    start += h.p;
+  }
+  e.packed = ids;
+  e.p_total = p_total;
+  if (K > 0) {
+    HPC_HIPCHK(hipMemcpyAsync(e.d_inst, e.h_inst.data(), K * sizeof(Inst), hipMemcpyHostToDevice, e.stream));
+    HPC_HIPCHK(hipMemcpyAsync(e.d_row_inst, e.h_row_inst.data(), p_total * sizeof(int32_t),
+                              hipMemcpyHostToDevice, e.stream));
+  }
+  return HPC_OK;
+}
+
+static Layout layout(Engine &e) {
+  Layout L;
+  L.inst = e.d_inst;
+  L.ictr = e.d_ictr;
+  L.row_inst = e.d_row_inst;
+  L.K = (int)e.packed.size();
+  L.p_total = e.p_total;
+  L.max_deg = e.max_deg;
+  L.pad_ = 0;
+  L.deg = e.d_deg;
+  L.rowptr = e.d_rowptr;
+  L.nbr = e.d_nbr;
+  L.erow = e.d_erow;
+  L.fp = e.d_fp;
+  L.binom = e.d_binom;
+  L.ctr = e.d_ctr;
+  return L;
+}
+
+static float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
+  return ms;
+}
+
+static int read_ctr(Engine &e) {
+  HPC_HIPCHK(hipMemcpyAsync(e.h_ctr, e.d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, e.stream));
+  HPC_HIPCHK(hipStreamSynchronize(e.stream));
+  return HPC_OK;
+}
+
+int engine_read_inst_counters(Engine &e) {
+  const size_t K = e.packed.size();
+  if (K == 0) return HPC_OK;
+  HPC_HIPCHK(hipMemcpyAsync(e.h_ictr, e.d_ictr, K * sizeof(InstCounters), hipMemcpyDeviceToHost,
+                            e.stream));
+  HPC_HIPCHK(hipStreamSynchronize(e.stream));
+  return HPC_OK;
+}
+
+static int zero_counters(Engine &e) {
+  const size_t K = e.packed.size();
+  HPC_HIPCHK(hipMemsetAsync(e.d_ctr, 0, sizeof(Counters), e.stream));
+  if (K) HPC_HIPCHK(hipMemsetAsync(e.d_ictr, 0, K * sizeof(InstCounters), e.stream));
+  return HPC_OK;
+}
+
+int engine_corr(Engine &e, float *ms_moments, float *ms_corr) {
+  HPC_HIPCHK(hipSetDevice(e.device));
+  for (int id : e.packed)
+    if (!e.inst[id].have_x) return hpc_fail(HPC_ESTATE, "no observations uploaded");
+  Layout L = layout(e);
+  HPC_HIPCHK(hipEventRecord(e.ev[0], e.stream));
+  launch_moments(e.stream, L);
+  HPC_HIPCHK(hipGetLastError());
+  HPC_HIPCHK(hipEventRecord(e.ev[1], e.stream));
+  launch_corr(e.stream, L, e.h_inst, e.d_prefix, e.h_prefix);
+  HPC_HIPCHK(hipGetLastError());
+  HPC_HIPCHK(hipEventRecord(e.ev[2], e.stream));
+  for (int id : e.packed) e.inst[id].have_cor = true;
+  e.h_cor_valid = false;
+  if (ms_moments || ms_corr) {
+    HPC_HIPCHK(hipEventSynchronize(e.ev[2]));
+    if (ms_moments) *ms_moments = elapsed(e.ev[0], e.ev[1]);
+    if (ms_corr) *ms_corr = elapsed(e.ev[1], e.ev[2]);
+  }
+  return HPC_OK;
+}
+
+int engine_level0(Engine &e, float *ms) {
+  HPC_HIPCHK(hipSetDevice(e.device));
+  int rc = zero_counters(e);
+  if (rc) return rc;
+  Layout L = layout(e);
+  HPC_HIPCHK(hipEventRecord(e.ev[3], e.stream));
+  launch_level0(e.stream, L, e.h_inst, e.d_prefix + 2 * (L.K + 1), e.h_prefix);
+  HPC_HIPCHK(hipGetLastError());
+  HPC_HIPCHK(hipEventRecord(e.ev[4], e.stream));
+  rc = engine_read_inst_counters(e);
+  if (rc) return rc;
+  if (ms) *ms = elapsed(e.ev[3], e.ev[4]);
+  for (size_t k = 0; k < e.packed.size(); ++k) {
+    InstHost &h = e.inst[e.packed[k]];
+    if (e.h_inst[k].level != 0) continue;
+    h.undirected -= (int64_t)e.h_ictr[k].removed;
+    if (e.h_ictr[k].removed) h.compacted = false;
+  }
+  return HPC_OK;
+}
+
+int engine_compact(Engine &e, float *ms) {
+  HPC_HIPCHK(hipSetDevice(e.device));
+  int rc = zero_counters(e);
+  if (rc) return rc;
+  Layout L = layout(e);
+  HPC_HIPCHK(hipEventRecord(e.ev[3], e.stream));
+  launch_compact(e.stream, L);
+  HPC_HIPCHK(hipGetLastError());
+  HPC_HIPCHK(hipEventRecord(e.ev[4], e.stream));
+  rc = read_ctr(e);
+  if (!rc) rc = engine_read_inst_counters(e);
+  if (rc) return rc;
+  e.n_edges = (int64_t)e.h_ctr->n_edges;
+  e.max_deg = e.h_ctr->max_deg;
+  // every ordered edge of the new snapshot starts with no passing set
+  if (e.n_edges > 0)
+    HPC_HIPCHK(hipMemsetAsync(e.d_fp, 0xFF, (size_t)e.n_edges * sizeof(int64_t), e.stream));
+  // the host share scatters its ranks from the copy stream after this point
+  HPC_HIPCHK(hipEventRecord(e.fp_ready, e.stream));
+  for (size_t k = 0; k < e.packed.size(); ++k) {
+    if (e.h_inst[k].level < 1) continue;
+    InstHost &h = e.inst[e.packed[k]];
+    h.max_deg = e.h_ictr[k].max_deg;
+    h.compacted = true;
+    h.ever_compacted = true;
+  }
+  if (ms) *ms = elapsed(e.ev[3], e.ev[4]);
+  return HPC_OK;
+}
+
+int engine_search(Engine &e, int level, const int32_t *rows, int nrows, float *ms) {
+  HPC_HIPCHK(hipSetDevice(e.device));
+  Layout L = layout(e);
+  const int32_t *mask = nullptr;
+  if (rows) {
+    std::vector<int32_t> m(e.p_total, 0);
+    for (int k = 0; k < nrows; ++k) {
+      if (rows[k] < 0 || rows[k] >= e.p_total) return hpc_fail(HPC_EINVAL, "row %d out of range", rows[k]);
+      m[rows[k]] = 1;
+    }
+    HPC_HIPCHK(hipMemcpyAsync(e.d_rowmask, m.data(), e.p_total * sizeof(int32_t),
+                              hipMemcpyHostToDevice, e.stream));
+    HPC_HIPCHK(hipStreamSynchronize(e.stream));
+    mask = e.d_rowmask;
+  }
+  HPC_HIPCHK(hipEventRecord(e.ev[5], e.stream));
+  if (e.n_edges > 0 && (!rows || nrows > 0)) {
+    if (level == 1) {
+      launch_search_l1(e.stream, L, e.n_edges, mask);
+    } else {
+      const int rpt = e.ranks_per_task;
+      launch_task_plan(e.stream, level, L, rpt, mask, e.d_row_tasks, e.d_task_off);
+      HPC_HIPCHK(hipGetLastError());
+      int rc = read_ctr(e);
+      if (rc) return rc;
+      const size_t n_tasks = (size_t)e.h_ctr->n_tasks;
+      if (n_tasks > 0) {
+        rc = grow(e.d_tasks, e.cap_tasks, n_tasks);
+        if (rc) return rc;
+        launch_task_fill(e.stream, L, e.d_row_tasks, e.d_task_off, e.d_tasks);
+        HPC_HIPCHK(hipGetLastError());
+        launch_search_ln(e.stream, level, L, e.d_tasks, (long long)n_tasks, rpt);
+      }
+    }
+  }
+  HPC_HIPCHK(hipGetLastError());
+  HPC_HIPCHK(hipEventRecord(e.ev[6], e.stream));
+  if (ms) {
+    HPC_HIPCHK(hipEventSynchronize(e.ev[6]));
+    *ms = elapsed(e.ev[5], e.ev[6]);
+  }
+  return HPC_OK;
+}
+
+int engine_commit(Engine &e, float *ms) {
+  HPC_HIPCHK(hipSetDevice(e.device));
+  Layout L = layout(e);
+  HPC_HIPCHK(hipEventRecord(e.ev[6], e.stream));
+  launch_commit(e.stream, L, e.n_edges);
+  HPC_HIPCHK(hipGetLastError());
+  HPC_HIPCHK(hipEventRecord(e.ev[7], e.stream));
+  int rc = engine_read_inst_counters(e);
+  if (rc) return rc;
+  if (ms) *ms = elapsed(e.ev[6], e.ev[7]);
+  for (size_t k = 0; k < e.packed.size(); ++k) {
+    if (e.h_inst[k].level < 1) continue;
+    InstHost &h = e.inst[e.packed[k]];
+    h.undirected -= (int64_t)e.h_ictr[k].removed;
+    if (e.h_ictr[k].removed) h.compacted = false;
+  }
+  return HPC_OK;
+}
+
+}  // namespace hpc
+
+namespace hpc {
+
+size_t engine_footprint(int p, int n, int max_level) {
+  const size_t pp = (size_t)p * p;
+  const int mc = max_level > 1 ? std::min(max_level, kMaxLevel) : (max_level < 0 ? 4 : 1);
+  const size_t npad = ((size_t)n + 63) & ~(size_t)63;
+  const size_t W = ((size_t)p + 63) / 64;
+  size_t b = pp * 8                 // correlation
+             + pp * (1 + 4 * mc)    // removal levels, sepset planes
+             + (size_t)p * W * 8    // bit adjacency
+             + (size_t)p * n * 8    // observations
+             + (size_t)p * npad * 9 // z-scores and NaN mask
+             + (size_t)p * 16;      // mean, sd
+  b += pp * (4 + 4 + 8);            // worst-case share of nbr / erow / first-pass arrays
+  b += (size_t)p * 40;              // row arrays
+  return b;
+}
+
+int engine_step(Engine &e, const std::vector<int> &ids) {
+  HPC_HIPCHK(hipSetDevice(e.device));
+  int rc = 0;
+  // correlation of instances that have none yet
+  std::vector<int> need, zero;
+  for (int id : ids) {
+    InstHost &h = e.inst[id];
+    if (h.finished || h.have_cor) continue;
+    if (!h.have_x) return hpc_fail(HPC_ESTATE, "instance %d has no observations", id);
+    need.push_back(id);
+    zero.push_back(0);
+  }
+  if (!need.empty()) {
+    rc = engine_pack(e, need, zero);
+    if (!rc) rc = engine_corr(e, nullptr, nullptr);
+    if (rc) return rc;
+  }
+  std::vector<int> act, lev;
+  for (int id : ids) {
+    InstHost &h = e.inst[id];
+    if (h.finished) continue;
+    if (h.next_level > h.mc && h.next_level <= kMaxLevel) {
+      rc = engine_grow_sep(e, id, std::min(kMaxLevel, std::max(h.next_level, 2 * h.mc)));
+      if (rc) return rc;
+    }
+    act.push_back(id);
+    lev.push_back(h.next_level);
+  }
+  if (act.empty()) return HPC_OK;
+  auto t0 = std::chrono::steady_clock::now();
+  rc = engine_pack(e, act, lev);
+  if (!rc) rc = zero_counters(e);
+  if (rc) return rc;
+  const int K = (int)act.size();
+  Layout L = layout(e);
+  bool any0 = false, any1 = false;
+  for (int l : lev) { any0 |= l == 0; any1 |= l >= 1; }
+  HPC_HIPCHK(hipEventRecord(e.ev[0], e.stream));
+  if (any0) launch_level0(e.stream, L, e.h_inst, e.d_prefix + 2 * (K + 1), e.h_prefix);
+  HPC_HIPCHK(hipGetLastError());
+  HPC_HIPCHK(hipEventRecord(e.ev[1], e.stream));
+  if (any1) launch_compact(e.stream, L);
+  HPC_HIPCHK(hipGetLastError());
+  HPC_HIPCHK(hipEventRecord(e.ev[2], e.stream));
+  rc = read_ctr(e);
+  if (!rc) rc = engine_read_inst_counters(e);
+  if (rc) return rc;
+  e.n_edges = any1 ? (int64_t)e.h_ctr->n_edges : 0;
+  e.max_deg = any1 ? e.h_ctr->max_deg : 0;
+  L.max_deg = e.max_deg;
+  if (e.n_edges > 0)
+    HPC_HIPCHK(hipMemsetAsync(e.d_fp, 0xFF, (size_t)e.n_edges * sizeof(int64_t), e.stream));
+  // instances whose level has nothing to test and no cap stop here; a level
+  // beyond the kernels' reach is an error for its instance when it has tests,
+  // else an empty record.  skip: 1 = no record, 2 = empty record.
+  std::vector<char> skip(K, 0);
+  bool retable = false;
+  for (int k = 0; k < K; ++k) {
+    if (lev[k] < 1) continue;
+    InstHost &h = e.inst[act[k]];
+    h.max_deg = e.h_ictr[k].max_deg;
+    h.compacted = true;
+    h.ever_compacted = true;
+    const bool testable = h.max_deg - 1 >= lev[k];
+    const bool beyond = lev[k] > kMaxLevel;
+    if ((!testable && h.max_level < 0) || beyond) {
+      if (testable) h.error = HPC_ELEVEL;
+      skip[k] = (testable || h.max_level < 0) ? 1 : 2;
+      e.h_inst[k].level = -1;
+      retable = true;
+    }
+  }
+  if (retable)
+    HPC_HIPCHK(hipMemcpyAsync(e.d_inst, e.h_inst.data(), K * sizeof(Inst), hipMemcpyHostToDevice,
+                              e.stream));
+  std::vector<int> levels_ge2;
+  bool run1 = false;
+  for (int k = 0; k < K; ++k) {
+    const int l = e.h_inst[k].level;
+    if (l == 1) run1 = true;
+    if (l >= 2 && l <= kMaxLevel &&
+        std::find(levels_ge2.begin(), levels_ge2.end(), l) == levels_ge2.end())
+      levels_ge2.push_back(l);
+  }
+  std::sort(levels_ge2.begin(), levels_ge2.end());
+  HPC_HIPCHK(hipEventRecord(e.ev[3], e.stream));
+  if (e.n_edges > 0) {
+    if (run1) launch_search_l1(e.stream, L, e.n_edges, nullptr);
+    for (int l : levels_ge2) {
+      launch_task_plan(e.stream, l, L, e.ranks_per_task, nullptr, e.d_row_tasks, e.d_task_off);
+      HPC_HIPCHK(hipGetLastError());
+      rc = read_ctr(e);
+      if (rc) return rc;
+      const size_t n_tasks = (size_t)e.h_ctr->n_tasks;
+      if (n_tasks == 0) continue;
+      rc = grow(e.d_tasks, e.cap_tasks, n_tasks);
+      if (rc) return rc;
+      launch_task_fill(e.stream, L, e.d_row_tasks, e.d_task_off, e.d_tasks);
+      launch_search_ln(e.stream, l, L, e.d_tasks, (long long)n_tasks, e.ranks_per_task);
+    }
+    HPC_HIPCHK(hipGetLastError());
+  }
+  HPC_HIPCHK(hipEventRecord(e.ev[4], e.stream));
+  if (any1) launch_commit(e.stream, L, e.n_edges);
+  HPC_HIPCHK(hipGetLastError());
+  HPC_HIPCHK(hipEventRecord(e.ev[5], e.stream));
+  rc = engine_read_inst_counters(e);
+  if (rc) return rc;
+  const double wall =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  const double ms_l0 = elapsed(e.ev[0], e.ev[1]), ms_c = elapsed(e.ev[1], e.ev[2]);
+  const double ms_s = elapsed(e.ev[3], e.ev[4]), ms_m = elapsed(e.ev[4], e.ev[5]);
+  for (int k = 0; k < K; ++k) {
+    InstHost &h = e.inst[act[k]];
+    const InstCounters &c = e.h_ictr[k];
+    if (skip[k] == 2) {
+      hpc_level_stats s;
+      memset(&s, 0, sizeof(s));
+      s.level = lev[k];
+      s.max_deg = h.max_deg;
+      s.edges_left = h.undirected;
+      s.ms_compact = ms_c;
+      s.ms_wall = wall;
+      h.hist.push_back(s);
+    } else if (!skip[k]) {
+      hpc_level_stats s;
+      memset(&s, 0, sizeof(s));
+      s.level = lev[k];
+      s.max_deg = lev[k] == 0 ? h.p : h.max_deg;
+      s.tests = (int64_t)c.tests;
+      s.tests_max = (int64_t)c.tests_max;
+      s.edges = (int64_t)c.edges;
+      s.removed = (int64_t)c.removed;
+      h.undirected -= s.removed;
+      s.edges_left = h.undirected;
+      if (lev[k] == 0) {
+        s.ms_test = ms_l0;
+      } else {
+        s.ms_compact = ms_c;
+        s.ms_test = ms_s;
+        s.ms_commit = ms_m;
+      }
+      s.ms_wall = wall;  // of the whole step (shared by the instances in it)
+      h.hist.push_back(s);
+      if (s.removed) h.compacted = false;
+    }
+    h.next_level += 1;
+    if (skip[k] == 1 || h.error || (h.max_level >= 0 && h.next_level > h.max_level))
+      h.finished = true;
+  }
+  return HPC_OK;
+}
+
+}  // namespace hpc

@@ -1,8 +1,10 @@
-// Adjacency kernels: initial state, the level-0 test and per-level compaction.
+// Adjacency kernels: initial state, the level-0 test and per-level compaction,
+// for every active instance of the layout at once.
 //
-// The adjacency lives on the device as a bit matrix: row i is W = ceil(p/64)
-// 64-bit words, bit j of word j/64 set while edge i-j is present.  p = 2000
-// takes 512 KB instead of the reference's 16 MB int32 matrix (state.cu:21-38).
+// The adjacency of an instance lives on the device as a bit matrix: row i is
+// W = ceil(p/64) 64-bit words, bit j of word j/64 set while edge i-j is
+// present.  p = 2000 takes 512 KB instead of the reference's 16 MB int32
+// matrix (state.cu:21-38).
 #pragma clang fp contract(off)
 
 #include "hpc_internal.hpp"
@@ -10,6 +12,15 @@
 
 namespace hpc {
 namespace {
+
+__device__ __forceinline__ int find_inst(const long long *prefix, int K, long long idx) {
+  int lo = 0, hi = K - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= idx) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
 
 // state.cu:31-38: every off-diagonal pair adjacent.
 __global__ void adj_init_kernel(unsigned long long *adj, int p, int W) {
@@ -27,22 +38,24 @@ __global__ void adj_init_kernel(unsigned long long *adj, int p, int W) {
 // adjacent pair j < i is tested on cor[i][j] (the lower triangle, as the
 // reference reads it) and removed iff p >= alpha.
 //
-// One wavefront per 64 x 64 tile (I, J) with I >= J of the lower block
-// triangle.  Iteration r reads row i = 64I + r, columns 64J..64J+63 (one
-// coalesced 512 B load); the ballot of surviving pairs is row i's word J, and
-// lane c collects bit r of row (64J + c)'s word I.  Each adjacency word has
-// exactly one owner, so no atomics are needed.
-__global__ __launch_bounds__(256) void level0_kernel(
-    const double *__restrict__ cor, unsigned long long *__restrict__ adj, int p,
-    int W, double r_crit, long long ntiles, Counters *__restrict__ ctr) {
+// One wavefront per 64 x 64 tile (I, J) with I >= J of an instance's lower
+// block triangle.  Iteration r reads row i = 64I + r, columns 64J..64J+63
+// (one coalesced 512 B load); the ballot of surviving pairs is row i's word
+// J, and lane c collects bit r of row (64J + c)'s word I.  Each adjacency
+// word has exactly one owner, so no atomics are needed.
+__global__ __launch_bounds__(256) void level0_kernel(Layout L, const long long *__restrict__ prefix) {
   const int lane = threadIdx.x & 63;
   const long long t = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (t >= ntiles) return;
-  // lower block triangle, row-major: tile t -> (I, J), J <= I
-  int I = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-  while ((long long)I * (I + 1) / 2 > t) --I;
-  while ((long long)(I + 1) * (I + 2) / 2 <= t) ++I;
-  const int J = (int)(t - (long long)I * (I + 1) / 2);
+  if (t >= prefix[L.K]) return;
+  const int k = find_inst(prefix, L.K, t);
+  const Inst In = L.inst[k];
+  const int p = In.p, W = In.W;
+  const long long tl = t - prefix[k];
+  // lower block triangle, row-major: tile tl -> (I, J), J <= I
+  int I = (int)((sqrt(8.0 * (double)tl + 1.0) - 1.0) * 0.5);
+  while ((long long)I * (I + 1) / 2 > tl) --I;
+  while ((long long)(I + 1) * (I + 2) / 2 <= tl) ++I;
+  const int J = (int)(tl - (long long)I * (I + 1) / 2);
   const int j = J * 64 + lane;
   const bool diag = (I == J);
   unsigned long long colword = 0ull;   // bits r: pair (64I + r, j) kept
@@ -51,13 +64,13 @@ __global__ __launch_bounds__(256) void level0_kernel(
   for (int r = 0; r < 64; ++r) {
     const int i = I * 64 + r;
     if (i >= p) break;
-    const unsigned long long aw = adj[(size_t)i * W + J];
+    const unsigned long long aw = In.adj[(size_t)i * W + J];
     bool valid = (j < p) && (diag ? (lane < r) : true);
     bool present = valid && ((aw >> lane) & 1ull);
     bool keep = false;
     if (present) {
-      double c = cor[(size_t)i * p + j];
-      keep = !removes(c, r_crit);
+      double c = In.cor[(size_t)i * p + j];
+      keep = !removes(c, In.r_crit);
     }
     const unsigned long long kb = __ballot(keep);
     tested += __popcll(__ballot(present));
@@ -67,47 +80,69 @@ __global__ __launch_bounds__(256) void level0_kernel(
   }
   const int i_lane = I * 64 + lane;
   if (diag) {
-    if (i_lane < p) adj[(size_t)i_lane * W + J] = rowword | colword;
+    if (i_lane < p) In.adj[(size_t)i_lane * W + J] = rowword | colword;
   } else {
-    if (i_lane < p) adj[(size_t)i_lane * W + J] = rowword;
-    if (j < p) adj[(size_t)j * W + I] = colword;
+    if (i_lane < p) In.adj[(size_t)i_lane * W + J] = rowword;
+    if (j < p) In.adj[(size_t)j * W + I] = colword;
   }
-  if (lane == 0) {
-    atomicAdd(&ctr->tests, tested);
-    atomicAdd(&ctr->tests_max, tested);
-    atomicAdd(&ctr->edges, tested);
-    atomicAdd(&ctr->removed, removed);
+  if (lane == 0 && tested) {
+    atomicAdd(&L.ictr[k].tests, tested);
+    atomicAdd(&L.ictr[k].tests_max, tested);
+    atomicAdd(&L.ictr[k].edges, tested);
+    atomicAdd(&L.ictr[k].removed, removed);
   }
 }
 
-// Row lengths (compact.cu:80-88 stores them at adj_compact[i][p-1]).
-__global__ __launch_bounds__(256) void degree_kernel(
-    const unsigned long long *__restrict__ adj, int p, int W,
-    int32_t *__restrict__ deg) {
-  const int lane = threadIdx.x & 63;
-  const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (i >= p) return;
-  int c = 0;
-  for (int w = lane; w < W; w += 64) c += __popcll(adj[(size_t)i * W + w]);
+// Row lengths (compact.cu:80-88 stores them at adj_compact[i][p-1]); rows of
+// inactive instances count as empty.  Per-instance maxima: the scan kernel
+// takes the single-instance case; otherwise one atomic per instance and
+// workgroup.
+constexpr int kDegWaves = 16;
+__global__ __launch_bounds__(64 * kDegWaves) void degree_kernel(Layout L) {
+  __shared__ int s_c[kDegWaves], s_k[kDegWaves];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int g = blockIdx.x * kDegWaves + wv;
+  int c = 0, k = -1;
+  if (g < L.p_total) {
+    k = L.row_inst[g];
+    const Inst In = L.inst[k];
+    if ((In.flags & kActive) && In.level >= 1) {
+      const unsigned long long *row = In.adj + (size_t)(g - In.start) * In.W;
+      for (int w = lane; w < In.W; w += 64) c += __popcll(row[w]);
+    }
 #pragma unroll
-  for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s, 64);
-  if (lane == 0) deg[i] = c;
+    for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s, 64);
+    if (lane == 0) L.deg[g] = c;
+  }
+  if (L.K == 1) return;
+  if (lane == 0) { s_c[wv] = c; s_k[wv] = c ? k : -1; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int cur = -1, m = 0;
+    for (int w = 0; w <= kDegWaves; ++w) {
+      const int kw = w < kDegWaves ? s_k[w] : -2;
+      if (kw == -1) continue;
+      if (kw != cur && cur >= 0) { atomicMax(&L.ictr[cur].max_deg, m); m = 0; }
+      if (kw < 0) break;
+      cur = kw;
+      m = max(m, s_c[w]);
+    }
+  }
 }
 
 // Exclusive scan of row lengths -> CSR row pointers, total ordered edges and
 // the maximum row length (compact.cu:81 atomicMax_system(max_adj)).
 constexpr int kScanThreads = 1024;
-__global__ __launch_bounds__(kScanThreads) void scan_kernel(
-    const int32_t *__restrict__ deg, int p, int32_t *__restrict__ rowptr,
-    Counters *__restrict__ ctr) {
+__global__ __launch_bounds__(kScanThreads) void scan_kernel(Layout L) {
   __shared__ long long part[kScanThreads];
   __shared__ int mx[kScanThreads];
   const int tid = threadIdx.x;
+  const int p = L.p_total;
   const int per = (p + kScanThreads - 1) / kScanThreads;
   const int b = tid * per, e = min(p, b + per);
   long long s = 0;
   int m = 0;
-  for (int i = b; i < e; ++i) { s += deg[i]; m = max(m, deg[i]); }
+  for (int i = b; i < e; ++i) { s += L.deg[i]; m = max(m, L.deg[i]); }
   part[tid] = s;
   mx[tid] = m;
   __syncthreads();
@@ -120,29 +155,30 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(
     __syncthreads();
   }
   long long run = part[tid] - s;
-  for (int i = b; i < e; ++i) { rowptr[i] = (int32_t)run; run += deg[i]; }
+  for (int i = b; i < e; ++i) { L.rowptr[i] = (int32_t)run; run += L.deg[i]; }
   if (tid == kScanThreads - 1) {
-    rowptr[p] = (int32_t)part[tid];
-    ctr->n_edges = (unsigned long long)part[tid];
-    ctr->max_deg = mx[tid];
+    L.rowptr[p] = (int32_t)part[tid];
+    L.ctr->n_edges = (unsigned long long)part[tid];
+    L.ctr->max_deg = mx[tid];
+    if (L.K == 1) L.ictr[0].max_deg = mx[tid];
   }
 }
 
-// compactCUTiled (compact.cu:4-89): the ascending neighbour ids of row i,
-// written at rowptr[i] (CSR instead of a p-wide padded row), plus the row id
-// of every ordered edge.  Wave prefix sum of per-word popcounts gives each
-// lane its output offset.
-__global__ __launch_bounds__(256) void fill_kernel(
-    const unsigned long long *__restrict__ adj, int p, int W,
-    const int32_t *__restrict__ rowptr, int32_t *__restrict__ nbr,
-    int32_t *__restrict__ erow) {
+// compactCUTiled (compact.cu:4-89): the ascending neighbour ids of row g
+// (global ids), written at rowptr[g] (CSR instead of a p-wide padded row),
+// plus the row of every ordered edge.  A wave prefix sum of per-word
+// popcounts gives each lane its output offset.
+__global__ __launch_bounds__(256) void fill_kernel(Layout L) {
   const int lane = threadIdx.x & 63;
-  const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (i >= p) return;
-  int base = rowptr[i];
-  for (int w0 = 0; w0 < W; w0 += 64) {
+  const int g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (g >= L.p_total) return;
+  const Inst In = L.inst[L.row_inst[g]];
+  if (!(In.flags & kActive) || In.level < 1) return;
+  const unsigned long long *row = In.adj + (size_t)(g - In.start) * In.W;
+  int base = L.rowptr[g];
+  for (int w0 = 0; w0 < In.W; w0 += 64) {
     const int w = w0 + lane;
-    unsigned long long word = w < W ? adj[(size_t)i * W + w] : 0ull;
+    unsigned long long word = w < In.W ? row[w] : 0ull;
     int c = __popcll(word);
     int incl = c;
 #pragma unroll
@@ -154,8 +190,8 @@ __global__ __launch_bounds__(256) void fill_kernel(
     while (word) {
       int b = __ffsll((long long)word) - 1;
       word &= word - 1ull;
-      nbr[off] = w * 64 + b;
-      erow[off] = i;
+      L.nbr[off] = In.start + w * 64 + b;
+      L.erow[off] = g;
       ++off;
     }
     base += __shfl(incl, 63, 64);
@@ -166,29 +202,32 @@ __global__ __launch_bounds__(256) void fill_kernel(
 
 void launch_adj_init(hipStream_t s, unsigned long long *adj, int p, int W) {
   long long n = (long long)p * W;
-  dim3 grid((unsigned)((n + 255) / 256)), block(256);
-  hipLaunchKernelGGL(adj_init_kernel, grid, block, 0, s, adj, p, W);
+  hipLaunchKernelGGL(adj_init_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, adj, p, W);
 }
 
-void launch_level0(hipStream_t s, const double *cor, unsigned long long *adj,
-                   int p, int W, double r_crit, Counters *ctr) {
-  const long long nt = (long long)W * (W + 1) / 2;
+void launch_level0(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst,
+                   long long *d_prefix, std::vector<long long> &h_prefix) {
+  const int K = L.K;
+  h_prefix.assign(K + 1, 0);
+  for (int k = 0; k < K; ++k) {
+    const Inst &I = host_inst[k];
+    const bool on = (I.flags & kActive) && I.level == 0;
+    h_prefix[k + 1] = h_prefix[k] + (on ? (long long)I.W * (I.W + 1) / 2 : 0);
+  }
+  if (h_prefix[K] == 0) return;
+  (void)hipMemcpyAsync(d_prefix, h_prefix.data(), h_prefix.size() * sizeof(long long),
+                       hipMemcpyHostToDevice, s);
   const int wpb = 4;
-  dim3 grid((unsigned)((nt + wpb - 1) / wpb)), block(64 * wpb);
-  hipLaunchKernelGGL(level0_kernel, grid, block, 0, s, cor, adj, p, W, r_crit,
-                     nt, ctr);
+  hipLaunchKernelGGL(level0_kernel, dim3((unsigned)((h_prefix[K] + wpb - 1) / wpb)), dim3(64 * wpb),
+                     0, s, L, d_prefix);
 }
 
-void launch_compact(hipStream_t s, const unsigned long long *adj, int p, int W,
-                    int32_t *deg, int32_t *rowptr, int32_t *nbr, int32_t *erow,
-                    Counters *ctr) {
+void launch_compact(hipStream_t s, const Layout &L) {
+  hipLaunchKernelGGL(degree_kernel, dim3((L.p_total + kDegWaves - 1) / kDegWaves),
+                     dim3(64 * kDegWaves), 0, s, L);
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(kScanThreads), 0, s, L);
   const int wpb = 4;
-  dim3 grid((p + wpb - 1) / wpb), block(64 * wpb);
-  hipLaunchKernelGGL(degree_kernel, grid, block, 0, s, adj, p, W, deg);
-  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(kScanThreads), 0, s, deg, p,
-                     rowptr, ctr);
-  hipLaunchKernelGGL(fill_kernel, grid, block, 0, s, adj, p, W, rowptr, nbr,
-                     erow);
+  hipLaunchKernelGGL(fill_kernel, dim3((L.p_total + wpb - 1) / wpb), dim3(64 * wpb), 0, s, L);
 }
 
 }  // namespace hpc

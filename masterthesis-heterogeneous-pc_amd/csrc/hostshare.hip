@@ -128,41 +128,38 @@ SearchFn search_for(int level) {
   }
 }
 
-__global__ void scatter_kernel(const int64_t *__restrict__ idx, const int64_t *__restrict__ val,
-                               long long count, int64_t *__restrict__ fp) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < count) fp[idx[t]] = val[t];
-}
 
 }  // namespace
 
 extern "C" int hpc_search_level_host(hpc_state *st, int level, const int32_t *rows,
                                      int nrows, int threads, hpc_level_stats *stats) {
   if (!st) return hpc_fail(HPC_EINVAL, "state is NULL");
+  Engine &e = st->eng;
+  InstHost &h = e.inst[st->id];
   if (level < 1 || level > kMaxLevel)
     return hpc_fail(HPC_ELEVEL, "host share supports levels 1..%d, not %d", kMaxLevel, level);
-  if (!st->compacted) return hpc_fail(HPC_ESTATE, "level %d needs hpc_compact first", level);
+  if (!h.compacted) return hpc_fail(HPC_ESTATE, "level %d needs hpc_compact first", level);
   if (!rows && nrows > 0) return hpc_fail(HPC_EINVAL, "rows is NULL");
   auto t0 = std::chrono::steady_clock::now();
-  HIPCHK(hipSetDevice(st->device));
-  const int p = st->p;
-  // host copies of the snapshot (and the correlation, once per matrix)
-  st->h_rowptr.resize(p + 1);
-  st->h_nbr.resize(st->n_edges > 0 ? st->n_edges : 1);
-  HIPCHK(hipMemcpyAsync(st->h_rowptr.data(), st->d_rowptr, (p + 1) * sizeof(int32_t),
-                        hipMemcpyDeviceToHost, st->copy_stream));
-  if (st->n_edges > 0)
-    HIPCHK(hipMemcpyAsync(st->h_nbr.data(), st->d_nbr, st->n_edges * sizeof(int32_t),
-                          hipMemcpyDeviceToHost, st->copy_stream));
-  if (!st->h_cor_valid) {
-    st->h_cor.resize((size_t)p * p);
-    HIPCHK(hipMemcpyAsync(st->h_cor.data(), st->d_cor, (size_t)p * p * sizeof(double),
-                          hipMemcpyDeviceToHost, st->copy_stream));
+  HIPCHK(hipSetDevice(e.device));
+  const int p = h.p;
+  // host copies of the snapshot (and the correlation, once per matrix); the
+  // snapshot of a one-instance engine uses local ids
+  e.h_rowptr.resize(p + 1);
+  e.h_nbr.resize(e.n_edges > 0 ? e.n_edges : 1);
+  HIPCHK(hipMemcpyAsync(e.h_rowptr.data(), e.d_rowptr, (p + 1) * sizeof(int32_t),
+                        hipMemcpyDeviceToHost, e.copy_stream));
+  if (e.n_edges > 0)
+    HIPCHK(hipMemcpyAsync(e.h_nbr.data(), e.d_nbr, e.n_edges * sizeof(int32_t),
+                          hipMemcpyDeviceToHost, e.copy_stream));
+  if (!e.h_cor_valid) {
+    e.h_cor.resize((size_t)p * p);
+    HIPCHK(hipMemcpyAsync(e.h_cor.data(), h.cor, (size_t)p * p * sizeof(double),
+                          hipMemcpyDeviceToHost, e.copy_stream));
   }
-  HIPCHK(hipStreamSynchronize(st->copy_stream));
-  st->h_cor_valid = true;
-  HostSnap snap{st->h_cor.data(), st->h_rowptr.data(), st->h_nbr.data(), st->h_binom.data(),
-                p, st->r_crit};
+  HIPCHK(hipStreamSynchronize(e.copy_stream));
+  e.h_cor_valid = true;
+  HostSnap snap{e.h_cor.data(), e.h_rowptr.data(), e.h_nbr.data(), e.h_binom.data(), p, h.r_crit};
   // work items: (row, neighbour position), rows longest first
   std::vector<std::pair<int, int>> order;
   for (int k = 0; k < nrows; ++k) {
@@ -181,7 +178,7 @@ extern "C" int hpc_search_level_host(hpc_state *st, int level, const int32_t *ro
   std::atomic<unsigned long long> T{0}, U{0}, E{0}, F{0};
   if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
   auto worker = [&]() {
-    unsigned long long t = 0, u = 0, e = 0, f = 0;
+    unsigned long long t = 0, u = 0, ed = 0, f = 0;
     for (;;) {
       const size_t b = next.fetch_add(10);
       if (b >= items.size()) break;
@@ -195,11 +192,11 @@ extern "C" int hpc_search_level_host(hpc_state *st, int level, const int32_t *ro
         if (total > 0) {
           t += f0 >= 0 ? (unsigned long long)f0 + 1 : total;
           u += total;
-          e += 1;
+          ed += 1;
         }
       }
     }
-    T += t; U += u; E += e; F += f;
+    T += t; U += u; E += ed; F += f;
   };
   std::vector<std::thread> pool;
   for (int k = 1; k < threads; ++k) pool.emplace_back(worker);
@@ -208,34 +205,35 @@ extern "C" int hpc_search_level_host(hpc_state *st, int level, const int32_t *ro
   // scatter the ranks into the device array
   const size_t cnt = items.size();
   if (cnt > 0) {
-    if (cnt > st->scatter_cap) {
-      if (st->d_scatter_idx) HIPCHK(hipFree(st->d_scatter_idx));
-      if (st->d_scatter_val) HIPCHK(hipFree(st->d_scatter_val));
-      st->d_scatter_idx = st->d_scatter_val = nullptr;
-      HIPCHK(hipMalloc(&st->d_scatter_idx, cnt * sizeof(int64_t)));
-      HIPCHK(hipMalloc(&st->d_scatter_val, cnt * sizeof(int64_t)));
-      st->scatter_cap = cnt;
+    if (cnt > e.scatter_cap) {
+      if (e.d_scatter_idx) HIPCHK(hipFree(e.d_scatter_idx));
+      if (e.d_scatter_val) HIPCHK(hipFree(e.d_scatter_val));
+      e.d_scatter_idx = e.d_scatter_val = nullptr;
+      HIPCHK(hipMalloc(&e.d_scatter_idx, cnt * sizeof(int64_t)));
+      HIPCHK(hipMalloc(&e.d_scatter_val, cnt * sizeof(int64_t)));
+      e.scatter_cap = cnt;
     }
-    HIPCHK(hipMemcpyAsync(st->d_scatter_idx, idx.data(), cnt * sizeof(int64_t),
-                          hipMemcpyHostToDevice, st->copy_stream));
-    HIPCHK(hipMemcpyAsync(st->d_scatter_val, val.data(), cnt * sizeof(int64_t),
-                          hipMemcpyHostToDevice, st->copy_stream));
-    hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0,
-                       st->copy_stream, st->d_scatter_idx, st->d_scatter_val, (long long)cnt,
-                       st->d_fp);
+    // after the compaction's reset of the array (the device share may still
+    // be searching on the main stream: its edges are disjoint from these)
+    HIPCHK(hipStreamWaitEvent(e.copy_stream, e.fp_ready, 0));
+    HIPCHK(hipMemcpyAsync(e.d_scatter_idx, idx.data(), cnt * sizeof(int64_t),
+                          hipMemcpyHostToDevice, e.copy_stream));
+    HIPCHK(hipMemcpyAsync(e.d_scatter_val, val.data(), cnt * sizeof(int64_t),
+                          hipMemcpyHostToDevice, e.copy_stream));
+    launch_scatter_fp(e.copy_stream, e.d_scatter_idx, e.d_scatter_val, (long long)cnt, e.d_fp);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(st->copy_stream));
+    HIPCHK(hipStreamSynchronize(e.copy_stream));
   }
   if (stats) {
     hpc_level_stats s;
     memset(&s, 0, sizeof(s));
     s.level = level;
-    s.max_deg = st->max_deg;
+    s.max_deg = h.max_deg;
     s.tests = (int64_t)T.load();
     s.tests_max = (int64_t)U.load();
     s.edges = (int64_t)E.load();
     s.pinv_fallbacks = (int64_t)F.load();
-    s.edges_left = st->undirected;
+    s.edges_left = h.undirected;
     s.ms_wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *stats = s;
   }

@@ -1,5 +1,13 @@
-// Internal declarations shared by the HIP translation units of
-// libheterogpc_hip.so.  Not part of the C ABI (include/heterogpc_hip.h).
+// Internal declarations shared by the translation units of libheterogpc_hip.so.
+// Not part of the C ABI (include/heterogpc_hip.h).
+//
+// Every kernel works on a set of skeleton instances at once.  Instance k owns
+// dense per-instance blocks (observations, z-scores, correlation, bit
+// adjacency, removal levels, sepset planes) addressed with local variable
+// ids 0..p_k-1; its variables also get global ids start_k..start_k+p_k-1 so
+// that the per-level neighbour lists (CSR), first passing ranks and search
+// tasks of all instances share one set of global arrays.  A single skeleton
+// is the one-instance case.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -12,116 +20,212 @@
 
 namespace hpc {
 
-// Device-wide counters one level writes; read back by the host once per level.
-struct Counters {
-  unsigned long long tests;      // canonical test count T_l
-  unsigned long long tests_max;  // U_l
-  unsigned long long edges;      // ordered edges searched
-  unsigned long long removed;    // undirected edges removed
-  unsigned long long n_edges;    // ordered edges at level start (CSR size)
-  int max_deg;                   // largest row length at level start
-  int pad_;
-  unsigned long long pinv_fallbacks;  // tests that took the pinv form
-  unsigned long long n_tasks;         // search tasks of a level >= 2
-};
-
 // Largest level the device kernels accept (conditioning-set size).
 constexpr int kMaxLevel = 16;
 // Binomial table columns: C(n, k) for k <= kMaxLevel.
 constexpr int kBinomK = kMaxLevel + 1;
 
-struct Views {
-  // read-only inputs of the test kernels
-  const double *cor;        // p x p
-  const int32_t *rowptr;    // p + 1
-  const int32_t *nbr;       // CSR neighbour ids (level-start snapshot)
-  const int32_t *erow;      // row of every ordered edge
-  const unsigned long long *binom;  // (p + 1) x kBinomK, saturating
-  int64_t *fp;              // first passing rank per ordered edge, -1 = none
-  Counters *ctr;
-  int p;
-  int max_deg;              // largest row length of the snapshot
-  double r_crit;
+// Device-wide counters of one phase; read back by the host.
+struct Counters {
+  unsigned long long n_edges;         // ordered edges of the snapshot (CSR size)
+  unsigned long long n_tasks;         // search tasks of a level >= 2
+  unsigned long long pinv_fallbacks;  // tests that took the pinv form
+  int max_deg;                        // largest row length of the snapshot
+  int pad_;
 };
 
-// ---- launchers (each defined in the .hip file that owns the kernel) --------
-void launch_moments(hipStream_t s, const double *x, int p, int n, int npad,
-                    double *mean, double *sd, double *z, uint8_t *nanm,
-                    int32_t *hasnan);
-void launch_corr_canon(hipStream_t s, const double *z, const uint8_t *nanm,
-                       const int32_t *hasnan, int p, int n, int npad,
-                       double *cor);
-void launch_corr_tiled(hipStream_t s, const double *z, int p, int n, int npad,
-                       double *cor);
+// Per-instance counters of one level.
+struct InstCounters {
+  unsigned long long tests;      // canonical CI tests T_l
+  unsigned long long tests_max;  // all conditioning sets U_l
+  unsigned long long edges;      // ordered edges searched
+  unsigned long long removed;    // undirected edges removed
+  int max_deg;                   // largest row length at level start
+  int pad_;
+};
+
+enum InstFlags {
+  kActive = 1,   // takes part in this phase
+  kHasNan = 2,   // observations hold NaN cells
+};
+
+// Device-side descriptor of one instance.
+struct Inst {
+  const double *x;        // [p][n] observations
+  double *z;              // [p][npad] z-scores
+  uint8_t *nanm;          // [p][npad] NaN mask
+  double *mean, *sd;      // [p]
+  double *cor;            // [p][p]
+  unsigned long long *adj;  // [p][W] bit adjacency
+  int8_t *rmlev;          // [p][p] removal level (pairs i < j), -1 none
+  int32_t *sep;           // [mc][p][p] sepset member planes
+  double r_crit;          // Fisher-z decision boundary of (n, alpha)
+  int start;              // first global variable id
+  int p, n, npad, W, mc;
+  int level;              // level of this phase
+  int flags;
+};
+
+// What every kernel sees: the instance table and the global arrays.
+struct Layout {
+  const Inst *inst;             // [K]
+  InstCounters *ictr;           // [K]
+  const int32_t *row_inst;      // [p_total] instance of each global variable
+  int K;
+  int p_total;
+  int max_deg;                  // largest row length over all instances
+  int pad_;
+  int32_t *deg;                 // [p_total]
+  int32_t *rowptr;              // [p_total + 1]
+  int32_t *nbr;                 // [E] global neighbour ids
+  int32_t *erow;                // [E] row of each ordered edge
+  int64_t *fp;                  // [E] first passing rank, -1 = none
+  const unsigned long long *binom;  // [(pmax + 1) x kBinomK], saturating
+  Counters *ctr;
+};
+
+// ---- launchers (each defined in the .hip file that owns the kernel) ------
+void launch_moments(hipStream_t s, const Layout &L);
+// pairwise sums: tiled kernel for NaN-free instances, masked kernel for the
+// others; prefix arrays hold per-instance block counts (device and host)
+void launch_corr(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst,
+                 long long *d_prefix, std::vector<long long> &h_prefix);
 void launch_adj_init(hipStream_t s, unsigned long long *adj, int p, int W);
-void launch_level0(hipStream_t s, const double *cor, unsigned long long *adj,
-                   int p, int W, double r_crit, Counters *ctr);
-void launch_compact(hipStream_t s, const unsigned long long *adj, int p, int W,
-                    int32_t *deg, int32_t *rowptr, int32_t *nbr, int32_t *erow,
-                    Counters *ctr);
-void launch_search_l1(hipStream_t s, const Views &v, int64_t n_edges,
+void launch_level0(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst,
+                   long long *d_prefix, std::vector<long long> &h_prefix);
+void launch_compact(hipStream_t s, const Layout &L);
+void launch_search_l1(hipStream_t s, const Layout &L, int64_t n_edges,
                       const int32_t *row_mask);
-// Levels >= 2: per-row task counts and their scan (total to ctr->n_tasks),
-// then the task list, then the search over it.
-void launch_task_plan(hipStream_t s, int level, const Views &v, int rpt,
+void launch_task_plan(hipStream_t s, int level, const Layout &L, int rpt,
                       const int32_t *row_mask, long long *row_tasks, long long *task_off);
-void launch_task_fill(hipStream_t s, int level, const Views &v, int rpt,
-                      const long long *row_tasks, const long long *task_off,
-                      unsigned long long *tasks);
-void launch_search_ln(hipStream_t s, int level, const Views &v,
+void launch_task_fill(hipStream_t s, const Layout &L, const long long *row_tasks,
+                      const long long *task_off, unsigned long long *tasks);
+void launch_search_ln(hipStream_t s, int level, const Layout &L,
                       const unsigned long long *tasks, long long n_tasks, int rpt);
-void launch_commit(hipStream_t s, int level, const Views &v, int64_t n_edges,
-                   unsigned long long *adj, int W, int32_t *sep, int8_t *rmlev);
+void launch_commit(hipStream_t s, const Layout &L, int64_t n_edges);
+void launch_scatter_fp(hipStream_t s, const int64_t *idx, const int64_t *val,
+                       long long count, int64_t *fp);
 
-}  // namespace hpc
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
 
-// One skeleton instance (include/heterogpc_hip.h).
-struct hpc_state {
+// Host bookkeeping of one instance.
+struct InstHost {
+  bool used = false;
+  int p = 0, n = 0, npad = 0, W = 0, mc = 1, max_level = 0;
+  double alpha = 0.0, r_crit = 0.0;
+  // device blocks
+  double *x = nullptr, *z = nullptr, *mean = nullptr, *sd = nullptr, *cor = nullptr;
+  uint8_t *nanm = nullptr;
+  unsigned long long *adj = nullptr;
+  int8_t *rmlev = nullptr;
+  int32_t *sep = nullptr;
+  bool have_x = false, have_cor = false, x_has_nan = false;
+  // level state
+  int start = 0;           // global id of local variable 0 in the last pack
+  int next_level = 0;      // level the next step runs
+  bool finished = false;   // no level left to run
+  bool compacted = false;  // snapshot matches the adjacency
+  bool ever_compacted = false;
+  int max_deg = 0;         // of the last compaction
+  int64_t undirected = 0;
+  size_t bytes = 0;        // device footprint
+  int error = 0;           // first error of a batched run (HPC_E*), 0 = none
+  std::vector<hpc_level_stats> hist;  // one record per level run
+};
+
+// A set of instances on one device with their shared per-level arrays and
+// one stream.
+struct Engine {
   int device = 0;
   hipStream_t stream = nullptr;
-  int p = 0, n = 0, npad = 0, W = 0;
-  int max_level = 0, mc = 1;
-  double alpha = 0.0, r_crit = 0.0;
-  // observations and correlation
-  double *d_x = nullptr, *d_z = nullptr, *d_mean = nullptr, *d_sd = nullptr;
-  uint8_t *d_nanm = nullptr;
-  int32_t *d_hasnan = nullptr;
-  double *d_cor = nullptr;
-  bool have_x = false, have_cor = false, x_has_nan = false;
-  // graph
-  unsigned long long *d_adj = nullptr;
+  hipStream_t copy_stream = nullptr;
+  std::vector<InstHost> inst;  // slot per instance id
+  // global arrays
+  Inst *d_inst = nullptr;
+  InstCounters *d_ictr = nullptr, *h_ictr = nullptr;
+  int32_t *d_row_inst = nullptr;
   int32_t *d_deg = nullptr, *d_rowptr = nullptr, *d_nbr = nullptr, *d_erow = nullptr;
   int32_t *d_rowmask = nullptr;
   int64_t *d_fp = nullptr;
-  int32_t *d_sep = nullptr;  // [mc][p][p]
-  int8_t *d_rmlev = nullptr;
+  long long *d_row_tasks = nullptr, *d_task_off = nullptr;
+  unsigned long long *d_tasks = nullptr;
   unsigned long long *d_binom = nullptr;
-  hpc::Counters *d_ctr = nullptr;
-  hpc::Counters *h_ctr = nullptr;  // pinned
-  // snapshot bookkeeping
-  bool compacted = false;     // neighbour lists match the adjacency
-  bool ever_compacted = false;
-  int64_t n_edges = 0;        // ordered edges in the snapshot
+  long long *d_prefix = nullptr;
+  Counters *d_ctr = nullptr, *h_ctr = nullptr;
+  size_t cap_inst = 0, cap_rows = 0, cap_edges = 0, cap_tasks = 0, cap_binom = 0;
+  std::vector<unsigned long long> h_binom;
+  std::vector<long long> h_prefix;
+  // the packed set of the current phase
+  std::vector<Inst> h_inst;
+  std::vector<int> packed;
+  std::vector<int32_t> h_row_inst;
+  int p_total = 0;
+  int64_t n_edges = 0;
   int max_deg = 0;
-  int64_t undirected = 0;     // edges currently present
-  hipEvent_t ev[7] = {};
-  bool corr_timed = false;
-  // host share of a level (CPU executor): a second stream for its copies and
-  // host copies of the snapshot and correlation
-  hipStream_t copy_stream = nullptr;
+  int ranks_per_task = 512;
+  hipEvent_t ev[8] = {};
+  hipEvent_t fp_ready = nullptr;  // recorded after the first-pass array is reset
+  // host share scratch (single-instance use)
   std::vector<double> h_cor;
   bool h_cor_valid = false;
   std::vector<int32_t> h_rowptr, h_nbr;
-  std::vector<unsigned long long> h_binom;
-  // level >= 2 search tasks
-  long long *d_row_tasks = nullptr, *d_task_off = nullptr;
-  unsigned long long *d_tasks = nullptr;
-  size_t task_cap = 0;
-  int ranks_per_task = 512;
-  int64_t *d_scatter_idx = nullptr;
-  int64_t *d_scatter_val = nullptr;
+  int64_t *d_scatter_idx = nullptr, *d_scatter_val = nullptr;
   size_t scatter_cap = 0;
+};
+
+double fisher_r_crit(int n, double alpha);
+int engine_init(Engine &e, int device);
+void engine_free(Engine &e);
+// New instance with device blocks and the initial adjacency; returns its id.
+int engine_add(Engine &e, int p, int n, double alpha, int max_level, int *id);
+void engine_remove(Engine &e, int id);
+int engine_set_observations(Engine &e, int id, const double *h_x);
+int engine_set_corr(Engine &e, int id, const double *h_cor);
+int engine_reset(Engine &e, int id);
+// Instance table of `ids` with the given per-instance levels; every phase
+// below works on the packed set.
+int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &levels);
+int engine_corr(Engine &e, float *ms_moments, float *ms_corr);
+// Compaction of the packed instances; updates their max_deg and n_edges.
+int engine_compact(Engine &e, float *ms);
+// Search of the packed instances at `level` (rows: global ids, NULL = all).
+int engine_search(Engine &e, int level, const int32_t *rows, int nrows, float *ms);
+// Level 0 of the packed instances at level 0 (tests and removes).
+int engine_level0(Engine &e, float *ms);
+int engine_commit(Engine &e, float *ms);
+int engine_read_inst_counters(Engine &e);
+int engine_grow_sep(Engine &e, int id, int mc);
+// One level for every id (correlation first for those without one): level 0,
+// compaction, the searches of every level present and the commit, with
+// three host synchronisations for the whole set.  Finished instances are
+// marked; their per-level records go to InstHost::hist.
+int engine_step(Engine &e, const std::vector<int> &ids);
+// Device bytes one instance needs (its blocks plus its share of the per-level
+// arrays).
+size_t engine_footprint(int p, int n, int max_level);
+int export_cor(Engine &e, int id, double *cor);
+int export_adj(Engine &e, int id, int32_t *adj);
+int export_sepsets(Engine &e, int id, int32_t *sep, int mc);
+
+}  // namespace hpc
+
+// One skeleton instance behind the single-instance ABI.
+struct hpc_state {
+  hpc::Engine eng;
+  int id = 0;
+  bool corr_timed = false;
+  float ms_moments = 0.f, ms_corr = 0.f;
 };
 
 // Error reporting shared by the ABI translation units.
 int hpc_fail(int code, const char *fmt, ...);
+
+#define HPC_HIPCHK(call)                                                    \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      return hpc_fail(HPC_EHIP, "%s failed: %s (%s:%d)", #call,             \
+                      hipGetErrorString(e_), __FILE__, __LINE__);           \
+  } while (0)

@@ -1,6 +1,6 @@
 // Level >= 1 CI tests (replaces testRowL1 / testRowLN<> of gpuRowTests.cu:29-205
 // and the workstealing variants of gpuWorkstealingTests.cu) and the level-end
-// edge removal.
+// edge removal, for every active instance of the layout at once.
 //
 // Search: for every ordered edge (i, j) of the level-start snapshot, the
 // lowest rank whose conditioning set S (an l-subset of N(i)\{j} in
@@ -18,6 +18,9 @@
 // sepset is the lower-id row's lowest passing set, else the higher-id row's.
 // It also counts the canonical tests (per ordered edge: first passing rank +
 // 1, else all its sets), which depend only on the first passing ranks.
+//
+// Neighbour lists hold global variable ids; an instance's dense blocks are
+// indexed with local ids (global - start).
 #pragma clang fp contract(off)
 
 #include "hpc_internal.hpp"
@@ -28,128 +31,115 @@ namespace {
 
 constexpr int kWavesPerBlock = 4;
 
-__device__ __forceinline__ void block_add(Counters *ctr, unsigned long long t,
-                                          unsigned long long u,
-                                          unsigned long long e,
-                                          unsigned long long f) {
-  __shared__ unsigned long long red[4][kWavesPerBlock];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (lane == 0) { red[0][wv] = t; red[1][wv] = u; red[2][wv] = e; red[3][wv] = f; }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-    for (int k = 0; k < kWavesPerBlock; ++k) {
-      s0 += red[0][k]; s1 += red[1][k]; s2 += red[2][k]; s3 += red[3][k];
-    }
-    if (s0) atomicAdd(&ctr->tests, s0);
-    if (s1) atomicAdd(&ctr->tests_max, s1);
-    if (s2) atomicAdd(&ctr->edges, s2);
-    if (s3) atomicAdd(&ctr->pinv_fallbacks, s3);
-  }
-}
-
 // Level 1 (testRowL1, gpuRowTests.cu:29-100; cpuRowTests.cpp:23-56).
 __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_kernel(
-    Views v, long long n_edges, const int32_t *__restrict__ row_mask) {
+    Layout L, long long n_edges, const int32_t *__restrict__ row_mask) {
   const int lane = threadIdx.x & 63;
   const long long wave0 = (long long)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   const long long nw = (long long)gridDim.x * kWavesPerBlock;
-  const int p = v.p;
   for (long long e = wave0; e < n_edges; e += nw) {
-    const int i = v.erow[e];
+    const int i = L.erow[e];
     if (row_mask && !row_mask[i]) continue;
-    const int base = v.rowptr[i];
-    const int nn = v.rowptr[i + 1] - base - 1;
+    const Inst &In = L.inst[L.row_inst[i]];
+    if (In.level != 1) continue;
+    const int base = L.rowptr[i];
+    const int nn = L.rowptr[i + 1] - base - 1;
     const int y = (int)(e - base);
     if (nn < 1) continue;
-    const int j = v.nbr[e];
-    const double x1 = v.cor[(size_t)i * p + j];
+    const int p = In.p, s0 = In.start;
+    const double *cor = In.cor;
+    const double r_crit = In.r_crit;
+    const int li = i - s0, lj = L.nbr[e] - s0;
+    const double x1 = cor[(size_t)li * p + lj];
     long long found = -1;
     for (int q0 = 0; q0 < nn; q0 += 64) {
       const int q = q0 + lane;
       bool pass = false;
       if (q < nn) {
-        const int k = v.nbr[base + (q < y ? q : q + 1)];
-        const double x2 = v.cor[(size_t)i * p + k];
-        const double x3 = v.cor[(size_t)j * p + k];
-        if (!(isnan(x1) || isnan(x2) || isnan(x3)))
-          pass = removes(pcor_l1(x1, x2, x3), v.r_crit);
+        const int lk = L.nbr[base + (q < y ? q : q + 1)] - s0;
+        const double x2 = cor[(size_t)li * p + lk];
+        const double x3 = cor[(size_t)lj * p + lk];
+        if (!(__builtin_isnan(x1) || __builtin_isnan(x2) || __builtin_isnan(x3)))
+          pass = removes(pcor_l1(x1, x2, x3), r_crit);
       }
       const unsigned long long m = __ballot(pass);
       if (m) { found = q0 + __ffsll((long long)m) - 1; break; }
     }
-    if (lane == 0) v.fp[e] = found;
+    if (lane == 0) L.fp[e] = found;
   }
 }
 
 // Level L >= 2 (testRowLN<>, gpuRowTests.cu:102-205; the general-level form of
 // cpuWorkstealingTests.cpp:39-112).  tasks[t] = (ordered edge << 32) | chunk;
 // chunk c covers ranks [c * rpt, (c + 1) * rpt).
-template <int L>
+template <int LV>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_kernel(
-    Views v, const unsigned long long *__restrict__ tasks, long long n_tasks, int rpt,
+    Layout L, const unsigned long long *__restrict__ tasks, long long n_tasks, int rpt,
     int lds_binom) {
-  // C(m, k) for m <= max_deg, k <= L staged in LDS (columns 0..L)
+  // C(m, k) for m <= max_deg, k <= LV staged in LDS (columns 0..LV)
   extern __shared__ unsigned long long s_binom[];
-  const unsigned long long *tab = v.binom;
+  const unsigned long long *tab = L.binom;
   int kc = kBinomK;
   if (lds_binom) {
-    const int rows = v.max_deg + 1;
-    for (int t = threadIdx.x; t < rows * (L + 1); t += blockDim.x) {
-      const int m = t / (L + 1), k = t % (L + 1);
-      s_binom[t] = v.binom[(size_t)m * kBinomK + k];
+    const int rows = L.max_deg + 1;
+    for (int t = threadIdx.x; t < rows * (LV + 1); t += blockDim.x) {
+      const int m = t / (LV + 1), k = t % (LV + 1);
+      s_binom[t] = L.binom[(size_t)m * kBinomK + k];
     }
     __syncthreads();
     tab = s_binom;
-    kc = L + 1;
+    kc = LV + 1;
   }
   const int lane = threadIdx.x & 63;
   const long long wave0 = (long long)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   const long long nw = (long long)gridDim.x * kWavesPerBlock;
   unsigned long long F = 0;
-  const int p = v.p;
-  unsigned long long *fpu = reinterpret_cast<unsigned long long *>(v.fp);
+  unsigned long long *fpu = reinterpret_cast<unsigned long long *>(L.fp);
   for (long long t = wave0; t < n_tasks; t += nw) {
     const unsigned long long task = tasks[t];
     const long long e = (long long)(task >> 32);
     const unsigned long long r_begin = (task & 0xffffffffull) * (unsigned long long)rpt;
     // a lower rank of this edge already passed: nothing here can win
     if (__atomic_load_n(&fpu[e], __ATOMIC_RELAXED) < r_begin) continue;
-    const int i = v.erow[e];
-    const int base = v.rowptr[i];
-    const int nn = v.rowptr[i + 1] - base - 1;
+    const int i = L.erow[e];
+    const Inst &In = L.inst[L.row_inst[i]];
+    const int p = In.p, s0 = In.start;
+    const double *cor = In.cor;
+    const double r_crit = In.r_crit;
+    const int base = L.rowptr[i];
+    const int nn = L.rowptr[i + 1] - base - 1;
     const int y = (int)(e - base);
-    const unsigned long long total = binom(tab, kc, nn, L);
+    const unsigned long long total = binom(tab, kc, nn, LV);
     const unsigned long long r_end = min(total, r_begin + (unsigned long long)rpt);
-    const int j = v.nbr[e];
-    const double a = v.cor[(size_t)i * p + j];
+    const int li = i - s0, lj = L.nbr[e] - s0;
+    const double a = cor[(size_t)li * p + lj];
     for (unsigned long long r0 = r_begin; r0 < r_end; r0 += 64) {
       const unsigned long long rank = r0 + lane;
       bool pass = false;
       if (rank < r_end) {
-        int q[L], S[L];
-        unrank<L>(tab, kc, rank, nn, q);
+        int q[LV], S[LV];
+        unrank<LV>(tab, kc, rank, nn, q);
 #pragma unroll
-        for (int c = 0; c < L; ++c) S[c] = v.nbr[base + (q[c] < y ? q[c] : q[c] + 1)];
-        double x[L], yv[L], C[L][L];
+        for (int c = 0; c < LV; ++c) S[c] = L.nbr[base + (q[c] < y ? q[c] : q[c] + 1)] - s0;
+        double x[LV], yv[LV], C[LV][LV];
         bool anynan = __builtin_isnan(a);
 #pragma unroll
-        for (int c = 0; c < L; ++c) {
-          x[c] = v.cor[(size_t)i * p + S[c]];
-          yv[c] = v.cor[(size_t)j * p + S[c]];
+        for (int c = 0; c < LV; ++c) {
+          x[c] = cor[(size_t)li * p + S[c]];
+          yv[c] = cor[(size_t)lj * p + S[c]];
           anynan |= __builtin_isnan(x[c]) || __builtin_isnan(yv[c]);
         }
 #pragma unroll
-        for (int c = 0; c < L; ++c)
+        for (int c = 0; c < LV; ++c)
 #pragma unroll
-          for (int r = c + 1; r < L; ++r) {
-            C[r][c] = v.cor[(size_t)S[c] * p + S[r]];
+          for (int r = c + 1; r < LV; ++r) {
+            C[r][c] = cor[(size_t)S[c] * p + S[r]];
             anynan |= __builtin_isnan(C[r][c]);
           }
         if (!anynan) {
           bool fb;
-          const double r = pcor_canon<L>(a, x, yv, C, fb);
-          pass = removes(r, v.r_crit);
+          const double r = pcor_canon<LV>(a, x, yv, C, fb);
+          pass = removes(r, r_crit);
           F += fb ? 1 : 0;
         }
       }
@@ -160,23 +150,25 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_kernel(
       }
     }
   }
-  // per-lane fallback counts -> wave sum
 #pragma unroll
   for (int s = 32; s > 0; s >>= 1) F += __shfl_xor(F, s, 64);
-  block_add(v.ctr, 0, 0, 0, F);
+  if (lane == 0 && F) atomicAdd(&L.ctr->pinv_fallbacks, F);
 }
 
-// Tasks per row: deg * ceil(C(deg - 1, L) / rpt) (0 for masked-out rows).
-__global__ void task_count_kernel(Views v, int level, int rpt, const int32_t *__restrict__ row_mask,
+// Tasks per row: deg * ceil(C(deg - 1, level) / rpt) for rows of instances at
+// this level (and in the row mask).
+__global__ void task_count_kernel(Layout L, int level, int rpt, const int32_t *__restrict__ row_mask,
                                   long long *__restrict__ row_tasks) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= v.p) return;
-  const int deg = v.rowptr[i + 1] - v.rowptr[i];
-  const unsigned long long total = binom(v.binom, kBinomK, deg - 1, level);
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= L.p_total) return;
+  const Inst &In = L.inst[L.row_inst[g]];
+  const int deg = L.rowptr[g + 1] - L.rowptr[g];
   long long n = 0;
-  if ((!row_mask || row_mask[i]) && total > 0)
-    n = (long long)deg * (long long)((total + rpt - 1) / rpt);
-  row_tasks[i] = n;
+  if ((In.flags & kActive) && In.level == level && (!row_mask || row_mask[g]) && deg > 0) {
+    const unsigned long long total = binom(L.binom, kBinomK, deg - 1, level);
+    if (total > 0) n = (long long)deg * (long long)((total + rpt - 1) / rpt);
+  }
+  row_tasks[g] = n;
 }
 
 // Exclusive scan of the per-row task counts (one workgroup), total to
@@ -205,19 +197,18 @@ __global__ __launch_bounds__(kScan) void task_scan_kernel(const long long *__res
 }
 
 // One wavefront per row writes its tasks, edge-major.
-__global__ __launch_bounds__(256) void task_fill_kernel(Views v, int level, int rpt,
-                                                        const long long *__restrict__ row_tasks,
+__global__ __launch_bounds__(256) void task_fill_kernel(Layout L, const long long *__restrict__ row_tasks,
                                                         const long long *__restrict__ off,
                                                         unsigned long long *__restrict__ tasks) {
   const int lane = threadIdx.x & 63;
-  const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (i >= v.p) return;
-  const long long n = row_tasks[i];
+  const int g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (g >= L.p_total) return;
+  const long long n = row_tasks[g];
   if (n == 0) return;
-  const int base = v.rowptr[i];
-  const int deg = v.rowptr[i + 1] - base;
+  const int base = L.rowptr[g];
+  const int deg = L.rowptr[g + 1] - base;
   const long long nch = n / deg;
-  const long long o = off[i];
+  const long long o = off[g];
   for (long long t = lane; t < n; t += 64) {
     const long long y = t / nch, c = t % nch;
     tasks[o + t] = ((unsigned long long)(base + y) << 32) | (unsigned long long)c;
@@ -233,139 +224,196 @@ __device__ __forceinline__ int find_pos(const int32_t *row, int len, int node) {
   return lo;
 }
 
-// Removal and sepset recording (cpuUtil.cpp:23-48).  The device keeps sepset
-// member k of pair (min, max) at sep[k*p*p + min*p + max] (one plane per
+// Removal and sepset recording (cpuUtil.cpp:23-48).  An instance keeps sepset
+// member m of pair (min, max) at sep[m*p*p + min*p + max] (one plane per
 // member, so the width can grow by appending planes) and the removal level
 // at rmlev[min*p + max]; the host export rebuilds the reference layout.
-__global__ __launch_bounds__(256) void commit_kernel(
-    Views v, long long n_edges, int level, unsigned long long *__restrict__ adj,
-    int W, int32_t *__restrict__ sep, int8_t *__restrict__ rmlev) {
+// Instances at level 0 were handled by the level-0 kernel.
+__global__ __launch_bounds__(256) void commit_kernel(Layout L, long long n_edges) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long T = 0, U = 0, E = 0;
+  int k = -1;
+  unsigned long long T = 0, U = 0, E = 0, R = 0;
   if (e < n_edges) {
-    const int i = v.erow[e];
-    const int j = v.nbr[e];
-    const int bi = v.rowptr[i];
-    const int nni = v.rowptr[i + 1] - bi - 1;
-    const unsigned long long total = binom(v.binom, kBinomK, nni, level);
-    const long long fi = v.fp[e];
-    if (total > 0) {
-      T = fi >= 0 ? (unsigned long long)fi + 1 : total;
-      U = total;
-      E = 1;
-    }
-    if (j > i) {
-      const int bj = v.rowptr[j];
-      const int yi = (int)(e - bi);
-      const int lenj = v.rowptr[j + 1] - bj;
-      const int yj = find_pos(v.nbr + bj, lenj, i);
-      // the adjacency is symmetric, so i is in N(j); guard anyway
-      const long long fj = (yj < lenj && v.nbr[bj + yj] == i) ? v.fp[bj + yj] : -1;
-      int row = -1, y = 0, base = 0;
-      unsigned long long rank = 0;
-      if (fi >= 0) { row = i; y = yi; base = bi; rank = (unsigned long long)fi; }
-      else if (fj >= 0) { row = j; y = yj; base = bj; rank = (unsigned long long)fj; }
-      if (row >= 0) {
-        const int nn = v.rowptr[row + 1] - base - 1;
-        atomicAnd(&adj[(size_t)i * W + (j >> 6)], ~(1ull << (j & 63)));
-        atomicAnd(&adj[(size_t)j * W + (i >> 6)], ~(1ull << (i & 63)));
-        rmlev[(size_t)i * v.p + j] = (int8_t)level;
-        const size_t plane = (size_t)v.p * v.p;
-        int32_t *out = sep + (size_t)i * v.p + j;
-        int val = 0;
-        for (int k = 0; k < level; ++k) {
-          for (;;) {
-            unsigned long long c = binom(v.binom, kBinomK, nn - 1 - val, level - 1 - k);
-            if (rank < c) break;
-            rank -= c;
-            ++val;
+    const int i = L.erow[e];
+    k = L.row_inst[i];
+    const Inst &In = L.inst[k];
+    const int level = In.level;
+    // levels the binomial table and the sepset planes cover (the host packs
+    // level <= mc <= kMaxLevel)
+    if (level >= 1 && level <= kMaxLevel && level <= In.mc) {
+      const int j = L.nbr[e];
+      const int bi = L.rowptr[i];
+      const int nni = L.rowptr[i + 1] - bi - 1;
+      const unsigned long long total = binom(L.binom, kBinomK, nni, level);
+      const long long fi = L.fp[e];
+      if (total > 0) {
+        T = fi >= 0 ? (unsigned long long)fi + 1 : total;
+        U = total;
+        E = 1;
+      }
+      if (j > i) {
+        const int bj = L.rowptr[j];
+        const int yi = (int)(e - bi);
+        const int lenj = L.rowptr[j + 1] - bj;
+        const int yj = find_pos(L.nbr + bj, lenj, i);
+        // the adjacency is symmetric, so i is in N(j); guard anyway
+        const long long fj = (yj < lenj && L.nbr[bj + yj] == i) ? L.fp[bj + yj] : -1;
+        int row = -1, y = 0, base = 0;
+        unsigned long long rank = 0;
+        if (fi >= 0) { row = i; y = yi; base = bi; rank = (unsigned long long)fi; }
+        else if (fj >= 0) { row = j; y = yj; base = bj; rank = (unsigned long long)fj; }
+        const int nn = row >= 0 ? L.rowptr[row + 1] - base - 1 : 0;
+        // a rank is valid for this level and row (the unranking below ends)
+        if (row >= 0 && rank < binom(L.binom, kBinomK, nn, level)) {
+          const int p = In.p, W = In.W, s0 = In.start;
+          const int li = i - s0, lj = j - s0;
+          atomicAnd(&In.adj[(size_t)li * W + (lj >> 6)], ~(1ull << (lj & 63)));
+          atomicAnd(&In.adj[(size_t)lj * W + (li >> 6)], ~(1ull << (li & 63)));
+          In.rmlev[(size_t)li * p + lj] = (int8_t)level;
+          const size_t plane = (size_t)p * p;
+          int32_t *out = In.sep + (size_t)li * p + lj;
+          int val = 0;
+          for (int m = 0; m < level; ++m) {
+            for (;;) {
+              unsigned long long c = binom(L.binom, kBinomK, nn - 1 - val, level - 1 - m);
+              if (rank < c) break;
+              rank -= c;
+              ++val;
+            }
+            const int q = val++;
+            out[(size_t)m * plane] = L.nbr[base + (q < y ? q : q + 1)] - s0;
           }
-          const int q = val++;
-          out[(size_t)k * plane] = v.nbr[base + (q < y ? q : q + 1)];
+          R = 1;
         }
-        atomicAdd(&v.ctr->removed, 1ull);
       }
     }
   }
+  // per-instance counts: reduced per wave when the wave holds one instance,
+  // then merged across the workgroup, so a one-instance launch issues one
+  // atomic per counter and workgroup
+  __shared__ int sk[4];
+  __shared__ unsigned long long sT[4], sU[4], sE[4], sR[4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int kk = k;
+  for (int s = 32; s > 0; s >>= 1) kk = max(kk, __shfl_xor(kk, s, 64));  // wave's instance, if uniform
+  const bool uniform = __all(k == kk || k < 0);
+  if (uniform) {
 #pragma unroll
-  for (int s = 32; s > 0; s >>= 1) {
-    T += __shfl_xor(T, s, 64);
-    U += __shfl_xor(U, s, 64);
-    E += __shfl_xor(E, s, 64);
+    for (int s = 32; s > 0; s >>= 1) {
+      T += __shfl_xor(T, s, 64);
+      U += __shfl_xor(U, s, 64);
+      E += __shfl_xor(E, s, 64);
+      R += __shfl_xor(R, s, 64);
+    }
+    if (lane == 0) {
+      sk[wv] = (E | R) ? kk : -1;
+      sT[wv] = T; sU[wv] = U; sE[wv] = E; sR[wv] = R;
+    }
+  } else {
+    if (E | R) {
+      atomicAdd(&L.ictr[k].tests, T);
+      atomicAdd(&L.ictr[k].tests_max, U);
+      atomicAdd(&L.ictr[k].edges, E);
+      if (R) atomicAdd(&L.ictr[k].removed, R);
+    }
+    if (lane == 0) sk[wv] = -1;
   }
-  if ((threadIdx.x & 63) == 0 && E) {
-    atomicAdd(&v.ctr->tests, T);
-    atomicAdd(&v.ctr->tests_max, U);
-    atomicAdd(&v.ctr->edges, E);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int cur = -1;
+    unsigned long long t = 0, u = 0, ed = 0, rm = 0;
+    for (int w = 0; w <= 4; ++w) {
+      const int kw = w < 4 ? sk[w] : -2;
+      if (kw == -1) continue;
+      if (kw != cur && cur >= 0) {
+        atomicAdd(&L.ictr[cur].tests, t);
+        atomicAdd(&L.ictr[cur].tests_max, u);
+        atomicAdd(&L.ictr[cur].edges, ed);
+        if (rm) atomicAdd(&L.ictr[cur].removed, rm);
+        t = u = ed = rm = 0;
+      }
+      if (kw < 0) break;
+      cur = kw;
+      t += sT[w]; u += sU[w]; ed += sE[w]; rm += sR[w];
+    }
   }
 }
 
-template <int L>
-void launch_ln(hipStream_t s, const Views &v, const unsigned long long *tasks,
+__global__ void scatter_kernel(const int64_t *__restrict__ idx, const int64_t *__restrict__ val,
+                               long long count, int64_t *__restrict__ fp) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) fp[idx[t]] = val[t];
+}
+
+template <int LV>
+void launch_ln(hipStream_t s, const Layout &L, const unsigned long long *tasks,
                long long n_tasks, int rpt) {
   long long blocks = (n_tasks + kWavesPerBlock - 1) / kWavesPerBlock;
   if (blocks > 8192) blocks = 8192;
-  const size_t lds = (size_t)(v.max_deg + 1) * (L + 1) * sizeof(unsigned long long);
+  const size_t lds = (size_t)(L.max_deg + 1) * (LV + 1) * sizeof(unsigned long long);
   const int use = lds <= 48 * 1024;
-  hipLaunchKernelGGL(search_ln_kernel<L>, dim3((unsigned)blocks), dim3(64 * kWavesPerBlock),
-                     use ? lds : 0, s, v, tasks, n_tasks, rpt, use);
+  hipLaunchKernelGGL(search_ln_kernel<LV>, dim3((unsigned)blocks), dim3(64 * kWavesPerBlock),
+                     use ? lds : 0, s, L, tasks, n_tasks, rpt, use);
 }
 
 }  // namespace
 
-void launch_search_l1(hipStream_t s, const Views &v, int64_t n_edges,
-                      const int32_t *row_mask) {
+void launch_search_l1(hipStream_t s, const Layout &L, int64_t n_edges, const int32_t *row_mask) {
   if (n_edges <= 0) return;
   long long blocks = (n_edges + kWavesPerBlock - 1) / kWavesPerBlock;
   if (blocks > 16384) blocks = 16384;
   hipLaunchKernelGGL(search_l1_kernel, dim3((unsigned)blocks), dim3(64 * kWavesPerBlock), 0, s,
-                     v, (long long)n_edges, row_mask);
+                     L, (long long)n_edges, row_mask);
 }
 
-void launch_task_plan(hipStream_t s, int level, const Views &v, int rpt,
+void launch_task_plan(hipStream_t s, int level, const Layout &L, int rpt,
                       const int32_t *row_mask, long long *row_tasks, long long *task_off) {
-  hipLaunchKernelGGL(task_count_kernel, dim3((v.p + 255) / 256), dim3(256), 0, s, v, level, rpt,
-                     row_mask, row_tasks);
-  hipLaunchKernelGGL(task_scan_kernel, dim3(1), dim3(kScan), 0, s, row_tasks, v.p, task_off,
-                     v.ctr);
+  hipLaunchKernelGGL(task_count_kernel, dim3((L.p_total + 255) / 256), dim3(256), 0, s, L, level,
+                     rpt, row_mask, row_tasks);
+  hipLaunchKernelGGL(task_scan_kernel, dim3(1), dim3(kScan), 0, s, row_tasks, L.p_total, task_off,
+                     L.ctr);
 }
 
-void launch_task_fill(hipStream_t s, int level, const Views &v, int rpt,
-                      const long long *row_tasks, const long long *task_off,
-                      unsigned long long *tasks) {
-  hipLaunchKernelGGL(task_fill_kernel, dim3((v.p + 3) / 4), dim3(256), 0, s, v, level, rpt,
-                     row_tasks, task_off, tasks);
+void launch_task_fill(hipStream_t s, const Layout &L, const long long *row_tasks,
+                      const long long *task_off, unsigned long long *tasks) {
+  hipLaunchKernelGGL(task_fill_kernel, dim3((L.p_total + 3) / 4), dim3(256), 0, s, L, row_tasks,
+                     task_off, tasks);
 }
 
-void launch_search_ln(hipStream_t s, int level, const Views &v,
+void launch_search_ln(hipStream_t s, int level, const Layout &L,
                       const unsigned long long *tasks, long long n_tasks, int rpt) {
   if (n_tasks <= 0) return;
   switch (level) {
-    case 2: launch_ln<2>(s, v, tasks, n_tasks, rpt); break;
-    case 3: launch_ln<3>(s, v, tasks, n_tasks, rpt); break;
-    case 4: launch_ln<4>(s, v, tasks, n_tasks, rpt); break;
-    case 5: launch_ln<5>(s, v, tasks, n_tasks, rpt); break;
-    case 6: launch_ln<6>(s, v, tasks, n_tasks, rpt); break;
-    case 7: launch_ln<7>(s, v, tasks, n_tasks, rpt); break;
-    case 8: launch_ln<8>(s, v, tasks, n_tasks, rpt); break;
-    case 9: launch_ln<9>(s, v, tasks, n_tasks, rpt); break;
-    case 10: launch_ln<10>(s, v, tasks, n_tasks, rpt); break;
-    case 11: launch_ln<11>(s, v, tasks, n_tasks, rpt); break;
-    case 12: launch_ln<12>(s, v, tasks, n_tasks, rpt); break;
-    case 13: launch_ln<13>(s, v, tasks, n_tasks, rpt); break;
-    case 14: launch_ln<14>(s, v, tasks, n_tasks, rpt); break;
-    case 15: launch_ln<15>(s, v, tasks, n_tasks, rpt); break;
-    case 16: launch_ln<16>(s, v, tasks, n_tasks, rpt); break;
+    case 2: launch_ln<2>(s, L, tasks, n_tasks, rpt); break;
+    case 3: launch_ln<3>(s, L, tasks, n_tasks, rpt); break;
+    case 4: launch_ln<4>(s, L, tasks, n_tasks, rpt); break;
+    case 5: launch_ln<5>(s, L, tasks, n_tasks, rpt); break;
+    case 6: launch_ln<6>(s, L, tasks, n_tasks, rpt); break;
+    case 7: launch_ln<7>(s, L, tasks, n_tasks, rpt); break;
+    case 8: launch_ln<8>(s, L, tasks, n_tasks, rpt); break;
+    case 9: launch_ln<9>(s, L, tasks, n_tasks, rpt); break;
+    case 10: launch_ln<10>(s, L, tasks, n_tasks, rpt); break;
+    case 11: launch_ln<11>(s, L, tasks, n_tasks, rpt); break;
+    case 12: launch_ln<12>(s, L, tasks, n_tasks, rpt); break;
+    case 13: launch_ln<13>(s, L, tasks, n_tasks, rpt); break;
+    case 14: launch_ln<14>(s, L, tasks, n_tasks, rpt); break;
+    case 15: launch_ln<15>(s, L, tasks, n_tasks, rpt); break;
+    case 16: launch_ln<16>(s, L, tasks, n_tasks, rpt); break;
     default: break;  // rejected by the caller (kMaxLevel)
   }
 }
 
-void launch_commit(hipStream_t s, int level, const Views &v, int64_t n_edges,
-                   unsigned long long *adj, int W, int32_t *sep, int8_t *rmlev) {
+void launch_commit(hipStream_t s, const Layout &L, int64_t n_edges) {
   if (n_edges <= 0) return;
-  dim3 g((unsigned)((n_edges + 255) / 256)), b(256);
-  hipLaunchKernelGGL(commit_kernel, g, b, 0, s, v, (long long)n_edges, level, adj, W, sep,
-                     rmlev);
+  hipLaunchKernelGGL(commit_kernel, dim3((unsigned)((n_edges + 255) / 256)), dim3(256), 0, s, L,
+                     (long long)n_edges);
+}
+
+void launch_scatter_fp(hipStream_t s, const int64_t *idx, const int64_t *val, long long count,
+                       int64_t *fp) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, idx,
+                     val, count, fp);
 }
 
 }  // namespace hpc

@@ -37,6 +37,30 @@ class LevelStats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class RequestInfo(C.Structure):
+    _fields_ = [
+        ("p", C.c_int32),
+        ("levels", C.c_int32),
+        ("max_cond_size", C.c_int32),
+        ("pad_", C.c_int32),
+        ("tests", C.c_int64),
+        ("queue_ms", C.c_double),
+        ("latency_ms", C.c_double),
+    ]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int64),
+        ("peak_active", C.c_int64),
+        ("peak_bytes", C.c_uint64),
+        ("budget", C.c_uint64),
+        ("busy_ms", C.c_double),
+        ("queued", C.c_int32),
+        ("running", C.c_int32),
+    ]
+
+
 class HpcError(RuntimeError):
     pass
 
@@ -73,6 +97,28 @@ SIGNATURES = {
     "hpc_get_compact": (_i, [_vp, _vp]),
     "hpc_get_sepsets": (_i, [_vp, _vp, _i]),
     "hpc_get_first_pass": (_i, [_vp, _vp, _vp, C.POINTER(_i64)]),
+    "hpc_footprint": (_i, [_u64, _i, _i, C.POINTER(_u64)]),
+    "hpc_batch_create": (_i, [_i, C.POINTER(_vp)]),
+    "hpc_batch_destroy": (_i, [_vp]),
+    "hpc_batch_add": (_i, [_vp, _u64, _i, _d, _i, _vp, C.POINTER(C.c_int)]),
+    "hpc_batch_remove": (_i, [_vp, _i]),
+    "hpc_batch_step": (_i, [_vp, C.POINTER(C.c_int)]),
+    "hpc_batch_run": (_i, [_vp, C.POINTER(C.c_int)]),
+    "hpc_batch_status": (_i, [_vp, _i, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                              C.POINTER(C.c_int)]),
+    "hpc_batch_get_stats": (_i, [_vp, _i, _ps, _i, C.POINTER(C.c_int)]),
+    "hpc_batch_max_cond_size": (_i, [_vp, _i]),
+    "hpc_batch_get_cor": (_i, [_vp, _i, _vp]),
+    "hpc_batch_get_adj": (_i, [_vp, _i, _vp]),
+    "hpc_batch_get_sepsets": (_i, [_vp, _i, _vp, _i]),
+    "hpc_batch_set_tuning": (_i, [_vp, _i, _i]),
+    "hpc_server_create": (_i, [_i, _u64, _i, C.POINTER(_vp)]),
+    "hpc_server_destroy": (_i, [_vp]),
+    "hpc_server_submit": (_i, [_vp, _u64, _i, _d, _i, _vp, C.POINTER(_i64)]),
+    "hpc_server_wait": (_i, [_vp, _i64, _d, C.POINTER(C.c_int)]),
+    "hpc_server_result": (_i, [_vp, _i64, _vp, _vp, _i, C.POINTER(RequestInfo)]),
+    "hpc_server_release": (_i, [_vp, _i64]),
+    "hpc_server_stats": (_i, [_vp, C.POINTER(ServerInfo)]),
 }
 
 _lib = None

@@ -1,0 +1,171 @@
+"""numpy-facing wrappers of hpc_batch (many skeletons stepped together) and
+hpc_server (continuous batching with HBM-footprint admission)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import LevelStats, RequestInfo, ServerInfo, check, load
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def footprint(p: int, n: int, max_level: int) -> int:
+    """Device bytes one instance of (p, n, max_level) needs at most."""
+    out = C.c_uint64()
+    check(load().hpc_footprint(int(p), int(n), int(max_level), C.byref(out)))
+    return int(out.value)
+
+
+class Batch:
+    """Independent skeleton instances on one GPU, one level of each per step."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        check(self.lib.hpc_batch_create(int(device), C.byref(h)))
+        self.h = h
+        self.shape: dict[int, tuple[int, int]] = {}
+
+    def close(self) -> None:
+        if self.h:
+            check(self.lib.hpc_batch_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, x: np.ndarray, alpha: float, max_level: int = 4) -> int:
+        """New instance on observations x ([p][n] float64); returns its id."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        p, n = x.shape
+        out = C.c_int()
+        check(self.lib.hpc_batch_add(self.h, p, n, float(alpha), int(max_level), _ptr(x),
+                                     C.byref(out)))
+        self.shape[out.value] = (p, n)
+        return out.value
+
+    def remove(self, iid: int) -> None:
+        check(self.lib.hpc_batch_remove(self.h, int(iid)))
+        self.shape.pop(iid, None)
+
+    def step(self) -> int:
+        left = C.c_int()
+        check(self.lib.hpc_batch_step(self.h, C.byref(left)))
+        return left.value
+
+    def run(self) -> int:
+        steps = C.c_int()
+        check(self.lib.hpc_batch_run(self.h, C.byref(steps)))
+        return steps.value
+
+    def status(self, iid: int) -> tuple[bool, int, int]:
+        f, lv, err = C.c_int(), C.c_int(), C.c_int()
+        check(self.lib.hpc_batch_status(self.h, int(iid), C.byref(f), C.byref(lv), C.byref(err)))
+        return bool(f.value), lv.value, err.value
+
+    def stats(self, iid: int) -> list[dict]:
+        buf = (LevelStats * 64)()
+        n = C.c_int()
+        check(self.lib.hpc_batch_get_stats(self.h, int(iid), buf, 64, C.byref(n)))
+        return [buf[k].as_dict() for k in range(min(n.value, 64))]
+
+    def max_cond_size(self, iid: int) -> int:
+        return int(self.lib.hpc_batch_max_cond_size(self.h, int(iid)))
+
+    def cor(self, iid: int) -> np.ndarray:
+        p = self.shape[iid][0]
+        out = np.empty((p, p), dtype=np.float64)
+        check(self.lib.hpc_batch_get_cor(self.h, int(iid), _ptr(out)))
+        return out
+
+    def adj(self, iid: int) -> np.ndarray:
+        p = self.shape[iid][0]
+        out = np.empty((p, p), dtype=np.int32)
+        check(self.lib.hpc_batch_get_adj(self.h, int(iid), _ptr(out)))
+        return out
+
+    def sepsets(self, iid: int, mc: int | None = None) -> np.ndarray:
+        p = self.shape[iid][0]
+        mc = self.max_cond_size(iid) if mc is None else int(mc)
+        out = np.empty((p, p, mc), dtype=np.int32)
+        check(self.lib.hpc_batch_get_sepsets(self.h, int(iid), _ptr(out), mc))
+        return out
+
+    def set_ranks_per_task(self, value: int) -> None:
+        check(self.lib.hpc_batch_set_tuning(self.h, 1, int(value)))
+
+
+class Server:
+    """Continuous-batching skeleton server on one GPU."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0, max_active: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        check(self.lib.hpc_server_create(int(device), int(hbm_budget), int(max_active),
+                                         C.byref(h)))
+        self.h = h
+        self.shape: dict[int, int] = {}
+
+    def close(self) -> None:
+        if self.h:
+            check(self.lib.hpc_server_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, x: np.ndarray, alpha: float, max_level: int = 4) -> int:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        p, n = x.shape
+        req = C.c_int64()
+        check(self.lib.hpc_server_submit(self.h, p, n, float(alpha), int(max_level), _ptr(x),
+                                         C.byref(req)))
+        self.shape[req.value] = p
+        return req.value
+
+    def wait(self, req: int, timeout_s: float = -1.0) -> bool:
+        done = C.c_int()
+        check(self.lib.hpc_server_wait(self.h, int(req), float(timeout_s), C.byref(done)))
+        return bool(done.value)
+
+    def result(self, req: int, mc: int | None = None, release: bool = True):
+        """(adj [p][p], sepsets [p][p][mc], info dict) of a finished request."""
+        p = self.shape[req]
+        info = RequestInfo()
+        check(self.lib.hpc_server_result(self.h, int(req), None, None, 0, C.byref(info)))
+        mc = info.max_cond_size if mc is None else int(mc)
+        adj = np.empty((p, p), dtype=np.int32)
+        sep = np.empty((p, p, mc), dtype=np.int32)
+        check(self.lib.hpc_server_result(self.h, int(req), _ptr(adj), _ptr(sep), mc,
+                                         C.byref(info)))
+        if release:
+            check(self.lib.hpc_server_release(self.h, int(req)))
+            self.shape.pop(req, None)
+        return adj, sep, {name: getattr(info, name) for name, _ in info._fields_}
+
+    def stats(self) -> dict:
+        info = ServerInfo()
+        check(self.lib.hpc_server_stats(self.h, C.byref(info)))
+        return {name: getattr(info, name) for name, _ in info._fields_}

@@ -71,3 +71,14 @@ def test_errors_are_reported_not_fatal(hpc):
     assert rc != 0
     with pytest.raises(hpc.HpcError):
         hpc.sem_generate(1, 5, 2.0)
+
+
+def test_footprint_is_host_only_and_monotone(hpc):
+    b = hpc.footprint(2000, 5000, 4)
+    # correlation + sepset planes + observations are a lower bound
+    assert b >= 2000 * 2000 * (8 + 4 * 4) + 2000 * 5000 * 8
+    assert hpc.footprint(2001, 5000, 4) > b
+    assert hpc.footprint(2000, 5001, 4) > b
+    assert hpc.footprint(2000, 5000, 8) > b
+    with pytest.raises(hpc.HpcError):
+        hpc.footprint(1, 10, 4)

@@ -1,0 +1,177 @@
+"""Batched and served skeletons through the C ABI (hpc_batch_*, hpc_server_*).
+
+Bar: every instance of a mixed batch -- different p, n, alpha, level caps,
+NaN cells, collinear columns -- gives the oracle's correlation, adjacency,
+sepsets and per-level test counts bit for bit, and the same as a single
+hpc_state run on its input; a served request gives the same result alone and
+in a mix, whatever the admission budget.
+"""
+import numpy as np
+import pytest
+
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed_inputs(hpc):
+    """(label, x, alpha, max_level) of a heterogeneous set of instances."""
+    out = []
+    for name in ("c1", "small", "dense", "deep"):
+        x, n, alpha, ml = workloads.sem(name)
+        out.append((name, x, alpha, ml))
+    x, n, alpha, _ = workloads.sem("small")
+    out.append(("small/cap2", x[:200].copy(), alpha, 2))
+    out.append(("small/cap0", x[:90].copy(), 0.01, 0))
+    xn = x[:80].copy()
+    rng = np.random.default_rng(3)
+    xn[rng.random(xn.shape) < 0.01] = np.nan
+    xn[5] = 1.25
+    xn[9, :] = np.nan
+    out.append(("nan", xn, alpha, 3))
+    xc = x[:60].copy()
+    xc[7] = xc[3]
+    xc[11] = 2.0 * xc[3] - 1.0
+    xc[20] = xc[21] + xc[22]
+    out.append(("collinear", xc, alpha, -1))
+    out.append(("cooling_house", workloads.cooling_house(), 0.05, 4))
+    rng = np.random.default_rng(11)
+    for k in range(5):
+        p = int(rng.integers(20, 240))
+        n = int(rng.integers(200, 3000))
+        d = float(rng.uniform(2, 6))
+        a = float(rng.choice([0.01, 0.05]))
+        xs = hpc.sem_generate(p, n, d, 0.1, 1.0, seed=1000 + k, fp32=True)
+        out.append((f"rand{k}", xs, a, int(rng.choice([-1, 3, 4]))))
+    return out
+
+
+def _oracle(orc, x, alpha, ml):
+    n = x.shape[1]
+    cor = orc.corr(x)
+    ref = orc.Skeleton(cor, n, alpha, max_level=ml)
+    recs = ref.run(ml)
+    return cor, ref, recs
+
+
+def _check_instance(b, iid, label, orc, x, alpha, ml, single=None):
+    print(f"[batch] {label}: p={x.shape[0]} n={x.shape[1]} alpha={alpha} max_level={ml}",
+          flush=True)
+    cor, ref, recs = _oracle(orc, x, alpha, ml)
+    assert np.array_equal(b.cor(iid), cor, equal_nan=True), f"{label}: correlation differs"
+    assert np.array_equal(b.adj(iid), ref.adj()), f"{label}: adjacency differs"
+    mc = ref.mc()
+    assert np.array_equal(b.sepsets(iid, mc), ref.sepsets()), f"{label}: sepsets differ"
+    finished, levels, err = b.status(iid)
+    assert finished and err == 0, (label, finished, err)
+    st = b.stats(iid)
+    assert len(st) == levels
+    # the oracle's unlimited run also records the final level with no tests
+    assert len(recs) in (levels, levels + 1), (label, len(recs), levels)
+    for lv, (d, o) in enumerate(zip(st, recs)):
+        assert d["level"] == lv
+        for key in ("tests", "tests_max", "removed"):
+            assert d[key] == o[key], (label, lv, key, d, o)
+    if single is not None:
+        s_adj, s_sep, s_stats = single
+        assert np.array_equal(b.adj(iid), s_adj), f"{label}: differs from the single run"
+        assert np.array_equal(b.sepsets(iid, s_sep.shape[2]), s_sep)
+        assert [d["tests"] for d in st] == [d["tests"] for d in s_stats]
+
+
+def _single(hpc, x, alpha, ml):
+    p, n = x.shape
+    with hpc.DeviceSkeleton(p, n, alpha, max_level=ml) as dev:
+        dev.pmcc(x)
+        stats = dev.run(ml)
+        return dev.adj(), dev.sepsets(), stats
+
+
+def test_batch_mixed_matches_oracle_and_single(hpc, orc):
+    inputs = _mixed_inputs(hpc)
+    singles = [_single(hpc, x, a, ml) for _, x, a, ml in inputs]
+    with hpc.Batch() as b:
+        ids = [b.add(x, a, ml) for _, x, a, ml in inputs]
+        steps = b.run()
+        print(f"[batch] {len(ids)} instances in {steps} steps", flush=True)
+        for iid, (label, x, a, ml), single in zip(ids, inputs, singles):
+            _check_instance(b, iid, label, orc, x, a, ml, single)
+
+
+def test_batch_continuous_admission_and_slot_reuse(hpc, orc):
+    inputs = _mixed_inputs(hpc)
+    first, later = inputs[:6], inputs[6:]
+    with hpc.Batch() as b:
+        ids = {}
+        for label, x, a, ml in first:
+            ids[label] = b.add(x, a, ml)
+        b.step()
+        b.step()
+        # drop one mid-run; its slot is reused by a later instance
+        gone = ids.pop("dense")
+        b.remove(gone)
+        for label, x, a, ml in later:
+            ids[label] = b.add(x, a, ml)
+            b.step()
+        b.run()
+        for label, x, a, ml in inputs:
+            if label in ids:
+                _check_instance(b, ids[label], label, orc, x, a, ml)
+
+
+def test_batch_many_small_instances(hpc, orc):
+    rng = np.random.default_rng(5)
+    inputs = []
+    for k in range(64):
+        p = int(rng.integers(10, 80))
+        n = int(rng.integers(100, 1500))
+        x = hpc.sem_generate(p, n, float(rng.uniform(1.5, 5)), 0.1, 1.0, seed=50 + k, fp32=True)
+        inputs.append((f"tiny{k}", x, float(rng.choice([0.01, 0.05])), -1))
+    with hpc.Batch() as b:
+        ids = [b.add(x, a, ml) for _, x, a, ml in inputs]
+        b.run()
+        for iid, (label, x, a, ml) in zip(ids, inputs):
+            _check_instance(b, iid, label, orc, x, a, ml)
+
+
+def _serve(hpc, inputs, budget=0, max_active=0, one_at_a_time=False):
+    out = {}
+    with hpc.Server(hbm_budget=budget, max_active=max_active) as s:
+        reqs = []
+        for label, x, a, ml in inputs:
+            r = s.submit(x, a, ml)
+            if one_at_a_time:
+                assert s.wait(r, 120.0)
+            reqs.append((label, r))
+        for label, r in reqs:
+            assert s.wait(r, 120.0), f"{label} did not finish"
+            out[label] = s.result(r)
+        stats = s.stats()
+    return out, stats
+
+
+def test_server_result_same_alone_and_in_mix(hpc, orc):
+    inputs = _mixed_inputs(hpc)
+    alone, _ = _serve(hpc, inputs, one_at_a_time=True)
+    mixed, st = _serve(hpc, inputs)
+    print(f"[server] mixed: {st}", flush=True)
+    biggest = max(hpc.footprint(x.shape[0], x.shape[1], ml) for _, x, _, ml in inputs)
+    tight, st2 = _serve(hpc, inputs, budget=int(biggest * 1.5), max_active=4)
+    print(f"[server] tight budget: {st2}", flush=True)
+    assert st2["peak_active"] <= 4 and st2["peak_bytes"] <= st2["budget"]
+    for label, x, a, ml in inputs:
+        cor, ref, recs = _oracle(orc, x, a, ml)
+        mc = ref.mc()
+        for mode, res in (("alone", alone), ("mixed", mixed), ("tight", tight)):
+            adj, sep, info = res[label]
+            assert np.array_equal(adj, ref.adj()), f"{label} ({mode}): adjacency differs"
+            assert np.array_equal(sep[:, :, :mc], ref.sepsets()), f"{label} ({mode}): sepsets"
+            assert info["tests"] == sum(o["tests"] for o in recs), (label, mode)
+            assert info["latency_ms"] >= info["queue_ms"] >= 0.0
+
+
+def test_server_rejects_request_beyond_budget(hpc):
+    x, n, alpha, _ = workloads.sem("small")
+    with hpc.Server(hbm_budget=1 << 20) as s:
+        with pytest.raises(hpc.HpcError):
+            s.submit(x, alpha, 4)

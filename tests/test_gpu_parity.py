@@ -166,3 +166,29 @@ def test_row_subset_then_rest(hpc, orc):
             assert np.array_equal(fp_half[seg], fp_full[seg])
         else:
             assert np.all(fp_half[seg] == -1)
+
+
+def test_host_share_alone_matches_device(hpc, orc):
+    """Every row searched on the host (the CPU-only executor) at levels 1..5:
+    the commit removes the same edges with the same sepsets as the device."""
+    x, n, alpha, _ = workloads.sem("small")
+    x = x[:150].copy()
+    p = x.shape[0]
+    full = hpc.DeviceSkeleton(p, n, alpha, max_level=5)
+    host = hpc.DeviceSkeleton(p, n, alpha, max_level=5)
+    full.pmcc(x)
+    host.pmcc(x)
+    full.level(0)
+    host.level(0)
+    rows = np.arange(p)
+    for level in range(1, 6):
+        print(f"[host-only] level {level}", flush=True)
+        full.level(level)
+        host.compact()
+        host.search_level(level, rows, host=True, threads=4)
+        host.commit_level(level)
+        fa, _ = full.first_pass()
+        fb, _ = host.first_pass()
+        assert np.array_equal(fa, fb), level
+        assert np.array_equal(full.adj(), host.adj()), level
+    assert np.array_equal(full.sepsets(5), host.sepsets(5))

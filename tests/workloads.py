@@ -42,3 +42,43 @@ def cooling_house() -> np.ndarray:
     """data/cooling_house.csv as variable-major [6][10000]."""
     X = np.loadtxt(os.path.join(GOLDEN, "cooling_house.csv"), delimiter=",")
     return np.ascontiguousarray(X.T)
+
+
+def batch_mix(count: int = 256, seed: int = 2024):
+    """Batched form: `count` independent SEM instances, p uniform in [20,400],
+    n uniform in [200,5000], expected degree uniform in [2,6], alpha in
+    {0.01,0.05}, FP32 data, levels until no edge has enough neighbours.
+    Returns [(x [p][n], alpha, max_level)]."""
+    from heterogpc import sem_generate
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        p = int(rng.integers(20, 401))
+        n = int(rng.integers(200, 5001))
+        d = float(rng.uniform(2.0, 6.0))
+        a = float(rng.choice([0.01, 0.05]))
+        x = sem_generate(p, n, d, 0.1, 1.0, seed=seed * 1000 + k, fp32=True)
+        out.append((x, a, -1))
+    return out
+
+
+def served_trace(count: int = 200, rate: float = 100.0, seed: int = 77, max_level: int = 4):
+    """Served form: Poisson arrivals at `rate` requests/s of SEM skeleton
+    requests, p log-uniform in [50,2000], n uniform in [500,10000], expected
+    degree uniform in [2,5], alpha in {0.01,0.05}, FP32 data.
+    Returns [(t_arrival_s, x [p][n], alpha, max_level)]."""
+    from heterogpc import sem_generate
+
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for k in range(count):
+        t += float(rng.exponential(1.0 / rate))
+        p = int(round(float(np.exp(rng.uniform(np.log(50.0), np.log(2000.0))))))
+        n = int(rng.integers(500, 10001))
+        d = float(rng.uniform(2.0, 5.0))
+        a = float(rng.choice([0.01, 0.05]))
+        x = sem_generate(p, n, d, 0.1, 1.0, seed=seed * 1000 + k, fp32=True)
+        out.append((t, x, a, max_level))
+    return out
