@@ -12,6 +12,8 @@ from __future__ import annotations
 
 import argparse
 import json
+
+import numpy as np
 import os
 import sys
 import time
@@ -87,12 +89,13 @@ def _pct(vals, q):
 
 def _plain_skeleton(heterogpc, x, alpha, max_level):
     """One instance through the plain single-instance path: state, upload,
-    correlation, levels, results to the host."""
+    correlation, levels, results (adjacency matrix and sepset list) to the
+    host."""
     p, n = x.shape
     with heterogpc.DeviceSkeleton(p, n, alpha, max_level=max_level) as dev:
         dev.pmcc(x)
         stats = dev.run(-2)
-        return dev.adj(), dev.sepsets(), stats
+        return dev.adj(), dev.sepset_list(), stats
 
 
 def bench_batched(heterogpc, insts):
@@ -112,9 +115,10 @@ def bench_batched(heterogpc, insts):
     with heterogpc.Batch() as b:
         ids = [b.add(x, a, ml) for x, a, ml in insts]
         steps = b.run()
-        got = [(b.adj(i), b.sepsets(i, pl[1].shape[2])) for i, pl in zip(ids, plain)]
+        got = [(b.adj(i), b.sepset_list(i)) for i in ids]
     t_batch = time.perf_counter() - t0
-    same = all((ga == pa).all() and (gs == ps).all() for (ga, gs), (pa, ps, _) in zip(got, plain))
+    same = all(np.array_equal(ga, pa) and np.array_equal(gs, ps)
+               for (ga, gs), (pa, ps, _) in zip(got, plain))
     tests = sum(lv["tests"] for _, _, st in plain for lv in st)
     k = len(insts)
     return {
@@ -130,21 +134,23 @@ def bench_batched(heterogpc, insts):
         "tests_total": tests,
         "batched_ci_tests_per_s": tests / t_batch,
         "results_identical": bool(same),
-        "scope": "host observations -> device, correlation, all levels, adjacency and "
-                 "sepsets back on the host",
+        "scope": "host observations -> device, correlation, all levels, adjacency matrix "
+                 "and sepset list back on the host",
     }
 
 
 def bench_served(heterogpc, trace, rate):
     """Served form: the arrival trace replayed in real time through the
     continuous-batching server and through the plain path (one request at a
-    time, in arrival order); latency = arrival -> result ready."""
+    time, in arrival order).  Latency = arrival -> adjacency matrix and sepset
+    list in the caller's hands, for both."""
     import threading
 
-    x0 = trace[0]
-    _plain_skeleton(heterogpc, x0[1], x0[2], x0[3])
+    _, x0, a0, m0 = trace[0]
+    _plain_skeleton(heterogpc, x0, a0, m0)
     n_req = len(trace)
     sub = [None] * n_req
+    served = [None] * n_req
     with heterogpc.Server() as srv:
         t0 = time.perf_counter()
 
@@ -153,21 +159,27 @@ def bench_served(heterogpc, trace, rate):
                 d = ta - (time.perf_counter() - t0)
                 if d > 0:
                     time.sleep(d)
-                ts = time.perf_counter() - t0
-                sub[k] = (srv.submit(x, a, ml), ts)
+                sub[k] = srv.submit(x, a, ml)
 
         th = threading.Thread(target=feeder)
         th.start()
+        left = set(range(n_req))
+        while left:
+            got = 0
+            for k in sorted(left):
+                if sub[k] is None:
+                    break
+                if srv.wait(sub[k], 0.0):
+                    adj, lst, info = srv.result(sub[k], dense_sepsets=False)
+                    served[k] = (adj, lst, info, time.perf_counter() - t0)
+                    left.discard(k)
+                    got += 1
+            if not got:
+                time.sleep(2e-4)
         th.join()
-        served = []
-        for k in range(n_req):
-            req, ts = sub[k]
-            srv.wait(req)
-            adj, sep, info = srv.result(req)
-            served.append((adj, sep, info, ts))
         stats = srv.stats()
-    s_lat = [info["latency_ms"] for _, _, info, _ in served]
-    s_end = max(ts + info["latency_ms"] * 1e-3 for _, _, info, ts in served)
+    s_lat = [(served[k][3] - trace[k][0]) * 1e3 for k in range(n_req)]
+    s_end = max(v[3] for v in served)
     t0 = time.perf_counter()
     plain_lat, plain = [], []
     for ta, x, a, ml in trace:
@@ -177,7 +189,7 @@ def bench_served(heterogpc, trace, rate):
         plain.append(_plain_skeleton(heterogpc, x, a, ml))
         plain_lat.append((time.perf_counter() - t0 - ta) * 1e3)
     p_end = time.perf_counter() - t0
-    same = all((sa == pa).all() and (ss[:, :, :ps.shape[2]] == ps).all()
+    same = all(np.array_equal(sa, pa) and np.array_equal(ss, ps)
                for (sa, ss, _, _), (pa, ps, _) in zip(served, plain))
     return {
         "workload": "%d requests, Poisson arrivals at %.0f/s, p log-uniform [50,2000], n U[500,"

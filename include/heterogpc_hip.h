@@ -16,6 +16,11 @@
  *                 row length at [i][p-1] (compact.cu:57-88)
  *   sepSets       p x p x mc ints at ((min*p + max)*mc + k); -1 empty or end,
  *                 -2 removed at level 0 (cpuUtil.cpp:3-48)
+ *
+ * Sepset list (the compact form of sepSets, no counterpart in the
+ * reference): one record per edge removed at a level l >= 1, l + 3 int32
+ * words: i, j (i < j), l, then the l members in ascending order; records
+ * sorted by (i, j).  Absent pairs without a record were removed at level 0.
  */
 #ifndef HETEROGPC_HIP_H
 #define HETEROGPC_HIP_H
@@ -158,6 +163,8 @@ int hpc_get_adj(hpc_state *st, int32_t *adj);              /* p*p */
 int hpc_get_compact(hpc_state *st, int32_t *adj_compact);  /* p*p */
 /* sepSets with width mc (pass hpc_max_cond_size(st) for the full width). */
 int hpc_get_sepsets(hpc_state *st, int32_t *sep, int mc);
+/* Sepset list; *words receives its length (out may be NULL to query it). */
+int hpc_get_sepset_list(hpc_state *st, int32_t *out, int64_t cap, int64_t *words);
 /* First passing rank per ordered edge of the last level (CSR order, -1 =
  * none), rowptr[p+1]; *n_edges receives the CSR size (fp may be NULL to
  * query it). */
@@ -197,6 +204,8 @@ int hpc_batch_max_cond_size(hpc_batch *b, int id);
 int hpc_batch_get_cor(hpc_batch *b, int id, double *cor);
 int hpc_batch_get_adj(hpc_batch *b, int id, int32_t *adj);
 int hpc_batch_get_sepsets(hpc_batch *b, int id, int32_t *sep, int mc);
+int hpc_batch_get_sepset_list(hpc_batch *b, int id, int32_t *out, int64_t cap,
+                              int64_t *words);
 int hpc_batch_set_tuning(hpc_batch *b, int param, int value);
 
 /* ---- continuous-batching server -------------------------------------------
@@ -242,6 +251,9 @@ int hpc_server_wait(hpc_server *s, int64_t req, double timeout_s, int *done);
  * NULL.  Returns the request's error when it failed. */
 int hpc_server_result(hpc_server *s, int64_t req, int32_t *adj, int32_t *sep,
                       int mc, hpc_request_info *info);
+/* Sepset list of a finished request (out may be NULL to query *words). */
+int hpc_server_sepset_list(hpc_server *s, int64_t req, int32_t *out, int64_t cap,
+                           int64_t *words);
 /* Drops a finished request's results. */
 int hpc_server_release(hpc_server *s, int64_t req);
 int hpc_server_stats(hpc_server *s, hpc_server_info *info);

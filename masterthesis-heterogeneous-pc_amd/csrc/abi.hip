@@ -8,6 +8,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <string>
 #include <vector>
@@ -278,10 +279,7 @@ extern "C" int hpc_search_level(hpc_state *st, int level, const int32_t *rows, i
     if (rc) return rc;
     fill_stats(st, 0, s);
   } else {
-    if (level > H(st).mc) {
-      rc = engine_grow_sep(st->eng, st->id, level);
-      if (rc) return rc;
-    }
+    engine_note_level(st->eng, st->id, level);
     rc = pack_at(st, level);  // counters of the compaction stay
     if (!rc && H(st).max_deg - 1 >= level) rc = engine_search(st->eng, level, rows, nrows, &ms);
     if (rc) return rc;
@@ -309,9 +307,9 @@ extern "C" int hpc_commit_level(hpc_state *st, int level, hpc_level_stats *stats
   s.max_deg = level == 0 ? H(st).p : H(st).max_deg;
   if (level >= 1) {
     // the host share searches without the device table: the commit kernel
-    // takes the level (and the sepset width) from a table packed here
-    if (level > H(st).mc) rc = engine_grow_sep(st->eng, st->id, level);
-    if (!rc) rc = pack_at(st, level);
+    // takes the level from a table packed here
+    engine_note_level(st->eng, st->id, level);
+    rc = pack_at(st, level);
     if (rc) return rc;
     float ms = 0.f;
     rc = engine_commit(st->eng, &ms);
@@ -418,7 +416,9 @@ int export_cor(Engine &e, int id, double *cor) {
   return HPC_OK;
 }
 
-static int fetch_adj(Engine &e, InstHost &h, std::vector<unsigned long long> &bits) {
+int fetch_adj_bits(Engine &e, int id, std::vector<unsigned long long> &bits) {
+  InstHost &h = e.inst[id];
+  HIPCHK(hipSetDevice(e.device));
   bits.resize((size_t)h.p * h.W);
   HIPCHK(hipMemcpyAsync(bits.data(), h.adj, bits.size() * sizeof(bits[0]), hipMemcpyDeviceToHost,
                         e.stream));
@@ -426,42 +426,88 @@ static int fetch_adj(Engine &e, InstHost &h, std::vector<unsigned long long> &bi
   return HPC_OK;
 }
 
-int export_adj(Engine &e, int id, int32_t *adj) {
+int fetch_records(Engine &e, int id, std::vector<int32_t> &rec) {
   InstHost &h = e.inst[id];
   HIPCHK(hipSetDevice(e.device));
-  std::vector<unsigned long long> bits;
-  int rc = fetch_adj(e, h, bits);
-  if (rc) return rc;
-  const int p = h.p, W = h.W;
+  std::vector<int32_t> raw(h.rec_used);
+  if (h.rec_used) {
+    HIPCHK(hipMemcpyAsync(raw.data(), h.rec, h.rec_used * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          e.stream));
+    HIPCHK(hipStreamSynchronize(e.stream));
+  }
+  // records are appended in scheduling order: sort by (i, j)
+  std::vector<std::pair<int64_t, size_t>> order;
+  for (size_t a = 0; a + 3 <= raw.size();) {
+    const int l = raw[a + 2];
+    if (l < 1 || a + 3 + (size_t)l > raw.size())
+      return fail(HPC_EHIP, "corrupt sepset record at word %zu", a);
+    order.push_back({(int64_t)raw[a] * h.p + raw[a + 1], a});
+    a += 3 + (size_t)l;
+  }
+  std::sort(order.begin(), order.end());
+  rec.clear();
+  rec.reserve(raw.size());
+  for (auto &o : order) {
+    const size_t a = o.second;
+    rec.insert(rec.end(), raw.begin() + a, raw.begin() + a + 3 + raw[a + 2]);
+  }
+  return HPC_OK;
+}
+
+void dense_adj(const std::vector<unsigned long long> &bits, int p, int32_t *adj) {
+  const int W = (p + 63) / 64;
+  for (int i = 0; i < p; ++i) {
+    const unsigned long long *row = bits.data() + (size_t)i * W;
+    int32_t *out = adj + (size_t)i * p;
+    for (int j = 0; j < p; ++j) out[j] = (int32_t)((row[j >> 6] >> (j & 63)) & 1ull);
+  }
+}
+
+void dense_sepsets(const std::vector<unsigned long long> &bits, const std::vector<int32_t> &rec,
+                   int p, int32_t *sep, int mc) {
+  const int W = (p + 63) / 64;
+  const size_t plane = (size_t)p * p;
+  std::fill(sep, sep + plane * mc, -1);
+  // every absent pair was removed at level 0 unless a record says otherwise
   for (int i = 0; i < p; ++i)
-    for (int j = 0; j < p; ++j)
-      adj[(size_t)i * p + j] = (int32_t)((bits[(size_t)i * W + (j >> 6)] >> (j & 63)) & 1ull);
+    for (int j = i + 1; j < p; ++j)
+      if (!((bits[(size_t)i * W + (j >> 6)] >> (j & 63)) & 1ull)) sep[((size_t)i * p + j) * mc] = -2;
+  for (size_t a = 0; a + 3 <= rec.size(); a += 3 + (size_t)rec[a + 2]) {
+    const int i = rec[a], j = rec[a + 1], l = rec[a + 2];
+    int32_t *slot = sep + ((size_t)i * p + j) * mc;
+    slot[0] = -1;
+    for (int m = 0; m < l && m < mc; ++m) slot[m] = rec[a + 3 + m];
+  }
+}
+
+int export_adj(Engine &e, int id, int32_t *adj) {
+  std::vector<unsigned long long> bits;
+  int rc = fetch_adj_bits(e, id, bits);
+  if (rc) return rc;
+  dense_adj(bits, e.inst[id].p, adj);
   return HPC_OK;
 }
 
 int export_sepsets(Engine &e, int id, int32_t *sep, int mc) {
-  InstHost &h = e.inst[id];
-  HIPCHK(hipSetDevice(e.device));
-  const int p = h.p, W = h.W;
-  const size_t plane = (size_t)p * p;
   std::vector<unsigned long long> bits;
-  int rc = fetch_adj(e, h, bits);
+  std::vector<int32_t> rec;
+  int rc = fetch_adj_bits(e, id, bits);
+  if (!rc) rc = fetch_records(e, id, rec);
   if (rc) return rc;
-  std::vector<int8_t> lev(plane);
-  std::vector<int32_t> dev((size_t)h.mc * plane);
-  HIPCHK(hipMemcpyAsync(lev.data(), h.rmlev, plane, hipMemcpyDeviceToHost, e.stream));
-  HIPCHK(hipMemcpyAsync(dev.data(), h.sep, dev.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
-                        e.stream));
-  HIPCHK(hipStreamSynchronize(e.stream));
-  for (size_t a = 0; a < plane * mc; ++a) sep[a] = -1;
-  for (int i = 0; i < p; ++i)
-    for (int j = i + 1; j < p; ++j) {
-      if ((bits[(size_t)i * W + (j >> 6)] >> (j & 63)) & 1ull) continue;
-      int32_t *slot = sep + ((size_t)i * p + j) * mc;
-      const int l = lev[(size_t)i * p + j];
-      if (l <= 0) { slot[0] = -2; continue; }  // removed at level 0
-      for (int k = 0; k < l && k < mc; ++k) slot[k] = dev[(size_t)k * plane + (size_t)i * p + j];
-    }
+  dense_sepsets(bits, rec, e.inst[id].p, sep, mc);
+  return HPC_OK;
+}
+
+int export_sepset_list(Engine &e, int id, int32_t *out, int64_t cap, int64_t *words) {
+  std::vector<int32_t> rec;
+  int rc = fetch_records(e, id, rec);
+  if (rc) return rc;
+  if (words) *words = (int64_t)rec.size();
+  if (out) {
+    if ((int64_t)rec.size() > cap)
+      return fail(HPC_EINVAL, "sepset list needs %zu words, cap is %lld", rec.size(), (long long)cap);
+    std::copy(rec.begin(), rec.end(), out);
+  }
   return HPC_OK;
 }
 
@@ -480,6 +526,11 @@ extern "C" int hpc_get_adj(hpc_state *st, int32_t *adj) {
 extern "C" int hpc_get_sepsets(hpc_state *st, int32_t *sep, int mc) {
   if (!st || !sep || mc < 1) return fail(HPC_EINVAL, "bad argument");
   return export_sepsets(st->eng, st->id, sep, mc);
+}
+
+extern "C" int hpc_get_sepset_list(hpc_state *st, int32_t *out, int64_t cap, int64_t *words) {
+  if (!st) return fail(HPC_EINVAL, "state is NULL");
+  return export_sepset_list(st->eng, st->id, out, cap, words);
 }
 
 extern "C" int hpc_get_compact(hpc_state *st, int32_t *out) {

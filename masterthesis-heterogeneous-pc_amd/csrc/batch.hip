@@ -37,6 +37,9 @@ struct hpc_batch {
   int steps = 0;
 };
 
+// Instance blocks of batched and served engines come from 512 MiB slabs.
+static constexpr size_t kSlabBytes = (size_t)512 << 20;
+
 extern "C" int hpc_footprint(uint64_t p, int n, int max_level, uint64_t *bytes) {
   if (!bytes || p < 2 || n < 1) return hpc_fail(HPC_EINVAL, "bad argument");
   *bytes = engine_footprint((int)p, n, max_level);
@@ -47,6 +50,7 @@ extern "C" int hpc_batch_create(int device, hpc_batch **out) {
   if (!out) return hpc_fail(HPC_EINVAL, "out is NULL");
   *out = nullptr;
   hpc_batch *b = new hpc_batch();
+  b->eng.arena.slab_min = kSlabBytes;
   int rc = engine_init(b->eng, device);
   if (rc) { engine_free(b->eng); delete b; return rc; }
   *out = b;
@@ -74,6 +78,7 @@ extern "C" int hpc_batch_add(hpc_batch *b, uint64_t p, int n, double alpha, int 
   if (rc) return rc;
   rc = engine_set_observations(b->eng, *id, x);
   if (rc) { engine_remove(b->eng, *id); return rc; }
+  b->eng.inst[*id].drop_obs = true;
   return HPC_OK;
 }
 
@@ -151,6 +156,13 @@ extern "C" int hpc_batch_get_sepsets(hpc_batch *b, int id, int32_t *sep, int mc)
   return export_sepsets(b->eng, id, sep, mc);
 }
 
+extern "C" int hpc_batch_get_sepset_list(hpc_batch *b, int id, int32_t *out, int64_t cap,
+                                         int64_t *words) {
+  int rc = check_id(b, id);
+  if (rc) return rc;
+  return export_sepset_list(b->eng, id, out, cap, words);
+}
+
 extern "C" int hpc_batch_get_cor(hpc_batch *b, int id, double *cor) {
   int rc = check_id(b, id);
   if (rc) return rc;
@@ -178,13 +190,15 @@ struct Request {
   int p = 0, n = 0, max_level = 0;
   double alpha = 0.0;
   std::vector<double> x;
+  int has_nan = 0;
   size_t bytes = 0;
   int slot = -1;       // engine instance while admitted
   int state = 0;       // 0 queued, 1 running, 2 done, 3 failed
   int error = 0;
   std::string message;
   Clock::time_point t_submit, t_admit, t_done;
-  std::vector<int32_t> adj, sep;
+  std::vector<unsigned long long> bits;  // adjacency, [p][W] words
+  std::vector<int32_t> rec;              // sepset records sorted by (i, j)
   int mc = 0;
   int levels = 0;
   int64_t tests = 0;
@@ -211,21 +225,21 @@ struct hpc_server {
   double busy_ms = 0.0;
 };
 
-static void server_finish(hpc_server *s, std::shared_ptr<Request> &r) {
-  InstHost &h = s->eng.inst[r->slot];
-  r->levels = (int)h.hist.size();
-  for (auto &l : h.hist) r->tests += l.tests;
-  r->mc = h.mc;
-  r->adj.resize((size_t)r->p * r->p);
-  r->sep.resize((size_t)r->p * r->p * r->mc);
+// Copies a finished request's results to the host and frees its instance;
+// returns its error code.  Runs on the worker without the lock: the request
+// is published (state set) afterwards, under the lock.
+static int server_collect(hpc_server *s, Request &r) {
+  InstHost &h = s->eng.inst[r.slot];
+  r.levels = (int)h.hist.size();
+  for (auto &l : h.hist) r.tests += l.tests;
+  r.mc = h.mc;
   int rc = h.error;
-  if (!rc) rc = export_adj(s->eng, r->slot, r->adj.data());
-  if (!rc) rc = export_sepsets(s->eng, r->slot, r->sep.data(), r->mc);
-  engine_remove(s->eng, r->slot);
-  r->slot = -1;
-  r->error = rc;
-  r->t_done = Clock::now();
-  r->state = rc ? 3 : 2;
+  if (!rc) rc = fetch_adj_bits(s->eng, r.slot, r.bits);
+  if (!rc) rc = fetch_records(s->eng, r.slot, r.rec);
+  if (rc && r.message.empty()) r.message = hpc_last_error();
+  engine_remove(s->eng, r.slot);
+  r.slot = -1;
+  return rc;
 }
 
 static void server_loop(hpc_server *s) {
@@ -253,8 +267,9 @@ static void server_loop(hpc_server *s) {
       int slot = -1;
       int rc = engine_add(s->eng, r->p, r->n, r->alpha, r->max_level, &slot);
       if (!rc) {
-        rc = engine_set_observations(s->eng, slot, r->x.data());
+        rc = engine_set_observations(s->eng, slot, r->x.data(), r->has_nan);
         if (rc) engine_remove(s->eng, slot);
+        else s->eng.inst[slot].drop_obs = true;
       }
       std::lock_guard<std::mutex> lk(s->mu);
       r->t_admit = Clock::now();
@@ -284,20 +299,29 @@ static void server_loop(hpc_server *s) {
     auto t0 = Clock::now();
     const int rc = engine_step(s->eng, ids);
     const double ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+    const std::string msg = rc ? hpc_last_error() : "";
+    // only the worker writes `running`, so it reads it without the lock
+    std::vector<std::shared_ptr<Request>> done, keep;
+    for (auto &r : s->running) {
+      InstHost &h = s->eng.inst[r->slot];
+      if (rc) {
+        h.error = rc;
+        r->message = msg;
+      }
+      (rc || h.finished ? done : keep).push_back(r);
+    }
+    std::vector<int> codes;
+    for (auto &r : done) codes.push_back(server_collect(s, *r));
+    const auto now = Clock::now();
     std::lock_guard<std::mutex> lk(s->mu);
     s->busy_ms += ms;
     ++s->steps;
-    std::vector<std::shared_ptr<Request>> keep;
-    for (auto &r : s->running) {
-      InstHost &h = s->eng.inst[r->slot];
-      if (rc) h.error = rc;
-      if (rc || h.finished) {
-        s->in_use -= r->bytes;
-        if (rc) r->message = hpc_last_error();
-        server_finish(s, r);
-      } else {
-        keep.push_back(r);
-      }
+    for (size_t k = 0; k < done.size(); ++k) {
+      Request &r = *done[k];
+      s->in_use -= r.bytes;
+      r.error = codes[k];
+      r.t_done = now;
+      r.state = codes[k] ? 3 : 2;
     }
     s->running.swap(keep);
     s->cv_done.notify_all();
@@ -309,6 +333,7 @@ extern "C" int hpc_server_create(int device, uint64_t hbm_budget, int max_active
   if (!out) return hpc_fail(HPC_EINVAL, "out is NULL");
   *out = nullptr;
   hpc_server *s = new hpc_server();
+  s->eng.arena.slab_min = kSlabBytes;
   int rc = engine_init(s->eng, device);
   if (rc) { engine_free(s->eng); delete s; return rc; }
   if (hbm_budget == 0) {
@@ -346,7 +371,19 @@ extern "C" int hpc_server_submit(hpc_server *s, uint64_t p, int n, double alpha,
   r->n = n;
   r->alpha = alpha;
   r->max_level = max_level;
-  r->x.assign(x, x + (size_t)p * n);
+  // copy and NaN scan on the caller's thread, off the worker's path
+  r->x.resize((size_t)p * n);
+  {
+    const double *src = x;
+    double *dst = r->x.data();
+    bool nan = false;
+    for (size_t k = 0; k < (size_t)p * n; ++k) {
+      const double v = src[k];
+      dst[k] = v;
+      nan |= v != v;
+    }
+    r->has_nan = nan ? 1 : 0;
+  }
   r->bytes = engine_footprint((int)p, n, max_level);
   r->t_submit = Clock::now();
   {
@@ -378,27 +415,46 @@ extern "C" int hpc_server_wait(hpc_server *s, int64_t req, double timeout_s, int
 extern "C" int hpc_server_result(hpc_server *s, int64_t req, int32_t *adj, int32_t *sep, int mc,
                                  hpc_request_info *info) {
   if (!s) return hpc_fail(HPC_EINVAL, "server is NULL");
-  std::lock_guard<std::mutex> lk(s->mu);
-  auto it = s->all.find(req);
-  if (it == s->all.end()) return hpc_fail(HPC_EINVAL, "no request %lld", (long long)req);
-  auto &r = it->second;
-  if (r->state < 2) return hpc_fail(HPC_ESTATE, "request %lld not finished", (long long)req);
-  if (r->state == 3)
-    return hpc_fail(r->error ? r->error : HPC_EHIP, "request %lld failed: %s", (long long)req,
-                    r->message.c_str());
-  if (adj) memcpy(adj, r->adj.data(), r->adj.size() * sizeof(int32_t));
-  if (sep) {
-    const size_t pp = (size_t)r->p * r->p;
-    for (size_t e = 0; e < pp; ++e)
-      for (int k = 0; k < mc; ++k) sep[e * mc + k] = k < r->mc ? r->sep[e * r->mc + k] : -1;
+  if (sep && mc < 1) return hpc_fail(HPC_EINVAL, "sepset width must be >= 1");
+  std::shared_ptr<Request> r;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(req);
+    if (it == s->all.end()) return hpc_fail(HPC_EINVAL, "no request %lld", (long long)req);
+    r = it->second;
+    if (r->state < 2) return hpc_fail(HPC_ESTATE, "request %lld not finished", (long long)req);
+    if (r->state == 3)
+      return hpc_fail(r->error ? r->error : HPC_EHIP, "request %lld failed: %s", (long long)req,
+                      r->message.c_str());
   }
+  // a finished request is no longer written by the worker: the dense
+  // layouts are built without the lock
+  if (adj) dense_adj(r->bits, r->p, adj);
+  if (sep) dense_sepsets(r->bits, r->rec, r->p, sep, mc);
   if (info) {
     info->p = r->p;
     info->levels = r->levels;
     info->max_cond_size = r->mc;
+    info->pad_ = 0;
     info->tests = r->tests;
     info->queue_ms = std::chrono::duration<double, std::milli>(r->t_admit - r->t_submit).count();
     info->latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+  }
+  return HPC_OK;
+}
+
+extern "C" int hpc_server_sepset_list(hpc_server *s, int64_t req, int32_t *out, int64_t cap,
+                                      int64_t *words) {
+  if (!s) return hpc_fail(HPC_EINVAL, "server is NULL");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->all.find(req);
+  if (it == s->all.end()) return hpc_fail(HPC_EINVAL, "no request %lld", (long long)req);
+  auto &r = it->second;
+  if (r->state != 2) return hpc_fail(HPC_ESTATE, "request %lld has no result", (long long)req);
+  if (words) *words = (int64_t)r->rec.size();
+  if (out) {
+    if ((int64_t)r->rec.size() > cap) return hpc_fail(HPC_EINVAL, "cap too small");
+    std::copy(r->rec.begin(), r->rec.end(), out);
   }
   return HPC_OK;
 }

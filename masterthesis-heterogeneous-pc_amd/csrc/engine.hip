@@ -56,6 +56,75 @@ static void dfree(T *&p) {
   p = nullptr;
 }
 
+// ---- arena ----------------------------------------------------------------
+constexpr size_t kArenaAlign = 256;
+
+static int arena_alloc(Engine &e, size_t bytes, void **out) {
+  Arena &A = e.arena;
+  const size_t need = std::max<size_t>((bytes + kArenaAlign - 1) & ~(kArenaAlign - 1), kArenaAlign);
+  for (size_t si = 0; si < A.slabs.size(); ++si) {
+    auto &fr = A.slabs[si].free;
+    for (auto it = fr.begin(); it != fr.end(); ++it) {
+      if (it->second < need) continue;
+      const size_t off = it->first, len = it->second;
+      fr.erase(it);
+      if (len > need) fr.emplace(off + need, len - need);
+      *out = A.slabs[si].base + off;
+      A.live[*out] = {(int)si, need};
+      A.used += need;
+      return HPC_OK;
+    }
+  }
+  Arena::Slab sl;
+  sl.size = std::max(need, A.slab_min);
+  void *base = nullptr;
+  hipError_t err = hipMalloc(&base, sl.size);
+  if (err != hipSuccess)
+    return hpc_fail(HPC_ENOMEM, "hipMalloc(%zu) failed: %s", sl.size, hipGetErrorString(err));
+  sl.base = (char *)base;
+  if (sl.size > need) sl.free.emplace(need, sl.size - need);
+  A.slabs.push_back(std::move(sl));
+  A.reserved += A.slabs.back().size;
+  *out = base;
+  A.live[base] = {(int)A.slabs.size() - 1, need};
+  A.used += need;
+  return HPC_OK;
+}
+
+static void arena_free(Engine &e, const void *ptr) {
+  if (!ptr) return;
+  Arena &A = e.arena;
+  auto it = A.live.find(ptr);
+  if (it == A.live.end()) return;
+  const int si = it->second.first;
+  const size_t len = it->second.second;
+  A.live.erase(it);
+  A.used -= len;
+  auto &fr = A.slabs[si].free;
+  size_t off = (size_t)((const char *)ptr - A.slabs[si].base), l = len;
+  auto nx = fr.lower_bound(off);
+  if (nx != fr.end() && nx->first == off + l) { l += nx->second; nx = fr.erase(nx); }
+  if (nx != fr.begin()) {
+    auto pv = std::prev(nx);
+    if (pv->first + pv->second == off) { off = pv->first; l += pv->second; fr.erase(pv); }
+  }
+  fr.emplace(off, l);
+}
+
+template <class T>
+static int ialloc(Engine &e, T *&p, size_t count) {
+  void *v = nullptr;
+  int rc = arena_alloc(e, count * sizeof(T), &v);
+  p = (T *)v;
+  return rc;
+}
+
+template <class T>
+static void ifree(Engine &e, T *&p) {
+  arena_free(e, (const void *)p);
+  p = nullptr;
+}
+
 // grows a device array to hold n elements (contents dropped)
 template <class T>
 static int grow(T *&p, size_t &cap, size_t n) {
@@ -86,17 +155,24 @@ int engine_init(Engine &e, int device) {
   return HPC_OK;
 }
 
-static void free_blocks(InstHost &h) {
-  dfree(h.x); dfree(h.z); dfree(h.nanm); dfree(h.mean); dfree(h.sd); dfree(h.cor);
-  dfree(h.adj); dfree(h.rmlev); dfree(h.sep);
+static void free_obs(Engine &e, InstHost &h) {
+  ifree(e, h.x); ifree(e, h.z); ifree(e, h.nanm);
+}
+
+static void free_blocks(Engine &e, InstHost &h) {
+  free_obs(e, h);
+  ifree(e, h.mean); ifree(e, h.sd); ifree(e, h.cor); ifree(e, h.adj); ifree(e, h.rec);
+  h.rec_cap = h.rec_used = 0;
 }
 
 void engine_free(Engine &e) {
   (void)hipSetDevice(e.device);
   if (e.stream) (void)hipStreamSynchronize(e.stream);
   if (e.copy_stream) (void)hipStreamSynchronize(e.copy_stream);
-  for (auto &h : e.inst) free_blocks(h);
   e.inst.clear();
+  for (auto &sl : e.arena.slabs)
+    if (sl.base) (void)hipFree(sl.base);
+  e.arena = Arena();
   dfree(e.d_inst); dfree(e.d_ictr); dfree(e.d_row_inst); dfree(e.d_deg); dfree(e.d_rowptr);
   dfree(e.d_nbr); dfree(e.d_erow); dfree(e.d_rowmask); dfree(e.d_fp); dfree(e.d_row_tasks);
   dfree(e.d_task_off); dfree(e.d_tasks); dfree(e.d_binom); dfree(e.d_prefix); dfree(e.d_ctr);
@@ -137,6 +213,12 @@ static int ensure_binom(Engine &e, int rows) {
   return HPC_OK;
 }
 
+// Sepset width before any level has run: the level cap (at least 1), or 4
+// with no cap (the reference's default maxCondSize).
+static int initial_mc(int max_level) {
+  return max_level > 1 ? std::min(max_level, kMaxLevel) : (max_level < 0 ? 4 : 1);
+}
+
 int engine_add(Engine &e, int p, int n, double alpha, int max_level, int *id) {
   if (p < 2 || p > 46340) return hpc_fail(HPC_EINVAL, "p = %d outside [2, 46340]", p);
   if (n < 1) return hpc_fail(HPC_EINVAL, "observations must be >= 1");
@@ -154,21 +236,19 @@ int engine_add(Engine &e, int p, int n, double alpha, int max_level, int *id) {
   h.W = (p + 63) / 64;
   h.alpha = alpha;
   h.max_level = max_level;
-  h.mc = max_level > 1 ? std::min(max_level, kMaxLevel) : (max_level < 0 ? 4 : 1);
+  h.mc = initial_mc(max_level);
   h.r_crit = fisher_r_crit(n, alpha);
   const size_t pp = (size_t)p * p;
   int rc = 0;
-  if (!rc) rc = dalloc((void **)&h.cor, pp * sizeof(double));
-  if (!rc) rc = dalloc((void **)&h.adj, (size_t)p * h.W * sizeof(unsigned long long));
-  if (!rc) rc = dalloc((void **)&h.rmlev, pp);
-  if (!rc) rc = dalloc((void **)&h.sep, (size_t)h.mc * pp * sizeof(int32_t));
-  if (!rc) rc = dalloc((void **)&h.mean, p * sizeof(double));
-  if (!rc) rc = dalloc((void **)&h.sd, p * sizeof(double));
+  if (!rc) rc = ialloc(e, h.cor, pp);
+  if (!rc) rc = ialloc(e, h.adj, (size_t)p * h.W);
+  if (!rc) rc = ialloc(e, h.mean, (size_t)p);
+  if (!rc) rc = ialloc(e, h.sd, (size_t)p);
   if (!rc) rc = ensure_binom(e, p + 1);
-  if (rc) { free_blocks(h); h.used = false; return rc; }
-  h.bytes = pp * (8 + 1 + 4 * h.mc) + (size_t)p * h.W * 8 + 16 * (size_t)p;
+  if (rc) { free_blocks(e, h); h.used = false; return rc; }
+  h.bytes = pp * 8 + (size_t)p * h.W * 8 + 16 * (size_t)p;
   rc = engine_reset(e, slot);
-  if (rc) { free_blocks(h); h.used = false; return rc; }
+  if (rc) { free_blocks(e, h); h.used = false; return rc; }
   *id = slot;
   return HPC_OK;
 }
@@ -176,8 +256,7 @@ int engine_add(Engine &e, int p, int n, double alpha, int max_level, int *id) {
 void engine_remove(Engine &e, int id) {
   if (id < 0 || id >= (int)e.inst.size() || !e.inst[id].used) return;
   (void)hipSetDevice(e.device);
-  (void)hipStreamSynchronize(e.stream);
-  free_blocks(e.inst[id]);
+  free_blocks(e, e.inst[id]);
   e.inst[id] = InstHost();
 }
 
@@ -186,7 +265,8 @@ int engine_reset(Engine &e, int id) {
   HPC_HIPCHK(hipSetDevice(e.device));
   launch_adj_init(e.stream, h.adj, h.p, h.W);
   HPC_HIPCHK(hipGetLastError());
-  HPC_HIPCHK(hipMemsetAsync(h.rmlev, 0xFF, (size_t)h.p * h.p, e.stream));
+  h.rec_used = 0;
+  h.mc = initial_mc(h.max_level);
   h.next_level = 0;
   h.finished = false;
   h.compacted = false;
@@ -196,24 +276,26 @@ int engine_reset(Engine &e, int id) {
   return HPC_OK;
 }
 
-int engine_set_observations(Engine &e, int id, const double *h_x) {
+int engine_set_observations(Engine &e, int id, const double *h_x, int nan_hint) {
   InstHost &h = e.inst[id];
   HPC_HIPCHK(hipSetDevice(e.device));
   int rc = 0;
-  if (!h.x) rc = dalloc((void **)&h.x, (size_t)h.p * h.n * sizeof(double));
-  if (!rc && !h.z) rc = dalloc((void **)&h.z, (size_t)h.p * h.npad * sizeof(double));
-  if (!rc && !h.nanm) rc = dalloc((void **)&h.nanm, (size_t)h.p * h.npad);
+  const bool fresh = !h.x;
+  if (!h.x) rc = ialloc(e, h.x, (size_t)h.p * h.n);
+  if (!rc && !h.z) rc = ialloc(e, h.z, (size_t)h.p * h.npad);
+  if (!rc && !h.nanm) rc = ialloc(e, h.nanm, (size_t)h.p * h.npad);
   if (rc) return rc;
   const size_t cnt = (size_t)h.p * h.n;
   HPC_HIPCHK(hipMemcpyAsync(h.x, h_x, cnt * sizeof(double), hipMemcpyHostToDevice, e.stream));
   // NaN cells need the masked pairwise kernel (corHelper.cu:101 skips them)
-  bool any_nan = false;
-  for (size_t k = 0; k < cnt; ++k)
-    if (h_x[k] != h_x[k]) { any_nan = true; break; }
+  bool any_nan = nan_hint > 0;
+  if (nan_hint < 0)
+    for (size_t k = 0; k < cnt; ++k)
+      if (h_x[k] != h_x[k]) { any_nan = true; break; }
   h.x_has_nan = any_nan;
   HPC_HIPCHK(hipStreamSynchronize(e.stream));
   h.have_x = true;
-  h.bytes += cnt * 8 + (size_t)h.p * h.npad * 9;
+  if (fresh) h.bytes += cnt * 8 + (size_t)h.p * h.npad * 9;
   return HPC_OK;
 }
 
@@ -228,20 +310,27 @@ int engine_set_corr(Engine &e, int id, const double *h_cor) {
   return HPC_OK;
 }
 
-int engine_grow_sep(Engine &e, int id, int mc) {
+void engine_note_level(Engine &e, int id, int level) {
   InstHost &h = e.inst[id];
-  if (mc <= h.mc) return HPC_OK;
-  const size_t plane = (size_t)h.p * h.p;
-  int32_t *ns = nullptr;
-  int rc = dalloc((void **)&ns, (size_t)mc * plane * sizeof(int32_t));
+  if (level > h.mc) h.mc = std::min(level, kMaxLevel);
+}
+
+// Room for the worst case of one level: every remaining edge removed, each
+// with a record of 3 + level words; contents kept.
+static int ensure_rec(Engine &e, InstHost &h, int level) {
+  const size_t need = h.rec_used + (size_t)std::max<int64_t>(h.undirected, 0) * (3 + level);
+  if (need <= h.rec_cap && h.rec) return HPC_OK;
+  const size_t cap = std::max(need, 2 * h.rec_cap);
+  int32_t *nr = nullptr;
+  int rc = ialloc(e, nr, cap);
   if (rc) return rc;
-  HPC_HIPCHK(hipMemcpyAsync(ns, h.sep, (size_t)h.mc * plane * sizeof(int32_t),
-                            hipMemcpyDeviceToDevice, e.stream));
-  HPC_HIPCHK(hipStreamSynchronize(e.stream));
-  dfree(h.sep);
-  h.sep = ns;
-  h.bytes += (size_t)(mc - h.mc) * plane * 4;
-  h.mc = mc;
+  if (h.rec_used)
+    HPC_HIPCHK(hipMemcpyAsync(nr, h.rec, h.rec_used * sizeof(int32_t), hipMemcpyDeviceToDevice,
+                              e.stream));
+  ifree(e, h.rec);
+  h.bytes += (cap - h.rec_cap) * sizeof(int32_t);
+  h.rec = nr;
+  h.rec_cap = cap;
   return HPC_OK;
 }
 
@@ -315,10 +404,10 @@ int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &
     InstHost &h = e.inst[ids[k]];
     Inst &I = e.h_inst[k];
     I.x = h.x; I.z = h.z; I.nanm = h.nanm; I.mean = h.mean; I.sd = h.sd; I.cor = h.cor;
-    I.adj = h.adj; I.rmlev = h.rmlev; I.sep = h.sep;
+    I.adj = h.adj; I.rec = h.rec; I.rec_base = h.rec_used;
     I.r_crit = h.r_crit;
     I.start = start;
-    I.p = h.p; I.n = h.n; I.npad = h.npad; I.W = h.W; I.mc = h.mc;
+    I.p = h.p; I.n = h.n; I.npad = h.npad; I.W = h.W;
     I.level = levels[k];
     I.flags = kActive | (h.x_has_nan ? kHasNan : 0);
     h.start = start;
@@ -501,6 +590,24 @@ int engine_search(Engine &e, int level, const int32_t *rows, int nrows, float *m
 
 int engine_commit(Engine &e, float *ms) {
   HPC_HIPCHK(hipSetDevice(e.device));
+  // room for the level's sepset records; the table is rewritten, so wait for
+  // earlier uploads of it
+  HPC_HIPCHK(hipStreamSynchronize(e.stream));
+  for (size_t k = 0; k < e.packed.size(); ++k) {
+    const int l = e.h_inst[k].level;
+    if (l < 1 || l > kMaxLevel) continue;
+    InstHost &h = e.inst[e.packed[k]];
+    int rc = ensure_rec(e, h, l);
+    if (rc) return rc;
+    e.h_inst[k].rec = h.rec;
+    e.h_inst[k].rec_base = h.rec_used;
+  }
+  if (!e.packed.empty()) {
+    HPC_HIPCHK(hipMemcpyAsync(e.d_inst, e.h_inst.data(), e.packed.size() * sizeof(Inst),
+                              hipMemcpyHostToDevice, e.stream));
+    // the removal kernel's counts (and record offsets) start from zero
+    HPC_HIPCHK(hipMemsetAsync(e.d_ictr, 0, e.packed.size() * sizeof(InstCounters), e.stream));
+  }
   Layout L = layout(e);
   HPC_HIPCHK(hipEventRecord(e.ev[6], e.stream));
   launch_commit(e.stream, L, e.n_edges);
@@ -513,6 +620,7 @@ int engine_commit(Engine &e, float *ms) {
     if (e.h_inst[k].level < 1) continue;
     InstHost &h = e.inst[e.packed[k]];
     h.undirected -= (int64_t)e.h_ictr[k].removed;
+    h.rec_used += (size_t)e.h_ictr[k].rec_words;
     if (e.h_ictr[k].removed) h.compacted = false;
   }
   return HPC_OK;
@@ -527,8 +635,9 @@ size_t engine_footprint(int p, int n, int max_level) {
   const int mc = max_level > 1 ? std::min(max_level, kMaxLevel) : (max_level < 0 ? 4 : 1);
   const size_t npad = ((size_t)n + 63) & ~(size_t)63;
   const size_t W = ((size_t)p + 63) / 64;
+  (void)mc;
   size_t b = pp * 8                 // correlation
-             + pp * (1 + 4 * mc)    // removal levels, sepset planes
+             + pp * 8               // sepset records (level 1, every edge removed)
              + (size_t)p * W * 8    // bit adjacency
              + (size_t)p * n * 8    // observations
              + (size_t)p * npad * 9 // z-scores and NaN mask
@@ -555,14 +664,18 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
     if (!rc) rc = engine_corr(e, nullptr, nullptr);
     if (rc) return rc;
   }
+  // observations and z-scores are only read by the correlation
+  for (int id : need)
+    if (e.inst[id].drop_obs) {
+      InstHost &h = e.inst[id];
+      free_obs(e, h);
+      h.have_x = false;
+      h.bytes -= std::min(h.bytes, (size_t)h.p * h.n * 8 + (size_t)h.p * h.npad * 9);
+    }
   std::vector<int> act, lev;
   for (int id : ids) {
     InstHost &h = e.inst[id];
     if (h.finished) continue;
-    if (h.next_level > h.mc && h.next_level <= kMaxLevel) {
-      rc = engine_grow_sep(e, id, std::min(kMaxLevel, std::max(h.next_level, 2 * h.mc)));
-      if (rc) return rc;
-    }
     act.push_back(id);
     lev.push_back(h.next_level);
   }
@@ -608,7 +721,13 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
       skip[k] = (testable || h.max_level < 0) ? 1 : 2;
       e.h_inst[k].level = -1;
       retable = true;
+      continue;
     }
+    rc = ensure_rec(e, h, lev[k]);
+    if (rc) return rc;
+    e.h_inst[k].rec = h.rec;
+    e.h_inst[k].rec_base = h.rec_used;
+    retable = true;
   }
   if (retable)
     HPC_HIPCHK(hipMemcpyAsync(e.d_inst, e.h_inst.data(), K * sizeof(Inst), hipMemcpyHostToDevice,
@@ -682,6 +801,8 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
       }
       s.ms_wall = wall;  // of the whole step (shared by the instances in it)
       h.hist.push_back(s);
+      h.rec_used += (size_t)c.rec_words;
+      if (lev[k] >= 1) engine_note_level(e, act[k], lev[k]);
       if (s.removed) h.compacted = false;
     }
     h.next_level += 1;

@@ -13,7 +13,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/heterogpc_hip.h"
@@ -40,8 +42,9 @@ struct InstCounters {
   unsigned long long tests_max;  // all conditioning sets U_l
   unsigned long long edges;      // ordered edges searched
   unsigned long long removed;    // undirected edges removed
+  unsigned long long rec_words;  // sepset record words appended
   int max_deg;                   // largest row length at level start
-  int pad_;
+  int has_nan;                   // observations hold NaN cells (correlation phase)
 };
 
 enum InstFlags {
@@ -57,11 +60,13 @@ struct Inst {
   double *mean, *sd;      // [p]
   double *cor;            // [p][p]
   unsigned long long *adj;  // [p][W] bit adjacency
-  int8_t *rmlev;          // [p][p] removal level (pairs i < j), -1 none
-  int32_t *sep;           // [mc][p][p] sepset member planes
+  // sepset records of edges removed at levels >= 1, appended by the removal
+  // kernel: i, j (i < j, local ids), level, then the level members
+  int32_t *rec;
+  unsigned long long rec_base;  // words written by earlier levels
   double r_crit;          // Fisher-z decision boundary of (n, alpha)
   int start;              // first global variable id
-  int p, n, npad, W, mc;
+  int p, n, npad, W;
   int level;              // level of this phase
   int flags;
 };
@@ -115,13 +120,14 @@ struct InstHost {
   bool used = false;
   int p = 0, n = 0, npad = 0, W = 0, mc = 1, max_level = 0;
   double alpha = 0.0, r_crit = 0.0;
-  // device blocks
+  // device blocks (from the engine's arena)
   double *x = nullptr, *z = nullptr, *mean = nullptr, *sd = nullptr, *cor = nullptr;
   uint8_t *nanm = nullptr;
   unsigned long long *adj = nullptr;
-  int8_t *rmlev = nullptr;
-  int32_t *sep = nullptr;
+  int32_t *rec = nullptr;
+  size_t rec_cap = 0, rec_used = 0;  // sepset record words
   bool have_x = false, have_cor = false, x_has_nan = false;
+  bool drop_obs = false;   // free observations and z-scores once correlated
   // level state
   int start = 0;           // global id of local variable 0 in the last pack
   int next_level = 0;      // level the next step runs
@@ -135,10 +141,27 @@ struct InstHost {
   std::vector<hpc_level_stats> hist;  // one record per level run
 };
 
+// Device memory of instance blocks: slabs from hipMalloc, sub-allocated
+// first fit with coalescing frees, returned to HIP only when the engine is
+// freed.  Frees are stream-ordered safe because all device work that reads a
+// block is on the engine's stream, ahead of any later reuse.
+struct Arena {
+  struct Slab {
+    char *base = nullptr;
+    size_t size = 0;
+    std::map<size_t, size_t> free;  // offset -> length
+  };
+  std::vector<Slab> slabs;
+  std::map<const void *, std::pair<int, size_t>> live;  // block -> (slab, length)
+  size_t slab_min = 0;  // smallest slab (0: exactly the request)
+  size_t reserved = 0, used = 0;
+};
+
 // A set of instances on one device with their shared per-level arrays and
 // one stream.
 struct Engine {
   int device = 0;
+  Arena arena;
   hipStream_t stream = nullptr;
   hipStream_t copy_stream = nullptr;
   std::vector<InstHost> inst;  // slot per instance id
@@ -181,7 +204,9 @@ void engine_free(Engine &e);
 // New instance with device blocks and the initial adjacency; returns its id.
 int engine_add(Engine &e, int p, int n, double alpha, int max_level, int *id);
 void engine_remove(Engine &e, int id);
-int engine_set_observations(Engine &e, int id, const double *h_x);
+// Uploads [p][n] observations; nan_hint 0/1 says whether they hold NaN
+// cells (-1: scan them here).
+int engine_set_observations(Engine &e, int id, const double *h_x, int nan_hint = -1);
 int engine_set_corr(Engine &e, int id, const double *h_cor);
 int engine_reset(Engine &e, int id);
 // Instance table of `ids` with the given per-instance levels; every phase
@@ -196,7 +221,8 @@ int engine_search(Engine &e, int level, const int32_t *rows, int nrows, float *m
 int engine_level0(Engine &e, float *ms);
 int engine_commit(Engine &e, float *ms);
 int engine_read_inst_counters(Engine &e);
-int engine_grow_sep(Engine &e, int id, int mc);
+// Sepset width of an instance's exports covers `level`.
+void engine_note_level(Engine &e, int id, int level);
 // One level for every id (correlation first for those without one): level 0,
 // compaction, the searches of every level present and the commit, with
 // three host synchronisations for the whole set.  Finished instances are
@@ -208,6 +234,16 @@ size_t engine_footprint(int p, int n, int max_level);
 int export_cor(Engine &e, int id, double *cor);
 int export_adj(Engine &e, int id, int32_t *adj);
 int export_sepsets(Engine &e, int id, int32_t *sep, int mc);
+int export_sepset_list(Engine &e, int id, int32_t *out, int64_t cap, int64_t *words);
+// Host copies of an instance's bit adjacency ([p][W] words) and of its sepset
+// records sorted by (i, j).
+int fetch_adj_bits(Engine &e, int id, std::vector<unsigned long long> &bits);
+int fetch_records(Engine &e, int id, std::vector<int32_t> &rec);
+// Reference layouts from those host copies (sepSets: -1 empty, -2 removed
+// at level 0, cpuUtil.cpp:3-48).
+void dense_adj(const std::vector<unsigned long long> &bits, int p, int32_t *adj);
+void dense_sepsets(const std::vector<unsigned long long> &bits, const std::vector<int32_t> &rec,
+                   int p, int32_t *sep, int mc);
 
 }  // namespace hpc
 

@@ -224,24 +224,27 @@ __device__ __forceinline__ int find_pos(const int32_t *row, int len, int node) {
   return lo;
 }
 
-// Removal and sepset recording (cpuUtil.cpp:23-48).  An instance keeps sepset
-// member m of pair (min, max) at sep[m*p*p + min*p + max] (one plane per
-// member, so the width can grow by appending planes) and the removal level
-// at rmlev[min*p + max]; the host export rebuilds the reference layout.
-// Instances at level 0 were handled by the level-0 kernel.
+// Removal and sepset recording (cpuUtil.cpp:23-48).  Each removed edge at
+// level l appends a record of 3 + l words to its instance's record buffer:
+// i, j (local ids, i < j), l, then the members in ascending order.  A
+// wavefront reserves the space of all its records with one atomic when it
+// holds one instance; the record order is scheduling-dependent, the records
+// are not.
 __global__ __launch_bounds__(256) void commit_kernel(Layout L, long long n_edges) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  int k = -1;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int k = -1, level = 0;
   unsigned long long T = 0, U = 0, E = 0, R = 0;
+  bool rm = false;
+  int i = 0, j = 0, y = 0, base = 0, nn = 0;
+  unsigned long long rank = 0;
   if (e < n_edges) {
-    const int i = L.erow[e];
+    i = L.erow[e];
     k = L.row_inst[i];
-    const Inst &In = L.inst[k];
-    const int level = In.level;
-    // levels the binomial table and the sepset planes cover (the host packs
-    // level <= mc <= kMaxLevel)
-    if (level >= 1 && level <= kMaxLevel && level <= In.mc) {
-      const int j = L.nbr[e];
+    level = L.inst[k].level;
+    // levels the binomial table covers (the host never packs a higher one)
+    if (level >= 1 && level <= kMaxLevel) {
+      j = L.nbr[e];
       const int bi = L.rowptr[i];
       const int nni = L.rowptr[i + 1] - bi - 1;
       const unsigned long long total = binom(L.binom, kBinomK, nni, level);
@@ -258,45 +261,66 @@ __global__ __launch_bounds__(256) void commit_kernel(Layout L, long long n_edges
         const int yj = find_pos(L.nbr + bj, lenj, i);
         // the adjacency is symmetric, so i is in N(j); guard anyway
         const long long fj = (yj < lenj && L.nbr[bj + yj] == i) ? L.fp[bj + yj] : -1;
-        int row = -1, y = 0, base = 0;
-        unsigned long long rank = 0;
+        int row = -1;
         if (fi >= 0) { row = i; y = yi; base = bi; rank = (unsigned long long)fi; }
         else if (fj >= 0) { row = j; y = yj; base = bj; rank = (unsigned long long)fj; }
-        const int nn = row >= 0 ? L.rowptr[row + 1] - base - 1 : 0;
-        // a rank is valid for this level and row (the unranking below ends)
-        if (row >= 0 && rank < binom(L.binom, kBinomK, nn, level)) {
-          const int p = In.p, W = In.W, s0 = In.start;
-          const int li = i - s0, lj = j - s0;
-          atomicAnd(&In.adj[(size_t)li * W + (lj >> 6)], ~(1ull << (lj & 63)));
-          atomicAnd(&In.adj[(size_t)lj * W + (li >> 6)], ~(1ull << (li & 63)));
-          In.rmlev[(size_t)li * p + lj] = (int8_t)level;
-          const size_t plane = (size_t)p * p;
-          int32_t *out = In.sep + (size_t)li * p + lj;
-          int val = 0;
-          for (int m = 0; m < level; ++m) {
-            for (;;) {
-              unsigned long long c = binom(L.binom, kBinomK, nn - 1 - val, level - 1 - m);
-              if (rank < c) break;
-              rank -= c;
-              ++val;
-            }
-            const int q = val++;
-            out[(size_t)m * plane] = L.nbr[base + (q < y ? q : q + 1)] - s0;
-          }
-          R = 1;
-        }
+        nn = row >= 0 ? L.rowptr[row + 1] - base - 1 : 0;
+        // a rank valid for this level and row (so the unranking below ends)
+        rm = row >= 0 && rank < binom(L.binom, kBinomK, nn, level);
       }
     }
+  }
+  // the wave's instance, if it holds only one
+  int kk = k;
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) kk = max(kk, __shfl_xor(kk, s, 64));
+  const bool uniform = __all(k == kk || k < 0);
+  // record space
+  const unsigned long long need = rm ? (unsigned long long)(3 + level) : 0ull;
+  unsigned long long off = 0;
+  if (uniform) {
+    unsigned long long incl = need;
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+      const unsigned long long v = __shfl_up(incl, s, 64);
+      if (lane >= s) incl += v;
+    }
+    const unsigned long long tot = __shfl(incl, 63, 64);
+    unsigned long long wbase = 0;
+    if (lane == 63 && tot) wbase = atomicAdd(&L.ictr[kk].rec_words, tot);
+    wbase = __shfl(wbase, 63, 64);
+    off = wbase + incl - need;
+  } else if (rm) {
+    off = atomicAdd(&L.ictr[k].rec_words, need);
+  }
+  if (rm) {
+    const Inst &In = L.inst[k];
+    const int W = In.W, s0 = In.start;
+    const int li = i - s0, lj = j - s0;
+    atomicAnd(&In.adj[(size_t)li * W + (lj >> 6)], ~(1ull << (lj & 63)));
+    atomicAnd(&In.adj[(size_t)lj * W + (li >> 6)], ~(1ull << (li & 63)));
+    int32_t *out = In.rec + In.rec_base + off;
+    out[0] = li;
+    out[1] = lj;
+    out[2] = level;
+    int val = 0;
+    for (int m = 0; m < level; ++m) {
+      for (;;) {
+        const unsigned long long c = binom(L.binom, kBinomK, nn - 1 - val, level - 1 - m);
+        if (rank < c) break;
+        rank -= c;
+        ++val;
+      }
+      const int q = val++;
+      out[3 + m] = L.nbr[base + (q < y ? q : q + 1)] - s0;
+    }
+    R = 1;
   }
   // per-instance counts: reduced per wave when the wave holds one instance,
   // then merged across the workgroup, so a one-instance launch issues one
   // atomic per counter and workgroup
   __shared__ int sk[4];
   __shared__ unsigned long long sT[4], sU[4], sE[4], sR[4];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  int kk = k;
-  for (int s = 32; s > 0; s >>= 1) kk = max(kk, __shfl_xor(kk, s, 64));  // wave's instance, if uniform
-  const bool uniform = __all(k == kk || k < 0);
   if (uniform) {
 #pragma unroll
     for (int s = 32; s > 0; s >>= 1) {
@@ -321,7 +345,7 @@ __global__ __launch_bounds__(256) void commit_kernel(Layout L, long long n_edges
   __syncthreads();
   if (threadIdx.x == 0) {
     int cur = -1;
-    unsigned long long t = 0, u = 0, ed = 0, rm = 0;
+    unsigned long long t = 0, u = 0, ed = 0, rmv = 0;
     for (int w = 0; w <= 4; ++w) {
       const int kw = w < 4 ? sk[w] : -2;
       if (kw == -1) continue;
@@ -329,12 +353,12 @@ __global__ __launch_bounds__(256) void commit_kernel(Layout L, long long n_edges
         atomicAdd(&L.ictr[cur].tests, t);
         atomicAdd(&L.ictr[cur].tests_max, u);
         atomicAdd(&L.ictr[cur].edges, ed);
-        if (rm) atomicAdd(&L.ictr[cur].removed, rm);
-        t = u = ed = rm = 0;
+        if (rmv) atomicAdd(&L.ictr[cur].removed, rmv);
+        t = u = ed = rmv = 0;
       }
       if (kw < 0) break;
       cur = kw;
-      t += sT[w]; u += sU[w]; ed += sE[w]; rm += sR[w];
+      t += sT[w]; u += sU[w]; ed += sE[w]; rmv += sR[w];
     }
   }
 }

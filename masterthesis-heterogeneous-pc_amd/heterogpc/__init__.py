@@ -5,4 +5,4 @@ the benchmark.  The C++ command-line tool `heterogpc` drives the same ABI.
 """
 from ._lib import HpcError, LevelStats, load  # noqa: F401
 from .batch import Batch, Server, footprint  # noqa: F401
-from .skeleton import DeviceSkeleton, r_crit, sem_generate  # noqa: F401
+from .skeleton import DeviceSkeleton, r_crit, sem_generate, sepset_records  # noqa: F401

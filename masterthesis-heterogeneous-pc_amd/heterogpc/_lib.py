@@ -96,6 +96,7 @@ SIGNATURES = {
     "hpc_get_adj": (_i, [_vp, _vp]),
     "hpc_get_compact": (_i, [_vp, _vp]),
     "hpc_get_sepsets": (_i, [_vp, _vp, _i]),
+    "hpc_get_sepset_list": (_i, [_vp, _vp, _i64, C.POINTER(_i64)]),
     "hpc_get_first_pass": (_i, [_vp, _vp, _vp, C.POINTER(_i64)]),
     "hpc_footprint": (_i, [_u64, _i, _i, C.POINTER(_u64)]),
     "hpc_batch_create": (_i, [_i, C.POINTER(_vp)]),
@@ -111,12 +112,14 @@ SIGNATURES = {
     "hpc_batch_get_cor": (_i, [_vp, _i, _vp]),
     "hpc_batch_get_adj": (_i, [_vp, _i, _vp]),
     "hpc_batch_get_sepsets": (_i, [_vp, _i, _vp, _i]),
+    "hpc_batch_get_sepset_list": (_i, [_vp, _i, _vp, _i64, C.POINTER(_i64)]),
     "hpc_batch_set_tuning": (_i, [_vp, _i, _i]),
     "hpc_server_create": (_i, [_i, _u64, _i, C.POINTER(_vp)]),
     "hpc_server_destroy": (_i, [_vp]),
     "hpc_server_submit": (_i, [_vp, _u64, _i, _d, _i, _vp, C.POINTER(_i64)]),
     "hpc_server_wait": (_i, [_vp, _i64, _d, C.POINTER(C.c_int)]),
     "hpc_server_result": (_i, [_vp, _i64, _vp, _vp, _i, C.POINTER(RequestInfo)]),
+    "hpc_server_sepset_list": (_i, [_vp, _i64, _vp, _i64, C.POINTER(_i64)]),
     "hpc_server_release": (_i, [_vp, _i64]),
     "hpc_server_stats": (_i, [_vp, C.POINTER(ServerInfo)]),
 }

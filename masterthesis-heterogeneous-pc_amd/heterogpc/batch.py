@@ -7,6 +7,7 @@ import ctypes as C
 import numpy as np
 
 from ._lib import LevelStats, RequestInfo, ServerInfo, check, load
+from .skeleton import _fetch_list
 
 
 def _ptr(a: np.ndarray) -> int:
@@ -104,6 +105,10 @@ class Batch:
         check(self.lib.hpc_batch_get_sepsets(self.h, int(iid), _ptr(out), mc))
         return out
 
+    def sepset_list(self, iid: int) -> np.ndarray:
+        return _fetch_list(lambda out, cap, w: self.lib.hpc_batch_get_sepset_list(
+            self.h, int(iid), out, cap, w))
+
     def set_ranks_per_task(self, value: int) -> None:
         check(self.lib.hpc_batch_set_tuning(self.h, 1, int(value)))
 
@@ -150,16 +155,28 @@ class Server:
         check(self.lib.hpc_server_wait(self.h, int(req), float(timeout_s), C.byref(done)))
         return bool(done.value)
 
-    def result(self, req: int, mc: int | None = None, release: bool = True):
-        """(adj [p][p], sepsets [p][p][mc], info dict) of a finished request."""
+    def sepset_list(self, req: int) -> np.ndarray:
+        return _fetch_list(lambda out, cap, w: self.lib.hpc_server_sepset_list(
+            self.h, int(req), out, cap, w))
+
+    def result(self, req: int, mc: int | None = None, release: bool = True,
+               dense_sepsets: bool = True):
+        """(adj [p][p], sepsets, info dict) of a finished request: sepsets in
+        the reference layout [p][p][mc], or the compact list when
+        dense_sepsets is False."""
         p = self.shape[req]
         info = RequestInfo()
         check(self.lib.hpc_server_result(self.h, int(req), None, None, 0, C.byref(info)))
         mc = info.max_cond_size if mc is None else int(mc)
         adj = np.empty((p, p), dtype=np.int32)
-        sep = np.empty((p, p, mc), dtype=np.int32)
-        check(self.lib.hpc_server_result(self.h, int(req), _ptr(adj), _ptr(sep), mc,
-                                         C.byref(info)))
+        if dense_sepsets:
+            sep = np.empty((p, p, mc), dtype=np.int32)
+            check(self.lib.hpc_server_result(self.h, int(req), _ptr(adj), _ptr(sep), mc,
+                                             C.byref(info)))
+        else:
+            check(self.lib.hpc_server_result(self.h, int(req), _ptr(adj), None, 0,
+                                             C.byref(info)))
+            sep = self.sepset_list(req)
         if release:
             check(self.lib.hpc_server_release(self.h, int(req)))
             self.shape.pop(req, None)

@@ -12,6 +12,24 @@ def _ptr(a: np.ndarray) -> int:
     return a.ctypes.data
 
 
+def _fetch_list(call) -> np.ndarray:
+    words = C.c_int64()
+    check(call(None, 0, C.byref(words)))
+    out = np.empty(max(words.value, 1), dtype=np.int32)
+    check(call(_ptr(out), words.value, C.byref(words)))
+    return out[: words.value]
+
+
+def sepset_records(lst: np.ndarray) -> dict:
+    """{(i, j): (level, members)} of a compact sepset list."""
+    out, a = {}, 0
+    while a < len(lst):
+        i, j, lv = int(lst[a]), int(lst[a + 1]), int(lst[a + 2])
+        out[(i, j)] = (lv, [int(v) for v in lst[a + 3: a + 3 + lv]])
+        a += 3 + lv
+    return out
+
+
 def r_crit(n: int, alpha: float) -> float:
     """Largest |r| whose Fisher-z p-value (cpuInd.cpp:8-14) is >= alpha."""
     return load().hpc_fisher_r_crit(int(n), float(alpha))
@@ -162,6 +180,11 @@ class DeviceSkeleton:
         out = np.empty((self.p, self.p, mc), dtype=np.int32)
         check(self.lib.hpc_get_sepsets(self.h, _ptr(out), mc))
         return out
+
+    def sepset_list(self) -> np.ndarray:
+        """Compact sepsets: per edge removed at level l >= 1, the words
+        i, j, l, members..., sorted by (i, j)."""
+        return _fetch_list(lambda out, cap, w: self.lib.hpc_get_sepset_list(self.h, out, cap, w))
 
     def first_pass(self):
         ne = C.c_int64()

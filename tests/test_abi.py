@@ -75,10 +75,11 @@ def test_errors_are_reported_not_fatal(hpc):
 
 def test_footprint_is_host_only_and_monotone(hpc):
     b = hpc.footprint(2000, 5000, 4)
-    # correlation + sepset planes + observations are a lower bound
-    assert b >= 2000 * 2000 * (8 + 4 * 4) + 2000 * 5000 * 8
+    # correlation + level-1 sepset records + observations are a lower bound
+    assert b >= 2000 * 2000 * (8 + 8) + 2000 * 5000 * 8
     assert hpc.footprint(2001, 5000, 4) > b
     assert hpc.footprint(2000, 5001, 4) > b
-    assert hpc.footprint(2000, 5000, 8) > b
+    # sepsets are records of removed edges: no per-level planes
+    assert hpc.footprint(2000, 5000, 8) == b
     with pytest.raises(hpc.HpcError):
         hpc.footprint(1, 10, 4)

@@ -61,7 +61,9 @@ def _check_instance(b, iid, label, orc, x, alpha, ml, single=None):
     assert np.array_equal(b.cor(iid), cor, equal_nan=True), f"{label}: correlation differs"
     assert np.array_equal(b.adj(iid), ref.adj()), f"{label}: adjacency differs"
     mc = ref.mc()
-    assert np.array_equal(b.sepsets(iid, mc), ref.sepsets()), f"{label}: sepsets differ"
+    so = ref.sepsets()
+    assert np.array_equal(b.sepsets(iid, mc), so), f"{label}: sepsets differ"
+    _check_list(b.sepset_list(iid), so, label)
     finished, levels, err = b.status(iid)
     assert finished and err == 0, (label, finished, err)
     st = b.stats(iid)
@@ -77,6 +79,25 @@ def _check_instance(b, iid, label, orc, x, alpha, ml, single=None):
         assert np.array_equal(b.adj(iid), s_adj), f"{label}: differs from the single run"
         assert np.array_equal(b.sepsets(iid, s_sep.shape[2]), s_sep)
         assert [d["tests"] for d in st] == [d["tests"] for d in s_stats]
+
+
+def _check_list(lst, dense, label):
+    """The compact sepset list holds exactly the dense layout's records at
+    levels >= 1, sorted by (i, j)."""
+    import heterogpc
+
+    rec = heterogpc.sepset_records(lst)
+    keys = list(rec)
+    assert keys == sorted(keys), f"{label}: sepset list not sorted"
+    want = {}
+    p = dense.shape[0]
+    for i in range(p):
+        for j in range(i + 1, p):
+            s = dense[i, j]
+            if s[0] >= 0:
+                mem = [int(v) for v in s if v >= 0]
+                want[(i, j)] = (len(mem), mem)
+    assert rec == want, f"{label}: sepset list differs from the dense sepsets"
 
 
 def _single(hpc, x, alpha, ml):
@@ -145,7 +166,8 @@ def _serve(hpc, inputs, budget=0, max_active=0, one_at_a_time=False):
             reqs.append((label, r))
         for label, r in reqs:
             assert s.wait(r, 120.0), f"{label} did not finish"
-            out[label] = s.result(r)
+            lst = s.sepset_list(r)
+            out[label] = s.result(r, mc=16) + (lst,)
         stats = s.stats()
     return out, stats
 
@@ -162,10 +184,13 @@ def test_server_result_same_alone_and_in_mix(hpc, orc):
     for label, x, a, ml in inputs:
         cor, ref, recs = _oracle(orc, x, a, ml)
         mc = ref.mc()
+        so = ref.sepsets()
         for mode, res in (("alone", alone), ("mixed", mixed), ("tight", tight)):
-            adj, sep, info = res[label]
+            adj, sep, info, lst = res[label]
             assert np.array_equal(adj, ref.adj()), f"{label} ({mode}): adjacency differs"
-            assert np.array_equal(sep[:, :, :mc], ref.sepsets()), f"{label} ({mode}): sepsets"
+            assert np.array_equal(sep[:, :, :mc], so), f"{label} ({mode}): sepsets"
+            assert (sep[:, :, mc:] == -1).all()
+            _check_list(lst, so, f"{label} ({mode})")
             assert info["tests"] == sum(o["tests"] for o in recs), (label, mode)
             assert info["latency_ms"] >= info["queue_ms"] >= 0.0
 

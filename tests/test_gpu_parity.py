@@ -52,6 +52,10 @@ def _compare_run(hpc, orc, x, n, alpha, max_level, cor=None, label=""):
     sd = dev.sepsets(mc)
     so = ref.sepsets()
     assert np.array_equal(sd, so), "sepsets differ"
+    # the compact list carries the same records
+    from test_gpu_batch import _check_list
+
+    _check_list(dev.sepset_list(), so, label)
     # every recorded sepset is one the reference's test accepts
     adj = ref.adj()
     checked = 0
