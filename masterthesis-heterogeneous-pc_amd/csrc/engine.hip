@@ -575,7 +575,8 @@ int engine_search(Engine &e, int level, const int32_t *rows, int nrows, float *m
         if (rc) return rc;
         launch_task_fill(e.stream, L, e.d_row_tasks, e.d_task_off, e.d_tasks);
         HPC_HIPCHK(hipGetLastError());
-        launch_search_ln(e.stream, level, L, e.d_tasks, (long long)n_tasks, rpt);
+        launch_search_ln(e.stream, level, L, e.d_tasks, (long long)n_tasks, rpt, e.d_row_tasks,
+                         e.d_task_off);
       }
     }
   }
@@ -755,7 +756,8 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
       rc = grow(e.d_tasks, e.cap_tasks, n_tasks);
       if (rc) return rc;
       launch_task_fill(e.stream, L, e.d_row_tasks, e.d_task_off, e.d_tasks);
-      launch_search_ln(e.stream, l, L, e.d_tasks, (long long)n_tasks, e.ranks_per_task);
+      launch_search_ln(e.stream, l, L, e.d_tasks, (long long)n_tasks, e.ranks_per_task,
+                       e.d_row_tasks, e.d_task_off);
     }
     HPC_HIPCHK(hipGetLastError());
   }

@@ -38,14 +38,23 @@ __global__ void adj_init_kernel(unsigned long long *adj, int p, int W) {
 // adjacent pair j < i is tested on cor[i][j] (the lower triangle, as the
 // reference reads it) and removed iff p >= alpha.
 //
-// One wavefront per 64 x 64 tile (I, J) with I >= J of an instance's lower
-// block triangle.  Iteration r reads row i = 64I + r, columns 64J..64J+63
-// (one coalesced 512 B load); the ballot of surviving pairs is row i's word
-// J, and lane c collects bit r of row (64J + c)'s word I.  Each adjacency
-// word has exactly one owner, so no atomics are needed.
-__global__ __launch_bounds__(256) void level0_kernel(Layout L, const long long *__restrict__ prefix) {
-  const int lane = threadIdx.x & 63;
-  const long long t = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+// One workgroup of kL0Waves wavefronts per 64 x 64 tile (I, J), I >= J, of an
+// instance's lower block triangle; wavefront w takes rows 64I + r for r in
+// [w * kL0Rows, (w + 1) * kL0Rows).  Iteration r reads row i = 64I + r,
+// columns 64J..64J+63 (one coalesced 512 B load); the ballot of surviving
+// pairs is row i's word J, and lane c collects bit r of row (64J + c)'s word
+// I.  The column words of the wavefronts are OR-ed in LDS; each adjacency
+// word has exactly one writer, so no atomics are needed.
+constexpr int kL0Waves = 8;
+constexpr int kL0Rows = 64 / kL0Waves;
+
+__global__ __launch_bounds__(64 * kL0Waves) void level0_kernel(Layout L,
+                                                              const long long *__restrict__ prefix) {
+  __shared__ unsigned long long s_row[64];
+  __shared__ unsigned long long s_col[kL0Waves][64];
+  __shared__ unsigned long long s_cnt[kL0Waves][2];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long long t = blockIdx.x;
   if (t >= prefix[L.K]) return;
   const int k = find_inst(prefix, L.K, t);
   const Inst In = L.inst[k];
@@ -61,16 +70,17 @@ __global__ __launch_bounds__(256) void level0_kernel(Layout L, const long long *
   unsigned long long colword = 0ull;   // bits r: pair (64I + r, j) kept
   unsigned long long rowword = 0ull;   // captured by lane r: row 64I + r, word J
   unsigned long long tested = 0ull, removed = 0ull;
-  for (int r = 0; r < 64; ++r) {
+  const int r0 = wv * kL0Rows;
+#pragma unroll
+  for (int rr = 0; rr < kL0Rows; ++rr) {
+    const int r = r0 + rr;
     const int i = I * 64 + r;
-    if (i >= p) break;
-    const unsigned long long aw = In.adj[(size_t)i * W + J];
-    bool valid = (j < p) && (diag ? (lane < r) : true);
-    bool present = valid && ((aw >> lane) & 1ull);
-    bool keep = false;
-    if (present) {
-      double c = In.cor[(size_t)i * p + j];
-      keep = !removes(c, In.r_crit);
+    bool present = false, keep = false;
+    if (i < p) {
+      const unsigned long long aw = In.adj[(size_t)i * W + J];
+      const bool valid = (j < p) && (diag ? (lane < r) : true);
+      present = valid && ((aw >> lane) & 1ull);
+      if (present) keep = !removes(In.cor[(size_t)i * p + j], In.r_crit);
     }
     const unsigned long long kb = __ballot(keep);
     tested += __popcll(__ballot(present));
@@ -78,18 +88,32 @@ __global__ __launch_bounds__(256) void level0_kernel(Layout L, const long long *
     if (lane == r) rowword = kb;
     colword |= (unsigned long long)(keep ? 1ull : 0ull) << r;
   }
+  if (lane >= r0 && lane < r0 + kL0Rows) s_row[lane] = rowword;
+  s_col[wv][lane] = colword;
+  if (lane == 0) { s_cnt[wv][0] = tested; s_cnt[wv][1] = removed; }
+  __syncthreads();
+  if (wv != 0) return;
+  unsigned long long cw = 0ull;
+#pragma unroll
+  for (int w = 0; w < kL0Waves; ++w) cw |= s_col[w][lane];
+  const unsigned long long rw = s_row[lane];
   const int i_lane = I * 64 + lane;
   if (diag) {
-    if (i_lane < p) In.adj[(size_t)i_lane * W + J] = rowword | colword;
+    if (i_lane < p) In.adj[(size_t)i_lane * W + J] = rw | cw;
   } else {
-    if (i_lane < p) In.adj[(size_t)i_lane * W + J] = rowword;
-    if (j < p) In.adj[(size_t)j * W + I] = colword;
+    if (i_lane < p) In.adj[(size_t)i_lane * W + J] = rw;
+    if (j < p) In.adj[(size_t)j * W + I] = cw;
   }
-  if (lane == 0 && tested) {
-    atomicAdd(&L.ictr[k].tests, tested);
-    atomicAdd(&L.ictr[k].tests_max, tested);
-    atomicAdd(&L.ictr[k].edges, tested);
-    atomicAdd(&L.ictr[k].removed, removed);
+  if (lane == 0) {
+    unsigned long long te = 0, re = 0;
+#pragma unroll
+    for (int w = 0; w < kL0Waves; ++w) { te += s_cnt[w][0]; re += s_cnt[w][1]; }
+    if (te) {
+      atomicAdd(&L.ictr[k].tests, te);
+      atomicAdd(&L.ictr[k].tests_max, te);
+      atomicAdd(&L.ictr[k].edges, te);
+      atomicAdd(&L.ictr[k].removed, re);
+    }
   }
 }
 
@@ -217,9 +241,8 @@ void launch_level0(hipStream_t s, const Layout &L, const std::vector<Inst> &host
   if (h_prefix[K] == 0) return;
   (void)hipMemcpyAsync(d_prefix, h_prefix.data(), h_prefix.size() * sizeof(long long),
                        hipMemcpyHostToDevice, s);
-  const int wpb = 4;
-  hipLaunchKernelGGL(level0_kernel, dim3((unsigned)((h_prefix[K] + wpb - 1) / wpb)), dim3(64 * wpb),
-                     0, s, L, d_prefix);
+  hipLaunchKernelGGL(level0_kernel, dim3((unsigned)h_prefix[K]), dim3(64 * kL0Waves), 0, s, L,
+                     d_prefix);
 }
 
 void launch_compact(hipStream_t s, const Layout &L) {

@@ -105,8 +105,11 @@ void launch_task_plan(hipStream_t s, int level, const Layout &L, int rpt,
                       const int32_t *row_mask, long long *row_tasks, long long *task_off);
 void launch_task_fill(hipStream_t s, const Layout &L, const long long *row_tasks,
                       const long long *task_off, unsigned long long *tasks);
+// tasks sorted by row with per-row counts and offsets (row_tasks, task_off)
+// let the kernel stage each row's neighbourhood in LDS; NULL = gather form
 void launch_search_ln(hipStream_t s, int level, const Layout &L,
-                      const unsigned long long *tasks, long long n_tasks, int rpt);
+                      const unsigned long long *tasks, long long n_tasks, int rpt,
+                      const long long *row_tasks, const long long *task_off);
 void launch_commit(hipStream_t s, const Layout &L, int64_t n_edges);
 void launch_scatter_fp(hipStream_t s, const int64_t *idx, const int64_t *val,
                        long long count, int64_t *fp);

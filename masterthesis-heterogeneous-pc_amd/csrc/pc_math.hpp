@@ -94,53 +94,56 @@ __host__ __device__ __noinline__ double pcor_pinv(double (&A)[M][M]) {
 
 // Partial correlation of i and j given S (|S| = L) from a = r_ij,
 // x[c] = r_i,S[c], y[c] = r_j,S[c] and the strictly lower triangle
-// C[r][c] = r_S[c],S[r] (r > c).  With C = L L^T, u = L^-1 x, v = L^-1 y:
-// r = (a - u.v) / __builtin_sqrt((1 - u.u)(1 - v.v)), the same value as -P01/__builtin_sqrt(P00
-// P11) for P = inverse of the (L+2)-square submatrix [i, j, S]
-// (gpuRowTests.cu:143-170).  A pivot at or below 1e-12 or a numerically
-// singular 2x2 Schur block switches to the pinv form.
+// C[r][c] = r_S[c],S[r] (r > c).  With C = U D U^T (U unit lower, E = U D),
+// u = U^-1 x, v = U^-1 y and w = D^-1 u:
+// r = (a - w.v) / sqrt((1 - w.u)(1 - (D^-1 v).v)), the same value as
+// -P01/sqrt(P00 P11) for P = inverse of the (L+2)-square submatrix [i, j, S]
+// (gpuRowTests.cu:143-170).  One reciprocal per pivot and one square root per
+// test.  A pivot at or below 1e-12 or a numerically singular 2x2 Schur block
+// switches to the pinv form.
 template <int L>
 __host__ __device__ __forceinline__ double pcor_canon(double a, const double (&x)[L],
                                              const double (&y)[L],
                                              const double (&C)[L][L],
                                              bool &used_pinv) {
-  double Lm[L][L];
+  double Um[L][L], Em[L][L], inv[L];
   double u[L], v[L];
   used_pinv = false;
   bool sing = false;
 #pragma unroll
-  for (int c = 0; c < L; ++c) {
+  for (int r = 0; r < L; ++r) {
+#pragma unroll
+    for (int c = 0; c < r; ++c) {
+      double e = C[r][c];
+#pragma unroll
+      for (int k = 0; k < c; ++k) e = __builtin_fma(-Em[r][k], Um[c][k], e);
+      Em[r][c] = e;
+      Um[r][c] = e * inv[c];
+    }
     double d = 1.0;
 #pragma unroll
-    for (int k = 0; k < c; ++k) d = __builtin_fma(-Lm[c][k], Lm[c][k], d);
+    for (int k = 0; k < r; ++k) d = __builtin_fma(-Em[r][k], Um[r][k], d);
     sing |= !(d > kPivTol);
-    double s = __builtin_sqrt(d);
-    Lm[c][c] = s;
-#pragma unroll
-    for (int r = c + 1; r < L; ++r) {
-      double acc = C[r][c];
-#pragma unroll
-      for (int k = 0; k < c; ++k) acc = __builtin_fma(-Lm[r][k], Lm[c][k], acc);
-      Lm[r][c] = acc / s;
-    }
+    inv[r] = 1.0 / d;
   }
 #pragma unroll
   for (int c = 0; c < L; ++c) {
     double su = x[c], sv = y[c];
 #pragma unroll
     for (int k = 0; k < c; ++k) {
-      su = __builtin_fma(-Lm[c][k], u[k], su);
-      sv = __builtin_fma(-Lm[c][k], v[k], sv);
+      su = __builtin_fma(-Um[c][k], u[k], su);
+      sv = __builtin_fma(-Um[c][k], v[k], sv);
     }
-    u[c] = su / Lm[c][c];
-    v[c] = sv / Lm[c][c];
+    u[c] = su;
+    v[c] = sv;
   }
   double s00 = 1.0, s11 = 1.0, s01 = a;
 #pragma unroll
   for (int k = 0; k < L; ++k) {
-    s00 = __builtin_fma(-u[k], u[k], s00);
-    s11 = __builtin_fma(-v[k], v[k], s11);
-    s01 = __builtin_fma(-u[k], v[k], s01);
+    const double wu = u[k] * inv[k], wv = v[k] * inv[k];
+    s00 = __builtin_fma(-wu, u[k], s00);
+    s11 = __builtin_fma(-wv, v[k], s11);
+    s01 = __builtin_fma(-wu, v[k], s01);
   }
   double q = s00 * s11;
   sing |= !(s00 > kPivTol) || !(s11 > kPivTol) ||

@@ -69,6 +69,62 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_kernel(
   }
 }
 
+// Level 1 with row i's correlations staged in LDS: workgroup b takes the
+// ordered edges [b * epb, (b + 1) * epb) (CSR order, so grouped by row),
+// copies ids and cor[i][N(i)] of each row it meets into LDS and gathers only
+// cor[j][k] per test.  Same tests, order and arithmetic as search_l1_kernel.
+__global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_lds_kernel(
+    Layout L, long long n_edges, const int32_t *__restrict__ row_mask, int epb) {
+  extern __shared__ unsigned long long smem[];
+  double *Mi = reinterpret_cast<double *>(smem);                 // max_deg
+  int *ids = reinterpret_cast<int *>(Mi + L.max_deg);             // max_deg
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  long long e0 = (long long)blockIdx.x * epb;
+  const long long e1 = min(n_edges, e0 + epb);
+  while (e0 < e1) {
+    const int i = L.erow[e0];
+    const int base = L.rowptr[i];
+    const int deg = L.rowptr[i + 1] - base;
+    const long long seg_end = min(e1, (long long)base + deg);
+    const Inst &In = L.inst[L.row_inst[i]];
+    // uniform across the workgroup: the row is skipped or staged by all
+    if (In.level == 1 && (!row_mask || row_mask[i]) && deg >= 2) {
+      const int p = In.p, s0 = In.start, li = i - s0;
+      __syncthreads();  // the previous row is done with the staged values
+      for (int a = tid; a < deg; a += blockDim.x) {
+        const int lk = L.nbr[base + a] - s0;
+        ids[a] = lk;
+        Mi[a] = In.cor[(size_t)li * p + lk];
+      }
+      __syncthreads();
+      const double r_crit = In.r_crit;
+      const int nn = deg - 1;
+      for (long long e = e0 + wave; e < seg_end; e += kWavesPerBlock) {
+        const int y = (int)(e - base);
+        const int lj = ids[y];
+        const double x1 = Mi[y];
+        const double *cj = In.cor + (size_t)lj * p;
+        long long found = -1;
+        for (int q0 = 0; q0 < nn; q0 += 64) {
+          const int q = q0 + lane;
+          bool pass = false;
+          if (q < nn) {
+            const int pos = q < y ? q : q + 1;
+            const double x2 = Mi[pos];
+            const double x3 = cj[ids[pos]];
+            if (!(__builtin_isnan(x1) || __builtin_isnan(x2) || __builtin_isnan(x3)))
+              pass = removes(pcor_l1(x1, x2, x3), r_crit);
+          }
+          const unsigned long long m = __ballot(pass);
+          if (m) { found = q0 + __ffsll((long long)m) - 1; break; }
+        }
+        if (lane == 0) L.fp[e] = found;
+      }
+    }
+    e0 = seg_end;
+  }
+}
+
 // Level L >= 2 (testRowLN<>, gpuRowTests.cu:102-205; the general-level form of
 // cpuWorkstealingTests.cpp:39-112).  tasks[t] = (ordered edge << 32) | chunk;
 // chunk c covers ranks [c * rpt, (c + 1) * rpt).
@@ -149,6 +205,107 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_kernel(
         break;
       }
     }
+  }
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) F += __shfl_xor(F, s, 64);
+  if (lane == 0 && F) atomicAdd(&L.ctr->pinv_fallbacks, F);
+}
+
+// Level L >= 2 with the row's neighbourhood staged in LDS.  Every operand of
+// a test on ordered edge (i, j) lies in cor[{i} u N(i), {i} u N(i)] (j and S
+// are neighbours of i), so a workgroup copies that (deg + 1)^2 block of the
+// correlation matrix into LDS once -- position a < deg is neighbour N(i)[a],
+// position deg is i -- and runs the row's tasks against it.  Workgroup b
+// takes tasks [b * tpb, (b + 1) * tpb) of the row-sorted task list, one
+// staging per row it meets.  Same ranks, same arithmetic and the same
+// orientation of every correlation read as search_ln_kernel.
+template <int LV>
+__global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
+    Layout L, const unsigned long long *__restrict__ tasks, long long n_tasks, int rpt,
+    const long long *__restrict__ task_off, const long long *__restrict__ row_tasks, int tpb,
+    int dmax) {
+  extern __shared__ unsigned long long smem[];
+  unsigned long long *bt = smem;                                   // (max_deg+1) x (LV+1)
+  const int brows = L.max_deg + 1;
+  double *M = reinterpret_cast<double *>(smem + (size_t)brows * (LV + 1));  // dmax x dmax
+  int *ids = reinterpret_cast<int *>(M + (size_t)dmax * dmax);     // dmax local ids
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int t = tid; t < brows * (LV + 1); t += blockDim.x) {
+    const int m = t / (LV + 1), k = t % (LV + 1);
+    bt[t] = L.binom[(size_t)m * kBinomK + k];
+  }
+  unsigned long long F = 0;
+  unsigned long long *fpu = reinterpret_cast<unsigned long long *>(L.fp);
+  long long t0 = (long long)blockIdx.x * tpb;
+  const long long t1 = min(n_tasks, t0 + tpb);
+  while (t0 < t1) {
+    // the segment of this block's tasks on row i (uniform across the block)
+    const int i = L.erow[(long long)(tasks[t0] >> 32)];
+    const long long seg_end = min(t1, task_off[i] + row_tasks[i]);
+    const Inst &In = L.inst[L.row_inst[i]];
+    const int p = In.p, s0 = In.start;
+    const int base = L.rowptr[i];
+    const int deg = L.rowptr[i + 1] - base;
+    const int D = deg + 1;
+    __syncthreads();  // the previous segment is done with M (and bt is staged)
+    for (int a = tid; a < D; a += blockDim.x) ids[a] = (a < deg ? L.nbr[base + a] : i) - s0;
+    __syncthreads();
+    for (int idx = tid; idx < D * D; idx += blockDim.x) {
+      const int a = idx / D, b = idx - a * D;
+      M[idx] = In.cor[(size_t)ids[a] * p + ids[b]];
+    }
+    __syncthreads();
+    const double r_crit = In.r_crit;
+    const int nn = deg - 1;
+    const unsigned long long total = binom(bt, LV + 1, nn, LV);
+    const double *Mi = M + (size_t)deg * D;  // row i
+    for (long long t = t0 + wave; t < seg_end; t += kWavesPerBlock) {
+      const unsigned long long task = tasks[t];
+      const long long e = (long long)(task >> 32);
+      const unsigned long long r_begin = (task & 0xffffffffull) * (unsigned long long)rpt;
+      if (__atomic_load_n(&fpu[e], __ATOMIC_RELAXED) < r_begin) continue;
+      const int y = (int)(e - base);
+      const unsigned long long r_end = min(total, r_begin + (unsigned long long)rpt);
+      const double a = Mi[y];
+      const double *Mj = M + (size_t)y * D;  // row j
+      for (unsigned long long r0 = r_begin; r0 < r_end; r0 += 64) {
+        const unsigned long long rank = r0 + lane;
+        bool pass = false;
+        if (rank < r_end) {
+          int q[LV], pos[LV];
+          unrank<LV>(bt, LV + 1, rank, nn, q);
+#pragma unroll
+          for (int c = 0; c < LV; ++c) pos[c] = q[c] < y ? q[c] : q[c] + 1;
+          double x[LV], yv[LV], C[LV][LV];
+          bool anynan = __builtin_isnan(a);
+#pragma unroll
+          for (int c = 0; c < LV; ++c) {
+            x[c] = Mi[pos[c]];
+            yv[c] = Mj[pos[c]];
+            anynan |= __builtin_isnan(x[c]) || __builtin_isnan(yv[c]);
+          }
+#pragma unroll
+          for (int c = 0; c < LV; ++c)
+#pragma unroll
+            for (int r = c + 1; r < LV; ++r) {
+              C[r][c] = M[(size_t)pos[c] * D + pos[r]];
+              anynan |= __builtin_isnan(C[r][c]);
+            }
+          if (!anynan) {
+            bool fb;
+            const double r = pcor_canon<LV>(a, x, yv, C, fb);
+            pass = removes(r, r_crit);
+            F += fb ? 1 : 0;
+          }
+        }
+        const unsigned long long m = __ballot(pass);
+        if (m) {
+          if (lane == 0) atomicMin(&fpu[e], r0 + (unsigned long long)(__ffsll((long long)m) - 1));
+          break;
+        }
+      }
+    }
+    t0 = seg_end;
   }
 #pragma unroll
   for (int s = 32; s > 0; s >>= 1) F += __shfl_xor(F, s, 64);
@@ -369,9 +526,27 @@ __global__ void scatter_kernel(const int64_t *__restrict__ idx, const int64_t *_
   if (t < count) fp[idx[t]] = val[t];
 }
 
+// LDS bytes of search_ln_lds_kernel: binomials, the neighbourhood block and
+// its ids.
+size_t lds_bytes(int max_deg, int lv) {
+  const size_t d = (size_t)max_deg + 1;
+  return d * (lv + 1) * 8 + d * d * 8 + ((d * 4 + 7) & ~(size_t)7);
+}
+
+constexpr size_t kLdsLimit = 64 * 1024;
+constexpr int kTasksPerBlock = 16;
+
 template <int LV>
 void launch_ln(hipStream_t s, const Layout &L, const unsigned long long *tasks,
-               long long n_tasks, int rpt) {
+               long long n_tasks, int rpt, const long long *row_tasks, const long long *task_off) {
+  const size_t staged = lds_bytes(L.max_deg, LV);
+  if (row_tasks && task_off && staged <= kLdsLimit) {
+    const long long blocks = (n_tasks + kTasksPerBlock - 1) / kTasksPerBlock;
+    hipLaunchKernelGGL(search_ln_lds_kernel<LV>, dim3((unsigned)blocks),
+                       dim3(64 * kWavesPerBlock), staged, s, L, tasks, n_tasks, rpt, task_off,
+                       row_tasks, kTasksPerBlock, L.max_deg + 1);
+    return;
+  }
   long long blocks = (n_tasks + kWavesPerBlock - 1) / kWavesPerBlock;
   if (blocks > 8192) blocks = 8192;
   const size_t lds = (size_t)(L.max_deg + 1) * (LV + 1) * sizeof(unsigned long long);
@@ -384,6 +559,14 @@ void launch_ln(hipStream_t s, const Layout &L, const unsigned long long *tasks,
 
 void launch_search_l1(hipStream_t s, const Layout &L, int64_t n_edges, const int32_t *row_mask) {
   if (n_edges <= 0) return;
+  const size_t staged = (size_t)L.max_deg * 12;
+  if (staged <= kLdsLimit) {
+    const int epb = 32;
+    const long long blocks = (n_edges + epb - 1) / epb;
+    hipLaunchKernelGGL(search_l1_lds_kernel, dim3((unsigned)blocks), dim3(64 * kWavesPerBlock),
+                       staged, s, L, (long long)n_edges, row_mask, epb);
+    return;
+  }
   long long blocks = (n_edges + kWavesPerBlock - 1) / kWavesPerBlock;
   if (blocks > 16384) blocks = 16384;
   hipLaunchKernelGGL(search_l1_kernel, dim3((unsigned)blocks), dim3(64 * kWavesPerBlock), 0, s,
@@ -405,24 +588,25 @@ void launch_task_fill(hipStream_t s, const Layout &L, const long long *row_tasks
 }
 
 void launch_search_ln(hipStream_t s, int level, const Layout &L,
-                      const unsigned long long *tasks, long long n_tasks, int rpt) {
+                      const unsigned long long *tasks, long long n_tasks, int rpt,
+                      const long long *row_tasks, const long long *task_off) {
   if (n_tasks <= 0) return;
   switch (level) {
-    case 2: launch_ln<2>(s, L, tasks, n_tasks, rpt); break;
-    case 3: launch_ln<3>(s, L, tasks, n_tasks, rpt); break;
-    case 4: launch_ln<4>(s, L, tasks, n_tasks, rpt); break;
-    case 5: launch_ln<5>(s, L, tasks, n_tasks, rpt); break;
-    case 6: launch_ln<6>(s, L, tasks, n_tasks, rpt); break;
-    case 7: launch_ln<7>(s, L, tasks, n_tasks, rpt); break;
-    case 8: launch_ln<8>(s, L, tasks, n_tasks, rpt); break;
-    case 9: launch_ln<9>(s, L, tasks, n_tasks, rpt); break;
-    case 10: launch_ln<10>(s, L, tasks, n_tasks, rpt); break;
-    case 11: launch_ln<11>(s, L, tasks, n_tasks, rpt); break;
-    case 12: launch_ln<12>(s, L, tasks, n_tasks, rpt); break;
-    case 13: launch_ln<13>(s, L, tasks, n_tasks, rpt); break;
-    case 14: launch_ln<14>(s, L, tasks, n_tasks, rpt); break;
-    case 15: launch_ln<15>(s, L, tasks, n_tasks, rpt); break;
-    case 16: launch_ln<16>(s, L, tasks, n_tasks, rpt); break;
+    case 2: launch_ln<2>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
+    case 3: launch_ln<3>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
+    case 4: launch_ln<4>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
+    case 5: launch_ln<5>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
+    case 6: launch_ln<6>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
+    case 7: launch_ln<7>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
+    case 8: launch_ln<8>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
+    case 9: launch_ln<9>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
+    case 10: launch_ln<10>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
+    case 11: launch_ln<11>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
+    case 12: launch_ln<12>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
+    case 13: launch_ln<13>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
+    case 14: launch_ln<14>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
+    case 15: launch_ln<15>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
+    case 16: launch_ln<16>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
     default: break;  // rejected by the caller (kMaxLevel)
   }
 }

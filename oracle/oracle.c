@@ -129,43 +129,47 @@ double orc_pcor_pinv(int m, const double *M) {
 }
 
 /* Canonical partial correlation of 0 and 1 given 2..m-1 (m = l + 2).
- * With C = M[2:,2:] = L L^T, u = L^-1 M[0,2:], v = L^-1 M[1,2:]:
- *   s00 = 1 - u.u, s11 = 1 - v.v, s01 = M01 - u.v, r = s01 / sqrt(s00 s11),
+ * With C = M[2:,2:] = U D U^T (U unit lower), u = U^-1 M[0,2:],
+ * v = U^-1 M[1,2:]:
+ *   s00 = 1 - u.D^-1 u, s11 = 1 - v.D^-1 v, s01 = M01 - u.D^-1 v,
+ *   r = s01 / sqrt(s00 s11),
  * which equals -P01/sqrt(P00 P11) for P = M^-1 (the 2x2 block of M^-1 is the
  * inverse of the Schur complement [[s00,s01],[s01,s11]]).  Any pivot at or
  * below 1e-12, or a numerically singular Schur block, selects the pinv form
  * so that rank-deficient sets follow Armadillo's truncation. */
 double orc_pcor_canon(int m, const double *M, int *used_pinv) {
   const int l = m - 2;
-  double L[ORC_MMAX][ORC_MMAX], u[ORC_MMAX], v[ORC_MMAX];
+  /* LDL^T of the S block: U unit lower, E = U D, inv = 1/D */
+  double U[ORC_MMAX][ORC_MMAX], E[ORC_MMAX][ORC_MMAX], inv[ORC_MMAX], u[ORC_MMAX], v[ORC_MMAX];
   if (used_pinv) *used_pinv = 0;
-  for (int c = 0; c < l; ++c) {
-    double d = 1.0;
-    for (int k = 0; k < c; ++k) d = fma(-L[c][k], L[c][k], d);
-    if (!(d > ORC_PIV_TOL)) goto singular;
-    double s = sqrt(d);
-    L[c][c] = s;
-    for (int r = c + 1; r < l; ++r) {
-      double acc = M[(r + 2) * m + (c + 2)];
-      for (int k = 0; k < c; ++k) acc = fma(-L[r][k], L[c][k], acc);
-      L[r][c] = acc / s;
+  for (int r = 0; r < l; ++r) {
+    for (int c = 0; c < r; ++c) {
+      double e = M[(r + 2) * m + (c + 2)];
+      for (int k = 0; k < c; ++k) e = fma(-E[r][k], U[c][k], e);
+      E[r][c] = e;
+      U[r][c] = e * inv[c];
     }
+    double d = 1.0;
+    for (int k = 0; k < r; ++k) d = fma(-E[r][k], U[r][k], d);
+    if (!(d > ORC_PIV_TOL)) goto singular;
+    inv[r] = 1.0 / d;
   }
   for (int c = 0; c < l; ++c) {
     double su = M[0 * m + (c + 2)], sv = M[1 * m + (c + 2)];
     for (int k = 0; k < c; ++k) {
-      su = fma(-L[c][k], u[k], su);
-      sv = fma(-L[c][k], v[k], sv);
+      su = fma(-U[c][k], u[k], su);
+      sv = fma(-U[c][k], v[k], sv);
     }
-    u[c] = su / L[c][c];
-    v[c] = sv / L[c][c];
+    u[c] = su;
+    v[c] = sv;
   }
   {
     double s00 = 1.0, s11 = 1.0, s01 = M[0 * m + 1];
     for (int k = 0; k < l; ++k) {
-      s00 = fma(-u[k], u[k], s00);
-      s11 = fma(-v[k], v[k], s11);
-      s01 = fma(-u[k], v[k], s01);
+      const double wu = u[k] * inv[k], wv = v[k] * inv[k];
+      s00 = fma(-wu, u[k], s00);
+      s11 = fma(-wv, v[k], s11);
+      s01 = fma(-wu, v[k], s01);
     }
     if (!(s00 > ORC_PIV_TOL) || !(s11 > ORC_PIV_TOL)) goto singular;
     double q = s00 * s11;
