@@ -364,41 +364,27 @@ extern "C" int hpc_level(hpc_state *st, int level, hpc_level_stats *stats) {
 extern "C" int hpc_run(hpc_state *st, int max_level, hpc_level_stats *stats, int cap,
                        int *levels_run) {
   if (!st) return fail(HPC_EINVAL, "state is NULL");
-  if (max_level == -2) max_level = H(st).max_level;
-  int ran = 0;
-  for (int l = 0; max_level < 0 || l <= max_level; ++l) {
-    hpc_level_stats s;
-    if (l >= 1) {
-      // levels 0..maxLevel (skeleton.cpp:172); with no cap, until no row has
-      // enough neighbours
-      auto t0 = std::chrono::steady_clock::now();
-      float ms_c = 0.f;
-      int rc = pack_at(st, l);
-      if (!rc) rc = engine_compact(st->eng, &ms_c);
-      if (rc) return rc;
-      if (max_level < 0 && H(st).max_deg - 1 < l) break;
-      if (l > kMaxLevel) {
-        if (H(st).max_deg - 1 >= l)
-          return fail(HPC_ELEVEL, "level %d exceeds the device maximum %d", l, kMaxLevel);
-        memset(&s, 0, sizeof(s));
-        s.level = l;
-        s.max_deg = H(st).max_deg;
-        s.edges_left = H(st).undirected;
-      } else {
-        rc = hpc_test_level(st, l, nullptr, 0, &s);
-        if (rc) return rc;
-      }
-      s.ms_compact = ms_c;
-      s.ms_wall = ms_since(t0);
-    } else {
-      int rc = hpc_test_level(st, 0, nullptr, 0, &s);
-      if (rc) return rc;
-    }
-    if (stats && ran < cap) stats[ran] = s;
-    ++ran;
-  }
+  InstHost &h = H(st);
+  if (!h.have_cor) return fail(HPC_ESTATE, "no correlation matrix");
+  // levels 0..maxLevel (skeleton.cpp:172) from the current adjacency; with no
+  // cap, until no row has enough neighbours.  The engine runs each level as
+  // one step: level 0 or compaction, the CI tests and the removal, with two
+  // host synchronisations.
+  const int saved = h.max_level;
+  if (max_level != -2) h.max_level = max_level;
+  h.next_level = 0;
+  h.finished = false;
+  h.error = 0;
+  const size_t first = h.hist.size();
+  int rc = HPC_OK;
+  while (!h.finished && !rc) rc = engine_step(st->eng, {st->id});
+  if (!rc && h.error) rc = fail(h.error, "level %d exceeds the device maximum %d", h.next_level - 1,
+                               kMaxLevel);
+  h.max_level = saved;
+  const int ran = (int)(h.hist.size() - first);
+  for (int k = 0; stats && k < ran && k < cap; ++k) stats[k] = h.hist[first + k];
   if (levels_run) *levels_run = ran;
-  return HPC_OK;
+  return rc;
 }
 
 // ---------------------------------------------------------------------------

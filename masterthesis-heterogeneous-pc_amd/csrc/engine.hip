@@ -148,6 +148,7 @@ int engine_init(Engine &e, int device) {
   HPC_HIPCHK(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
   HPC_HIPCHK(hipStreamCreateWithFlags(&e.copy_stream, hipStreamNonBlocking));
   for (auto &ev : e.ev) HPC_HIPCHK(hipEventCreate(&ev));
+  for (auto &ev : e.sev) HPC_HIPCHK(hipEventCreate(&ev));
   HPC_HIPCHK(hipEventCreateWithFlags(&e.fp_ready, hipEventDisableTiming));
   int rc = dalloc((void **)&e.d_ctr, sizeof(Counters));
   if (rc) return rc;
@@ -175,13 +176,15 @@ void engine_free(Engine &e) {
   e.arena = Arena();
   dfree(e.d_inst); dfree(e.d_ictr); dfree(e.d_row_inst); dfree(e.d_deg); dfree(e.d_rowptr);
   dfree(e.d_nbr); dfree(e.d_erow); dfree(e.d_rowmask); dfree(e.d_fp); dfree(e.d_row_tasks);
-  dfree(e.d_task_off); dfree(e.d_tasks); dfree(e.d_binom); dfree(e.d_prefix); dfree(e.d_ctr);
+  dfree(e.d_task_off); dfree(e.d_binom); dfree(e.d_prefix); dfree(e.d_ctr);
   dfree(e.d_scatter_idx); dfree(e.d_scatter_val);
   if (e.h_ctr) (void)hipHostFree(e.h_ctr);
   if (e.h_ictr) (void)hipHostFree(e.h_ictr);
   e.h_ctr = nullptr;
   e.h_ictr = nullptr;
   for (auto &ev : e.ev)
+    if (ev) (void)hipEventDestroy(ev);
+  for (auto &ev : e.sev)
     if (ev) (void)hipEventDestroy(ev);
   if (e.fp_ready) (void)hipEventDestroy(e.fp_ready);
   e.fp_ready = nullptr;
@@ -449,8 +452,13 @@ static float elapsed(hipEvent_t a, hipEvent_t b) {
   return ms;
 }
 
-static int read_ctr(Engine &e) {
+// Device counters and the packed instances' counters, one synchronisation.
+static int read_counters(Engine &e) {
   HPC_HIPCHK(hipMemcpyAsync(e.h_ctr, e.d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, e.stream));
+  const size_t K = e.packed.size();
+  if (K)
+    HPC_HIPCHK(hipMemcpyAsync(e.h_ictr, e.d_ictr, K * sizeof(InstCounters), hipMemcpyDeviceToHost,
+                              e.stream));
   HPC_HIPCHK(hipStreamSynchronize(e.stream));
   return HPC_OK;
 }
@@ -523,8 +531,7 @@ int engine_compact(Engine &e, float *ms) {
   launch_compact(e.stream, L);
   HPC_HIPCHK(hipGetLastError());
   HPC_HIPCHK(hipEventRecord(e.ev[4], e.stream));
-  rc = read_ctr(e);
-  if (!rc) rc = engine_read_inst_counters(e);
+  rc = read_counters(e);
   if (rc) return rc;
   e.n_edges = (int64_t)e.h_ctr->n_edges;
   e.max_deg = e.h_ctr->max_deg;
@@ -559,26 +566,17 @@ int engine_search(Engine &e, int level, const int32_t *rows, int nrows, float *m
     HPC_HIPCHK(hipStreamSynchronize(e.stream));
     mask = e.d_rowmask;
   }
+  // the event pair brackets the CI-test kernel alone (the task plan of a
+  // level >= 2 runs before it)
+  bool any = e.n_edges > 0 && (!rows || nrows > 0);
+  if (any && level >= 2) {
+    launch_task_plan(e.stream, level, L, e.ranks_per_task, mask, e.d_row_tasks, e.d_task_off);
+    HPC_HIPCHK(hipGetLastError());
+  }
   HPC_HIPCHK(hipEventRecord(e.ev[5], e.stream));
-  if (e.n_edges > 0 && (!rows || nrows > 0)) {
-    if (level == 1) {
-      launch_search_l1(e.stream, L, e.n_edges, mask);
-    } else {
-      const int rpt = e.ranks_per_task;
-      launch_task_plan(e.stream, level, L, rpt, mask, e.d_row_tasks, e.d_task_off);
-      HPC_HIPCHK(hipGetLastError());
-      int rc = read_ctr(e);
-      if (rc) return rc;
-      const size_t n_tasks = (size_t)e.h_ctr->n_tasks;
-      if (n_tasks > 0) {
-        rc = grow(e.d_tasks, e.cap_tasks, n_tasks);
-        if (rc) return rc;
-        launch_task_fill(e.stream, L, e.d_row_tasks, e.d_task_off, e.d_tasks);
-        HPC_HIPCHK(hipGetLastError());
-        launch_search_ln(e.stream, level, L, e.d_tasks, (long long)n_tasks, rpt, e.d_row_tasks,
-                         e.d_task_off);
-      }
-    }
+  if (any) {
+    if (level == 1) launch_search_l1(e.stream, L, e.n_edges, mask);
+    else launch_search_ln(e.stream, level, L, e.ranks_per_task, e.d_row_tasks, e.d_task_off);
   }
   HPC_HIPCHK(hipGetLastError());
   HPC_HIPCHK(hipEventRecord(e.ev[6], e.stream));
@@ -696,8 +694,7 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
   if (any1) launch_compact(e.stream, L);
   HPC_HIPCHK(hipGetLastError());
   HPC_HIPCHK(hipEventRecord(e.ev[2], e.stream));
-  rc = read_ctr(e);
-  if (!rc) rc = engine_read_inst_counters(e);
+  rc = read_counters(e);
   if (rc) return rc;
   e.n_edges = any1 ? (int64_t)e.h_ctr->n_edges : 0;
   e.max_deg = any1 ? e.h_ctr->max_deg : 0;
@@ -743,21 +740,22 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
       levels_ge2.push_back(l);
   }
   std::sort(levels_ge2.begin(), levels_ge2.end());
-  HPC_HIPCHK(hipEventRecord(e.ev[3], e.stream));
+  // search kernels, each bracketed by its own event pair (task plans of
+  // levels >= 2 run outside the pairs)
+  int npairs = 0;
   if (e.n_edges > 0) {
-    if (run1) launch_search_l1(e.stream, L, e.n_edges, nullptr);
+    if (run1) {
+      HPC_HIPCHK(hipEventRecord(e.sev[2 * npairs], e.stream));
+      launch_search_l1(e.stream, L, e.n_edges, nullptr);
+      HPC_HIPCHK(hipEventRecord(e.sev[2 * npairs + 1], e.stream));
+      ++npairs;
+    }
     for (int l : levels_ge2) {
       launch_task_plan(e.stream, l, L, e.ranks_per_task, nullptr, e.d_row_tasks, e.d_task_off);
-      HPC_HIPCHK(hipGetLastError());
-      rc = read_ctr(e);
-      if (rc) return rc;
-      const size_t n_tasks = (size_t)e.h_ctr->n_tasks;
-      if (n_tasks == 0) continue;
-      rc = grow(e.d_tasks, e.cap_tasks, n_tasks);
-      if (rc) return rc;
-      launch_task_fill(e.stream, L, e.d_row_tasks, e.d_task_off, e.d_tasks);
-      launch_search_ln(e.stream, l, L, e.d_tasks, (long long)n_tasks, e.ranks_per_task,
-                       e.d_row_tasks, e.d_task_off);
+      HPC_HIPCHK(hipEventRecord(e.sev[2 * npairs], e.stream));
+      launch_search_ln(e.stream, l, L, e.ranks_per_task, e.d_row_tasks, e.d_task_off);
+      HPC_HIPCHK(hipEventRecord(e.sev[2 * npairs + 1], e.stream));
+      ++npairs;
     }
     HPC_HIPCHK(hipGetLastError());
   }
@@ -765,12 +763,14 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
   if (any1) launch_commit(e.stream, L, e.n_edges);
   HPC_HIPCHK(hipGetLastError());
   HPC_HIPCHK(hipEventRecord(e.ev[5], e.stream));
-  rc = engine_read_inst_counters(e);
+  rc = read_counters(e);
   if (rc) return rc;
   const double wall =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   const double ms_l0 = elapsed(e.ev[0], e.ev[1]), ms_c = elapsed(e.ev[1], e.ev[2]);
-  const double ms_s = elapsed(e.ev[3], e.ev[4]), ms_m = elapsed(e.ev[4], e.ev[5]);
+  const double ms_m = elapsed(e.ev[4], e.ev[5]);
+  double ms_s = 0.0;
+  for (int k = 0; k < npairs; ++k) ms_s += elapsed(e.sev[2 * k], e.sev[2 * k + 1]);
   for (int k = 0; k < K; ++k) {
     InstHost &h = e.inst[act[k]];
     const InstCounters &c = e.h_ictr[k];
@@ -794,6 +794,7 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
       s.removed = (int64_t)c.removed;
       h.undirected -= s.removed;
       s.edges_left = h.undirected;
+      if (K == 1) s.pinv_fallbacks = (int64_t)e.h_ctr->pinv_fallbacks;
       if (lev[k] == 0) {
         s.ms_test = ms_l0;
       } else {

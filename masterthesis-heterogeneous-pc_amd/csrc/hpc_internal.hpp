@@ -103,12 +103,9 @@ void launch_search_l1(hipStream_t s, const Layout &L, int64_t n_edges,
                       const int32_t *row_mask);
 void launch_task_plan(hipStream_t s, int level, const Layout &L, int rpt,
                       const int32_t *row_mask, long long *row_tasks, long long *task_off);
-void launch_task_fill(hipStream_t s, const Layout &L, const long long *row_tasks,
-                      const long long *task_off, unsigned long long *tasks);
-// tasks sorted by row with per-row counts and offsets (row_tasks, task_off)
-// let the kernel stage each row's neighbourhood in LDS; NULL = gather form
-void launch_search_ln(hipStream_t s, int level, const Layout &L,
-                      const unsigned long long *tasks, long long n_tasks, int rpt,
+// The plan's per-row task counts and exclusive offsets define the tasks; the
+// kernels read the task count from the device counters.
+void launch_search_ln(hipStream_t s, int level, const Layout &L, int rpt,
                       const long long *row_tasks, const long long *task_off);
 void launch_commit(hipStream_t s, const Layout &L, int64_t n_edges);
 void launch_scatter_fp(hipStream_t s, const int64_t *idx, const int64_t *val,
@@ -176,11 +173,10 @@ struct Engine {
   int32_t *d_rowmask = nullptr;
   int64_t *d_fp = nullptr;
   long long *d_row_tasks = nullptr, *d_task_off = nullptr;
-  unsigned long long *d_tasks = nullptr;
   unsigned long long *d_binom = nullptr;
   long long *d_prefix = nullptr;
   Counters *d_ctr = nullptr, *h_ctr = nullptr;
-  size_t cap_inst = 0, cap_rows = 0, cap_edges = 0, cap_tasks = 0, cap_binom = 0;
+  size_t cap_inst = 0, cap_rows = 0, cap_edges = 0, cap_binom = 0;
   std::vector<unsigned long long> h_binom;
   std::vector<long long> h_prefix;
   // the packed set of the current phase
@@ -192,6 +188,7 @@ struct Engine {
   int max_deg = 0;
   int ranks_per_task = 512;
   hipEvent_t ev[8] = {};
+  hipEvent_t sev[2 * (kMaxLevel + 1)] = {};  // search kernel brackets of one step
   hipEvent_t fp_ready = nullptr;  // recorded after the first-pass array is reset
   // host share scratch (single-instance use)
   std::vector<double> h_cor;

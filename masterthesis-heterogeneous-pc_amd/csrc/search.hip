@@ -125,13 +125,29 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_lds_kernel(
   }
 }
 
+// Task t of a level's plan: row g with task_off[g] <= t < task_off[g] +
+// row_tasks[g] (the last row whose exclusive offset is <= t), then ordered
+// edge rowptr[g] + local / nch and rank chunk local % nch, where
+// nch = row_tasks[g] / deg.
+__device__ __forceinline__ int task_row(const long long *__restrict__ task_off, int rows,
+                                        long long t) {
+  int lo = 0, hi = rows - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (task_off[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
 // Level L >= 2 (testRowLN<>, gpuRowTests.cu:102-205; the general-level form of
-// cpuWorkstealingTests.cpp:39-112).  tasks[t] = (ordered edge << 32) | chunk;
-// chunk c covers ranks [c * rpt, (c + 1) * rpt).
+// cpuWorkstealingTests.cpp:39-112), gathering operands from the correlation
+// matrix (rows too long for the LDS form).  Each wavefront takes tasks
+// t = wave, wave + waves, ... < n_tasks (read on the device); task t covers
+// ranks [chunk * rpt, (chunk + 1) * rpt) of its edge.
 template <int LV>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_kernel(
-    Layout L, const unsigned long long *__restrict__ tasks, long long n_tasks, int rpt,
-    int lds_binom) {
+    Layout L, int rpt, const long long *__restrict__ task_off,
+    const long long *__restrict__ row_tasks, int lds_binom) {
   // C(m, k) for m <= max_deg, k <= LV staged in LDS (columns 0..LV)
   extern __shared__ unsigned long long s_binom[];
   const unsigned long long *tab = L.binom;
@@ -146,24 +162,27 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_kernel(
     tab = s_binom;
     kc = LV + 1;
   }
+  const long long n_tasks = (long long)L.ctr->n_tasks;
   const int lane = threadIdx.x & 63;
   const long long wave0 = (long long)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   const long long nw = (long long)gridDim.x * kWavesPerBlock;
   unsigned long long F = 0;
   unsigned long long *fpu = reinterpret_cast<unsigned long long *>(L.fp);
   for (long long t = wave0; t < n_tasks; t += nw) {
-    const unsigned long long task = tasks[t];
-    const long long e = (long long)(task >> 32);
-    const unsigned long long r_begin = (task & 0xffffffffull) * (unsigned long long)rpt;
+    const int i = task_row(task_off, L.p_total, t);
+    const int base = L.rowptr[i];
+    const int deg = L.rowptr[i + 1] - base;
+    const long long local = t - task_off[i];
+    const long long nch = row_tasks[i] / deg;
+    const long long e = base + local / nch;
+    const unsigned long long r_begin = (unsigned long long)(local % nch) * (unsigned long long)rpt;
     // a lower rank of this edge already passed: nothing here can win
     if (__atomic_load_n(&fpu[e], __ATOMIC_RELAXED) < r_begin) continue;
-    const int i = L.erow[e];
     const Inst &In = L.inst[L.row_inst[i]];
     const int p = In.p, s0 = In.start;
     const double *cor = In.cor;
     const double r_crit = In.r_crit;
-    const int base = L.rowptr[i];
-    const int nn = L.rowptr[i + 1] - base - 1;
+    const int nn = deg - 1;
     const int y = (int)(e - base);
     const unsigned long long total = binom(tab, kc, nn, LV);
     const unsigned long long r_end = min(total, r_begin + (unsigned long long)rpt);
@@ -215,15 +234,15 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_kernel(
 // a test on ordered edge (i, j) lies in cor[{i} u N(i), {i} u N(i)] (j and S
 // are neighbours of i), so a workgroup copies that (deg + 1)^2 block of the
 // correlation matrix into LDS once -- position a < deg is neighbour N(i)[a],
-// position deg is i -- and runs the row's tasks against it.  Workgroup b
-// takes tasks [b * tpb, (b + 1) * tpb) of the row-sorted task list, one
-// staging per row it meets.  Same ranks, same arithmetic and the same
-// orientation of every correlation read as search_ln_kernel.
+// position deg is i -- and runs the row's tasks against it.  Workgroups take
+// blocks of tpb consecutive tasks (grid-stride; the task count is read on
+// the device), one staging per row a block meets.  Same ranks, same
+// arithmetic and the same orientation of every correlation read as
+// search_ln_kernel.
 template <int LV>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
-    Layout L, const unsigned long long *__restrict__ tasks, long long n_tasks, int rpt,
-    const long long *__restrict__ task_off, const long long *__restrict__ row_tasks, int tpb,
-    int dmax) {
+    Layout L, int rpt, const long long *__restrict__ task_off,
+    const long long *__restrict__ row_tasks, int tpb, int dmax) {
   extern __shared__ unsigned long long smem[];
   unsigned long long *bt = smem;                                   // (max_deg+1) x (LV+1)
   const int brows = L.max_deg + 1;
@@ -234,78 +253,83 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
     const int m = t / (LV + 1), k = t % (LV + 1);
     bt[t] = L.binom[(size_t)m * kBinomK + k];
   }
+  const long long n_tasks = (long long)L.ctr->n_tasks;
   unsigned long long F = 0;
   unsigned long long *fpu = reinterpret_cast<unsigned long long *>(L.fp);
-  long long t0 = (long long)blockIdx.x * tpb;
-  const long long t1 = min(n_tasks, t0 + tpb);
-  while (t0 < t1) {
-    // the segment of this block's tasks on row i (uniform across the block)
-    const int i = L.erow[(long long)(tasks[t0] >> 32)];
-    const long long seg_end = min(t1, task_off[i] + row_tasks[i]);
-    const Inst &In = L.inst[L.row_inst[i]];
-    const int p = In.p, s0 = In.start;
-    const int base = L.rowptr[i];
-    const int deg = L.rowptr[i + 1] - base;
-    const int D = deg + 1;
-    __syncthreads();  // the previous segment is done with M (and bt is staged)
-    for (int a = tid; a < D; a += blockDim.x) ids[a] = (a < deg ? L.nbr[base + a] : i) - s0;
-    __syncthreads();
-    for (int idx = tid; idx < D * D; idx += blockDim.x) {
-      const int a = idx / D, b = idx - a * D;
-      M[idx] = In.cor[(size_t)ids[a] * p + ids[b]];
-    }
-    __syncthreads();
-    const double r_crit = In.r_crit;
-    const int nn = deg - 1;
-    const unsigned long long total = binom(bt, LV + 1, nn, LV);
-    const double *Mi = M + (size_t)deg * D;  // row i
-    for (long long t = t0 + wave; t < seg_end; t += kWavesPerBlock) {
-      const unsigned long long task = tasks[t];
-      const long long e = (long long)(task >> 32);
-      const unsigned long long r_begin = (task & 0xffffffffull) * (unsigned long long)rpt;
-      if (__atomic_load_n(&fpu[e], __ATOMIC_RELAXED) < r_begin) continue;
-      const int y = (int)(e - base);
-      const unsigned long long r_end = min(total, r_begin + (unsigned long long)rpt);
-      const double a = Mi[y];
-      const double *Mj = M + (size_t)y * D;  // row j
-      for (unsigned long long r0 = r_begin; r0 < r_end; r0 += 64) {
-        const unsigned long long rank = r0 + lane;
-        bool pass = false;
-        if (rank < r_end) {
-          int q[LV], pos[LV];
-          unrank<LV>(bt, LV + 1, rank, nn, q);
+  for (long long blk = blockIdx.x; blk * tpb < n_tasks; blk += gridDim.x) {
+    long long t0 = blk * tpb;
+    const long long t1 = min(n_tasks, t0 + tpb);
+    while (t0 < t1) {
+      // the segment of this block's tasks on row i (uniform across the block)
+      const int i = task_row(task_off, L.p_total, t0);
+      const long long off = task_off[i];
+      const long long seg_end = min(t1, off + row_tasks[i]);
+      const Inst &In = L.inst[L.row_inst[i]];
+      const int p = In.p, s0 = In.start;
+      const int base = L.rowptr[i];
+      const int deg = L.rowptr[i + 1] - base;
+      const int D = deg + 1;
+      const long long nch = row_tasks[i] / deg;
+      __syncthreads();  // the previous segment is done with M (and bt is staged)
+      for (int a = tid; a < D; a += blockDim.x) ids[a] = (a < deg ? L.nbr[base + a] : i) - s0;
+      __syncthreads();
+      for (int idx = tid; idx < D * D; idx += blockDim.x) {
+        const int a = idx / D, b = idx - a * D;
+        M[idx] = In.cor[(size_t)ids[a] * p + ids[b]];
+      }
+      __syncthreads();
+      const double r_crit = In.r_crit;
+      const int nn = deg - 1;
+      const unsigned long long total = binom(bt, LV + 1, nn, LV);
+      const double *Mi = M + (size_t)deg * D;  // row i
+      for (long long t = t0 + wave; t < seg_end; t += kWavesPerBlock) {
+        const long long local = t - off;
+        const int y = (int)(local / nch);
+        const long long e = base + y;
+        const unsigned long long r_begin = (unsigned long long)(local % nch) * (unsigned long long)rpt;
+        if (__atomic_load_n(&fpu[e], __ATOMIC_RELAXED) < r_begin) continue;
+        const unsigned long long r_end = min(total, r_begin + (unsigned long long)rpt);
+        const double a = Mi[y];
+        const double *Mj = M + (size_t)y * D;  // row j
+        for (unsigned long long r0 = r_begin; r0 < r_end; r0 += 64) {
+          const unsigned long long rank = r0 + lane;
+          bool pass = false;
+          if (rank < r_end) {
+            int q[LV], pos[LV];
+            unrank<LV>(bt, LV + 1, rank, nn, q);
 #pragma unroll
-          for (int c = 0; c < LV; ++c) pos[c] = q[c] < y ? q[c] : q[c] + 1;
-          double x[LV], yv[LV], C[LV][LV];
-          bool anynan = __builtin_isnan(a);
+            for (int c = 0; c < LV; ++c) pos[c] = q[c] < y ? q[c] : q[c] + 1;
+            double x[LV], yv[LV], C[LV][LV];
+            bool anynan = __builtin_isnan(a);
 #pragma unroll
-          for (int c = 0; c < LV; ++c) {
-            x[c] = Mi[pos[c]];
-            yv[c] = Mj[pos[c]];
-            anynan |= __builtin_isnan(x[c]) || __builtin_isnan(yv[c]);
-          }
-#pragma unroll
-          for (int c = 0; c < LV; ++c)
-#pragma unroll
-            for (int r = c + 1; r < LV; ++r) {
-              C[r][c] = M[(size_t)pos[c] * D + pos[r]];
-              anynan |= __builtin_isnan(C[r][c]);
+            for (int c = 0; c < LV; ++c) {
+              x[c] = Mi[pos[c]];
+              yv[c] = Mj[pos[c]];
+              anynan |= __builtin_isnan(x[c]) || __builtin_isnan(yv[c]);
             }
-          if (!anynan) {
-            bool fb;
-            const double r = pcor_canon<LV>(a, x, yv, C, fb);
-            pass = removes(r, r_crit);
-            F += fb ? 1 : 0;
+#pragma unroll
+            for (int c = 0; c < LV; ++c)
+#pragma unroll
+              for (int r = c + 1; r < LV; ++r) {
+                C[r][c] = M[(size_t)pos[c] * D + pos[r]];
+                anynan |= __builtin_isnan(C[r][c]);
+              }
+            if (!anynan) {
+              bool fb;
+              const double r = pcor_canon<LV>(a, x, yv, C, fb);
+              pass = removes(r, r_crit);
+              F += fb ? 1 : 0;
+            }
           }
-        }
-        const unsigned long long m = __ballot(pass);
-        if (m) {
-          if (lane == 0) atomicMin(&fpu[e], r0 + (unsigned long long)(__ffsll((long long)m) - 1));
-          break;
+          const unsigned long long m = __ballot(pass);
+          if (m) {
+            if (lane == 0) atomicMin(&fpu[e], r0 + (unsigned long long)(__ffsll((long long)m) - 1));
+            break;
+          }
         }
       }
+      t0 = seg_end;
     }
-    t0 = seg_end;
   }
 #pragma unroll
   for (int s = 32; s > 0; s >>= 1) F += __shfl_xor(F, s, 64);
@@ -354,24 +378,6 @@ __global__ __launch_bounds__(kScan) void task_scan_kernel(const long long *__res
 }
 
 // One wavefront per row writes its tasks, edge-major.
-__global__ __launch_bounds__(256) void task_fill_kernel(Layout L, const long long *__restrict__ row_tasks,
-                                                        const long long *__restrict__ off,
-                                                        unsigned long long *__restrict__ tasks) {
-  const int lane = threadIdx.x & 63;
-  const int g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (g >= L.p_total) return;
-  const long long n = row_tasks[g];
-  if (n == 0) return;
-  const int base = L.rowptr[g];
-  const int deg = L.rowptr[g + 1] - base;
-  const long long nch = n / deg;
-  const long long o = off[g];
-  for (long long t = lane; t < n; t += 64) {
-    const long long y = t / nch, c = t % nch;
-    tasks[o + t] = ((unsigned long long)(base + y) << 32) | (unsigned long long)c;
-  }
-}
-
 __device__ __forceinline__ int find_pos(const int32_t *row, int len, int node) {
   int lo = 0, hi = len;
   while (lo < hi) {
@@ -536,23 +542,23 @@ size_t lds_bytes(int max_deg, int lv) {
 constexpr size_t kLdsLimit = 64 * 1024;
 constexpr int kTasksPerBlock = 16;
 
+// Resident workgroups of the grid-stride search kernels (the task count is
+// only known on the device): enough to fill all 256 CUs several times.
+constexpr int kSearchBlocks = 256 * 8;
+
 template <int LV>
-void launch_ln(hipStream_t s, const Layout &L, const unsigned long long *tasks,
-               long long n_tasks, int rpt, const long long *row_tasks, const long long *task_off) {
+void launch_ln(hipStream_t s, const Layout &L, int rpt, const long long *row_tasks,
+               const long long *task_off) {
   const size_t staged = lds_bytes(L.max_deg, LV);
-  if (row_tasks && task_off && staged <= kLdsLimit) {
-    const long long blocks = (n_tasks + kTasksPerBlock - 1) / kTasksPerBlock;
-    hipLaunchKernelGGL(search_ln_lds_kernel<LV>, dim3((unsigned)blocks),
-                       dim3(64 * kWavesPerBlock), staged, s, L, tasks, n_tasks, rpt, task_off,
-                       row_tasks, kTasksPerBlock, L.max_deg + 1);
+  if (staged <= kLdsLimit) {
+    hipLaunchKernelGGL(search_ln_lds_kernel<LV>, dim3(kSearchBlocks), dim3(64 * kWavesPerBlock),
+                       staged, s, L, rpt, task_off, row_tasks, kTasksPerBlock, L.max_deg + 1);
     return;
   }
-  long long blocks = (n_tasks + kWavesPerBlock - 1) / kWavesPerBlock;
-  if (blocks > 8192) blocks = 8192;
   const size_t lds = (size_t)(L.max_deg + 1) * (LV + 1) * sizeof(unsigned long long);
   const int use = lds <= 48 * 1024;
-  hipLaunchKernelGGL(search_ln_kernel<LV>, dim3((unsigned)blocks), dim3(64 * kWavesPerBlock),
-                     use ? lds : 0, s, L, tasks, n_tasks, rpt, use);
+  hipLaunchKernelGGL(search_ln_kernel<LV>, dim3(kSearchBlocks), dim3(64 * kWavesPerBlock),
+                     use ? lds : 0, s, L, rpt, task_off, row_tasks, use);
 }
 
 }  // namespace
@@ -581,32 +587,24 @@ void launch_task_plan(hipStream_t s, int level, const Layout &L, int rpt,
                      L.ctr);
 }
 
-void launch_task_fill(hipStream_t s, const Layout &L, const long long *row_tasks,
-                      const long long *task_off, unsigned long long *tasks) {
-  hipLaunchKernelGGL(task_fill_kernel, dim3((L.p_total + 3) / 4), dim3(256), 0, s, L, row_tasks,
-                     task_off, tasks);
-}
-
-void launch_search_ln(hipStream_t s, int level, const Layout &L,
-                      const unsigned long long *tasks, long long n_tasks, int rpt,
+void launch_search_ln(hipStream_t s, int level, const Layout &L, int rpt,
                       const long long *row_tasks, const long long *task_off) {
-  if (n_tasks <= 0) return;
   switch (level) {
-    case 2: launch_ln<2>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
-    case 3: launch_ln<3>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
-    case 4: launch_ln<4>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
-    case 5: launch_ln<5>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
-    case 6: launch_ln<6>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
-    case 7: launch_ln<7>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
-    case 8: launch_ln<8>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
-    case 9: launch_ln<9>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
-    case 10: launch_ln<10>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
-    case 11: launch_ln<11>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
-    case 12: launch_ln<12>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
-    case 13: launch_ln<13>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
-    case 14: launch_ln<14>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
-    case 15: launch_ln<15>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
-    case 16: launch_ln<16>(s, L, tasks, n_tasks, rpt, row_tasks, task_off); break;
+    case 2: launch_ln<2>(s, L, rpt, row_tasks, task_off); break;
+    case 3: launch_ln<3>(s, L, rpt, row_tasks, task_off); break;
+    case 4: launch_ln<4>(s, L, rpt, row_tasks, task_off); break;
+    case 5: launch_ln<5>(s, L, rpt, row_tasks, task_off); break;
+    case 6: launch_ln<6>(s, L, rpt, row_tasks, task_off); break;
+    case 7: launch_ln<7>(s, L, rpt, row_tasks, task_off); break;
+    case 8: launch_ln<8>(s, L, rpt, row_tasks, task_off); break;
+    case 9: launch_ln<9>(s, L, rpt, row_tasks, task_off); break;
+    case 10: launch_ln<10>(s, L, rpt, row_tasks, task_off); break;
+    case 11: launch_ln<11>(s, L, rpt, row_tasks, task_off); break;
+    case 12: launch_ln<12>(s, L, rpt, row_tasks, task_off); break;
+    case 13: launch_ln<13>(s, L, rpt, row_tasks, task_off); break;
+    case 14: launch_ln<14>(s, L, rpt, row_tasks, task_off); break;
+    case 15: launch_ln<15>(s, L, rpt, row_tasks, task_off); break;
+    case 16: launch_ln<16>(s, L, rpt, row_tasks, task_off); break;
     default: break;  // rejected by the caller (kMaxLevel)
   }
 }
