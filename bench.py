@@ -139,23 +139,20 @@ def bench_batched(heterogpc, insts):
     }
 
 
-def bench_served(heterogpc, trace, rate):
-    """Served form: the arrival trace replayed in real time through the
-    continuous-batching server and through the plain path (one request at a
-    time, in arrival order).  Latency = arrival -> adjacency matrix and sepset
-    list in the caller's hands, for both."""
+def _serve_trace(heterogpc, reqs, arrivals):
+    """Replays (x, alpha, max_level) requests arriving at `arrivals` (s)
+    through the continuous-batching server.  Returns per-request
+    (adj, sepset list, completion s) and the server's statistics."""
     import threading
 
-    _, x0, a0, m0 = trace[0]
-    _plain_skeleton(heterogpc, x0, a0, m0)
-    n_req = len(trace)
+    n_req = len(reqs)
     sub = [None] * n_req
-    served = [None] * n_req
+    out = [None] * n_req
     with heterogpc.Server() as srv:
         t0 = time.perf_counter()
 
         def feeder():
-            for k, (ta, x, a, ml) in enumerate(trace):
+            for k, ((x, a, ml), ta) in enumerate(zip(reqs, arrivals)):
                 d = ta - (time.perf_counter() - t0)
                 if d > 0:
                     time.sleep(d)
@@ -170,40 +167,70 @@ def bench_served(heterogpc, trace, rate):
                 if sub[k] is None:
                     break
                 if srv.wait(sub[k], 0.0):
-                    adj, lst, info = srv.result(sub[k], dense_sepsets=False)
-                    served[k] = (adj, lst, info, time.perf_counter() - t0)
+                    adj, lst, _ = srv.result(sub[k], dense_sepsets=False)
+                    out[k] = (adj, lst, time.perf_counter() - t0)
                     left.discard(k)
                     got += 1
             if not got:
                 time.sleep(2e-4)
         th.join()
         stats = srv.stats()
-    s_lat = [(served[k][3] - trace[k][0]) * 1e3 for k in range(n_req)]
-    s_end = max(v[3] for v in served)
+    return out, stats
+
+
+def _plain_trace(heterogpc, reqs, arrivals):
+    """The same trace through the plain path, one request at a time in
+    arrival order."""
     t0 = time.perf_counter()
-    plain_lat, plain = [], []
-    for ta, x, a, ml in trace:
+    out = []
+    for (x, a, ml), ta in zip(reqs, arrivals):
         d = ta - (time.perf_counter() - t0)
         if d > 0:
             time.sleep(d)
-        plain.append(_plain_skeleton(heterogpc, x, a, ml))
-        plain_lat.append((time.perf_counter() - t0 - ta) * 1e3)
-    p_end = time.perf_counter() - t0
+        adj, lst, _ = _plain_skeleton(heterogpc, x, a, ml)
+        out.append((adj, lst, time.perf_counter() - t0))
+    return out
+
+
+def bench_served(heterogpc, trace, load):
+    """Served form.  The plain path's capacity C (requests/s, one at a time)
+    is measured on the requests first; Poisson arrivals at `load` x C are
+    then replayed in real time through the continuous-batching server and
+    through the plain path.  Latency = arrival -> adjacency matrix and sepset
+    list in the caller's hands.  Saturation (every request at t = 0) gives
+    each path's skeletons/hour."""
+    reqs = [(x, a, ml) for _, x, a, ml in trace]
+    gaps = np.diff([0.0] + [t for t, _, _, _ in trace])  # unit-rate exponential gaps
+    gaps = gaps / gaps.mean()
+    n_req = len(reqs)
+    _plain_skeleton(heterogpc, *reqs[0])
+    sat_plain = _plain_trace(heterogpc, reqs, [0.0] * n_req)
+    plain_sat_s = sat_plain[-1][2]
+    rate = load * n_req / plain_sat_s
+    arrivals = list(np.cumsum(gaps) / rate)
+    sat_served, _ = _serve_trace(heterogpc, reqs, [0.0] * n_req)
+    served_sat_s = max(v[2] for v in sat_served)
+    served, stats = _serve_trace(heterogpc, reqs, arrivals)
+    plain = _plain_trace(heterogpc, reqs, arrivals)
+    s_lat = [(served[k][2] - arrivals[k]) * 1e3 for k in range(n_req)]
+    p_lat = [(plain[k][2] - arrivals[k]) * 1e3 for k in range(n_req)]
     same = all(np.array_equal(sa, pa) and np.array_equal(ss, ps)
-               for (sa, ss, _, _), (pa, ps, _) in zip(served, plain))
+               for (sa, ss, _), (pa, ps, _) in zip(served + sat_served, plain + sat_plain))
     return {
-        "workload": "%d requests, Poisson arrivals at %.0f/s, p log-uniform [50,2000], n U[500,"
-                    "10000], degree U[2,5], alpha {0.01,0.05}, FP32 data, max level 4"
-                    % (n_req, rate),
+        "workload": "%d requests, p log-uniform [50,2000], n U[500,10000], degree U[2,5], alpha "
+                    "{0.01,0.05}, FP32 data, max level 4; Poisson arrivals at %.1f/s = %.2f x the "
+                    "plain path's capacity" % (n_req, rate, load),
         "requests": n_req,
+        "arrival_rate_per_s": rate,
         "served_p50_ms": _pct(s_lat, 50),
         "served_p99_ms": _pct(s_lat, 99),
-        "plain_p50_ms": _pct(plain_lat, 50),
-        "plain_p99_ms": _pct(plain_lat, 99),
-        "served_skeletons_per_hour": n_req / s_end * 3600.0,
-        "plain_skeletons_per_hour": n_req / p_end * 3600.0,
-        "served_makespan_s": s_end,
-        "plain_makespan_s": p_end,
+        "plain_p50_ms": _pct(p_lat, 50),
+        "plain_p99_ms": _pct(p_lat, 99),
+        "served_skeletons_per_hour_at_rate": n_req / max(v[2] for v in served) * 3600.0,
+        "plain_skeletons_per_hour_at_rate": n_req / plain[-1][2] * 3600.0,
+        "served_skeletons_per_hour_saturated": n_req / served_sat_s * 3600.0,
+        "plain_skeletons_per_hour_saturated": n_req / plain_sat_s * 3600.0,
+        "saturated_speedup": plain_sat_s / served_sat_s,
         "server": {k: stats[k] for k in ("steps", "peak_active", "peak_bytes", "budget", "busy_ms")},
         "results_identical": bool(same),
     }
@@ -220,8 +247,9 @@ def main() -> int:
                     help="instances of the batched form (0 = skip)")
     ap.add_argument("--served-count", type=int, default=200,
                     help="requests of the served form (0 = skip)")
-    ap.add_argument("--served-rate", type=float, default=100.0,
-                    help="Poisson arrival rate of the served form, requests/s")
+    ap.add_argument("--served-load", type=float, default=0.7,
+                    help="arrival rate of the served form as a fraction of the plain "
+                         "path's measured capacity")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -310,8 +338,8 @@ def main() -> int:
         out["batched"] = bench_batched(heterogpc, workloads.batch_mix(args.batch_count))
     if args.served_count > 0:
         print("[bench] served form", file=sys.stderr, flush=True)
-        out["served"] = bench_served(heterogpc, workloads.served_trace(
-            args.served_count, args.served_rate), args.served_rate)
+        out["served"] = bench_served(heterogpc, workloads.served_trace(args.served_count, 1.0),
+                                     args.served_load)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(x, n, alpha, max_level, tests_total)

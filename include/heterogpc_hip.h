@@ -245,6 +245,10 @@ int hpc_server_destroy(hpc_server *s);
  * exceeds the budget. */
 int hpc_server_submit(hpc_server *s, uint64_t p, int n, double alpha,
                       int max_level, const double *x, int64_t *req);
+/* As hpc_server_submit without the copy: x is read in place, so it must stay
+ * valid and unchanged until hpc_server_wait reports the request done. */
+int hpc_server_submit_borrowed(hpc_server *s, uint64_t p, int n, double alpha,
+                               int max_level, const double *x, int64_t *req);
 /* Waits up to timeout_s seconds (< 0: no limit); *done = 1 when finished. */
 int hpc_server_wait(hpc_server *s, int64_t req, double timeout_s, int *done);
 /* Adjacency (p*p) and sepsets (width mc) of a finished request; either may be

@@ -189,8 +189,11 @@ struct Request {
   int64_t id = 0;
   int p = 0, n = 0, max_level = 0;
   double alpha = 0.0;
-  std::vector<double> x;
-  int has_nan = 0;
+  std::vector<double> x;        // copy of the observations (unless borrowed)
+  const double *src = nullptr;  // observations to upload: x, or the caller's
+  size_t count = 0;
+  double *dst = nullptr;        // device block of the observations
+  hipError_t up_err = hipSuccess;
   size_t bytes = 0;
   int slot = -1;       // engine instance while admitted
   int state = 0;       // 0 queued, 1 running, 2 done, 3 failed
@@ -216,8 +219,18 @@ struct hpc_server {
   std::vector<std::shared_ptr<Request>> running;
   std::map<int64_t, std::shared_ptr<Request>> all;
   int64_t next_id = 1;
+  int64_t submits = 0;
   bool stop = false;
   std::thread worker;
+  // observation uploads: the uploader thread copies admitted requests' data
+  // on its own stream while the worker keeps stepping the running ones
+  hipStream_t up_stream = nullptr;
+  std::thread uploader;
+  std::condition_variable cv_up;
+  std::deque<std::shared_ptr<Request>> up_queue;
+  std::vector<std::shared_ptr<Request>> up_done;
+  int uploading = 0;
+  bool stop_up = false;
   // statistics
   int64_t steps = 0;
   int peak_active = 0;
@@ -242,51 +255,115 @@ static int server_collect(hpc_server *s, Request &r) {
   return rc;
 }
 
-static void server_loop(hpc_server *s) {
+static void fail_request(hpc_server *s, Request &r, int rc, const std::string &msg) {
+  r.error = rc;
+  r.message = msg;
+  r.state = 3;
+  r.t_done = Clock::now();
+  s->in_use -= r.bytes;
+}
+
+static void uploader_loop(hpc_server *s) {
   (void)hipSetDevice(s->eng.device);
   for (;;) {
-    // admission: queued requests in arrival order while they fit
-    std::vector<std::shared_ptr<Request>> admit;
+    std::shared_ptr<Request> r;
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty() || !s->running.empty(); });
-      if (s->stop && s->running.empty()) return;
+      s->cv_up.wait(lk, [&] { return s->stop_up || !s->up_queue.empty(); });
+      if (s->up_queue.empty()) return;
+      r = s->up_queue.front();
+      s->up_queue.pop_front();
+    }
+    hipError_t err = hipMemcpyAsync(r->dst, r->src, r->count * sizeof(double),
+                                    hipMemcpyHostToDevice, s->up_stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(s->up_stream);
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      r->up_err = err;
+      s->up_done.push_back(r);
+    }
+    s->cv_work.notify_all();
+  }
+}
+
+static void server_loop(hpc_server *s) {
+  (void)hipSetDevice(s->eng.device);
+  int64_t seen = -1;
+  for (;;) {
+    // admission: queued requests in arrival order while they fit; uploads
+    // that have completed
+    std::vector<std::shared_ptr<Request>> admit, ready;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_work.wait(lk, [&] {
+        return s->stop || !s->queue.empty() || !s->running.empty() || !s->up_done.empty() ||
+               s->uploading > 0;
+      });
+      if (s->stop && s->running.empty() && s->uploading == 0) return;
+      seen = s->submits;
       size_t used = s->in_use;
-      int active = (int)s->running.size();
+      int active = (int)s->running.size() + s->uploading;
       while (!s->queue.empty()) {
         auto &r = s->queue.front();
         if (active >= s->max_active) break;
         if (used + r->bytes > s->budget && active > 0) break;
         used += r->bytes;
         ++active;
+        s->in_use += r->bytes;
         admit.push_back(r);
         s->queue.pop_front();
       }
+      ready.swap(s->up_done);
     }
     for (auto &r : admit) {
       int slot = -1;
+      double *dst = nullptr;
       int rc = engine_add(s->eng, r->p, r->n, r->alpha, r->max_level, &slot);
       if (!rc) {
-        rc = engine_set_observations(s->eng, slot, r->x.data(), r->has_nan);
+        rc = engine_alloc_observations(s->eng, slot, &dst);
         if (rc) engine_remove(s->eng, slot);
-        else s->eng.inst[slot].drop_obs = true;
       }
+      // the adjacency initialisation is on the engine stream: done before the
+      // instance first runs
       std::lock_guard<std::mutex> lk(s->mu);
       r->t_admit = Clock::now();
       if (rc) {
-        r->error = rc;
-        r->message = hpc_last_error();
-        r->state = 3;
-        r->t_done = r->t_admit;
+        fail_request(s, *r, rc, hpc_last_error());
         s->cv_done.notify_all();
         continue;
       }
       r->slot = slot;
+      r->dst = dst;
       r->state = 1;
+      ++s->uploading;
+      s->up_queue.push_back(r);
+      s->cv_up.notify_one();
+    }
+    for (auto &r : ready) {
+      std::lock_guard<std::mutex> lk(s->mu);
+      --s->uploading;
+      if (r->up_err != hipSuccess) {
+        engine_remove(s->eng, r->slot);
+        r->slot = -1;
+        fail_request(s, *r, HPC_EHIP, std::string("observation upload failed: ") +
+                                          hipGetErrorString(r->up_err));
+        s->cv_done.notify_all();
+        continue;
+      }
+      engine_observations_ready(s->eng, r->slot);
+      s->eng.inst[r->slot].drop_obs = true;
       r->x.clear();
       r->x.shrink_to_fit();
-      s->in_use += r->bytes;
+      r->src = nullptr;
       s->running.push_back(r);
+    }
+    if (s->running.empty()) {
+      // nothing to step: wait for an upload, a submission or the stop
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_work.wait(lk, [&] {
+        return !s->up_done.empty() || s->submits != seen || (s->stop && s->uploading == 0);
+      });
+      continue;
     }
     std::vector<int> ids;
     {
@@ -343,6 +420,12 @@ extern "C" int hpc_server_create(int device, uint64_t hbm_budget, int max_active
   }
   s->budget = hbm_budget;
   s->max_active = max_active > 0 ? max_active : 1024;
+  if (hipStreamCreateWithFlags(&s->up_stream, hipStreamNonBlocking) != hipSuccess) {
+    engine_free(s->eng);
+    delete s;
+    return hpc_fail(HPC_EHIP, "stream creation failed");
+  }
+  s->uploader = std::thread(uploader_loop, s);
   s->worker = std::thread(server_loop, s);
   *out = s;
   return HPC_OK;
@@ -357,13 +440,20 @@ extern "C" int hpc_server_destroy(hpc_server *s) {
   }
   s->cv_work.notify_all();
   if (s->worker.joinable()) s->worker.join();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop_up = true;
+  }
+  s->cv_up.notify_all();
+  if (s->uploader.joinable()) s->uploader.join();
+  if (s->up_stream) (void)hipStreamDestroy(s->up_stream);
   engine_free(s->eng);
   delete s;
   return HPC_OK;
 }
 
-extern "C" int hpc_server_submit(hpc_server *s, uint64_t p, int n, double alpha, int max_level,
-                                 const double *x, int64_t *req) {
+static int server_submit(hpc_server *s, uint64_t p, int n, double alpha, int max_level,
+                         const double *x, bool borrow, int64_t *req) {
   if (!s || !x || !req) return hpc_fail(HPC_EINVAL, "NULL argument");
   if (p < 2 || p > 46340 || n < 1) return hpc_fail(HPC_EINVAL, "bad request shape");
   auto r = std::make_shared<Request>();
@@ -371,18 +461,12 @@ extern "C" int hpc_server_submit(hpc_server *s, uint64_t p, int n, double alpha,
   r->n = n;
   r->alpha = alpha;
   r->max_level = max_level;
-  // copy and NaN scan on the caller's thread, off the worker's path
-  r->x.resize((size_t)p * n);
-  {
-    const double *src = x;
-    double *dst = r->x.data();
-    bool nan = false;
-    for (size_t k = 0; k < (size_t)p * n; ++k) {
-      const double v = src[k];
-      dst[k] = v;
-      nan |= v != v;
-    }
-    r->has_nan = nan ? 1 : 0;
+  r->count = (size_t)p * n;
+  if (borrow) {
+    r->src = x;
+  } else {
+    r->x.assign(x, x + r->count);
+    r->src = r->x.data();
   }
   r->bytes = engine_footprint((int)p, n, max_level);
   r->t_submit = Clock::now();
@@ -393,10 +477,21 @@ extern "C" int hpc_server_submit(hpc_server *s, uint64_t p, int n, double alpha,
     r->id = s->next_id++;
     s->all[r->id] = r;
     s->queue.push_back(r);
+    ++s->submits;
     *req = r->id;
   }
   s->cv_work.notify_all();
   return HPC_OK;
+}
+
+extern "C" int hpc_server_submit(hpc_server *s, uint64_t p, int n, double alpha, int max_level,
+                                 const double *x, int64_t *req) {
+  return server_submit(s, p, n, alpha, max_level, x, false, req);
+}
+
+extern "C" int hpc_server_submit_borrowed(hpc_server *s, uint64_t p, int n, double alpha,
+                                          int max_level, const double *x, int64_t *req) {
+  return server_submit(s, p, n, alpha, max_level, x, true, req);
 }
 
 extern "C" int hpc_server_wait(hpc_server *s, int64_t req, double timeout_s, int *done) {

@@ -9,6 +9,8 @@
 // CPU evaluation of the same order.
 #pragma clang fp contract(off)
 
+#include <algorithm>
+
 #include "hpc_internal.hpp"
 
 namespace hpc {
@@ -38,12 +40,14 @@ __device__ __forceinline__ int find_inst(const long long *prefix, int K, long lo
 
 // One wavefront per variable: mean over non-NaN samples, sd rounded through
 // float (corHelper.cu:75), z-scores (x - mean) / sd padded with zeros to npad,
-// NaN mask.
+// NaN mask.  A variable with NaN cells flags its instance (ictr.has_nan),
+// which selects the masked pairwise kernel for it.
 __global__ __launch_bounds__(256) void moments_kernel(Layout L) {
   const int lane = threadIdx.x & 63;
   const int g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (g >= L.p_total) return;
-  const Inst I = L.inst[L.row_inst[g]];
+  const int kin = L.row_inst[g];
+  const Inst I = L.inst[kin];
   if (!(I.flags & kActive)) return;
   const int a = g - I.start, n = I.n, npad = I.npad;
   const double *xa = I.x + (size_t)a * n;
@@ -55,6 +59,7 @@ __global__ __launch_bounds__(256) void moments_kernel(Layout L) {
   }
   s = tree64(s);
   c = tree64i(c);
+  if (lane == 0 && c < n) atomicOr(&L.ictr[kin].has_nan, 1);
   const double mu = __shfl(s, 0, 64) / (double)__shfl(c, 0, 64);
   double q = 0.0;
   for (int k = lane; k < n; k += 64) {
@@ -88,15 +93,12 @@ __global__ __launch_bounds__(256) void moments_kernel(Layout L) {
 // ---------------------------------------------------------------------------
 constexpr int TB = 4;
 
-__global__ __launch_bounds__(256) void corr_canon_kernel(Layout L, const long long *__restrict__ prefix) {
-  const int lane = threadIdx.x & 63;
-  const long long w = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (w >= prefix[L.K]) return;
-  const int k = find_inst(prefix, L.K, w);
+// One 4 x 4 block (index `rem` of instance k's upper block triangle) of the
+// masked form.
+__device__ __forceinline__ void corr_masked_block(Layout L, int k, long long rem, int lane) {
   const Inst I = L.inst[k];
   const int p = I.p, npad = I.npad;
   const int nb = (p + TB - 1) / TB;
-  long long rem = w - prefix[k];
   int A = 0;
   while (rem >= nb - A) { rem -= nb - A; ++A; }
   const int B = A + (int)rem;
@@ -145,6 +147,22 @@ __global__ __launch_bounds__(256) void corr_canon_kernel(Layout L, const long lo
     }
 }
 
+__global__ __launch_bounds__(256) void corr_canon_kernel(Layout L, const long long *__restrict__ prefix) {
+  const int lane = threadIdx.x & 63;
+  const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
+  for (long long w = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+       w < prefix[L.K]; w += nw) {
+    const int k = find_inst(prefix, L.K, w);
+    if (!L.ictr[k].has_nan) {
+      // NaN-free instances go to the tiled kernel: jump past this one's blocks
+      const long long end = prefix[k + 1];
+      w += ((end - 1 - w) / nw) * nw;
+      continue;
+    }
+    corr_masked_block(L, k, w - prefix[k], lane);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Tiled form for NaN-free instances: a 256-thread workgroup owns a 16 x 16
 // tile of variable pairs; each wavefront an 8 x 8 sub-tile (64 accumulators
@@ -170,6 +188,7 @@ __global__ __launch_bounds__(256) void corr_tile_kernel(Layout L, const long lon
   const long long Lg = (g % kXcds) * per_xcd + g / kXcds;
   if (Lg >= prefix[L.K]) return;
   const int k = find_inst(prefix, L.K, Lg);
+  if (L.ictr[k].has_nan) return;  // the masked kernel takes it
   const Inst I = L.inst[k];
   const int p = I.p, npad = I.npad;
   const int nt = (p + CT - 1) / CT;
@@ -261,14 +280,15 @@ void launch_moments(hipStream_t s, const Layout &L) {
 void launch_corr(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst,
                  long long *d_prefix, std::vector<long long> &h_prefix) {
   const int K = L.K;
-  // two prefix arrays in one upload: tiled slots, then masked blocks
+  // two prefix arrays in one upload: tiled slots, then masked blocks, both
+  // over every active instance; the moments kernel's NaN flag decides which
+  // kernel computes an instance
   h_prefix.assign(2 * (K + 1), 0);
   for (int k = 0; k < K; ++k) {
     const Inst &I = host_inst[k];
     const bool on = I.flags & kActive;
-    const bool nan = I.flags & kHasNan;
-    h_prefix[k + 1] = h_prefix[k] + (on && !nan ? tiled_slots(I.p) : 0);
-    h_prefix[K + 1 + k + 1] = h_prefix[K + 1 + k] + (on && nan ? masked_blocks(I.p) : 0);
+    h_prefix[k + 1] = h_prefix[k] + (on ? tiled_slots(I.p) : 0);
+    h_prefix[K + 1 + k + 1] = h_prefix[K + 1 + k] + (on ? masked_blocks(I.p) : 0);
   }
   (void)hipMemcpyAsync(d_prefix, h_prefix.data(), h_prefix.size() * sizeof(long long),
                        hipMemcpyHostToDevice, s);
@@ -280,8 +300,9 @@ void launch_corr(hipStream_t s, const Layout &L, const std::vector<Inst> &host_i
   }
   if (masked > 0) {
     const int wpb = 4;
-    hipLaunchKernelGGL(corr_canon_kernel, dim3((unsigned)((masked + wpb - 1) / wpb)),
-                       dim3(64 * wpb), 0, s, L, d_prefix + K + 1);
+    const long long blocks = std::min<long long>((masked + wpb - 1) / wpb, 4096);
+    hipLaunchKernelGGL(corr_canon_kernel, dim3((unsigned)blocks), dim3(64 * wpb), 0, s, L,
+                       d_prefix + K + 1);
   }
 }
 

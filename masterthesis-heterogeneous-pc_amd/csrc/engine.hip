@@ -279,7 +279,7 @@ int engine_reset(Engine &e, int id) {
   return HPC_OK;
 }
 
-int engine_set_observations(Engine &e, int id, const double *h_x, int nan_hint) {
+int engine_alloc_observations(Engine &e, int id, double **x) {
   InstHost &h = e.inst[id];
   HPC_HIPCHK(hipSetDevice(e.device));
   int rc = 0;
@@ -288,17 +288,26 @@ int engine_set_observations(Engine &e, int id, const double *h_x, int nan_hint) 
   if (!rc && !h.z) rc = ialloc(e, h.z, (size_t)h.p * h.npad);
   if (!rc && !h.nanm) rc = ialloc(e, h.nanm, (size_t)h.p * h.npad);
   if (rc) return rc;
-  const size_t cnt = (size_t)h.p * h.n;
-  HPC_HIPCHK(hipMemcpyAsync(h.x, h_x, cnt * sizeof(double), hipMemcpyHostToDevice, e.stream));
-  // NaN cells need the masked pairwise kernel (corHelper.cu:101 skips them)
-  bool any_nan = nan_hint > 0;
-  if (nan_hint < 0)
-    for (size_t k = 0; k < cnt; ++k)
-      if (h_x[k] != h_x[k]) { any_nan = true; break; }
-  h.x_has_nan = any_nan;
-  HPC_HIPCHK(hipStreamSynchronize(e.stream));
+  if (fresh) h.bytes += (size_t)h.p * h.n * 8 + (size_t)h.p * h.npad * 9;
+  *x = h.x;
+  return HPC_OK;
+}
+
+void engine_observations_ready(Engine &e, int id) {
+  InstHost &h = e.inst[id];
   h.have_x = true;
-  if (fresh) h.bytes += cnt * 8 + (size_t)h.p * h.npad * 9;
+  h.have_cor = false;
+}
+
+int engine_set_observations(Engine &e, int id, const double *h_x) {
+  InstHost &h = e.inst[id];
+  double *dst = nullptr;
+  int rc = engine_alloc_observations(e, id, &dst);
+  if (rc) return rc;
+  const size_t cnt = (size_t)h.p * h.n;
+  HPC_HIPCHK(hipMemcpyAsync(dst, h_x, cnt * sizeof(double), hipMemcpyHostToDevice, e.stream));
+  HPC_HIPCHK(hipStreamSynchronize(e.stream));
+  engine_observations_ready(e, id);
   return HPC_OK;
 }
 
@@ -412,7 +421,7 @@ int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &
     I.start = start;
     I.p = h.p; I.n = h.n; I.npad = h.npad; I.W = h.W;
     I.level = levels[k];
-    I.flags = kActive | (h.x_has_nan ? kHasNan : 0);
+    I.flags = kActive;
     h.start = start;
     std::fill(e.h_row_inst.begin() + start, e.h_row_inst.begin() + start + h.p, k);
     start += h.p;
@@ -483,6 +492,9 @@ int engine_corr(Engine &e, float *ms_moments, float *ms_corr) {
   HPC_HIPCHK(hipSetDevice(e.device));
   for (int id : e.packed)
     if (!e.inst[id].have_x) return hpc_fail(HPC_ESTATE, "no observations uploaded");
+  // the moments kernel raises the per-instance NaN flags
+  int rc = zero_counters(e);
+  if (rc) return rc;
   Layout L = layout(e);
   HPC_HIPCHK(hipEventRecord(e.ev[0], e.stream));
   launch_moments(e.stream, L);
@@ -663,14 +675,6 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
     if (!rc) rc = engine_corr(e, nullptr, nullptr);
     if (rc) return rc;
   }
-  // observations and z-scores are only read by the correlation
-  for (int id : need)
-    if (e.inst[id].drop_obs) {
-      InstHost &h = e.inst[id];
-      free_obs(e, h);
-      h.have_x = false;
-      h.bytes -= std::min(h.bytes, (size_t)h.p * h.n * 8 + (size_t)h.p * h.npad * 9);
-    }
   std::vector<int> act, lev;
   for (int id : ids) {
     InstHost &h = e.inst[id];
@@ -683,6 +687,16 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
   rc = engine_pack(e, act, lev);
   if (!rc) rc = zero_counters(e);
   if (rc) return rc;
+  // observations and z-scores are only read by the correlation, which the
+  // pack above has waited for; blocks freed here may go to an upload on
+  // another stream
+  for (int id : need)
+    if (e.inst[id].drop_obs) {
+      InstHost &h = e.inst[id];
+      free_obs(e, h);
+      h.have_x = false;
+      h.bytes -= std::min(h.bytes, (size_t)h.p * h.n * 8 + (size_t)h.p * h.npad * 9);
+    }
   const int K = (int)act.size();
   Layout L = layout(e);
   bool any0 = false, any1 = false;

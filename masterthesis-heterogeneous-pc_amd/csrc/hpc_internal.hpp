@@ -49,7 +49,6 @@ struct InstCounters {
 
 enum InstFlags {
   kActive = 1,   // takes part in this phase
-  kHasNan = 2,   // observations hold NaN cells
 };
 
 // Device-side descriptor of one instance.
@@ -126,7 +125,7 @@ struct InstHost {
   unsigned long long *adj = nullptr;
   int32_t *rec = nullptr;
   size_t rec_cap = 0, rec_used = 0;  // sepset record words
-  bool have_x = false, have_cor = false, x_has_nan = false;
+  bool have_x = false, have_cor = false;
   bool drop_obs = false;   // free observations and z-scores once correlated
   // level state
   int start = 0;           // global id of local variable 0 in the last pack
@@ -204,9 +203,13 @@ void engine_free(Engine &e);
 // New instance with device blocks and the initial adjacency; returns its id.
 int engine_add(Engine &e, int p, int n, double alpha, int max_level, int *id);
 void engine_remove(Engine &e, int id);
-// Uploads [p][n] observations; nan_hint 0/1 says whether they hold NaN
-// cells (-1: scan them here).
-int engine_set_observations(Engine &e, int id, const double *h_x, int nan_hint = -1);
+// Uploads [p][n] observations (NaN cells are detected on the device).
+int engine_set_observations(Engine &e, int id, const double *h_x);
+// The two halves of an upload done elsewhere (another stream or thread):
+// device blocks for the observations (*x receives the [p][n] block), then,
+// once the copy has completed, the instance is marked ready.
+int engine_alloc_observations(Engine &e, int id, double **x);
+void engine_observations_ready(Engine &e, int id);
 int engine_set_corr(Engine &e, int id, const double *h_cor);
 int engine_reset(Engine &e, int id);
 // Instance table of `ids` with the given per-instance levels; every phase

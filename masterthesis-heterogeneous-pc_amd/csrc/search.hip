@@ -417,19 +417,27 @@ __global__ __launch_bounds__(256) void commit_kernel(Layout L, long long n_edges
         U = total;
         E = 1;
       }
-      if (j > i) {
-        const int bj = L.rowptr[j];
-        const int yi = (int)(e - bi);
-        const int lenj = L.rowptr[j + 1] - bj;
-        const int yj = find_pos(L.nbr + bj, lenj, i);
-        // the adjacency is symmetric, so i is in N(j); guard anyway
-        const long long fj = (yj < lenj && L.nbr[bj + yj] == i) ? L.fp[bj + yj] : -1;
-        int row = -1;
-        if (fi >= 0) { row = i; y = yi; base = bi; rank = (unsigned long long)fi; }
-        else if (fj >= 0) { row = j; y = yj; base = bj; rank = (unsigned long long)fj; }
-        nn = row >= 0 ? L.rowptr[row + 1] - base - 1 : 0;
-        // a rank valid for this level and row (so the unranking below ends)
-        rm = row >= 0 && rank < binom(L.binom, kBinomK, nn, level);
+      // the lower-id row's lowest passing set wins, else the higher-id
+      // row's: row i records when it passed and either it is the lower id or
+      // row j did not pass (the only case that needs the reverse edge)
+      if (fi >= 0) {
+        bool mine = true;
+        if (j < i) {
+          const int bj = L.rowptr[j];
+          const int lenj = L.rowptr[j + 1] - bj;
+          const int yj = find_pos(L.nbr + bj, lenj, i);
+          // the adjacency is symmetric, so i is in N(j); guard anyway
+          const long long fj = (yj < lenj && L.nbr[bj + yj] == i) ? L.fp[bj + yj] : -1;
+          mine = fj < 0;
+        }
+        if (mine) {
+          y = (int)(e - bi);
+          base = bi;
+          rank = (unsigned long long)fi;
+          nn = nni;
+          // a rank valid for this level and row (so the unranking below ends)
+          rm = rank < total;
+        }
       }
     }
   }
@@ -463,8 +471,8 @@ __global__ __launch_bounds__(256) void commit_kernel(Layout L, long long n_edges
     atomicAnd(&In.adj[(size_t)li * W + (lj >> 6)], ~(1ull << (lj & 63)));
     atomicAnd(&In.adj[(size_t)lj * W + (li >> 6)], ~(1ull << (li & 63)));
     int32_t *out = In.rec + In.rec_base + off;
-    out[0] = li;
-    out[1] = lj;
+    out[0] = min(li, lj);
+    out[1] = max(li, lj);
     out[2] = level;
     int val = 0;
     for (int m = 0; m < level; ++m) {

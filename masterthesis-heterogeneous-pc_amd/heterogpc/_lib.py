@@ -117,6 +117,7 @@ SIGNATURES = {
     "hpc_server_create": (_i, [_i, _u64, _i, C.POINTER(_vp)]),
     "hpc_server_destroy": (_i, [_vp]),
     "hpc_server_submit": (_i, [_vp, _u64, _i, _d, _i, _vp, C.POINTER(_i64)]),
+    "hpc_server_submit_borrowed": (_i, [_vp, _u64, _i, _d, _i, _vp, C.POINTER(_i64)]),
     "hpc_server_wait": (_i, [_vp, _i64, _d, C.POINTER(C.c_int)]),
     "hpc_server_result": (_i, [_vp, _i64, _vp, _vp, _i, C.POINTER(RequestInfo)]),
     "hpc_server_sepset_list": (_i, [_vp, _i64, _vp, _i64, C.POINTER(_i64)]),

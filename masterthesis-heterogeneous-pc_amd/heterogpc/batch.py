@@ -123,11 +123,13 @@ class Server:
                                          C.byref(h)))
         self.h = h
         self.shape: dict[int, int] = {}
+        self._borrowed: dict[int, np.ndarray] = {}
 
     def close(self) -> None:
         if self.h:
             check(self.lib.hpc_server_destroy(self.h))
             self.h = C.c_void_p()
+            self._borrowed.clear()
 
     def __enter__(self):
         return self
@@ -141,13 +143,18 @@ class Server:
         except Exception:
             pass
 
-    def submit(self, x: np.ndarray, alpha: float, max_level: int = 4) -> int:
+    def submit(self, x: np.ndarray, alpha: float, max_level: int = 4, copy: bool = False) -> int:
+        """Queues a skeleton request on observations x ([p][n] float64).
+        Without copy the server reads x in place; the wrapper keeps a
+        reference to it until the request's result is released."""
         x = np.ascontiguousarray(x, dtype=np.float64)
         p, n = x.shape
         req = C.c_int64()
-        check(self.lib.hpc_server_submit(self.h, p, n, float(alpha), int(max_level), _ptr(x),
-                                         C.byref(req)))
+        fn = self.lib.hpc_server_submit if copy else self.lib.hpc_server_submit_borrowed
+        check(fn(self.h, p, n, float(alpha), int(max_level), _ptr(x), C.byref(req)))
         self.shape[req.value] = p
+        if not copy:
+            self._borrowed[req.value] = x
         return req.value
 
     def wait(self, req: int, timeout_s: float = -1.0) -> bool:
@@ -180,6 +187,7 @@ class Server:
         if release:
             check(self.lib.hpc_server_release(self.h, int(req)))
             self.shape.pop(req, None)
+            self._borrowed.pop(req, None)
         return adj, sep, {name: getattr(info, name) for name, _ in info._fields_}
 
     def stats(self) -> dict:
