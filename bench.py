@@ -254,6 +254,64 @@ def bench_served(heterogpc, trace, load):
     }
 
 
+FP32_MFMA_PEAK_TFLOPS = 157.3
+
+
+def bench_precision(heterogpc, x, n, alpha, max_level, steps=3):
+    """Precision form: the same skeleton from the exact FP64 correlation and
+    from the MFMA FP32 one with exact rechecks.  Reports the correlation
+    kernels' time and throughput, the max |fast - exact| correlation error,
+    the rechecks, and whether the skeletons are identical."""
+    p = x.shape[0]
+    out = {}
+    res = {}
+    for mode, name in ((0, "fp64_exact"), (1, "f32_mfma")):
+        with heterogpc.DeviceSkeleton(p, n, alpha, max_level=max_level) as dev:
+            dev.set_observations(x)
+            dev.corr(mode)
+            dev.run(-2)  # warm-up
+            cors, e2e, lev = [], [], []
+            for _ in range(steps):
+                dev.reset()
+                t0 = time.perf_counter()
+                dev.corr(mode)
+                ms_m, ms_c = dev.corr_ms()
+                t1 = time.perf_counter()
+                st = dev.run(-2)
+                dev.sync()
+                t2 = time.perf_counter()
+                cors.append(ms_c)
+                e2e.append((t2 - t0) * 1e3)
+                lev.append((t2 - t1) * 1e3)
+            dev.reset()
+            dev.corr(mode)
+            cor = dev.cor()
+            st = dev.run(-2)
+            res[name] = (cor, dev.adj(), dev.sepset_list())
+            flops = 2.0 * n * p * (p + 1) / 2.0
+            ms_c = float(np.median(cors))
+            out[name] = {"corr_ms": ms_c, "corr_tflops": flops / (ms_c * 1e-3) / 1e12,
+                         "levels_ms": float(np.median(lev)), "e2e_ms": float(np.median(e2e)),
+                         "tests": int(sum(l["tests"] for l in st)),
+                         "rechecks": int(sum(l["rechecks"] for l in st))}
+    out["f32_mfma"]["corr_frac_of_fp32_mfma_peak"] = \
+        out["f32_mfma"]["corr_tflops"] / FP32_MFMA_PEAK_TFLOPS
+    out["fp64_exact"]["corr_frac_of_fp64_peak"] = \
+        out["fp64_exact"]["corr_tflops"] / FP64_PEAK_TFLOPS
+    ce, cf = res["fp64_exact"][0], res["f32_mfma"][0]
+    off = ~np.eye(p, dtype=bool)
+    d = np.abs(cf - ce)[off]
+    big = np.abs(ce)[off] >= 0.1
+    out["max_abs_cor_error"] = float(np.nanmax(d))
+    out["max_rel_cor_error_abs_r_ge_0.1"] = float(np.nanmax(d[big] / np.abs(ce[off][big])))
+    out["cor_error_bound"] = 74.0 * 2.0 ** -24 * 1.01
+    out["skeleton_identical"] = bool(np.array_equal(res["fp64_exact"][1], res["f32_mfma"][1]) and
+                                     np.array_equal(res["fp64_exact"][2], res["f32_mfma"][2]))
+    out["workload"] = ("SEM p=%d, n=%d, expected degree 4, weights +-U[0.05,1], alpha=%g, FP32 "
+                       "data, levels 0..%d" % (p, n, alpha, max_level))
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -265,6 +323,8 @@ def main() -> int:
                     help="instances of the batched form (0 = skip)")
     ap.add_argument("--served-count", type=int, default=200,
                     help="requests of the served form (0 = skip)")
+    ap.add_argument("--precision", type=int, default=1,
+                    help="run the precision form (p=4000 SEM, exact vs MFMA FP32)")
     ap.add_argument("--served-load", type=float, default=0.7,
                     help="arrival rate of the served form as a fraction of the plain "
                          "path's measured capacity")
@@ -361,6 +421,11 @@ def main() -> int:
         "corr_roofline": corr_roof,
     }
     dev.close()
+    if args.precision:
+        print("[bench] precision form", file=sys.stderr, flush=True)
+        xc, nc, ac, _ = workloads.sem("c3")
+        out["precision"] = bench_precision(heterogpc, xc, nc, ac, 4)
+        del xc
     if args.batch_count > 0:
         print("[bench] batched form", file=sys.stderr, flush=True)
         out["batched"] = bench_batched(heterogpc, workloads.batch_mix(args.batch_count))

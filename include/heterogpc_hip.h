@@ -42,7 +42,15 @@ enum {
 
 /* Correlation modes for hpc_corr / hpc_pmcc. */
 enum {
-  HPC_CORR_CANON = 0 /* FP64, the reference's summation order (bit-exact) */
+  HPC_CORR_CANON = 0, /* FP64, the reference's summation order (bit-exact) */
+  HPC_CORR_F32 = 1    /* z-scores rounded to float, products on MFMA with FP32
+                         accumulation over 64-sample chunks and FP64 across
+                         chunks; |cor - exact| <= 74 * 2^-24 * 1.01.  Every test
+                         whose statistic lies within its first-order error bound
+                         of the threshold (every level-0 pair within that bound)
+                         is re-run on exact FP64 correlations, so the adjacency,
+                         sepsets and test counts equal HPC_CORR_CANON's.
+                         Instances with NaN cells use the exact kernels. */
 };
 
 typedef struct hpc_state hpc_state;
@@ -63,6 +71,7 @@ typedef struct hpc_level_stats {
   double ms_test;        /* device time of the CI-test kernel(s) */
   double ms_commit;      /* device time of the removal kernel */
   double ms_wall;        /* host wall time of the whole level */
+  int64_t rechecks;      /* tests re-run on exact correlations (HPC_CORR_F32) */
 } hpc_level_stats;
 
 /* ---- runtime ----------------------------------------------------------- */
@@ -101,8 +110,11 @@ int hpc_max_cond_size(const hpc_state *st);
 
 /* Launch-shape knobs (set by the autotuner; results do not depend on them). */
 enum {
-  HPC_TUNE_RANKS_PER_TASK = 1 /* conditioning sets per search task at levels
-                                 >= 2: a multiple of 64, default 512 */
+  HPC_TUNE_RANKS_PER_TASK = 1, /* conditioning sets per search task at levels
+                                  >= 2: a multiple of 64, default 512 */
+  HPC_TUNE_RECHECK_QUEUE = 2   /* HPC_CORR_F32: length of the exact-recheck
+                                  queues (0 = 1 Mi); a level whose queue
+                                  overflows is redone on exact correlations */
 };
 int hpc_set_tuning(hpc_state *st, int param, int value);
 
@@ -207,6 +219,8 @@ int hpc_batch_get_sepsets(hpc_batch *b, int id, int32_t *sep, int mc);
 int hpc_batch_get_sepset_list(hpc_batch *b, int id, int32_t *out, int64_t cap,
                               int64_t *words);
 int hpc_batch_set_tuning(hpc_batch *b, int param, int value);
+/* Correlation mode (HPC_CORR_*) of an instance, before its first step. */
+int hpc_batch_set_corr_mode(hpc_batch *b, int id, int mode);
 
 /* ---- continuous-batching server -------------------------------------------
  * No counterpart in the reference.  A worker thread admits submitted requests
@@ -258,6 +272,8 @@ int hpc_server_result(hpc_server *s, int64_t req, int32_t *adj, int32_t *sep,
 /* Sepset list of a finished request (out may be NULL to query *words). */
 int hpc_server_sepset_list(hpc_server *s, int64_t req, int32_t *out, int64_t cap,
                            int64_t *words);
+/* Correlation mode (HPC_CORR_*) of the requests submitted from now on. */
+int hpc_server_set_corr_mode(hpc_server *s, int mode);
 /* Drops a finished request's results. */
 int hpc_server_release(hpc_server *s, int64_t req);
 int hpc_server_stats(hpc_server *s, hpc_server_info *info);

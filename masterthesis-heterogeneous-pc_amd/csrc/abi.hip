@@ -175,6 +175,10 @@ extern "C" int hpc_set_tuning(hpc_state *st, int param, int value) {
         return fail(HPC_EINVAL, "ranks per task must be a multiple of 64 in [64, 2^24]");
       st->eng.ranks_per_task = value;
       return HPC_OK;
+    case HPC_TUNE_RECHECK_QUEUE:
+      if (value < 0) return fail(HPC_EINVAL, "recheck queue length must be >= 0");
+      st->eng.rchk_limit = (size_t)value;
+      return HPC_OK;
     default:
       return fail(HPC_EINVAL, "unknown tuning parameter %d", param);
   }
@@ -196,8 +200,11 @@ extern "C" int hpc_set_observations(hpc_state *st, const double *h_mat) {
 extern "C" int hpc_corr(hpc_state *st, int mode) {
   if (!st) return fail(HPC_EINVAL, "state is NULL");
   if (!H(st).have_x) return fail(HPC_ESTATE, "no observations uploaded");
-  if (mode != HPC_CORR_CANON) return fail(HPC_EINVAL, "unknown correlation mode %d", mode);
-  int rc = pack_at(st, 0);
+  if (mode != HPC_CORR_CANON && mode != HPC_CORR_F32)
+    return fail(HPC_EINVAL, "unknown correlation mode %d", mode);
+  H(st).corr_mode = mode;
+  int rc = engine_prepare_corr(st->eng, st->id);
+  if (!rc) rc = pack_at(st, 0);
   if (!rc) rc = engine_corr(st->eng, &st->ms_moments, &st->ms_corr);
   if (rc) return rc;
   st->corr_timed = true;
@@ -291,6 +298,7 @@ extern "C" int hpc_search_level(hpc_state *st, int level, const int32_t *rows, i
                         st->eng.stream));
   HIPCHK(hipStreamSynchronize(st->eng.stream));
   s.pinv_fallbacks = (int64_t)st->eng.h_ctr->pinv_fallbacks;
+  s.rechecks = (int64_t)st->eng.rechecks;
   s.ms_test = ms;
   s.ms_wall = ms_since(t0);
   if (stats) *stats = s;

@@ -170,6 +170,17 @@ extern "C" int hpc_batch_get_cor(hpc_batch *b, int id, double *cor) {
   return export_cor(b->eng, id, cor);
 }
 
+extern "C" int hpc_batch_set_corr_mode(hpc_batch *b, int id, int mode) {
+  int rc = check_id(b, id);
+  if (rc) return rc;
+  if (mode != HPC_CORR_CANON && mode != HPC_CORR_F32)
+    return hpc_fail(HPC_EINVAL, "unknown correlation mode %d", mode);
+  InstHost &h = b->eng.inst[id];
+  if (h.have_cor) return hpc_fail(HPC_ESTATE, "instance %d already has its correlation", id);
+  h.corr_mode = mode;
+  return HPC_OK;
+}
+
 extern "C" int hpc_batch_set_tuning(hpc_batch *b, int param, int value) {
   if (!b) return hpc_fail(HPC_EINVAL, "batch is NULL");
   if (param != HPC_TUNE_RANKS_PER_TASK || value < 64 || value % 64 != 0 || value > (1 << 24))
@@ -193,6 +204,7 @@ struct Request {
   const double *src = nullptr;  // observations to upload: x, or the caller's
   size_t count = 0;
   double *dst = nullptr;        // device block of the observations
+  int corr_mode = HPC_CORR_CANON;
   hipError_t up_err = hipSuccess;
   size_t bytes = 0;
   int slot = -1;       // engine instance while admitted
@@ -220,6 +232,7 @@ struct hpc_server {
   std::map<int64_t, std::shared_ptr<Request>> all;
   int64_t next_id = 1;
   int64_t submits = 0;
+  int corr_mode = HPC_CORR_CANON;  // of requests admitted from now on
   bool stop = false;
   std::thread worker;
   // observation uploads: the uploader thread copies admitted requests' data
@@ -320,6 +333,7 @@ static void server_loop(hpc_server *s) {
       double *dst = nullptr;
       int rc = engine_add(s->eng, r->p, r->n, r->alpha, r->max_level, &slot);
       if (!rc) {
+        s->eng.inst[slot].corr_mode = r->corr_mode;
         rc = engine_alloc_observations(s->eng, slot, &dst);
         if (rc) engine_remove(s->eng, slot);
       }
@@ -462,6 +476,7 @@ static int server_submit(hpc_server *s, uint64_t p, int n, double alpha, int max
   r->alpha = alpha;
   r->max_level = max_level;
   r->count = (size_t)p * n;
+  r->corr_mode = s->corr_mode;
   if (borrow) {
     r->src = x;
   } else {
@@ -551,6 +566,15 @@ extern "C" int hpc_server_sepset_list(hpc_server *s, int64_t req, int32_t *out, 
     if ((int64_t)r->rec.size() > cap) return hpc_fail(HPC_EINVAL, "cap too small");
     std::copy(r->rec.begin(), r->rec.end(), out);
   }
+  return HPC_OK;
+}
+
+extern "C" int hpc_server_set_corr_mode(hpc_server *s, int mode) {
+  if (!s) return hpc_fail(HPC_EINVAL, "server is NULL");
+  if (mode != HPC_CORR_CANON && mode != HPC_CORR_F32)
+    return hpc_fail(HPC_EINVAL, "unknown correlation mode %d", mode);
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->corr_mode = mode;
   return HPC_OK;
 }
 

@@ -78,10 +78,13 @@ __global__ __launch_bounds__(256) void moments_kernel(Layout L) {
   }
   double *za = I.z + (size_t)a * npad;
   uint8_t *na = I.nanm + (size_t)a * npad;
+  float *z32 = I.z32 ? I.z32 + (size_t)a * npad : nullptr;
   for (int k = lane; k < npad; k += 64) {
     double v = k < n ? xa[k] : 0.0;
-    za[k] = k < n ? (v - mu) / sd : 0.0;
+    const double zk = k < n ? (v - mu) / sd : 0.0;
+    za[k] = zk;
     na[k] = (k < n && __builtin_isnan(v)) ? 1 : 0;
+    if (z32) z32[k] = (float)zk;
   }
 }
 
@@ -188,7 +191,9 @@ __global__ __launch_bounds__(256) void corr_tile_kernel(Layout L, const long lon
   const long long Lg = (g % kXcds) * per_xcd + g / kXcds;
   if (Lg >= prefix[L.K]) return;
   const int k = find_inst(prefix, L.K, Lg);
-  if (L.ictr[k].has_nan) return;  // the masked kernel takes it
+  // instances with NaN cells go to the masked kernel, fast ones to the MFMA
+  // kernel
+  if (L.ictr[k].has_nan || L.inst[k].fast) return;
   const Inst I = L.inst[k];
   const int p = I.p, npad = I.npad;
   const int nt = (p + CT - 1) / CT;
@@ -259,6 +264,109 @@ __global__ __launch_bounds__(256) void corr_tile_kernel(Layout L, const long lon
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fast form (HPC_CORR_F32): z-scores rounded to float, 64 x 64 output tiles
+// of the upper triangle per workgroup, each wavefront a 32 x 32 quarter as
+// 2 x 2 MFMA 16x16x4 FP32 blocks.  Samples stream through LDS 64 at a time;
+// the FP32 accumulators are added into FP64 ones after every 64 samples, so
+// the error of an entry is at most (64 + 8 + 2) * 2^-24 of sum |z_a z_b| /
+// (n - 1) <= 1.01 (the +2: rounding of the two operands).
+//
+// v_mfma_f32_16x16x4f32 lane layout: lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16]; it receives D[4 * (l / 16) + r][l % 16] in register r.
+// ---------------------------------------------------------------------------
+constexpr int FT = 64;   // output tile side
+constexpr int FK = 64;   // samples per LDS stage and per FP32 accumulation
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void corr_mfma_kernel(Layout L, const long long *__restrict__ prefix,
+                                                        long long per_xcd) {
+  __shared__ float sa[FT][FK + 4];
+  __shared__ float sb[FT][FK + 4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const long long g = blockIdx.x;
+  const long long Lg = (g % kXcds) * per_xcd + g / kXcds;
+  if (Lg >= prefix[L.K]) return;
+  const int k = find_inst(prefix, L.K, Lg);
+  if (L.ictr[k].has_nan) return;  // the masked kernel takes it
+  const Inst I = L.inst[k];
+  const int p = I.p, npad = I.npad;
+  const int nt = (p + FT - 1) / FT;
+  // upper-triangle tile (TA <= TB), row-major over TA
+  const long long t = Lg - prefix[k];
+  int TA = (int)((2.0 * nt + 1.0 - sqrt((2.0 * nt + 1.0) * (2.0 * nt + 1.0) - 8.0 * (double)t)) * 0.5);
+  if (TA < 0) TA = 0;
+  while (TA > 0 && (long long)TA * nt - (long long)TA * (TA - 1) / 2 > t) --TA;
+  while ((long long)(TA + 1) * nt - (long long)(TA + 1) * TA / 2 <= t) ++TA;
+  const int TBi = TA + (int)(t - ((long long)TA * nt - (long long)TA * (TA - 1) / 2));
+  const int a0 = TA * FT, b0 = TBi * FT;
+  f4v acc32[2][2];
+  double acc64[2][2][4];
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc64[u][v][r] = 0.0;
+  const int li = lane & 15, lk = lane >> 4;
+  for (int k0 = 0; k0 < npad; k0 += FK) {
+    // 64 rows x 64 samples of each operand: 1024 float4 per operand
+    for (int idx = tid; idx < FT * FK / 4; idx += blockDim.x) {
+      const int r = idx / (FK / 4), c4 = (idx % (FK / 4)) * 4;
+      const int ra = min(a0 + r, p - 1), rb = min(b0 + r, p - 1);
+      const f4v va = *reinterpret_cast<const f4v *>(I.z32 + (size_t)ra * npad + k0 + c4);
+      const f4v vb = *reinterpret_cast<const f4v *>(I.z32 + (size_t)rb * npad + k0 + c4);
+      *reinterpret_cast<f4v *>(&sa[r][c4]) = va;
+      *reinterpret_cast<f4v *>(&sb[r][c4]) = vb;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int v = 0; v < 2; ++v) acc32[u][v] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+    for (int kk = 0; kk < FK; kk += 4) {
+      const float x0 = sa[wr * 32 + li][kk + lk];
+      const float x1 = sa[wr * 32 + 16 + li][kk + lk];
+      const float y0 = sb[wc * 32 + li][kk + lk];
+      const float y1 = sb[wc * 32 + 16 + li][kk + lk];
+      acc32[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(x0, y0, acc32[0][0], 0, 0, 0);
+      acc32[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(x0, y1, acc32[0][1], 0, 0, 0);
+      acc32[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(x1, y0, acc32[1][0], 0, 0, 0);
+      acc32[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(x1, y1, acc32[1][1], 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int v = 0; v < 2; ++v)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc64[u][v][r] += (double)acc32[u][v][r];
+    __syncthreads();
+  }
+  const double div = (double)((float)I.n - 1.f);
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int a = a0 + wr * 32 + u * 16 + 4 * lk + r;
+        const int b = b0 + wc * 32 + v * 16 + li;
+        if (a < p && b < p && a <= b) {
+          const double c = acc64[u][v][r] / div;
+          I.cor[(size_t)a * p + b] = c;
+          I.cor[(size_t)b * p + a] = c;
+        }
+      }
+}
+
+long long mfma_tiles(int p) {
+  const long long nt = (p + FT - 1) / FT;
+  return nt * (nt + 1) / 2;
+}
+
 long long tiled_slots(int p) {
   const int nt = (p + CT - 1) / CT;
   const int ns = (nt + ST - 1) / ST;
@@ -304,6 +412,24 @@ void launch_corr(hipStream_t s, const Layout &L, const std::vector<Inst> &host_i
     hipLaunchKernelGGL(corr_canon_kernel, dim3((unsigned)blocks), dim3(64 * wpb), 0, s, L,
                        d_prefix + K + 1);
   }
+}
+
+void launch_corr_fast(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst,
+                      long long *d_prefix, std::vector<long long> &h_prefix) {
+  const int K = L.K;
+  h_prefix.assign(K + 1, 0);
+  for (int k = 0; k < K; ++k) {
+    const Inst &I = host_inst[k];
+    const bool on = (I.flags & kActive) && I.fast;
+    h_prefix[k + 1] = h_prefix[k] + (on ? mfma_tiles(I.p) : 0);
+  }
+  const long long tiles = h_prefix[K];
+  if (tiles == 0) return;
+  (void)hipMemcpyAsync(d_prefix, h_prefix.data(), h_prefix.size() * sizeof(long long),
+                       hipMemcpyHostToDevice, s);
+  const long long per_xcd = (tiles + kXcds - 1) / kXcds;
+  hipLaunchKernelGGL(corr_mfma_kernel, dim3((unsigned)(per_xcd * kXcds)), dim3(256), 0, s, L,
+                     d_prefix, per_xcd);
 }
 
 }  // namespace hpc

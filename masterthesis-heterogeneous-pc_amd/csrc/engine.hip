@@ -157,7 +157,7 @@ int engine_init(Engine &e, int device) {
 }
 
 static void free_obs(Engine &e, InstHost &h) {
-  ifree(e, h.x); ifree(e, h.z); ifree(e, h.nanm);
+  ifree(e, h.x); ifree(e, h.z); ifree(e, h.nanm); ifree(e, h.z32);
 }
 
 static void free_blocks(Engine &e, InstHost &h) {
@@ -177,7 +177,7 @@ void engine_free(Engine &e) {
   dfree(e.d_inst); dfree(e.d_ictr); dfree(e.d_row_inst); dfree(e.d_deg); dfree(e.d_rowptr);
   dfree(e.d_nbr); dfree(e.d_erow); dfree(e.d_rowmask); dfree(e.d_fp); dfree(e.d_row_tasks);
   dfree(e.d_task_off); dfree(e.d_binom); dfree(e.d_prefix); dfree(e.d_ctr);
-  dfree(e.d_scatter_idx); dfree(e.d_scatter_val);
+  dfree(e.d_scatter_idx); dfree(e.d_scatter_val); dfree(e.d_rchk); dfree(e.d_pairs);
   if (e.h_ctr) (void)hipHostFree(e.h_ctr);
   if (e.h_ictr) (void)hipHostFree(e.h_ictr);
   e.h_ctr = nullptr;
@@ -318,6 +318,7 @@ int engine_set_corr(Engine &e, int id, const double *h_cor) {
                             hipMemcpyHostToDevice, e.stream));
   HPC_HIPCHK(hipStreamSynchronize(e.stream));
   h.have_cor = true;
+  h.fast = false;
   e.h_cor_valid = false;
   return HPC_OK;
 }
@@ -392,6 +393,18 @@ static int ensure_edge_cap(Engine &e, size_t edges) {
   return rc;
 }
 
+// Recheck queues of fast instances (1 Mi entries each; an overflow voids the
+// phase and the host redoes it with exact correlations).
+static int ensure_recheck_cap(Engine &e) {
+  const size_t cap = (size_t)1 << 20;
+  if (e.d_rchk && e.d_pairs) return HPC_OK;
+  int rc = dalloc((void **)&e.d_rchk, cap * sizeof(Recheck));
+  if (!rc) rc = dalloc((void **)&e.d_pairs, cap * sizeof(int2));
+  if (rc) return rc;
+  e.cap_rchk = e.cap_pairs = cap;
+  return HPC_OK;
+}
+
 int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &levels) {
   HPC_HIPCHK(hipSetDevice(e.device));
   // the previous phase's uploads read these host vectors
@@ -408,6 +421,9 @@ int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &
   if (!rc) rc = ensure_row_cap(e, (size_t)p_total + 1);
   if (!rc) rc = ensure_edge_cap(e, edges + 1);
   if (!rc) rc = ensure_binom(e, pmax + 1);
+  bool any_fast = false;
+  for (int id : ids) any_fast |= e.inst[id].fast;
+  if (!rc && any_fast) rc = ensure_recheck_cap(e);
   if (rc) return rc;
   e.h_inst.assign(K, Inst());
   e.h_row_inst.resize(p_total);
@@ -417,6 +433,9 @@ int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &
     Inst &I = e.h_inst[k];
     I.x = h.x; I.z = h.z; I.nanm = h.nanm; I.mean = h.mean; I.sd = h.sd; I.cor = h.cor;
     I.adj = h.adj; I.rec = h.rec; I.rec_base = h.rec_used;
+    I.z32 = h.z32;
+    I.fast = h.fast ? 1 : 0;
+    I.eps = kFastEps;
     I.r_crit = h.r_crit;
     I.start = start;
     I.p = h.p; I.n = h.n; I.npad = h.npad; I.W = h.W;
@@ -436,6 +455,12 @@ int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &
   return HPC_OK;
 }
 
+static bool any_fast_at(const Engine &e, int level) {
+  for (const Inst &I : e.h_inst)
+    if (I.fast && (level < 0 || I.level == level)) return true;
+  return false;
+}
+
 static Layout layout(Engine &e) {
   Layout L;
   L.inst = e.d_inst;
@@ -452,6 +477,10 @@ static Layout layout(Engine &e) {
   L.fp = e.d_fp;
   L.binom = e.d_binom;
   L.ctr = e.d_ctr;
+  L.rchk = e.d_rchk;
+  L.rchk_cap = e.rchk_limit ? std::min(e.rchk_limit, e.cap_rchk) : e.cap_rchk;
+  L.pairs = e.d_pairs;
+  L.pairs_cap = e.rchk_limit ? std::min(e.rchk_limit, e.cap_pairs) : e.cap_pairs;
   return L;
 }
 
@@ -502,14 +531,55 @@ int engine_corr(Engine &e, float *ms_moments, float *ms_corr) {
   HPC_HIPCHK(hipEventRecord(e.ev[1], e.stream));
   launch_corr(e.stream, L, e.h_inst, e.d_prefix, e.h_prefix);
   HPC_HIPCHK(hipGetLastError());
+  if (any_fast_at(e, -1)) {
+    launch_corr_fast(e.stream, L, e.h_inst, e.d_prefix + 3 * (L.K + 1), e.h_prefix_fast);
+    HPC_HIPCHK(hipGetLastError());
+  }
   HPC_HIPCHK(hipEventRecord(e.ev[2], e.stream));
   for (int id : e.packed) e.inst[id].have_cor = true;
   e.h_cor_valid = false;
+  if (any_fast_at(e, -1)) {
+    // instances with NaN cells took the exact masked kernel; the float
+    // z-scores are no longer needed
+    rc = engine_read_inst_counters(e);
+    if (rc) return rc;
+    for (size_t k = 0; k < e.packed.size(); ++k) {
+      InstHost &h = e.inst[e.packed[k]];
+      if (!h.fast) continue;
+      if (e.h_ictr[k].has_nan) h.fast = false;
+      ifree(e, h.z32);
+    }
+  }
   if (ms_moments || ms_corr) {
     HPC_HIPCHK(hipEventSynchronize(e.ev[2]));
     if (ms_moments) *ms_moments = elapsed(e.ev[0], e.ev[1]);
     if (ms_corr) *ms_corr = elapsed(e.ev[1], e.ev[2]);
   }
+  return HPC_OK;
+}
+
+int engine_prepare_corr(Engine &e, int id) {
+  InstHost &h = e.inst[id];
+  h.fast = h.corr_mode == HPC_CORR_F32;
+  if (h.fast && !h.z32) return ialloc(e, h.z32, (size_t)h.p * h.npad);
+  return HPC_OK;
+}
+
+int engine_exact_from_z(Engine &e, const std::vector<int> &ids) {
+  std::vector<int> redo;
+  for (int id : ids)
+    if (e.inst[id].fast) {
+      e.inst[id].fast = false;
+      redo.push_back(id);
+    }
+  if (redo.empty()) return HPC_OK;
+  int rc = engine_pack(e, redo, std::vector<int>(redo.size(), 0));
+  if (!rc) rc = zero_counters(e);
+  if (rc) return rc;
+  Layout L = layout(e);
+  launch_corr(e.stream, L, e.h_inst, e.d_prefix, e.h_prefix);
+  HPC_HIPCHK(hipGetLastError());
+  HPC_HIPCHK(hipStreamSynchronize(e.stream));
   return HPC_OK;
 }
 
@@ -519,11 +589,21 @@ int engine_level0(Engine &e, float *ms) {
   if (rc) return rc;
   Layout L = layout(e);
   HPC_HIPCHK(hipEventRecord(e.ev[3], e.stream));
+  if (any_fast_at(e, 0)) launch_refine_l0(e.stream, L);
   launch_level0(e.stream, L, e.h_inst, e.d_prefix + 2 * (L.K + 1), e.h_prefix);
   HPC_HIPCHK(hipGetLastError());
   HPC_HIPCHK(hipEventRecord(e.ev[4], e.stream));
-  rc = engine_read_inst_counters(e);
+  rc = read_counters(e);
   if (rc) return rc;
+  if (e.h_ctr->overflow) {
+    // too many pairs near the threshold: exact correlations, then level 0 again
+    const std::vector<int> ids = e.packed;
+    const std::vector<int> lv(ids.size(), 0);
+    rc = engine_exact_from_z(e, ids);
+    if (!rc) rc = engine_pack(e, ids, lv);
+    if (rc) return rc;
+    return engine_level0(e, ms);
+  }
   if (ms) *ms = elapsed(e.ev[3], e.ev[4]);
   for (size_t k = 0; k < e.packed.size(); ++k) {
     InstHost &h = e.inst[e.packed[k]];
@@ -565,6 +645,7 @@ int engine_compact(Engine &e, float *ms) {
 
 int engine_search(Engine &e, int level, const int32_t *rows, int nrows, float *ms) {
   HPC_HIPCHK(hipSetDevice(e.device));
+  e.rechecks = 0;
   Layout L = layout(e);
   const int32_t *mask = nullptr;
   if (rows) {
@@ -586,12 +667,33 @@ int engine_search(Engine &e, int level, const int32_t *rows, int nrows, float *m
     HPC_HIPCHK(hipGetLastError());
   }
   HPC_HIPCHK(hipEventRecord(e.ev[5], e.stream));
+  const bool fast = any && any_fast_at(e, level);
   if (any) {
-    if (level == 1) launch_search_l1(e.stream, L, e.n_edges, mask);
-    else launch_search_ln(e.stream, level, L, e.ranks_per_task, e.d_row_tasks, e.d_task_off);
+    if (level == 1) launch_search_l1(e.stream, L, e.n_edges, mask, fast);
+    else launch_search_ln(e.stream, level, L, e.ranks_per_task, e.d_row_tasks, e.d_task_off, fast);
+    if (fast) launch_recheck(e.stream, level, L);
   }
   HPC_HIPCHK(hipGetLastError());
   HPC_HIPCHK(hipEventRecord(e.ev[6], e.stream));
+  if (fast) {
+    int rc = read_counters(e);
+    if (rc) return rc;
+    e.rechecks = e.h_ctr->rechecks;
+    if (e.h_ctr->overflow) {
+      // too many tests near the threshold: exact correlations, then the
+      // search again from scratch
+      const std::vector<int> ids = e.packed;
+      std::vector<int> lv;
+      for (const Inst &I : e.h_inst) lv.push_back(I.level);
+      rc = engine_exact_from_z(e, ids);
+      if (!rc) rc = engine_pack(e, ids, lv);
+      if (rc) return rc;
+      HPC_HIPCHK(hipMemsetAsync(e.d_ctr, 0, sizeof(Counters), e.stream));
+      if (e.n_edges > 0)
+        HPC_HIPCHK(hipMemsetAsync(e.d_fp, 0xFF, (size_t)e.n_edges * sizeof(int64_t), e.stream));
+      return engine_search(e, level, rows, nrows, ms);
+    }
+  }
   if (ms) {
     HPC_HIPCHK(hipEventSynchronize(e.ev[6]));
     *ms = elapsed(e.ev[5], e.ev[6]);
@@ -671,6 +773,10 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
     zero.push_back(0);
   }
   if (!need.empty()) {
+    for (int id : need) {
+      rc = engine_prepare_corr(e, id);
+      if (rc) return rc;
+    }
     rc = engine_pack(e, need, zero);
     if (!rc) rc = engine_corr(e, nullptr, nullptr);
     if (rc) return rc;
@@ -693,15 +799,22 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
   for (int id : need)
     if (e.inst[id].drop_obs) {
       InstHost &h = e.inst[id];
-      free_obs(e, h);
+      if (h.fast) {
+        // exact rechecks read the z-scores until the instance finishes
+        ifree(e, h.x); ifree(e, h.nanm);
+        h.bytes -= std::min(h.bytes, (size_t)h.p * h.n * 8 + (size_t)h.p * h.npad);
+      } else {
+        free_obs(e, h);
+        h.bytes -= std::min(h.bytes, (size_t)h.p * h.n * 8 + (size_t)h.p * h.npad * 9);
+      }
       h.have_x = false;
-      h.bytes -= std::min(h.bytes, (size_t)h.p * h.n * 8 + (size_t)h.p * h.npad * 9);
     }
   const int K = (int)act.size();
   Layout L = layout(e);
   bool any0 = false, any1 = false;
   for (int l : lev) { any0 |= l == 0; any1 |= l >= 1; }
   HPC_HIPCHK(hipEventRecord(e.ev[0], e.stream));
+  if (any0 && any_fast_at(e, 0)) launch_refine_l0(e.stream, L);
   if (any0) launch_level0(e.stream, L, e.h_inst, e.d_prefix + 2 * (K + 1), e.h_prefix);
   HPC_HIPCHK(hipGetLastError());
   HPC_HIPCHK(hipEventRecord(e.ev[1], e.stream));
@@ -760,14 +873,18 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
   if (e.n_edges > 0) {
     if (run1) {
       HPC_HIPCHK(hipEventRecord(e.sev[2 * npairs], e.stream));
-      launch_search_l1(e.stream, L, e.n_edges, nullptr);
+      const bool f1 = any_fast_at(e, 1);
+      launch_search_l1(e.stream, L, e.n_edges, nullptr, f1);
+      if (f1) launch_recheck(e.stream, 1, L);
       HPC_HIPCHK(hipEventRecord(e.sev[2 * npairs + 1], e.stream));
       ++npairs;
     }
     for (int l : levels_ge2) {
       launch_task_plan(e.stream, l, L, e.ranks_per_task, nullptr, e.d_row_tasks, e.d_task_off);
       HPC_HIPCHK(hipEventRecord(e.sev[2 * npairs], e.stream));
-      launch_search_ln(e.stream, l, L, e.ranks_per_task, e.d_row_tasks, e.d_task_off);
+      const bool fl = any_fast_at(e, l);
+      launch_search_ln(e.stream, l, L, e.ranks_per_task, e.d_row_tasks, e.d_task_off, fl);
+      if (fl) launch_recheck(e.stream, l, L);
       HPC_HIPCHK(hipEventRecord(e.sev[2 * npairs + 1], e.stream));
       ++npairs;
     }
@@ -779,6 +896,15 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
   HPC_HIPCHK(hipEventRecord(e.ev[5], e.stream));
   rc = read_counters(e);
   if (rc) return rc;
+  if (e.h_ctr->overflow) {
+    // a recheck queue overflowed: level 0 and the removal were skipped, so
+    // the step left no trace; exact correlations for the fast instances, then
+    // the same step again
+    rc = engine_exact_from_z(e, act);
+    if (rc) return rc;
+    return engine_step(e, ids);
+  }
+  e.rechecks = e.h_ctr->rechecks;
   const double wall =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   const double ms_l0 = elapsed(e.ev[0], e.ev[1]), ms_c = elapsed(e.ev[1], e.ev[2]);
@@ -808,7 +934,10 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
       s.removed = (int64_t)c.removed;
       h.undirected -= s.removed;
       s.edges_left = h.undirected;
-      if (K == 1) s.pinv_fallbacks = (int64_t)e.h_ctr->pinv_fallbacks;
+      if (K == 1) {
+        s.pinv_fallbacks = (int64_t)e.h_ctr->pinv_fallbacks;
+        s.rechecks = (int64_t)e.h_ctr->rechecks;
+      }
       if (lev[k] == 0) {
         s.ms_test = ms_l0;
       } else {

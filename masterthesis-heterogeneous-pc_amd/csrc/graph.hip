@@ -56,6 +56,8 @@ __global__ __launch_bounds__(64 * kL0Waves) void level0_kernel(Layout L,
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const long long t = blockIdx.x;
   if (t >= prefix[L.K]) return;
+  // an overflowed refinement queue leaves level 0 undecided: the host redoes it
+  if (L.ctr->overflow) return;
   const int k = find_inst(prefix, L.K, t);
   const Inst In = L.inst[k];
   const int p = In.p, W = In.W;

@@ -139,6 +139,8 @@ extern "C" int hpc_search_level_host(hpc_state *st, int level, const int32_t *ro
   if (level < 1 || level > kMaxLevel)
     return hpc_fail(HPC_ELEVEL, "host share supports levels 1..%d, not %d", kMaxLevel, level);
   if (!h.compacted) return hpc_fail(HPC_ESTATE, "level %d needs hpc_compact first", level);
+  if (h.fast)
+    return hpc_fail(HPC_ESTATE, "the host share needs an exact correlation (HPC_CORR_CANON)");
   if (!rows && nrows > 0) return hpc_fail(HPC_EINVAL, "rows is NULL");
   auto t0 = std::chrono::steady_clock::now();
   HIPCHK(hipSetDevice(e.device));

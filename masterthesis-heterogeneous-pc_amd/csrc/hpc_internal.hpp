@@ -22,6 +22,11 @@
 
 namespace hpc {
 
+// Bound on |fast correlation - exact correlation| (HPC_CORR_F32): 64-sample
+// FP32 accumulation (64 + 8 roundings of 2^-24), the float rounding of both
+// operands, and 1% for the float-rounded standard deviation.
+constexpr double kFastEps = 74.0 * 5.9604644775390625e-8 * 1.01;
+
 // Largest level the device kernels accept (conditioning-set size).
 constexpr int kMaxLevel = 16;
 // Binomial table columns: C(n, k) for k <= kMaxLevel.
@@ -32,8 +37,18 @@ struct Counters {
   unsigned long long n_edges;         // ordered edges of the snapshot (CSR size)
   unsigned long long n_tasks;         // search tasks of a level >= 2
   unsigned long long pinv_fallbacks;  // tests that took the pinv form
+  unsigned long long n_rchk;          // queued exact rechecks of the current search
+  unsigned long long rechecks;        // exact rechecks run in this phase
+  unsigned long long n_pairs;         // level-0 pairs queued for an exact correlation
   int max_deg;                        // largest row length of the snapshot
-  int pad_;
+  int overflow;                       // a recheck queue overflowed: the phase is void
+};
+
+// A test whose fast-correlation statistic lies within its error bound of the
+// threshold: ordered edge and rank, re-run on exact correlations.
+struct Recheck {
+  unsigned long long edge;
+  unsigned long long rank;
 };
 
 // Per-instance counters of one level.
@@ -58,16 +73,20 @@ struct Inst {
   uint8_t *nanm;          // [p][npad] NaN mask
   double *mean, *sd;      // [p]
   double *cor;            // [p][p]
+  float *z32;             // [p][npad] z-scores rounded to float (fast correlation)
   unsigned long long *adj;  // [p][W] bit adjacency
   // sepset records of edges removed at levels >= 1, appended by the removal
   // kernel: i, j (i < j, local ids), level, then the level members
   int32_t *rec;
   unsigned long long rec_base;  // words written by earlier levels
   double r_crit;          // Fisher-z decision boundary of (n, alpha)
+  double eps;             // bound on |cor - exact cor| when `fast`
   int start;              // first global variable id
   int p, n, npad, W;
   int level;              // level of this phase
   int flags;
+  int fast;               // cor holds fast (MFMA) values: decisions need rechecks
+  int pad_;
 };
 
 // What every kernel sees: the instance table and the global arrays.
@@ -86,6 +105,10 @@ struct Layout {
   int64_t *fp;                  // [E] first passing rank, -1 = none
   const unsigned long long *binom;  // [(pmax + 1) x kBinomK], saturating
   Counters *ctr;
+  Recheck *rchk;                // recheck queue of a search
+  unsigned long long rchk_cap;
+  int2 *pairs;                  // level-0 pairs (global ids) for exact correlations
+  unsigned long long pairs_cap;
 };
 
 // ---- launchers (each defined in the .hip file that owns the kernel) ------
@@ -94,18 +117,31 @@ void launch_moments(hipStream_t s, const Layout &L);
 // others; prefix arrays hold per-instance block counts (device and host)
 void launch_corr(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst,
                  long long *d_prefix, std::vector<long long> &h_prefix);
+// Fast correlation of the packed instances with `fast` set: MFMA FP32 on the
+// float z-scores, FP64 across 64-sample chunks (instances with NaN cells are
+// left to launch_corr's masked kernel).
+void launch_corr_fast(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst,
+                      long long *d_prefix, std::vector<long long> &h_prefix);
+// Exact (canonical-order) correlations of the level-0 pairs of fast instances
+// that lie within eps of the threshold, written into cor.
+void launch_refine_l0(hipStream_t s, const Layout &L);
+// Exact re-runs of the queued uncertain tests of `level`; then the queue is
+// emptied.
+void launch_recheck(hipStream_t s, int level, const Layout &L);
 void launch_adj_init(hipStream_t s, unsigned long long *adj, int p, int W);
 void launch_level0(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst,
                    long long *d_prefix, std::vector<long long> &h_prefix);
 void launch_compact(hipStream_t s, const Layout &L);
+// `fast`: the launch holds an instance with fast correlations (selects the
+// variant with uncertainty bounds and the recheck queue)
 void launch_search_l1(hipStream_t s, const Layout &L, int64_t n_edges,
-                      const int32_t *row_mask);
+                      const int32_t *row_mask, bool fast);
 void launch_task_plan(hipStream_t s, int level, const Layout &L, int rpt,
                       const int32_t *row_mask, long long *row_tasks, long long *task_off);
 // The plan's per-row task counts and exclusive offsets define the tasks; the
 // kernels read the task count from the device counters.
 void launch_search_ln(hipStream_t s, int level, const Layout &L, int rpt,
-                      const long long *row_tasks, const long long *task_off);
+                      const long long *row_tasks, const long long *task_off, bool fast);
 void launch_commit(hipStream_t s, const Layout &L, int64_t n_edges);
 void launch_scatter_fp(hipStream_t s, const int64_t *idx, const int64_t *val,
                        long long count, int64_t *fp);
@@ -125,6 +161,9 @@ struct InstHost {
   unsigned long long *adj = nullptr;
   int32_t *rec = nullptr;
   size_t rec_cap = 0, rec_used = 0;  // sepset record words
+  float *z32 = nullptr;
+  int corr_mode = 0;       // HPC_CORR_* of the next correlation
+  bool fast = false;       // cor holds fast values (mode F32 and no NaN cells)
   bool have_x = false, have_cor = false;
   bool drop_obs = false;   // free observations and z-scores once correlated
   // level state
@@ -177,7 +216,7 @@ struct Engine {
   Counters *d_ctr = nullptr, *h_ctr = nullptr;
   size_t cap_inst = 0, cap_rows = 0, cap_edges = 0, cap_binom = 0;
   std::vector<unsigned long long> h_binom;
-  std::vector<long long> h_prefix;
+  std::vector<long long> h_prefix, h_prefix_fast;
   // the packed set of the current phase
   std::vector<Inst> h_inst;
   std::vector<int> packed;
@@ -195,6 +234,12 @@ struct Engine {
   std::vector<int32_t> h_rowptr, h_nbr;
   int64_t *d_scatter_idx = nullptr, *d_scatter_val = nullptr;
   size_t scatter_cap = 0;
+  // recheck queues of fast instances
+  Recheck *d_rchk = nullptr;
+  int2 *d_pairs = nullptr;
+  size_t cap_rchk = 0, cap_pairs = 0;
+  size_t rchk_limit = 0;  // queue length in use (0 = capacity); a tuning knob
+  unsigned long long rechecks = 0;  // exact rechecks of the last step or level
 };
 
 double fisher_r_crit(int n, double alpha);
@@ -216,6 +261,12 @@ int engine_reset(Engine &e, int id);
 // below works on the packed set.
 int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &levels);
 int engine_corr(Engine &e, float *ms_moments, float *ms_corr);
+// Before packing an instance for its correlation: the fast flag from its
+// correlation mode, and its float z-score block.
+int engine_prepare_corr(Engine &e, int id);
+// Exact correlations (from the device-resident z-scores) for the fast
+// instances among ids, which then count as exact.
+int engine_exact_from_z(Engine &e, const std::vector<int> &ids);
 // Compaction of the packed instances; updates their max_deg and n_edges.
 int engine_compact(Engine &e, float *ms);
 // Search of the packed instances at `level` (rows: global ids, NULL = all).

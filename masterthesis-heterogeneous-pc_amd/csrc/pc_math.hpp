@@ -161,6 +161,91 @@ __host__ __device__ __forceinline__ double pcor_canon(double a, const double (&x
   return pcor_pinv<L + 2>(A);
 }
 
+// First-order sensitivity of the partial correlation to its inputs: with
+// P = M^-1 for the (L+2)-square matrix M = [i, j, S] (unit diagonal) and
+// rho_a the absolute sum of row a of P,
+//   |dr| <= eps * (rho_0 rho_1 / sqrt(P00 P11)
+//                  + |r| (rho_0^2 / P00 + rho_1^2 / P11) / 2)
+// when every off-diagonal entry of M moves by at most eps (dP = -P dM P).
+// Returns the factor in parentheses, or +inf when M is numerically singular.
+template <int L>
+__host__ __device__ __forceinline__ double pcor_sensitivity(double a, const double (&x)[L],
+                                                   const double (&y)[L],
+                                                   const double (&C)[L][L]) {
+  double Um[L][L], Em[L][L], inv[L], u[L], v[L], gx[L], gy[L];
+#pragma unroll
+  for (int r = 0; r < L; ++r) {
+#pragma unroll
+    for (int c = 0; c < r; ++c) {
+      double e = C[r][c];
+#pragma unroll
+      for (int k = 0; k < c; ++k) e = __builtin_fma(-Em[r][k], Um[c][k], e);
+      Em[r][c] = e;
+      Um[r][c] = e * inv[c];
+    }
+    double d = 1.0;
+#pragma unroll
+    for (int k = 0; k < r; ++k) d = __builtin_fma(-Em[r][k], Um[r][k], d);
+    if (!(d > kPivTol)) return __builtin_inf();
+    inv[r] = 1.0 / d;
+  }
+#pragma unroll
+  for (int c = 0; c < L; ++c) {
+    double su = x[c], sv = y[c];
+#pragma unroll
+    for (int k = 0; k < c; ++k) {
+      su = __builtin_fma(-Um[c][k], u[k], su);
+      sv = __builtin_fma(-Um[c][k], v[k], sv);
+    }
+    u[c] = su;
+    v[c] = sv;
+  }
+  double s00 = 1.0, s11 = 1.0, s01 = a;
+#pragma unroll
+  for (int k = 0; k < L; ++k) {
+    const double wu = u[k] * inv[k], wv = v[k] * inv[k];
+    s00 = __builtin_fma(-wu, u[k], s00);
+    s11 = __builtin_fma(-wv, v[k], s11);
+    s01 = __builtin_fma(-wu, v[k], s01);
+    gx[k] = wu;
+    gy[k] = wv;
+  }
+  // C^-1 x = U^-T D^-1 u (back substitution, in place)
+#pragma unroll
+  for (int c = L - 1; c >= 0; --c) {
+#pragma unroll
+    for (int k = c + 1; k < L; ++k) {
+      gx[c] = __builtin_fma(-Um[k][c], gx[k], gx[c]);
+      gy[c] = __builtin_fma(-Um[k][c], gy[k], gy[c]);
+    }
+  }
+  const double det = __builtin_fma(-s01, s01, s00 * s11);
+  if (!(s00 > kPivTol) || !(s11 > kPivTol) || !(det > kPivTol * s00 * s11)) return __builtin_inf();
+  const double id = 1.0 / det;
+  const double p00 = s11 * id, p11 = s00 * id, p01 = -s01 * id;
+  double rho0 = __builtin_fabs(p00) + __builtin_fabs(p01);
+  double rho1 = __builtin_fabs(p01) + __builtin_fabs(p11);
+#pragma unroll
+  for (int k = 0; k < L; ++k) {
+    rho0 += __builtin_fabs(p00 * gx[k] + p01 * gy[k]);
+    rho1 += __builtin_fabs(p01 * gx[k] + p11 * gy[k]);
+  }
+  const double r = s01 / __builtin_sqrt(s00 * s11);
+  return rho0 * rho1 / __builtin_sqrt(p00 * p11) +
+         __builtin_fabs(r) * 0.5 * (rho0 * rho0 / p00 + rho1 * rho1 / p11);
+}
+
+// Whether a test on fast correlations (error <= eps per entry) decides the
+// same as on exact ones: the statistic is farther from the threshold than
+// twice its first-order error bound, and that bound is small enough for the
+// first order to hold.
+__host__ __device__ __forceinline__ bool fast_certain(double r, double r_crit, double sens, double eps) {
+  const double bound = 2.0 * sens * eps;
+  if (!(bound < 0.05)) return false;
+  const double re = __builtin_isnan(r) ? 0.0 : __builtin_fmin(kCutThr, __builtin_fabs(r));
+  return __builtin_fabs(re - r_crit) > bound;
+}
+
 // Number of l-subsets of an nn-set, saturating (cuda_util.cu:20-32).
 __host__ __device__ __forceinline__ unsigned long long binom(const unsigned long long *tab,
                                                     int kcols, int nn, int l) {

@@ -31,7 +31,27 @@ namespace {
 
 constexpr int kWavesPerBlock = 4;
 
+// Lanes of a fast instance's test chunk whose decision is uncertain and whose
+// rank lies below the chunk's lowest certain pass (`certain_passes`) are
+// queued for an exact re-run.  Wave-uniform call.
+__device__ __forceinline__ void queue_uncertain(const Layout &L, bool unc,
+                                                unsigned long long certain_passes, long long e,
+                                                unsigned long long r0, int lane) {
+  unsigned long long m = __ballot(unc);
+  if (certain_passes) m &= (1ull << (__ffsll((long long)certain_passes) - 1)) - 1ull;
+  if (!m) return;
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd(&L.ctr->n_rchk, (unsigned long long)__popcll(m));
+  base = __shfl(base, 0, 64);
+  if ((m >> lane) & 1ull) {
+    const unsigned long long slot = base + __popcll(m & ((1ull << lane) - 1ull));
+    if (slot < L.rchk_cap) L.rchk[slot] = Recheck{(unsigned long long)e, r0 + lane};
+    else L.ctr->overflow = 1;
+  }
+}
+
 // Level 1 (testRowL1, gpuRowTests.cu:29-100; cpuRowTests.cpp:23-56).
+template <bool FAST>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_kernel(
     Layout L, long long n_edges, const int32_t *__restrict__ row_mask) {
   const int lane = threadIdx.x & 63;
@@ -49,20 +69,29 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_kernel(
     const int p = In.p, s0 = In.start;
     const double *cor = In.cor;
     const double r_crit = In.r_crit;
+    const bool fast = FAST && In.fast;
     const int li = i - s0, lj = L.nbr[e] - s0;
     const double x1 = cor[(size_t)li * p + lj];
     long long found = -1;
     for (int q0 = 0; q0 < nn; q0 += 64) {
       const int q = q0 + lane;
-      bool pass = false;
+      bool pass = false, unc = false;
       if (q < nn) {
         const int lk = L.nbr[base + (q < y ? q : q + 1)] - s0;
         const double x2 = cor[(size_t)li * p + lk];
         const double x3 = cor[(size_t)lj * p + lk];
-        if (!(__builtin_isnan(x1) || __builtin_isnan(x2) || __builtin_isnan(x3)))
-          pass = removes(pcor_l1(x1, x2, x3), r_crit);
+        if (!(__builtin_isnan(x1) || __builtin_isnan(x2) || __builtin_isnan(x3))) {
+          const double r = pcor_l1(x1, x2, x3);
+          pass = removes(r, r_crit);
+          if (fast) {
+            const double xa[1] = {x2}, ya[1] = {x3}, Cz[1][1] = {{1.0}};
+            unc = !fast_certain(r, r_crit, pcor_sensitivity<1>(x1, xa, ya, Cz), In.eps);
+            pass &= !unc;
+          }
+        }
       }
       const unsigned long long m = __ballot(pass);
+      if (fast) queue_uncertain(L, unc, m, e, (unsigned long long)q0, lane);
       if (m) { found = q0 + __ffsll((long long)m) - 1; break; }
     }
     if (lane == 0) L.fp[e] = found;
@@ -73,6 +102,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_kernel(
 // ordered edges [b * epb, (b + 1) * epb) (CSR order, so grouped by row),
 // copies ids and cor[i][N(i)] of each row it meets into LDS and gathers only
 // cor[j][k] per test.  Same tests, order and arithmetic as search_l1_kernel.
+template <bool FAST>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_lds_kernel(
     Layout L, long long n_edges, const int32_t *__restrict__ row_mask, int epb) {
   extern __shared__ unsigned long long smem[];
@@ -99,6 +129,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_lds_kernel(
       __syncthreads();
       const double r_crit = In.r_crit;
       const int nn = deg - 1;
+      const bool fast = FAST && In.fast;
       for (long long e = e0 + wave; e < seg_end; e += kWavesPerBlock) {
         const int y = (int)(e - base);
         const int lj = ids[y];
@@ -107,15 +138,23 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_lds_kernel(
         long long found = -1;
         for (int q0 = 0; q0 < nn; q0 += 64) {
           const int q = q0 + lane;
-          bool pass = false;
+          bool pass = false, unc = false;
           if (q < nn) {
             const int pos = q < y ? q : q + 1;
             const double x2 = Mi[pos];
             const double x3 = cj[ids[pos]];
-            if (!(__builtin_isnan(x1) || __builtin_isnan(x2) || __builtin_isnan(x3)))
-              pass = removes(pcor_l1(x1, x2, x3), r_crit);
+            if (!(__builtin_isnan(x1) || __builtin_isnan(x2) || __builtin_isnan(x3))) {
+              const double r = pcor_l1(x1, x2, x3);
+              pass = removes(r, r_crit);
+              if (fast) {
+                const double xa[1] = {x2}, ya[1] = {x3}, Cz[1][1] = {{1.0}};
+                unc = !fast_certain(r, r_crit, pcor_sensitivity<1>(x1, xa, ya, Cz), In.eps);
+                pass &= !unc;
+              }
+            }
           }
           const unsigned long long m = __ballot(pass);
+          if (fast) queue_uncertain(L, unc, m, e, (unsigned long long)q0, lane);
           if (m) { found = q0 + __ffsll((long long)m) - 1; break; }
         }
         if (lane == 0) L.fp[e] = found;
@@ -144,7 +183,7 @@ __device__ __forceinline__ int task_row(const long long *__restrict__ task_off, 
 // matrix (rows too long for the LDS form).  Each wavefront takes tasks
 // t = wave, wave + waves, ... < n_tasks (read on the device); task t covers
 // ranks [chunk * rpt, (chunk + 1) * rpt) of its edge.
-template <int LV>
+template <int LV, bool FAST>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_kernel(
     Layout L, int rpt, const long long *__restrict__ task_off,
     const long long *__restrict__ row_tasks, int lds_binom) {
@@ -182,6 +221,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_kernel(
     const int p = In.p, s0 = In.start;
     const double *cor = In.cor;
     const double r_crit = In.r_crit;
+    const bool fast = FAST && In.fast;
     const int nn = deg - 1;
     const int y = (int)(e - base);
     const unsigned long long total = binom(tab, kc, nn, LV);
@@ -190,7 +230,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_kernel(
     const double a = cor[(size_t)li * p + lj];
     for (unsigned long long r0 = r_begin; r0 < r_end; r0 += 64) {
       const unsigned long long rank = r0 + lane;
-      bool pass = false;
+      bool pass = false, unc = false;
       if (rank < r_end) {
         int q[LV], S[LV];
         unrank<LV>(tab, kc, rank, nn, q);
@@ -216,9 +256,14 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_kernel(
           const double r = pcor_canon<LV>(a, x, yv, C, fb);
           pass = removes(r, r_crit);
           F += fb ? 1 : 0;
+          if (fast) {
+            unc = fb || !fast_certain(r, r_crit, pcor_sensitivity<LV>(a, x, yv, C), In.eps);
+            pass &= !unc;
+          }
         }
       }
       const unsigned long long m = __ballot(pass);
+      if (fast) queue_uncertain(L, unc, m, e, r0, lane);
       if (m) {
         if (lane == 0) atomicMin(&fpu[e], r0 + (unsigned long long)(__ffsll((long long)m) - 1));
         break;
@@ -239,7 +284,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_kernel(
 // the device), one staging per row a block meets.  Same ranks, same
 // arithmetic and the same orientation of every correlation read as
 // search_ln_kernel.
-template <int LV>
+template <int LV, bool FAST>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
     Layout L, int rpt, const long long *__restrict__ task_off,
     const long long *__restrict__ row_tasks, int tpb, int dmax) {
@@ -279,6 +324,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
       }
       __syncthreads();
       const double r_crit = In.r_crit;
+      const bool fast = FAST && In.fast;
       const int nn = deg - 1;
       const unsigned long long total = binom(bt, LV + 1, nn, LV);
       const double *Mi = M + (size_t)deg * D;  // row i
@@ -293,7 +339,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
         const double *Mj = M + (size_t)y * D;  // row j
         for (unsigned long long r0 = r_begin; r0 < r_end; r0 += 64) {
           const unsigned long long rank = r0 + lane;
-          bool pass = false;
+          bool pass = false, unc = false;
           if (rank < r_end) {
             int q[LV], pos[LV];
             unrank<LV>(bt, LV + 1, rank, nn, q);
@@ -319,9 +365,14 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
               const double r = pcor_canon<LV>(a, x, yv, C, fb);
               pass = removes(r, r_crit);
               F += fb ? 1 : 0;
+              if (fast) {
+                unc = fb || !fast_certain(r, r_crit, pcor_sensitivity<LV>(a, x, yv, C), In.eps);
+                pass &= !unc;
+              }
             }
           }
           const unsigned long long m = __ballot(pass);
+          if (fast) queue_uncertain(L, unc, m, e, r0, lane);
           if (m) {
             if (lane == 0) atomicMin(&fpu[e], r0 + (unsigned long long)(__ffsll((long long)m) - 1));
             break;
@@ -394,6 +445,8 @@ __device__ __forceinline__ int find_pos(const int32_t *row, int len, int node) {
 // holds one instance; the record order is scheduling-dependent, the records
 // are not.
 __global__ __launch_bounds__(256) void commit_kernel(Layout L, long long n_edges) {
+  // an overflowed recheck queue leaves the level undecided: the host redoes it
+  if (L.ctr->overflow) return;
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   int k = -1, level = 0;
@@ -554,37 +607,56 @@ constexpr int kTasksPerBlock = 16;
 // only known on the device): enough to fill all 256 CUs several times.
 constexpr int kSearchBlocks = 256 * 8;
 
-template <int LV>
-void launch_ln(hipStream_t s, const Layout &L, int rpt, const long long *row_tasks,
-               const long long *task_off) {
+template <int LV, bool FAST>
+void launch_ln_as(hipStream_t s, const Layout &L, int rpt, const long long *row_tasks,
+                  const long long *task_off) {
   const size_t staged = lds_bytes(L.max_deg, LV);
   if (staged <= kLdsLimit) {
-    hipLaunchKernelGGL(search_ln_lds_kernel<LV>, dim3(kSearchBlocks), dim3(64 * kWavesPerBlock),
-                       staged, s, L, rpt, task_off, row_tasks, kTasksPerBlock, L.max_deg + 1);
+    hipLaunchKernelGGL((search_ln_lds_kernel<LV, FAST>), dim3(kSearchBlocks),
+                       dim3(64 * kWavesPerBlock), staged, s, L, rpt, task_off, row_tasks,
+                       kTasksPerBlock, L.max_deg + 1);
     return;
   }
   const size_t lds = (size_t)(L.max_deg + 1) * (LV + 1) * sizeof(unsigned long long);
   const int use = lds <= 48 * 1024;
-  hipLaunchKernelGGL(search_ln_kernel<LV>, dim3(kSearchBlocks), dim3(64 * kWavesPerBlock),
+  hipLaunchKernelGGL((search_ln_kernel<LV, FAST>), dim3(kSearchBlocks), dim3(64 * kWavesPerBlock),
                      use ? lds : 0, s, L, rpt, task_off, row_tasks, use);
+}
+
+// The fast-correlation variant (uncertainty bounds, recheck queue) is only
+// instantiated into launches that hold a fast instance.
+template <int LV>
+void launch_ln(hipStream_t s, const Layout &L, int rpt, const long long *row_tasks,
+               const long long *task_off, bool fast) {
+  if (fast) launch_ln_as<LV, true>(s, L, rpt, row_tasks, task_off);
+  else launch_ln_as<LV, false>(s, L, rpt, row_tasks, task_off);
 }
 
 }  // namespace
 
-void launch_search_l1(hipStream_t s, const Layout &L, int64_t n_edges, const int32_t *row_mask) {
+void launch_search_l1(hipStream_t s, const Layout &L, int64_t n_edges, const int32_t *row_mask,
+                      bool fast) {
   if (n_edges <= 0) return;
   const size_t staged = (size_t)L.max_deg * 12;
   if (staged <= kLdsLimit) {
     const int epb = 32;
     const long long blocks = (n_edges + epb - 1) / epb;
-    hipLaunchKernelGGL(search_l1_lds_kernel, dim3((unsigned)blocks), dim3(64 * kWavesPerBlock),
-                       staged, s, L, (long long)n_edges, row_mask, epb);
+    if (fast)
+      hipLaunchKernelGGL(search_l1_lds_kernel<true>, dim3((unsigned)blocks),
+                         dim3(64 * kWavesPerBlock), staged, s, L, (long long)n_edges, row_mask, epb);
+    else
+      hipLaunchKernelGGL(search_l1_lds_kernel<false>, dim3((unsigned)blocks),
+                         dim3(64 * kWavesPerBlock), staged, s, L, (long long)n_edges, row_mask, epb);
     return;
   }
   long long blocks = (n_edges + kWavesPerBlock - 1) / kWavesPerBlock;
   if (blocks > 16384) blocks = 16384;
-  hipLaunchKernelGGL(search_l1_kernel, dim3((unsigned)blocks), dim3(64 * kWavesPerBlock), 0, s,
-                     L, (long long)n_edges, row_mask);
+  if (fast)
+    hipLaunchKernelGGL(search_l1_kernel<true>, dim3((unsigned)blocks), dim3(64 * kWavesPerBlock), 0,
+                       s, L, (long long)n_edges, row_mask);
+  else
+    hipLaunchKernelGGL(search_l1_kernel<false>, dim3((unsigned)blocks), dim3(64 * kWavesPerBlock),
+                       0, s, L, (long long)n_edges, row_mask);
 }
 
 void launch_task_plan(hipStream_t s, int level, const Layout &L, int rpt,
@@ -596,23 +668,23 @@ void launch_task_plan(hipStream_t s, int level, const Layout &L, int rpt,
 }
 
 void launch_search_ln(hipStream_t s, int level, const Layout &L, int rpt,
-                      const long long *row_tasks, const long long *task_off) {
+                      const long long *row_tasks, const long long *task_off, bool fast) {
   switch (level) {
-    case 2: launch_ln<2>(s, L, rpt, row_tasks, task_off); break;
-    case 3: launch_ln<3>(s, L, rpt, row_tasks, task_off); break;
-    case 4: launch_ln<4>(s, L, rpt, row_tasks, task_off); break;
-    case 5: launch_ln<5>(s, L, rpt, row_tasks, task_off); break;
-    case 6: launch_ln<6>(s, L, rpt, row_tasks, task_off); break;
-    case 7: launch_ln<7>(s, L, rpt, row_tasks, task_off); break;
-    case 8: launch_ln<8>(s, L, rpt, row_tasks, task_off); break;
-    case 9: launch_ln<9>(s, L, rpt, row_tasks, task_off); break;
-    case 10: launch_ln<10>(s, L, rpt, row_tasks, task_off); break;
-    case 11: launch_ln<11>(s, L, rpt, row_tasks, task_off); break;
-    case 12: launch_ln<12>(s, L, rpt, row_tasks, task_off); break;
-    case 13: launch_ln<13>(s, L, rpt, row_tasks, task_off); break;
-    case 14: launch_ln<14>(s, L, rpt, row_tasks, task_off); break;
-    case 15: launch_ln<15>(s, L, rpt, row_tasks, task_off); break;
-    case 16: launch_ln<16>(s, L, rpt, row_tasks, task_off); break;
+    case 2: launch_ln<2>(s, L, rpt, row_tasks, task_off, fast); break;
+    case 3: launch_ln<3>(s, L, rpt, row_tasks, task_off, fast); break;
+    case 4: launch_ln<4>(s, L, rpt, row_tasks, task_off, fast); break;
+    case 5: launch_ln<5>(s, L, rpt, row_tasks, task_off, fast); break;
+    case 6: launch_ln<6>(s, L, rpt, row_tasks, task_off, fast); break;
+    case 7: launch_ln<7>(s, L, rpt, row_tasks, task_off, fast); break;
+    case 8: launch_ln<8>(s, L, rpt, row_tasks, task_off, fast); break;
+    case 9: launch_ln<9>(s, L, rpt, row_tasks, task_off, fast); break;
+    case 10: launch_ln<10>(s, L, rpt, row_tasks, task_off, fast); break;
+    case 11: launch_ln<11>(s, L, rpt, row_tasks, task_off, fast); break;
+    case 12: launch_ln<12>(s, L, rpt, row_tasks, task_off, fast); break;
+    case 13: launch_ln<13>(s, L, rpt, row_tasks, task_off, fast); break;
+    case 14: launch_ln<14>(s, L, rpt, row_tasks, task_off, fast); break;
+    case 15: launch_ln<15>(s, L, rpt, row_tasks, task_off, fast); break;
+    case 16: launch_ln<16>(s, L, rpt, row_tasks, task_off, fast); break;
     default: break;  // rejected by the caller (kMaxLevel)
   }
 }

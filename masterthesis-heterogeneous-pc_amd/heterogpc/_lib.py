@@ -14,7 +14,9 @@ HEADER = os.path.join(os.path.dirname(PKG_ROOT), "include", "heterogpc_hip.h")
 
 HPC_OK = 0
 HPC_CORR_CANON = 0
+HPC_CORR_F32 = 1
 HPC_TUNE_RANKS_PER_TASK = 1
+HPC_TUNE_RECHECK_QUEUE = 2
 
 
 class LevelStats(C.Structure):
@@ -31,6 +33,7 @@ class LevelStats(C.Structure):
         ("ms_test", C.c_double),
         ("ms_commit", C.c_double),
         ("ms_wall", C.c_double),
+        ("rechecks", C.c_int64),
     ]
 
     def as_dict(self) -> dict:
@@ -114,6 +117,8 @@ SIGNATURES = {
     "hpc_batch_get_sepsets": (_i, [_vp, _i, _vp, _i]),
     "hpc_batch_get_sepset_list": (_i, [_vp, _i, _vp, _i64, C.POINTER(_i64)]),
     "hpc_batch_set_tuning": (_i, [_vp, _i, _i]),
+    "hpc_batch_set_corr_mode": (_i, [_vp, _i, _i]),
+    "hpc_server_set_corr_mode": (_i, [_vp, _i]),
     "hpc_server_create": (_i, [_i, _u64, _i, C.POINTER(_vp)]),
     "hpc_server_destroy": (_i, [_vp]),
     "hpc_server_submit": (_i, [_vp, _u64, _i, _d, _i, _vp, C.POINTER(_i64)]),

@@ -48,7 +48,7 @@ class Batch:
         except Exception:
             pass
 
-    def add(self, x: np.ndarray, alpha: float, max_level: int = 4) -> int:
+    def add(self, x: np.ndarray, alpha: float, max_level: int = 4, corr_mode: int = 0) -> int:
         """New instance on observations x ([p][n] float64); returns its id."""
         x = np.ascontiguousarray(x, dtype=np.float64)
         p, n = x.shape
@@ -56,6 +56,8 @@ class Batch:
         check(self.lib.hpc_batch_add(self.h, p, n, float(alpha), int(max_level), _ptr(x),
                                      C.byref(out)))
         self.shape[out.value] = (p, n)
+        if corr_mode:
+            check(self.lib.hpc_batch_set_corr_mode(self.h, out.value, int(corr_mode)))
         return out.value
 
     def remove(self, iid: int) -> None:
@@ -116,12 +118,15 @@ class Batch:
 class Server:
     """Continuous-batching skeleton server on one GPU."""
 
-    def __init__(self, device: int = 0, hbm_budget: int = 0, max_active: int = 0):
+    def __init__(self, device: int = 0, hbm_budget: int = 0, max_active: int = 0,
+                 corr_mode: int = 0):
         self.lib = load()
         h = C.c_void_p()
         check(self.lib.hpc_server_create(int(device), int(hbm_budget), int(max_active),
                                          C.byref(h)))
         self.h = h
+        if corr_mode:
+            check(self.lib.hpc_server_set_corr_mode(self.h, int(corr_mode)))
         self.shape: dict[int, int] = {}
         self._borrowed: dict[int, np.ndarray] = {}
 

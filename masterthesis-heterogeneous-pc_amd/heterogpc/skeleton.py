@@ -103,6 +103,10 @@ class DeviceSkeleton:
     def set_ranks_per_task(self, value: int) -> None:
         check(self.lib.hpc_set_tuning(self.h, 1, int(value)))
 
+    def set_recheck_queue(self, value: int) -> None:
+        """Length of the HPC_CORR_F32 recheck queues (0 = full)."""
+        check(self.lib.hpc_set_tuning(self.h, 2, int(value)))
+
     # ---- levels ----------------------------------------------------------
     def reset(self) -> None:
         check(self.lib.hpc_reset(self.h))
