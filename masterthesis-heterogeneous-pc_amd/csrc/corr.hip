@@ -182,8 +182,8 @@ constexpr int kXcds = 8;
 
 __global__ __launch_bounds__(256) void corr_tile_kernel(Layout L, const long long *__restrict__ prefix,
                                                         long long per_xcd) {
-  __shared__ double sa[2][CT][64];
-  __shared__ double sb[2][CT][64];
+  __shared__ double sa[3][CT][64];
+  __shared__ double sb[3][CT][64];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wa = wave >> 1, wb = wave & 1;
@@ -211,44 +211,88 @@ __global__ __launch_bounds__(256) void corr_tile_kernel(Layout L, const long lon
   if (TA >= nt || TBt >= nt || TA > TBt) return;
   const int a0 = TA * CT, b0 = TBt * CT;
 
-  // loader mapping: 16 rows x 64 samples per operand = 256 threads x 4 doubles
-  const int lr = tid >> 4, lc = (tid & 15) * 4;
-  const double *ga = I.z + (size_t)min(a0 + lr, p - 1) * npad + lc;
-  const double *gb = I.z + (size_t)min(b0 + lr, p - 1) * npad + lc;
+  // loader mapping: 16 rows x 64 samples per operand = 512 chunks of 16 B;
+  // thread tid moves chunks tid and tid + 256 (row c / 32, column 2 (c % 32)),
+  // so the lanes of one LDS store write consecutive 16 B (no bank conflicts)
+  const int lr0 = tid >> 5, lr1 = lr0 + 8, lc = (tid & 31) * 2;
+  const double *ga0 = I.z + (size_t)min(a0 + lr0, p - 1) * npad + lc;
+  const double *ga1 = I.z + (size_t)min(a0 + lr1, p - 1) * npad + lc;
+  const double *gb0 = I.z + (size_t)min(b0 + lr0, p - 1) * npad + lc;
+  const double *gb1 = I.z + (size_t)min(b0 + lr1, p - 1) * npad + lc;
   const int nsteps = npad >> 6;
-
   double acc[8][8];
 #pragma unroll
   for (int u = 0; u < 8; ++u)
 #pragma unroll
     for (int v = 0; v < 8; ++v) acc[u][v] = 0.0;
-
-  double4 pa = *reinterpret_cast<const double4 *>(ga);
-  double4 pb = *reinterpret_cast<const double4 *>(gb);
-  *reinterpret_cast<double4 *>(&sa[0][lr][lc]) = pa;
-  *reinterpret_cast<double4 *>(&sb[0][lr][lc]) = pb;
-  __syncthreads();
-  for (int m = 0; m < nsteps; ++m) {
-    const int cur = m & 1;
-    if (m + 1 < nsteps) {
-      pa = *reinterpret_cast<const double4 *>(ga + (size_t)(m + 1) * 64);
-      pb = *reinterpret_cast<const double4 *>(gb + (size_t)(m + 1) * 64);
-    }
-    double za[8], zb[8];
+  // Step s of the samples lives in LDS buffer s % 3.  While a step's FMAs
+  // run, the wave's next operands are read from LDS and the global loads of
+  // the step after next are in flight; the loop body covers two steps with
+  // two sets of load registers, so no register copy waits on a global load.
+  double za[8], zb[8], zn[8], zm[8];
+  auto lds_read = [&](int buf, double (&xa)[8], double (&xb)[8]) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) za[u] = sa[cur][wa * 8 + u][lane];
+    for (int u = 0; u < 8; ++u) xa[u] = sa[buf][wa * 8 + u][lane];
 #pragma unroll
-    for (int v = 0; v < 8; ++v) zb[v] = sb[cur][wb * 8 + v][lane];
+    for (int v = 0; v < 8; ++v) xb[v] = sb[buf][wb * 8 + v][lane];
+  };
+  auto fmas = [&]() {
 #pragma unroll
     for (int u = 0; u < 8; ++u)
 #pragma unroll
       for (int v = 0; v < 8; ++v) acc[u][v] = __builtin_fma(za[u], zb[v], acc[u][v]);
-    if (m + 1 < nsteps) {
-      *reinterpret_cast<double4 *>(&sa[cur ^ 1][lr][lc]) = pa;
-      *reinterpret_cast<double4 *>(&sb[cur ^ 1][lr][lc]) = pb;
-    }
-    __syncthreads();
+  };
+  const int last = nsteps - 1;
+#define HPC_LOAD(st, A0, A1, B0, B1)                                       \
+  do {                                                                     \
+    const size_t o_ = (size_t)(st) * 64;                                   \
+    A0 = *reinterpret_cast<const double2 *>(ga0 + o_);                     \
+    A1 = *reinterpret_cast<const double2 *>(ga1 + o_);                     \
+    B0 = *reinterpret_cast<const double2 *>(gb0 + o_);                     \
+    B1 = *reinterpret_cast<const double2 *>(gb1 + o_);                     \
+  } while (0)
+#define HPC_STORE(buf, A0, A1, B0, B1)                                     \
+  do {                                                                     \
+    *reinterpret_cast<double2 *>(&sa[buf][lr0][lc]) = A0;                  \
+    *reinterpret_cast<double2 *>(&sa[buf][lr1][lc]) = A1;                  \
+    *reinterpret_cast<double2 *>(&sb[buf][lr0][lc]) = B0;                  \
+    *reinterpret_cast<double2 *>(&sb[buf][lr1][lc]) = B1;                  \
+  } while (0)
+  // one step: FMAs on (za, zb) = step m while reading step m + 1 into
+  // (zn, zm) and storing step m + 2 from the given registers
+#define HPC_STEP(m, A0, A1, B0, B1)                                        \
+  do {                                                                     \
+    lds_read(((m) + 1) % 3, zn, zm);                                       \
+    fmas();                                                                \
+    __builtin_amdgcn_sched_barrier(0);                                     \
+    HPC_STORE(((m) + 2) % 3, A0, A1, B0, B1);                              \
+    __syncthreads();                                                       \
+    _Pragma("unroll") for (int u_ = 0; u_ < 8; ++u_) {                     \
+      za[u_] = zn[u_];                                                     \
+      zb[u_] = zm[u_];                                                     \
+    }                                                                      \
+  } while (0)
+  double2 pa0, pa1, pb0, pb1, qa0, qa1, qb0, qb1;
+  HPC_LOAD(0, pa0, pa1, pb0, pb1);
+  HPC_STORE(0, pa0, pa1, pb0, pb1);
+  HPC_LOAD(min(1, last), pa0, pa1, pb0, pb1);
+  HPC_STORE(1, pa0, pa1, pb0, pb1);
+  HPC_LOAD(min(2, last), pa0, pa1, pb0, pb1);
+  __syncthreads();
+  lds_read(0, za, zb);
+  int m = 0;
+  for (; m + 1 < nsteps; m += 2) {
+    // the p registers hold step m + 2
+    HPC_LOAD(min(m + 3, last), qa0, qa1, qb0, qb1);
+    HPC_STEP(m, pa0, pa1, pb0, pb1);
+    // the q registers hold step m + 3
+    HPC_LOAD(min(m + 4, last), pa0, pa1, pb0, pb1);
+    HPC_STEP(m + 1, qa0, qa1, qb0, qb1);
   }
+  if (m < nsteps) fmas();  // odd step count: the last step
+#undef HPC_STEP
+#undef HPC_LOAD
+#undef HPC_STORE
   const double div = (double)((float)I.n - 1.f);
 #pragma unroll
   for (int u = 0; u < 8; ++u)
@@ -282,8 +326,8 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(256) void corr_mfma_kernel(Layout L, const long long *__restrict__ prefix,
                                                         long long per_xcd) {
-  __shared__ float sa[FT][FK + 4];
-  __shared__ float sb[FT][FK + 4];
+  __shared__ float sa[2][FT][FK + 4];
+  __shared__ float sb[2][FT][FK + 4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   const long long g = blockIdx.x;
@@ -311,27 +355,41 @@ __global__ __launch_bounds__(256) void corr_mfma_kernel(Layout L, const long lon
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc64[u][v][r] = 0.0;
   const int li = lane & 15, lk = lane >> 4;
-  for (int k0 = 0; k0 < npad; k0 += FK) {
-    // 64 rows x 64 samples of each operand: 1024 float4 per operand
-    for (int idx = tid; idx < FT * FK / 4; idx += blockDim.x) {
-      const int r = idx / (FK / 4), c4 = (idx % (FK / 4)) * 4;
-      const int ra = min(a0 + r, p - 1), rb = min(b0 + r, p - 1);
-      const f4v va = *reinterpret_cast<const f4v *>(I.z32 + (size_t)ra * npad + k0 + c4);
-      const f4v vb = *reinterpret_cast<const f4v *>(I.z32 + (size_t)rb * npad + k0 + c4);
-      *reinterpret_cast<f4v *>(&sa[r][c4]) = va;
-      *reinterpret_cast<f4v *>(&sb[r][c4]) = vb;
+  // staging: thread tid moves float4 number tid + 256 j (j < 4) of each
+  // operand's 64 x 64 chunk: row (tid + 256 j) / 16, column 4 ((tid) % 16)
+  const int sr = tid >> 4, sc = (tid & 15) * 4;
+  const float *ga[4], *gb[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    ga[j] = I.z32 + (size_t)min(a0 + sr + 16 * j, p - 1) * npad + sc;
+    gb[j] = I.z32 + (size_t)min(b0 + sr + 16 * j, p - 1) * npad + sc;
+  }
+  const int nsteps = npad / FK, last = nsteps - 1;
+  auto load = [&](int st, f4v (&ra)[4], f4v (&rb)[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      ra[j] = *reinterpret_cast<const f4v *>(ga[j] + (size_t)st * FK);
+      rb[j] = *reinterpret_cast<const f4v *>(gb[j] + (size_t)st * FK);
     }
-    __syncthreads();
+  };
+  auto store = [&](int buf, const f4v (&ra)[4], const f4v (&rb)[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      *reinterpret_cast<f4v *>(&sa[buf][sr + 16 * j][sc]) = ra[j];
+      *reinterpret_cast<f4v *>(&sb[buf][sr + 16 * j][sc]) = rb[j];
+    }
+  };
+  auto chunk = [&](int buf) {
 #pragma unroll
     for (int u = 0; u < 2; ++u)
 #pragma unroll
       for (int v = 0; v < 2; ++v) acc32[u][v] = f4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 4
     for (int kk = 0; kk < FK; kk += 4) {
-      const float x0 = sa[wr * 32 + li][kk + lk];
-      const float x1 = sa[wr * 32 + 16 + li][kk + lk];
-      const float y0 = sb[wc * 32 + li][kk + lk];
-      const float y1 = sb[wc * 32 + 16 + li][kk + lk];
+      const float x0 = sa[buf][wr * 32 + li][kk + lk];
+      const float x1 = sa[buf][wr * 32 + 16 + li][kk + lk];
+      const float y0 = sb[buf][wc * 32 + li][kk + lk];
+      const float y1 = sb[buf][wc * 32 + 16 + li][kk + lk];
       acc32[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(x0, y0, acc32[0][0], 0, 0, 0);
       acc32[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(x0, y1, acc32[0][1], 0, 0, 0);
       acc32[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(x1, y0, acc32[1][0], 0, 0, 0);
@@ -343,8 +401,28 @@ __global__ __launch_bounds__(256) void corr_mfma_kernel(Layout L, const long lon
       for (int v = 0; v < 2; ++v)
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc64[u][v][r] += (double)acc32[u][v][r];
+  };
+  // chunk s in LDS buffer s & 1, the next one or two in registers (loaded two
+  // chunks ahead; the body covers two chunks with two register sets)
+  f4v pa[4], pb[4], qa[4], qb[4];
+  load(0, pa, pb);
+  store(0, pa, pb);
+  load(min(1, last), pa, pb);
+  __syncthreads();
+  int m = 0;
+  for (; m + 1 < nsteps; m += 2) {
+    load(min(m + 2, last), qa, qb);
+    chunk(0);
+    __builtin_amdgcn_sched_barrier(0);
+    store(1, pa, pb);
+    __syncthreads();
+    load(min(m + 3, last), pa, pb);
+    chunk(1);
+    __builtin_amdgcn_sched_barrier(0);
+    store(0, qa, qb);
     __syncthreads();
   }
+  if (m < nsteps) chunk(0);
   const double div = (double)((float)I.n - 1.f);
 #pragma unroll
   for (int u = 0; u < 2; ++u)

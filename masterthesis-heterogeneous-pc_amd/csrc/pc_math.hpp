@@ -101,11 +101,11 @@ __host__ __device__ __noinline__ double pcor_pinv(double (&A)[M][M]) {
 // (gpuRowTests.cu:143-170).  One reciprocal per pivot and one square root per
 // test.  A pivot at or below 1e-12 or a numerically singular 2x2 Schur block
 // switches to the pinv form.
-template <int L>
-__host__ __device__ __forceinline__ double pcor_canon(double a, const double (&x)[L],
-                                             const double (&y)[L],
-                                             const double (&C)[L][L],
-                                             bool &used_pinv) {
+template <int L, bool SENS>
+__host__ __device__ __forceinline__ double pcor_canon_impl(double a, const double (&x)[L],
+                                                  const double (&y)[L],
+                                                  const double (&C)[L][L], bool &used_pinv,
+                                                  double *sens) {
   double Um[L][L], Em[L][L], inv[L];
   double u[L], v[L];
   used_pinv = false;
@@ -138,18 +138,48 @@ __host__ __device__ __forceinline__ double pcor_canon(double a, const double (&x
     v[c] = sv;
   }
   double s00 = 1.0, s11 = 1.0, s01 = a;
+  double gx[L], gy[L];
 #pragma unroll
   for (int k = 0; k < L; ++k) {
     const double wu = u[k] * inv[k], wv = v[k] * inv[k];
     s00 = __builtin_fma(-wu, u[k], s00);
     s11 = __builtin_fma(-wv, v[k], s11);
     s01 = __builtin_fma(-wu, v[k], s01);
+    gx[k] = wu;
+    gy[k] = wv;
   }
   double q = s00 * s11;
   sing |= !(s00 > kPivTol) || !(s11 > kPivTol) ||
           !(__builtin_fma(-s01, s01, q) > kPivTol * q);
-  if (!sing) return s01 / __builtin_sqrt(q);
+  if (!sing) {
+    const double r = s01 / __builtin_sqrt(q);
+    if (SENS) {
+      // rows 0 and 1 of P = M^-1: [S^-1, -S^-1 G^T] with S the 2x2 Schur
+      // block and G = C^-1 [x y] = U^-T D^-1 [u v] (back substitution)
+#pragma unroll
+      for (int c = L - 1; c >= 0; --c) {
+#pragma unroll
+        for (int k = c + 1; k < L; ++k) {
+          gx[c] = __builtin_fma(-Um[k][c], gx[k], gx[c]);
+          gy[c] = __builtin_fma(-Um[k][c], gy[k], gy[c]);
+        }
+      }
+      const double id = 1.0 / __builtin_fma(-s01, s01, q);
+      const double p00 = s11 * id, p11 = s00 * id, p01 = -s01 * id;
+      double rho0 = __builtin_fabs(p00) + __builtin_fabs(p01);
+      double rho1 = __builtin_fabs(p01) + __builtin_fabs(p11);
+#pragma unroll
+      for (int k = 0; k < L; ++k) {
+        rho0 += __builtin_fabs(p00 * gx[k] + p01 * gy[k]);
+        rho1 += __builtin_fabs(p01 * gx[k] + p11 * gy[k]);
+      }
+      *sens = rho0 * rho1 / __builtin_sqrt(p00 * p11) +
+              __builtin_fabs(r) * 0.5 * (rho0 * rho0 / p00 + rho1 * rho1 / p11);
+    }
+    return r;
+  }
   used_pinv = true;
+  if (SENS) *sens = __builtin_inf();
   double A[L + 2][L + 2];
   for (int r = 0; r < L + 2; ++r) A[r][r] = 1.0;
   A[0][1] = A[1][0] = a;
@@ -159,6 +189,24 @@ __host__ __device__ __forceinline__ double pcor_canon(double a, const double (&x
     for (int r = c + 1; r < L; ++r) A[r + 2][c + 2] = A[c + 2][r + 2] = C[r][c];
   }
   return pcor_pinv<L + 2>(A);
+}
+
+template <int L>
+__host__ __device__ __forceinline__ double pcor_canon(double a, const double (&x)[L],
+                                             const double (&y)[L],
+                                             const double (&C)[L][L],
+                                             bool &used_pinv) {
+  return pcor_canon_impl<L, false>(a, x, y, C, used_pinv, nullptr);
+}
+
+// pcor_canon plus the first-order sensitivity factor of pcor_sensitivity
+// (+inf when the pinv form is taken), sharing the factorisation.
+template <int L>
+__host__ __device__ __forceinline__ double pcor_canon_sens(double a, const double (&x)[L],
+                                                  const double (&y)[L],
+                                                  const double (&C)[L][L], bool &used_pinv,
+                                                  double &sens) {
+  return pcor_canon_impl<L, true>(a, x, y, C, used_pinv, &sens);
 }
 
 // First-order sensitivity of the partial correlation to its inputs: with

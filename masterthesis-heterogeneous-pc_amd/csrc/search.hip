@@ -253,13 +253,16 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_kernel(
           }
         if (!anynan) {
           bool fb;
-          const double r = pcor_canon<LV>(a, x, yv, C, fb);
-          pass = removes(r, r_crit);
-          F += fb ? 1 : 0;
+          double r;
           if (fast) {
-            unc = fb || !fast_certain(r, r_crit, pcor_sensitivity<LV>(a, x, yv, C), In.eps);
-            pass &= !unc;
+            double sens;
+            r = pcor_canon_sens<LV>(a, x, yv, C, fb, sens);
+            unc = fb || !fast_certain(r, r_crit, sens, In.eps);
+          } else {
+            r = pcor_canon<LV>(a, x, yv, C, fb);
           }
+          pass = removes(r, r_crit) && !unc;
+          F += fb ? 1 : 0;
         }
       }
       const unsigned long long m = __ballot(pass);
@@ -362,13 +365,16 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
               }
             if (!anynan) {
               bool fb;
-              const double r = pcor_canon<LV>(a, x, yv, C, fb);
-              pass = removes(r, r_crit);
-              F += fb ? 1 : 0;
+              double r;
               if (fast) {
-                unc = fb || !fast_certain(r, r_crit, pcor_sensitivity<LV>(a, x, yv, C), In.eps);
-                pass &= !unc;
+                double sens;
+                r = pcor_canon_sens<LV>(a, x, yv, C, fb, sens);
+                unc = fb || !fast_certain(r, r_crit, sens, In.eps);
+              } else {
+                r = pcor_canon<LV>(a, x, yv, C, fb);
               }
+              pass = removes(r, r_crit) && !unc;
+              F += fb ? 1 : 0;
             }
           }
           const unsigned long long m = __ballot(pass);
