@@ -118,6 +118,24 @@ enum {
 };
 int hpc_set_tuning(hpc_state *st, int param, int value);
 
+/* Autotuner (no counterpart in the reference): runs the state's levels
+ * 1..max_level once, timing each level's CI-test kernel on that level's
+ * snapshot for every candidate launch shape -- level 1: ordered edges per
+ * workgroup in {8..128}; levels >= 2: conditioning sets per task in
+ * {128..2048} x tasks per workgroup in {4..32} -- and keeps the fastest per
+ * level for this state's later runs.  Results do not depend on the shape.
+ * The state is left reset (adjacency and sepsets back to the start). */
+typedef struct hpc_tune_entry {
+  int32_t level;
+  int32_t ranks_per_task;    /* levels >= 2 */
+  int32_t tasks_per_block;   /* levels >= 2 */
+  int32_t edges_per_block;   /* level 1 */
+  double ms_best;            /* CI-test kernel time with the chosen shape */
+  double ms_default;         /* with the default shape */
+} hpc_tune_entry;
+int hpc_autotune(hpc_state *st, int max_level, hpc_tune_entry *out, int cap,
+                 int *levels);
+
 /* ---- correlation (src/correlation/corOwn.cu:10-48) ---------------------- */
 /* gpuPMCC(h_mat, p, n, state.cor, device): uploads the variable-major
  * observations and computes the correlation matrix on the device. */

@@ -184,6 +184,18 @@ extern "C" int hpc_set_tuning(hpc_state *st, int param, int value) {
   }
 }
 
+extern "C" int hpc_autotune(hpc_state *st, int max_level, hpc_tune_entry *out, int cap,
+                            int *levels) {
+  if (!st) return fail(HPC_EINVAL, "state is NULL");
+  if (H(st).fast) return fail(HPC_ESTATE, "the autotuner needs an exact correlation");
+  std::vector<hpc_tune_entry> res;
+  int rc = engine_autotune(st->eng, st->id, max_level, res);
+  if (rc) return rc;
+  for (int k = 0; out && k < (int)res.size() && k < cap; ++k) out[k] = res[k];
+  if (levels) *levels = (int)res.size();
+  return HPC_OK;
+}
+
 // packs the state's one instance at `level`
 static int pack_at(hpc_state *st, int level) {
   return engine_pack(st->eng, {st->id}, {level});

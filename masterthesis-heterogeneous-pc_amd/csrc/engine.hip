@@ -663,14 +663,16 @@ int engine_search(Engine &e, int level, const int32_t *rows, int nrows, float *m
   // level >= 2 runs before it)
   bool any = e.n_edges > 0 && (!rows || nrows > 0);
   if (any && level >= 2) {
-    launch_task_plan(e.stream, level, L, e.ranks_per_task, mask, e.d_row_tasks, e.d_task_off);
+    launch_task_plan(e.stream, level, L, engine_rpt(e, level), mask, e.d_row_tasks, e.d_task_off);
     HPC_HIPCHK(hipGetLastError());
   }
   HPC_HIPCHK(hipEventRecord(e.ev[5], e.stream));
   const bool fast = any && any_fast_at(e, level);
   if (any) {
-    if (level == 1) launch_search_l1(e.stream, L, e.n_edges, mask, fast);
-    else launch_search_ln(e.stream, level, L, e.ranks_per_task, e.d_row_tasks, e.d_task_off, fast);
+    if (level == 1) launch_search_l1(e.stream, L, e.n_edges, mask, fast, engine_epb(e));
+    else
+      launch_search_ln(e.stream, level, L, engine_rpt(e, level), e.d_row_tasks, e.d_task_off, fast,
+                       engine_tpb(e, level));
     if (fast) launch_recheck(e.stream, level, L);
   }
   HPC_HIPCHK(hipGetLastError());
@@ -874,16 +876,17 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
     if (run1) {
       HPC_HIPCHK(hipEventRecord(e.sev[2 * npairs], e.stream));
       const bool f1 = any_fast_at(e, 1);
-      launch_search_l1(e.stream, L, e.n_edges, nullptr, f1);
+      launch_search_l1(e.stream, L, e.n_edges, nullptr, f1, engine_epb(e));
       if (f1) launch_recheck(e.stream, 1, L);
       HPC_HIPCHK(hipEventRecord(e.sev[2 * npairs + 1], e.stream));
       ++npairs;
     }
     for (int l : levels_ge2) {
-      launch_task_plan(e.stream, l, L, e.ranks_per_task, nullptr, e.d_row_tasks, e.d_task_off);
+      launch_task_plan(e.stream, l, L, engine_rpt(e, l), nullptr, e.d_row_tasks, e.d_task_off);
       HPC_HIPCHK(hipEventRecord(e.sev[2 * npairs], e.stream));
       const bool fl = any_fast_at(e, l);
-      launch_search_ln(e.stream, l, L, e.ranks_per_task, e.d_row_tasks, e.d_task_off, fl);
+      launch_search_ln(e.stream, l, L, engine_rpt(e, l), e.d_row_tasks, e.d_task_off, fl,
+                       engine_tpb(e, l));
       if (fl) launch_recheck(e.stream, l, L);
       HPC_HIPCHK(hipEventRecord(e.sev[2 * npairs + 1], e.stream));
       ++npairs;
@@ -956,6 +959,80 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
       h.finished = true;
   }
   return HPC_OK;
+}
+
+int engine_rpt(const Engine &e, int level) {
+  const int v = (level >= 0 && level <= kMaxLevel) ? e.rpt_level[level] : 0;
+  return v > 0 ? v : e.ranks_per_task;
+}
+
+int engine_tpb(const Engine &e, int level) {
+  const int v = (level >= 0 && level <= kMaxLevel) ? e.tpb_level[level] : 0;
+  return v > 0 ? v : 16;
+}
+
+int engine_epb(const Engine &e) { return e.epb_level1 > 0 ? e.epb_level1 : 32; }
+
+int engine_autotune(Engine &e, int id, int max_level, std::vector<hpc_tune_entry> &out) {
+  InstHost &h = e.inst[id];
+  if (!h.have_cor) return hpc_fail(HPC_ESTATE, "no correlation matrix");
+  if (max_level < 0 || max_level > kMaxLevel) max_level = kMaxLevel;
+  out.clear();
+  int rc = engine_reset(e, id);
+  if (!rc) rc = engine_pack(e, {id}, {0});
+  if (!rc) rc = engine_level0(e, nullptr);
+  if (rc) return rc;
+  for (int l = 1; l <= max_level; ++l) {
+    rc = engine_pack(e, {id}, {l});
+    if (!rc) rc = engine_compact(e, nullptr);
+    if (rc) return rc;
+    if (h.max_deg - 1 < l || e.n_edges == 0) break;
+    hpc_tune_entry t;
+    memset(&t, 0, sizeof(t));
+    t.level = l;
+    // candidates (ranks per task, tasks per workgroup) or edges per workgroup
+    std::vector<std::pair<int, int>> cand;
+    if (l == 1) {
+      for (int epb : {8, 16, 32, 64, 128}) cand.push_back({0, epb});
+    } else {
+      for (int rpt : {128, 256, 512, 1024, 2048})
+        for (int tpb : {4, 8, 16, 32}) cand.push_back({rpt, tpb});
+    }
+    double best = 1e300;
+    std::pair<int, int> pick = cand[0];
+    for (const auto &c : cand) {
+      if (l == 1) e.epb_level1 = c.second;
+      else { e.rpt_level[l] = c.first; e.tpb_level[l] = c.second; }
+      double ms_min = 1e300;
+      for (int rep = 0; rep < 2; ++rep) {
+        HPC_HIPCHK(hipMemsetAsync(e.d_fp, 0xFF, (size_t)e.n_edges * sizeof(int64_t), e.stream));
+        float ms = 0.f;
+        rc = engine_search(e, l, nullptr, 0, &ms);
+        if (rc) return rc;
+        ms_min = std::min(ms_min, (double)ms);
+      }
+      const bool dflt = l == 1 ? c.second == 32 : (c.first == 512 && c.second == 16);
+      if (dflt) t.ms_default = ms_min;
+      if (ms_min < best) { best = ms_min; pick = c; }
+    }
+    if (l == 1) {
+      e.epb_level1 = pick.second;
+      t.edges_per_block = pick.second;
+    } else {
+      e.rpt_level[l] = pick.first;
+      e.tpb_level[l] = pick.second;
+      t.ranks_per_task = pick.first;
+      t.tasks_per_block = pick.second;
+    }
+    t.ms_best = best;
+    out.push_back(t);
+    // the level itself, with the chosen shape
+    HPC_HIPCHK(hipMemsetAsync(e.d_fp, 0xFF, (size_t)e.n_edges * sizeof(int64_t), e.stream));
+    rc = engine_search(e, l, nullptr, 0, nullptr);
+    if (!rc) rc = engine_commit(e, nullptr);
+    if (rc) return rc;
+  }
+  return engine_reset(e, id);
 }
 
 }  // namespace hpc

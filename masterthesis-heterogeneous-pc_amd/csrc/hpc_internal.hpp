@@ -134,14 +134,16 @@ void launch_level0(hipStream_t s, const Layout &L, const std::vector<Inst> &host
 void launch_compact(hipStream_t s, const Layout &L);
 // `fast`: the launch holds an instance with fast correlations (selects the
 // variant with uncertainty bounds and the recheck queue)
+// epb: ordered edges per workgroup of the LDS-staged form
 void launch_search_l1(hipStream_t s, const Layout &L, int64_t n_edges,
-                      const int32_t *row_mask, bool fast);
+                      const int32_t *row_mask, bool fast, int epb);
 void launch_task_plan(hipStream_t s, int level, const Layout &L, int rpt,
                       const int32_t *row_mask, long long *row_tasks, long long *task_off);
 // The plan's per-row task counts and exclusive offsets define the tasks; the
 // kernels read the task count from the device counters.
+// tpb: tasks per workgroup of the LDS-staged form
 void launch_search_ln(hipStream_t s, int level, const Layout &L, int rpt,
-                      const long long *row_tasks, const long long *task_off, bool fast);
+                      const long long *row_tasks, const long long *task_off, bool fast, int tpb);
 void launch_commit(hipStream_t s, const Layout &L, int64_t n_edges);
 void launch_scatter_fp(hipStream_t s, const int64_t *idx, const int64_t *val,
                        long long count, int64_t *fp);
@@ -225,6 +227,11 @@ struct Engine {
   int64_t n_edges = 0;
   int max_deg = 0;
   int ranks_per_task = 512;
+  // launch shapes per level (set by the autotuner; 0 = defaults: ranks per
+  // task `ranks_per_task`, 16 tasks and 32 ordered edges per workgroup)
+  int rpt_level[kMaxLevel + 1] = {};
+  int tpb_level[kMaxLevel + 1] = {};
+  int epb_level1 = 0;
   hipEvent_t ev[8] = {};
   hipEvent_t sev[2 * (kMaxLevel + 1)] = {};  // search kernel brackets of one step
   hipEvent_t fp_ready = nullptr;  // recorded after the first-pass array is reset
@@ -282,6 +289,15 @@ void engine_note_level(Engine &e, int id, int level);
 // three host synchronisations for the whole set.  Finished instances are
 // marked; their per-level records go to InstHost::hist.
 int engine_step(Engine &e, const std::vector<int> &ids);
+// Launch shapes of a level's CI-test kernel.
+int engine_rpt(const Engine &e, int level);
+int engine_tpb(const Engine &e, int level);
+int engine_epb(const Engine &e);
+// Autotuner: runs instance id's levels 1..max_level, timing each level's
+// CI-test kernel on the level's snapshot for every candidate launch shape
+// (results do not depend on the shape), keeps the fastest per level in the
+// engine and leaves the instance reset.
+int engine_autotune(Engine &e, int id, int max_level, std::vector<hpc_tune_entry> &out);
 // Device bytes one instance needs (its blocks plus its share of the per-level
 // arrays).
 size_t engine_footprint(int p, int n, int max_level);

@@ -607,7 +607,6 @@ size_t lds_bytes(int max_deg, int lv) {
 }
 
 constexpr size_t kLdsLimit = 64 * 1024;
-constexpr int kTasksPerBlock = 16;
 
 // Resident workgroups of the grid-stride search kernels (the task count is
 // only known on the device): enough to fill all 256 CUs several times.
@@ -615,12 +614,12 @@ constexpr int kSearchBlocks = 256 * 8;
 
 template <int LV, bool FAST>
 void launch_ln_as(hipStream_t s, const Layout &L, int rpt, const long long *row_tasks,
-                  const long long *task_off) {
+                  const long long *task_off, int tpb) {
   const size_t staged = lds_bytes(L.max_deg, LV);
   if (staged <= kLdsLimit) {
     hipLaunchKernelGGL((search_ln_lds_kernel<LV, FAST>), dim3(kSearchBlocks),
                        dim3(64 * kWavesPerBlock), staged, s, L, rpt, task_off, row_tasks,
-                       kTasksPerBlock, L.max_deg + 1);
+                       tpb, L.max_deg + 1);
     return;
   }
   const size_t lds = (size_t)(L.max_deg + 1) * (LV + 1) * sizeof(unsigned long long);
@@ -633,19 +632,18 @@ void launch_ln_as(hipStream_t s, const Layout &L, int rpt, const long long *row_
 // instantiated into launches that hold a fast instance.
 template <int LV>
 void launch_ln(hipStream_t s, const Layout &L, int rpt, const long long *row_tasks,
-               const long long *task_off, bool fast) {
-  if (fast) launch_ln_as<LV, true>(s, L, rpt, row_tasks, task_off);
-  else launch_ln_as<LV, false>(s, L, rpt, row_tasks, task_off);
+               const long long *task_off, bool fast, int tpb) {
+  if (fast) launch_ln_as<LV, true>(s, L, rpt, row_tasks, task_off, tpb);
+  else launch_ln_as<LV, false>(s, L, rpt, row_tasks, task_off, tpb);
 }
 
 }  // namespace
 
 void launch_search_l1(hipStream_t s, const Layout &L, int64_t n_edges, const int32_t *row_mask,
-                      bool fast) {
+                      bool fast, int epb) {
   if (n_edges <= 0) return;
   const size_t staged = (size_t)L.max_deg * 12;
   if (staged <= kLdsLimit) {
-    const int epb = 32;
     const long long blocks = (n_edges + epb - 1) / epb;
     if (fast)
       hipLaunchKernelGGL(search_l1_lds_kernel<true>, dim3((unsigned)blocks),
@@ -674,23 +672,23 @@ void launch_task_plan(hipStream_t s, int level, const Layout &L, int rpt,
 }
 
 void launch_search_ln(hipStream_t s, int level, const Layout &L, int rpt,
-                      const long long *row_tasks, const long long *task_off, bool fast) {
+                      const long long *row_tasks, const long long *task_off, bool fast, int tpb) {
   switch (level) {
-    case 2: launch_ln<2>(s, L, rpt, row_tasks, task_off, fast); break;
-    case 3: launch_ln<3>(s, L, rpt, row_tasks, task_off, fast); break;
-    case 4: launch_ln<4>(s, L, rpt, row_tasks, task_off, fast); break;
-    case 5: launch_ln<5>(s, L, rpt, row_tasks, task_off, fast); break;
-    case 6: launch_ln<6>(s, L, rpt, row_tasks, task_off, fast); break;
-    case 7: launch_ln<7>(s, L, rpt, row_tasks, task_off, fast); break;
-    case 8: launch_ln<8>(s, L, rpt, row_tasks, task_off, fast); break;
-    case 9: launch_ln<9>(s, L, rpt, row_tasks, task_off, fast); break;
-    case 10: launch_ln<10>(s, L, rpt, row_tasks, task_off, fast); break;
-    case 11: launch_ln<11>(s, L, rpt, row_tasks, task_off, fast); break;
-    case 12: launch_ln<12>(s, L, rpt, row_tasks, task_off, fast); break;
-    case 13: launch_ln<13>(s, L, rpt, row_tasks, task_off, fast); break;
-    case 14: launch_ln<14>(s, L, rpt, row_tasks, task_off, fast); break;
-    case 15: launch_ln<15>(s, L, rpt, row_tasks, task_off, fast); break;
-    case 16: launch_ln<16>(s, L, rpt, row_tasks, task_off, fast); break;
+    case 2: launch_ln<2>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
+    case 3: launch_ln<3>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
+    case 4: launch_ln<4>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
+    case 5: launch_ln<5>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
+    case 6: launch_ln<6>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
+    case 7: launch_ln<7>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
+    case 8: launch_ln<8>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
+    case 9: launch_ln<9>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
+    case 10: launch_ln<10>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
+    case 11: launch_ln<11>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
+    case 12: launch_ln<12>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
+    case 13: launch_ln<13>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
+    case 14: launch_ln<14>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
+    case 15: launch_ln<15>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
+    case 16: launch_ln<16>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
     default: break;  // rejected by the caller (kMaxLevel)
   }
 }

@@ -64,6 +64,17 @@ class ServerInfo(C.Structure):
     ]
 
 
+class TuneEntry(C.Structure):
+    _fields_ = [
+        ("level", C.c_int32),
+        ("ranks_per_task", C.c_int32),
+        ("tasks_per_block", C.c_int32),
+        ("edges_per_block", C.c_int32),
+        ("ms_best", C.c_double),
+        ("ms_default", C.c_double),
+    ]
+
+
 class HpcError(RuntimeError):
     pass
 
@@ -83,6 +94,7 @@ SIGNATURES = {
     "hpc_sync": (_i, [_vp]),
     "hpc_max_cond_size": (_i, [_vp]),
     "hpc_set_tuning": (_i, [_vp, _i, _i]),
+    "hpc_autotune": (_i, [_vp, _i, C.POINTER(TuneEntry), _i, C.POINTER(C.c_int)]),
     "hpc_pmcc": (_i, [_vp, _vp, _i]),
     "hpc_set_observations": (_i, [_vp, _vp]),
     "hpc_corr": (_i, [_vp, _i]),

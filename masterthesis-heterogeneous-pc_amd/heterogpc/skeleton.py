@@ -103,6 +103,17 @@ class DeviceSkeleton:
     def set_ranks_per_task(self, value: int) -> None:
         check(self.lib.hpc_set_tuning(self.h, 1, int(value)))
 
+    def autotune(self, max_level: int = 4) -> list[dict]:
+        """Per-level launch shapes of the CI-test kernels, timed on this
+        state's own levels; the state is left reset."""
+        from ._lib import TuneEntry
+
+        buf = (TuneEntry * 32)()
+        n = C.c_int()
+        check(self.lib.hpc_autotune(self.h, int(max_level), buf, 32, C.byref(n)))
+        return [{name: getattr(buf[k], name) for name, _ in TuneEntry._fields_}
+                for k in range(min(n.value, 32))]
+
     def set_recheck_queue(self, value: int) -> None:
         """Length of the HPC_CORR_F32 recheck queues (0 = full)."""
         check(self.lib.hpc_set_tuning(self.h, 2, int(value)))

@@ -196,3 +196,27 @@ def test_host_share_alone_matches_device(hpc, orc):
         assert np.array_equal(fa, fb), level
         assert np.array_equal(full.adj(), host.adj()), level
     assert np.array_equal(full.sepsets(5), host.sepsets(5))
+
+
+def test_autotune_picks_shapes_and_keeps_results(hpc, orc):
+    """The autotuner times every launch shape per level on the state's own
+    snapshots, leaves the state reset, and the tuned run equals the oracle."""
+    x, n, alpha, _ = workloads.sem("c2")
+    p = x.shape[0]
+    with hpc.DeviceSkeleton(p, n, alpha, max_level=4) as dev:
+        dev.pmcc(x)
+        entries = dev.autotune(4)
+        print(f"[autotune] {entries}", flush=True)
+        assert [t["level"] for t in entries] == list(range(1, len(entries) + 1))
+        assert entries[0]["edges_per_block"] in (8, 16, 32, 64, 128)
+        for t in entries[1:]:
+            assert t["ranks_per_task"] in (128, 256, 512, 1024, 2048)
+            assert t["tasks_per_block"] in (4, 8, 16, 32)
+            assert 0 < t["ms_best"] <= t["ms_default"]
+        assert dev.adj().sum() == p * (p - 1)  # reset
+        st = dev.run(4)
+        ref = orc.Skeleton(orc.corr(x), n, alpha, max_level=4)
+        recs = ref.run(4)
+        assert np.array_equal(dev.adj(), ref.adj())
+        assert np.array_equal(dev.sepsets(ref.mc()), ref.sepsets())
+        assert [s["tests"] for s in st] == [r["tests"] for r in recs]
