@@ -135,6 +135,12 @@ typedef struct hpc_tune_entry {
 } hpc_tune_entry;
 int hpc_autotune(hpc_state *st, int max_level, hpc_tune_entry *out, int cap,
                  int *levels);
+/* Launch shapes chosen by hpc_autotune (its entries, any subset of levels)
+ * for this state's later runs; no counterpart in the reference.  Every entry
+ * is checked before any is applied (level 1: edges_per_block in 1..4096;
+ * levels >= 2: ranks_per_task a multiple of 64 in [64, 2^24] and
+ * tasks_per_block in 1..1024); results do not depend on the shapes. */
+int hpc_apply_tuning(hpc_state *st, const hpc_tune_entry *entries, int count);
 
 /* ---- correlation (src/correlation/corOwn.cu:10-48) ---------------------- */
 /* gpuPMCC(h_mat, p, n, state.cor, device): uploads the variable-major
@@ -237,6 +243,8 @@ int hpc_batch_get_sepsets(hpc_batch *b, int id, int32_t *sep, int mc);
 int hpc_batch_get_sepset_list(hpc_batch *b, int id, int32_t *out, int64_t cap,
                               int64_t *words);
 int hpc_batch_set_tuning(hpc_batch *b, int param, int value);
+/* hpc_apply_tuning for every instance of the batch. */
+int hpc_batch_apply_tuning(hpc_batch *b, const hpc_tune_entry *entries, int count);
 /* Correlation mode (HPC_CORR_*) of an instance, before its first step. */
 int hpc_batch_set_corr_mode(hpc_batch *b, int id, int mode);
 
@@ -292,6 +300,9 @@ int hpc_server_sepset_list(hpc_server *s, int64_t req, int32_t *out, int64_t cap
                            int64_t *words);
 /* Correlation mode (HPC_CORR_*) of the requests submitted from now on. */
 int hpc_server_set_corr_mode(hpc_server *s, int mode);
+/* hpc_apply_tuning for the server's later steps (taken up by the worker
+ * before its next step). */
+int hpc_server_apply_tuning(hpc_server *s, const hpc_tune_entry *entries, int count);
 /* Drops a finished request's results. */
 int hpc_server_release(hpc_server *s, int64_t req);
 int hpc_server_stats(hpc_server *s, hpc_server_info *info);

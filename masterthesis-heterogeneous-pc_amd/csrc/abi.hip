@@ -184,6 +184,11 @@ extern "C" int hpc_set_tuning(hpc_state *st, int param, int value) {
   }
 }
 
+extern "C" int hpc_apply_tuning(hpc_state *st, const hpc_tune_entry *entries, int count) {
+  if (!st) return fail(HPC_EINVAL, "state is NULL");
+  return engine_apply_tuning(st->eng, entries, count);
+}
+
 extern "C" int hpc_autotune(hpc_state *st, int max_level, hpc_tune_entry *out, int cap,
                             int *levels) {
   if (!st) return fail(HPC_EINVAL, "state is NULL");

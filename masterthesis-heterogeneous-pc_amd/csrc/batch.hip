@@ -189,6 +189,11 @@ extern "C" int hpc_batch_set_tuning(hpc_batch *b, int param, int value) {
   return HPC_OK;
 }
 
+extern "C" int hpc_batch_apply_tuning(hpc_batch *b, const hpc_tune_entry *entries, int count) {
+  if (!b) return hpc_fail(HPC_EINVAL, "batch is NULL");
+  return engine_apply_tuning(b->eng, entries, count);
+}
+
 // ---------------------------------------------------------------------------
 // server
 // ---------------------------------------------------------------------------
@@ -233,6 +238,9 @@ struct hpc_server {
   int64_t next_id = 1;
   int64_t submits = 0;
   int corr_mode = HPC_CORR_CANON;  // of requests admitted from now on
+  // launch shapes for the worker to apply before its next step
+  std::vector<hpc_tune_entry> tune;
+  bool tune_pending = false;
   bool stop = false;
   std::thread worker;
   // observation uploads: the uploader thread copies admitted requests' data
@@ -382,6 +390,11 @@ static void server_loop(hpc_server *s) {
     std::vector<int> ids;
     {
       std::lock_guard<std::mutex> lk(s->mu);
+      if (s->tune_pending) {
+        // checked by hpc_server_apply_tuning
+        (void)engine_apply_tuning(s->eng, s->tune.data(), (int)s->tune.size());
+        s->tune_pending = false;
+      }
       for (auto &r : s->running) ids.push_back(r->slot);
       s->peak_active = std::max(s->peak_active, (int)ids.size());
       s->peak_bytes = std::max(s->peak_bytes, s->in_use);
@@ -575,6 +588,17 @@ extern "C" int hpc_server_set_corr_mode(hpc_server *s, int mode) {
     return hpc_fail(HPC_EINVAL, "unknown correlation mode %d", mode);
   std::lock_guard<std::mutex> lk(s->mu);
   s->corr_mode = mode;
+  return HPC_OK;
+}
+
+extern "C" int hpc_server_apply_tuning(hpc_server *s, const hpc_tune_entry *entries, int count) {
+  if (!s) return hpc_fail(HPC_EINVAL, "server is NULL");
+  // checked here; the worker applies them before its next step
+  const int rc = engine_check_tuning(entries, count);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->tune.insert(s->tune.end(), entries, entries + count);
+  s->tune_pending = true;
   return HPC_OK;
 }
 

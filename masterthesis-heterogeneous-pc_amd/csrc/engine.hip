@@ -138,6 +138,8 @@ static int grow(T *&p, size_t &cap, size_t n) {
   return HPC_OK;
 }
 
+static int ensure_inst_cap(Engine &e, size_t K);
+
 int engine_init(Engine &e, int device) {
   int ndev = 0;
   HPC_HIPCHK(hipGetDeviceCount(&ndev));
@@ -150,10 +152,11 @@ int engine_init(Engine &e, int device) {
   for (auto &ev : e.ev) HPC_HIPCHK(hipEventCreate(&ev));
   for (auto &ev : e.sev) HPC_HIPCHK(hipEventCreate(&ev));
   HPC_HIPCHK(hipEventCreateWithFlags(&e.fp_ready, hipEventDisableTiming));
-  int rc = dalloc((void **)&e.d_ctr, sizeof(Counters));
-  if (rc) return rc;
-  HPC_HIPCHK(hipHostMalloc((void **)&e.h_ctr, sizeof(Counters), hipHostMallocDefault));
-  return HPC_OK;
+  HPC_HIPCHK(hipHostMalloc((void **)&e.h_seq, 64, hipHostMallocMapped | hipHostMallocCoherent));
+  HPC_HIPCHK(hipHostGetDevicePointer((void **)&e.d_seq, e.h_seq, 0));
+  __atomic_store_n(e.h_seq, 0ull, __ATOMIC_RELEASE);
+  e.seq = 0;
+  return ensure_inst_cap(e, 1);
 }
 
 static void free_obs(Engine &e, InstHost &h) {
@@ -174,14 +177,16 @@ void engine_free(Engine &e) {
   for (auto &sl : e.arena.slabs)
     if (sl.base) (void)hipFree(sl.base);
   e.arena = Arena();
-  dfree(e.d_inst); dfree(e.d_ictr); dfree(e.d_row_inst); dfree(e.d_deg); dfree(e.d_rowptr);
+  dfree(e.d_inst); dfree(e.d_row_inst); dfree(e.d_deg); dfree(e.d_rowptr);
   dfree(e.d_nbr); dfree(e.d_erow); dfree(e.d_rowmask); dfree(e.d_fp); dfree(e.d_row_tasks);
   dfree(e.d_task_off); dfree(e.d_binom); dfree(e.d_prefix); dfree(e.d_ctr);
+  e.d_ictr = nullptr;
   dfree(e.d_scatter_idx); dfree(e.d_scatter_val); dfree(e.d_rchk); dfree(e.d_pairs);
   if (e.h_ctr) (void)hipHostFree(e.h_ctr);
-  if (e.h_ictr) (void)hipHostFree(e.h_ictr);
-  e.h_ctr = nullptr;
+  e.h_ctr = e.h_ctr_dev = nullptr;
   e.h_ictr = nullptr;
+  if (e.h_seq) (void)hipHostFree(e.h_seq);
+  e.h_seq = e.d_seq = nullptr;
   for (auto &ev : e.ev)
     if (ev) (void)hipEventDestroy(ev);
   for (auto &ev : e.sev)
@@ -352,17 +357,50 @@ static int ensure_rec(Engine &e, InstHost &h, int level) {
 static int ensure_inst_cap(Engine &e, size_t K) {
   if (K <= e.cap_inst && e.d_inst) return HPC_OK;
   const size_t c = std::max<size_t>(K + K / 4, 8);
-  dfree(e.d_inst); dfree(e.d_ictr); dfree(e.d_prefix);
-  if (e.h_ictr) (void)hipHostFree(e.h_ictr);
-  e.h_ictr = nullptr;
+  dfree(e.d_inst); dfree(e.d_ctr); dfree(e.d_prefix);
+  if (e.h_ctr) (void)hipHostFree(e.h_ctr);
+  e.h_ctr = e.h_ctr_dev = nullptr;
+  e.d_ictr = e.h_ictr = nullptr;
   e.cap_inst = 0;
+  const size_t block = kIctrOffset + c * sizeof(InstCounters);
   int rc = dalloc((void **)&e.d_inst, c * sizeof(Inst));
-  if (!rc) rc = dalloc((void **)&e.d_ictr, c * sizeof(InstCounters));
+  if (!rc) rc = dalloc((void **)&e.d_ctr, block);
   if (!rc) rc = dalloc((void **)&e.d_prefix, 4 * (c + 1) * sizeof(long long));
-  if (!rc && hipHostMalloc((void **)&e.h_ictr, c * sizeof(InstCounters), hipHostMallocDefault) != hipSuccess)
+  // fine-grained, so the publication kernel's stores reach host memory
+  if (!rc && hipHostMalloc((void **)&e.h_ctr, block, hipHostMallocMapped | hipHostMallocCoherent) !=
+                 hipSuccess) {
+    e.h_ctr = nullptr;
     rc = hpc_fail(HPC_ENOMEM, "hipHostMalloc failed");
-  if (!rc) e.cap_inst = c;
-  return rc;
+  }
+  if (!rc && hipHostGetDevicePointer((void **)&e.h_ctr_dev, e.h_ctr, 0) != hipSuccess)
+    rc = hpc_fail(HPC_EHIP, "hipHostGetDevicePointer failed");
+  if (rc) return rc;
+  e.d_ictr = reinterpret_cast<InstCounters *>(reinterpret_cast<char *>(e.d_ctr) + kIctrOffset);
+  e.h_ictr = reinterpret_cast<InstCounters *>(reinterpret_cast<char *>(e.h_ctr) + kIctrOffset);
+  // a fresh block holds no counts
+  HPC_HIPCHK(hipMemset(e.d_ctr, 0, block));
+  e.cap_inst = c;
+  return HPC_OK;
+}
+
+// The packed instance table to the device: a kernel argument when it is
+// small, else a copy of e.h_inst (the next pack waits for it).
+static int upload_table(Engine &e) {
+  const int K = (int)e.h_inst.size();
+  if (K == 0) return HPC_OK;
+  if (K <= kTableArg) {
+    TableArg a;
+    memset(&a, 0, sizeof(a));
+    for (int k = 0; k < K; ++k) a.inst[k] = e.h_inst[k];
+    a.K = K;
+    launch_table(e.stream, e.d_inst, a);
+    HPC_HIPCHK(hipGetLastError());
+    return HPC_OK;
+  }
+  HPC_HIPCHK(hipMemcpyAsync(e.d_inst, e.h_inst.data(), K * sizeof(Inst), hipMemcpyHostToDevice,
+                            e.stream));
+  e.host_upload_pending = true;
+  return HPC_OK;
 }
 
 static int ensure_row_cap(Engine &e, size_t rows) {
@@ -371,6 +409,7 @@ static int ensure_row_cap(Engine &e, size_t rows) {
   dfree(e.d_row_inst); dfree(e.d_deg); dfree(e.d_rowptr); dfree(e.d_rowmask);
   dfree(e.d_row_tasks); dfree(e.d_task_off);
   e.cap_rows = 0;
+  e.row_inst_valid = false;
   int rc = dalloc((void **)&e.d_row_inst, c * sizeof(int32_t));
   if (!rc) rc = dalloc((void **)&e.d_deg, c * sizeof(int32_t));
   if (!rc) rc = dalloc((void **)&e.d_rowptr, (c + 1) * sizeof(int32_t));
@@ -407,8 +446,11 @@ static int ensure_recheck_cap(Engine &e) {
 
 int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &levels) {
   HPC_HIPCHK(hipSetDevice(e.device));
-  // the previous phase's uploads read these host vectors
-  HPC_HIPCHK(hipStreamSynchronize(e.stream));
+  // a previous phase's upload may still read these host vectors
+  if (e.host_upload_pending) {
+    HPC_HIPCHK(hipStreamSynchronize(e.stream));
+    e.host_upload_pending = false;
+  }
   const int K = (int)ids.size();
   int p_total = 0, pmax = 0;
   size_t edges = 0;
@@ -448,9 +490,16 @@ int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &
   e.packed = ids;
   e.p_total = p_total;
   if (K > 0) {
-    HPC_HIPCHK(hipMemcpyAsync(e.d_inst, e.h_inst.data(), K * sizeof(Inst), hipMemcpyHostToDevice, e.stream));
-    HPC_HIPCHK(hipMemcpyAsync(e.d_row_inst, e.h_row_inst.data(), p_total * sizeof(int32_t),
-                              hipMemcpyHostToDevice, e.stream));
+    rc = upload_table(e);
+    if (rc) return rc;
+    // the row map changes only with the set of instances
+    if (!e.row_inst_valid || e.h_row_inst != e.h_row_inst_dev) {
+      HPC_HIPCHK(hipMemcpyAsync(e.d_row_inst, e.h_row_inst.data(), p_total * sizeof(int32_t),
+                                hipMemcpyHostToDevice, e.stream));
+      e.h_row_inst_dev = e.h_row_inst;
+      e.row_inst_valid = true;
+      e.host_upload_pending = true;
+    }
   }
   return HPC_OK;
 }
@@ -486,34 +535,48 @@ static Layout layout(Engine &e) {
 
 static float elapsed(hipEvent_t a, hipEvent_t b) {
   float ms = 0.f;
-  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
-  return ms;
+  hipError_t r = hipEventElapsedTime(&ms, a, b);
+  if (r == hipErrorNotReady && hipEventSynchronize(b) == hipSuccess)
+    r = hipEventElapsedTime(&ms, a, b);
+  return r == hipSuccess ? ms : 0.f;
 }
 
-// Device counters and the packed instances' counters, one synchronisation.
+// Device counters and the packed instances' counters into the host mirror,
+// and a wait until all work so far on the stream is done.  A one-workgroup
+// kernel stores them into fine-grained host memory and then a sequence
+// number; the host spins on that number, which comes back sooner than a
+// copy plus a stream synchronisation.  The spin asks the stream every few
+// hundred polls, so a failed launch or a stream error ends it.
 static int read_counters(Engine &e) {
-  HPC_HIPCHK(hipMemcpyAsync(e.h_ctr, e.d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, e.stream));
   const size_t K = e.packed.size();
-  if (K)
-    HPC_HIPCHK(hipMemcpyAsync(e.h_ictr, e.d_ictr, K * sizeof(InstCounters), hipMemcpyDeviceToHost,
-                              e.stream));
-  HPC_HIPCHK(hipStreamSynchronize(e.stream));
+  const size_t bytes = K ? kIctrOffset + K * sizeof(InstCounters) : sizeof(Counters);
+  const unsigned long long seq = ++e.seq;
+  launch_publish(e.stream, reinterpret_cast<const unsigned long long *>(e.d_ctr), bytes / 8,
+                 reinterpret_cast<unsigned long long *>(e.h_ctr_dev), e.d_seq, seq);
+  HPC_HIPCHK(hipGetLastError());
+  for (unsigned it = 1;; ++it) {
+    if (__atomic_load_n(e.h_seq, __ATOMIC_ACQUIRE) == seq) break;
+    if ((it & 255u) == 0) {
+      const hipError_t q = hipStreamQuery(e.stream);
+      if (q == hipSuccess) {
+        if (__atomic_load_n(e.h_seq, __ATOMIC_ACQUIRE) == seq) break;
+        return hpc_fail(HPC_EHIP, "counter publication %llu did not arrive", seq);
+      }
+      if (q != hipErrorNotReady)
+        return hpc_fail(HPC_EHIP, "HIP error: %s", hipGetErrorString(q));
+    }
+    __builtin_ia32_pause();
+  }
+  e.host_upload_pending = false;  // everything before the publication is done
   return HPC_OK;
 }
 
-int engine_read_inst_counters(Engine &e) {
-  const size_t K = e.packed.size();
-  if (K == 0) return HPC_OK;
-  HPC_HIPCHK(hipMemcpyAsync(e.h_ictr, e.d_ictr, K * sizeof(InstCounters), hipMemcpyDeviceToHost,
-                            e.stream));
-  HPC_HIPCHK(hipStreamSynchronize(e.stream));
-  return HPC_OK;
-}
+int engine_read_inst_counters(Engine &e) { return read_counters(e); }
 
 static int zero_counters(Engine &e) {
   const size_t K = e.packed.size();
-  HPC_HIPCHK(hipMemsetAsync(e.d_ctr, 0, sizeof(Counters), e.stream));
-  if (K) HPC_HIPCHK(hipMemsetAsync(e.d_ictr, 0, K * sizeof(InstCounters), e.stream));
+  const size_t bytes = K ? kIctrOffset + K * sizeof(InstCounters) : sizeof(Counters);
+  HPC_HIPCHK(hipMemsetAsync(e.d_ctr, 0, bytes, e.stream));
   return HPC_OK;
 }
 
@@ -530,9 +593,11 @@ int engine_corr(Engine &e, float *ms_moments, float *ms_corr) {
   HPC_HIPCHK(hipGetLastError());
   HPC_HIPCHK(hipEventRecord(e.ev[1], e.stream));
   launch_corr(e.stream, L, e.h_inst, e.d_prefix, e.h_prefix);
+  e.host_upload_pending = true;  // the block prefixes
   HPC_HIPCHK(hipGetLastError());
   if (any_fast_at(e, -1)) {
     launch_corr_fast(e.stream, L, e.h_inst, e.d_prefix + 3 * (L.K + 1), e.h_prefix_fast);
+    e.host_upload_pending = true;
     HPC_HIPCHK(hipGetLastError());
   }
   HPC_HIPCHK(hipEventRecord(e.ev[2], e.stream));
@@ -578,6 +643,7 @@ int engine_exact_from_z(Engine &e, const std::vector<int> &ids) {
   if (rc) return rc;
   Layout L = layout(e);
   launch_corr(e.stream, L, e.h_inst, e.d_prefix, e.h_prefix);
+  e.host_upload_pending = true;  // the block prefixes
   HPC_HIPCHK(hipGetLastError());
   HPC_HIPCHK(hipStreamSynchronize(e.stream));
   return HPC_OK;
@@ -591,6 +657,7 @@ int engine_level0(Engine &e, float *ms) {
   HPC_HIPCHK(hipEventRecord(e.ev[3], e.stream));
   if (any_fast_at(e, 0)) launch_refine_l0(e.stream, L);
   launch_level0(e.stream, L, e.h_inst, e.d_prefix + 2 * (L.K + 1), e.h_prefix);
+  e.host_upload_pending = true;
   HPC_HIPCHK(hipGetLastError());
   HPC_HIPCHK(hipEventRecord(e.ev[4], e.stream));
   rc = read_counters(e);
@@ -627,10 +694,9 @@ int engine_compact(Engine &e, float *ms) {
   if (rc) return rc;
   e.n_edges = (int64_t)e.h_ctr->n_edges;
   e.max_deg = e.h_ctr->max_deg;
-  // every ordered edge of the new snapshot starts with no passing set
-  if (e.n_edges > 0)
-    HPC_HIPCHK(hipMemsetAsync(e.d_fp, 0xFF, (size_t)e.n_edges * sizeof(int64_t), e.stream));
-  // the host share scatters its ranks from the copy stream after this point
+  // every ordered edge of the new snapshot starts with no passing set (set
+  // by the compaction); the host share scatters its ranks from the copy
+  // stream after this point
   HPC_HIPCHK(hipEventRecord(e.fp_ready, e.stream));
   for (size_t k = 0; k < e.packed.size(); ++k) {
     if (e.h_inst[k].level < 1) continue;
@@ -707,7 +773,10 @@ int engine_commit(Engine &e, float *ms) {
   HPC_HIPCHK(hipSetDevice(e.device));
   // room for the level's sepset records; the table is rewritten, so wait for
   // earlier uploads of it
-  HPC_HIPCHK(hipStreamSynchronize(e.stream));
+  if (e.host_upload_pending) {
+    HPC_HIPCHK(hipStreamSynchronize(e.stream));
+    e.host_upload_pending = false;
+  }
   for (size_t k = 0; k < e.packed.size(); ++k) {
     const int l = e.h_inst[k].level;
     if (l < 1 || l > kMaxLevel) continue;
@@ -718,8 +787,8 @@ int engine_commit(Engine &e, float *ms) {
     e.h_inst[k].rec_base = h.rec_used;
   }
   if (!e.packed.empty()) {
-    HPC_HIPCHK(hipMemcpyAsync(e.d_inst, e.h_inst.data(), e.packed.size() * sizeof(Inst),
-                              hipMemcpyHostToDevice, e.stream));
+    int rc = upload_table(e);
+    if (rc) return rc;
     // the removal kernel's counts (and record offsets) start from zero
     HPC_HIPCHK(hipMemsetAsync(e.d_ictr, 0, e.packed.size() * sizeof(InstCounters), e.stream));
   }
@@ -755,7 +824,7 @@ size_t engine_footprint(int p, int n, int max_level) {
              + pp * 8               // sepset records (level 1, every edge removed)
              + (size_t)p * W * 8    // bit adjacency
              + (size_t)p * n * 8    // observations
-             + (size_t)p * npad * 9 // z-scores and NaN mask
+             + (size_t)p * npad * 13 // z-scores (FP64, and FP32 for HPC_CORR_F32) and NaN mask
              + (size_t)p * 16;      // mean, sd
   b += pp * (4 + 4 + 8);            // worst-case share of nbr / erow / first-pass arrays
   b += (size_t)p * 40;              // row arrays
@@ -792,6 +861,13 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
   }
   if (act.empty()) return HPC_OK;
   auto t0 = std::chrono::steady_clock::now();
+  // room for each level's sepset records before the table is packed (the
+  // worst case depends only on the edges left, known from the last step)
+  for (size_t k = 0; k < act.size(); ++k) {
+    if (lev[k] < 1 || lev[k] > kMaxLevel) continue;
+    rc = ensure_rec(e, e.inst[act[k]], lev[k]);
+    if (rc) return rc;
+  }
   rc = engine_pack(e, act, lev);
   if (!rc) rc = zero_counters(e);
   if (rc) return rc;
@@ -817,7 +893,10 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
   for (int l : lev) { any0 |= l == 0; any1 |= l >= 1; }
   HPC_HIPCHK(hipEventRecord(e.ev[0], e.stream));
   if (any0 && any_fast_at(e, 0)) launch_refine_l0(e.stream, L);
-  if (any0) launch_level0(e.stream, L, e.h_inst, e.d_prefix + 2 * (K + 1), e.h_prefix);
+  if (any0) {
+    launch_level0(e.stream, L, e.h_inst, e.d_prefix + 2 * (K + 1), e.h_prefix);
+    e.host_upload_pending = true;
+  }
   HPC_HIPCHK(hipGetLastError());
   HPC_HIPCHK(hipEventRecord(e.ev[1], e.stream));
   if (any1) launch_compact(e.stream, L);
@@ -828,8 +907,7 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
   e.n_edges = any1 ? (int64_t)e.h_ctr->n_edges : 0;
   e.max_deg = any1 ? e.h_ctr->max_deg : 0;
   L.max_deg = e.max_deg;
-  if (e.n_edges > 0)
-    HPC_HIPCHK(hipMemsetAsync(e.d_fp, 0xFF, (size_t)e.n_edges * sizeof(int64_t), e.stream));
+  // (the compaction set every first passing rank to -1)
   // instances whose level has nothing to test and no cap stop here; a level
   // beyond the kernels' reach is an error for its instance when it has tests,
   // else an empty record.  skip: 1 = no record, 2 = empty record.
@@ -850,15 +928,11 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
       retable = true;
       continue;
     }
-    rc = ensure_rec(e, h, lev[k]);
-    if (rc) return rc;
-    e.h_inst[k].rec = h.rec;
-    e.h_inst[k].rec_base = h.rec_used;
-    retable = true;
   }
-  if (retable)
-    HPC_HIPCHK(hipMemcpyAsync(e.d_inst, e.h_inst.data(), K * sizeof(Inst), hipMemcpyHostToDevice,
-                              e.stream));
+  if (retable) {
+    rc = upload_table(e);
+    if (rc) return rc;
+  }
   std::vector<int> levels_ge2;
   bool run1 = false;
   for (int k = 0; k < K; ++k) {
@@ -972,6 +1046,44 @@ int engine_tpb(const Engine &e, int level) {
 }
 
 int engine_epb(const Engine &e) { return e.epb_level1 > 0 ? e.epb_level1 : 32; }
+
+int engine_check_tuning(const hpc_tune_entry *t, int count) {
+  if (count < 0 || (count > 0 && !t)) return hpc_fail(HPC_EINVAL, "no tuning entries");
+  for (int k = 0; k < count; ++k) {
+    const hpc_tune_entry &x = t[k];
+    if (x.level < 1 || x.level > kMaxLevel)
+      return hpc_fail(HPC_EINVAL, "tuning entry %d: level %d outside 1..%d", k, x.level, kMaxLevel);
+    if (x.level == 1) {
+      if (x.edges_per_block < 1 || x.edges_per_block > 4096)
+        return hpc_fail(HPC_EINVAL, "tuning entry %d: edges per workgroup %d outside 1..4096", k,
+                        x.edges_per_block);
+    } else {
+      if (x.ranks_per_task < 64 || x.ranks_per_task % 64 != 0 || x.ranks_per_task > (1 << 24))
+        return hpc_fail(HPC_EINVAL,
+                        "tuning entry %d: ranks per task %d is not a multiple of 64 in [64, 2^24]",
+                        k, x.ranks_per_task);
+      if (x.tasks_per_block < 1 || x.tasks_per_block > 1024)
+        return hpc_fail(HPC_EINVAL, "tuning entry %d: tasks per workgroup %d outside 1..1024", k,
+                        x.tasks_per_block);
+    }
+  }
+  return HPC_OK;
+}
+
+int engine_apply_tuning(Engine &e, const hpc_tune_entry *t, int count) {
+  const int rc = engine_check_tuning(t, count);
+  if (rc) return rc;
+  for (int k = 0; k < count; ++k) {
+    const hpc_tune_entry &x = t[k];
+    if (x.level == 1) {
+      e.epb_level1 = x.edges_per_block;
+    } else {
+      e.rpt_level[x.level] = x.ranks_per_task;
+      e.tpb_level[x.level] = x.tasks_per_block;
+    }
+  }
+  return HPC_OK;
+}
 
 int engine_autotune(Engine &e, int id, int max_level, std::vector<hpc_tune_entry> &out) {
   InstHost &h = e.inst[id];

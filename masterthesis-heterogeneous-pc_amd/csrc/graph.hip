@@ -192,8 +192,8 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(Layout L) {
 
 // compactCUTiled (compact.cu:4-89): the ascending neighbour ids of row g
 // (global ids), written at rowptr[g] (CSR instead of a p-wide padded row),
-// plus the row of every ordered edge.  A wave prefix sum of per-word
-// popcounts gives each lane its output offset.
+// plus the row of every ordered edge, whose first passing rank starts at -1.
+// A wave prefix sum of per-word popcounts gives each lane its output offset.
 __global__ __launch_bounds__(256) void fill_kernel(Layout L) {
   const int lane = threadIdx.x & 63;
   const int g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -218,13 +218,40 @@ __global__ __launch_bounds__(256) void fill_kernel(Layout L) {
       word &= word - 1ull;
       L.nbr[off] = In.start + w * 64 + b;
       L.erow[off] = g;
+      L.fp[off] = -1;  // no passing set found yet
       ++off;
     }
     base += __shfl(incl, 63, 64);
   }
 }
 
+__global__ void table_kernel(Inst *__restrict__ dst, TableArg arg) {
+  const int k = threadIdx.x;
+  if (k < arg.K) dst[k] = arg.inst[k];
+}
+
+// One workgroup: the words, a system-scope fence per thread, then the flag.
+__global__ __launch_bounds__(256) void publish_kernel(const unsigned long long *__restrict__ src,
+                                                      size_t words,
+                                                      unsigned long long *__restrict__ dst,
+                                                      unsigned long long *flag,
+                                                      unsigned long long seq) {
+  for (size_t w = threadIdx.x; w < words; w += blockDim.x) dst[w] = src[w];
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 }  // namespace
+
+void launch_publish(hipStream_t s, const unsigned long long *src, size_t words,
+                    unsigned long long *dst, unsigned long long *flag, unsigned long long seq) {
+  hipLaunchKernelGGL(publish_kernel, dim3(1), dim3(256), 0, s, src, words, dst, flag, seq);
+}
+
+void launch_table(hipStream_t s, Inst *dst, const TableArg &arg) {
+  hipLaunchKernelGGL(table_kernel, dim3(1), dim3(64), 0, s, dst, arg);
+}
 
 void launch_adj_init(hipStream_t s, unsigned long long *adj, int p, int W) {
   long long n = (long long)p * W;

@@ -129,6 +129,19 @@ void launch_refine_l0(hipStream_t s, const Layout &L);
 // emptied.
 void launch_recheck(hipStream_t s, int level, const Layout &L);
 void launch_adj_init(hipStream_t s, unsigned long long *adj, int p, int W);
+// Instance tables of up to kTableArg entries travel as a kernel argument (one
+// launch instead of a host-to-device copy); larger ones are copied.
+constexpr int kTableArg = 8;
+struct TableArg {
+  Inst inst[kTableArg];
+  int K;
+};
+void launch_table(hipStream_t s, Inst *dst, const TableArg &arg);
+// Copies `words` 8-byte words from device memory to fine-grained host memory
+// (device-visible pointers), then stores `seq` to *flag: the host sees the
+// words once it sees the flag.
+void launch_publish(hipStream_t s, const unsigned long long *src, size_t words,
+                    unsigned long long *dst, unsigned long long *flag, unsigned long long seq);
 void launch_level0(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst,
                    long long *d_prefix, std::vector<long long> &h_prefix);
 void launch_compact(hipStream_t s, const Layout &L);
@@ -197,6 +210,9 @@ struct Arena {
   size_t reserved = 0, used = 0;
 };
 
+// Byte offset of the instance counters in the counter block.
+constexpr size_t kIctrOffset = (sizeof(Counters) + 63) & ~(size_t)63;
+
 // A set of instances on one device with their shared per-level arrays and
 // one stream.
 struct Engine {
@@ -215,7 +231,18 @@ struct Engine {
   long long *d_row_tasks = nullptr, *d_task_off = nullptr;
   unsigned long long *d_binom = nullptr;
   long long *d_prefix = nullptr;
+  // device counters and the packed instances' counters in one block (and
+  // its pinned host mirror), so one copy reads and one fill clears them all:
+  // d_ctr at the start, d_ictr at kIctrOffset
   Counters *d_ctr = nullptr, *h_ctr = nullptr;
+  Counters *h_ctr_dev = nullptr;  // h_ctr as the device addresses it
+  // publication flag (fine-grained host memory): the last sequence number a
+  // counter publication wrote, its device address, and the next number
+  unsigned long long *h_seq = nullptr, *d_seq = nullptr;
+  unsigned long long seq = 0;
+  // an asynchronous copy from a host vector (table, row map, prefixes) may
+  // still read it: the next pack waits for the stream first
+  bool host_upload_pending = false;
   size_t cap_inst = 0, cap_rows = 0, cap_edges = 0, cap_binom = 0;
   std::vector<unsigned long long> h_binom;
   std::vector<long long> h_prefix, h_prefix_fast;
@@ -223,6 +250,8 @@ struct Engine {
   std::vector<Inst> h_inst;
   std::vector<int> packed;
   std::vector<int32_t> h_row_inst;
+  std::vector<int32_t> h_row_inst_dev;  // what d_row_inst holds (when row_inst_valid)
+  bool row_inst_valid = false;
   int p_total = 0;
   int64_t n_edges = 0;
   int max_deg = 0;
@@ -298,6 +327,10 @@ int engine_epb(const Engine &e);
 // (results do not depend on the shape), keeps the fastest per level in the
 // engine and leaves the instance reset.
 int engine_autotune(Engine &e, int id, int max_level, std::vector<hpc_tune_entry> &out);
+// Per-level launch shapes (entries as hpc_autotune reports them): the check
+// alone, and check-then-apply (nothing is applied unless every entry is valid).
+int engine_check_tuning(const hpc_tune_entry *t, int count);
+int engine_apply_tuning(Engine &e, const hpc_tune_entry *t, int count);
 // Device bytes one instance needs (its blocks plus its share of the per-level
 // arrays).
 size_t engine_footprint(int p, int n, int max_level);

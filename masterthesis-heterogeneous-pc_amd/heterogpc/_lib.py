@@ -75,6 +75,15 @@ class TuneEntry(C.Structure):
     ]
 
 
+def tune_array(entries):
+    """ctypes array of TuneEntry from dicts as autotune() returns them."""
+    arr = (TuneEntry * max(len(entries), 1))()
+    for k, e in enumerate(entries):
+        for name, _ in TuneEntry._fields_:
+            setattr(arr[k], name, e.get(name, 0))
+    return arr, len(entries)
+
+
 class HpcError(RuntimeError):
     pass
 
@@ -95,6 +104,7 @@ SIGNATURES = {
     "hpc_max_cond_size": (_i, [_vp]),
     "hpc_set_tuning": (_i, [_vp, _i, _i]),
     "hpc_autotune": (_i, [_vp, _i, C.POINTER(TuneEntry), _i, C.POINTER(C.c_int)]),
+    "hpc_apply_tuning": (_i, [_vp, C.POINTER(TuneEntry), _i]),
     "hpc_pmcc": (_i, [_vp, _vp, _i]),
     "hpc_set_observations": (_i, [_vp, _vp]),
     "hpc_corr": (_i, [_vp, _i]),
@@ -129,8 +139,10 @@ SIGNATURES = {
     "hpc_batch_get_sepsets": (_i, [_vp, _i, _vp, _i]),
     "hpc_batch_get_sepset_list": (_i, [_vp, _i, _vp, _i64, C.POINTER(_i64)]),
     "hpc_batch_set_tuning": (_i, [_vp, _i, _i]),
+    "hpc_batch_apply_tuning": (_i, [_vp, C.POINTER(TuneEntry), _i]),
     "hpc_batch_set_corr_mode": (_i, [_vp, _i, _i]),
     "hpc_server_set_corr_mode": (_i, [_vp, _i]),
+    "hpc_server_apply_tuning": (_i, [_vp, C.POINTER(TuneEntry), _i]),
     "hpc_server_create": (_i, [_i, _u64, _i, C.POINTER(_vp)]),
     "hpc_server_destroy": (_i, [_vp]),
     "hpc_server_submit": (_i, [_vp, _u64, _i, _d, _i, _vp, C.POINTER(_i64)]),

@@ -6,7 +6,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import LevelStats, RequestInfo, ServerInfo, check, load
+from ._lib import LevelStats, RequestInfo, ServerInfo, check, load, tune_array
 from .skeleton import _fetch_list
 
 
@@ -114,6 +114,11 @@ class Batch:
     def set_ranks_per_task(self, value: int) -> None:
         check(self.lib.hpc_batch_set_tuning(self.h, 1, int(value)))
 
+    def apply_tuning(self, entries: list[dict]) -> None:
+        """Launch shapes (as DeviceSkeleton.autotune() returns them)."""
+        arr, n = tune_array(entries)
+        check(self.lib.hpc_batch_apply_tuning(self.h, arr, n))
+
 
 class Server:
     """Continuous-batching skeleton server on one GPU."""
@@ -194,6 +199,11 @@ class Server:
             self.shape.pop(req, None)
             self._borrowed.pop(req, None)
         return adj, sep, {name: getattr(info, name) for name, _ in info._fields_}
+
+    def apply_tuning(self, entries: list[dict]) -> None:
+        """Launch shapes for the server's later steps."""
+        arr, n = tune_array(entries)
+        check(self.lib.hpc_server_apply_tuning(self.h, arr, n))
 
     def stats(self) -> dict:
         info = ServerInfo()

@@ -114,6 +114,13 @@ class DeviceSkeleton:
         return [{name: getattr(buf[k], name) for name, _ in TuneEntry._fields_}
                 for k in range(min(n.value, 32))]
 
+    def apply_tuning(self, entries: list[dict]) -> None:
+        """Launch shapes (as autotune() returns them) for later runs."""
+        from ._lib import tune_array
+
+        arr, n = tune_array(entries)
+        check(self.lib.hpc_apply_tuning(self.h, arr, n))
+
     def set_recheck_queue(self, value: int) -> None:
         """Length of the HPC_CORR_F32 recheck queues (0 = full)."""
         check(self.lib.hpc_set_tuning(self.h, 2, int(value)))

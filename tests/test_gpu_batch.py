@@ -123,6 +123,12 @@ def test_batch_continuous_admission_and_slot_reuse(hpc, orc):
     inputs = _mixed_inputs(hpc)
     first, later = inputs[:6], inputs[6:]
     with hpc.Batch() as b:
+        # launch shapes other than the defaults; results must not change
+        b.apply_tuning([{"level": 1, "edges_per_block": 8},
+                        {"level": 2, "ranks_per_task": 128, "tasks_per_block": 4},
+                        {"level": 3, "ranks_per_task": 2048, "tasks_per_block": 32}])
+        with pytest.raises(hpc.HpcError):
+            b.apply_tuning([{"level": 17, "ranks_per_task": 128, "tasks_per_block": 4}])
         ids = {}
         for label, x, a, ml in first:
             ids[label] = b.add(x, a, ml)
@@ -201,3 +207,28 @@ def test_server_rejects_request_beyond_budget(hpc):
     with hpc.Server(hbm_budget=1 << 20) as s:
         with pytest.raises(hpc.HpcError):
             s.submit(x, alpha, 4)
+
+
+def test_server_applied_tuning_keeps_results(hpc, orc):
+    """Launch shapes handed to a running server apply from its next step on
+    and leave every result unchanged."""
+    inputs = _mixed_inputs(hpc)[:6]
+    with hpc.Server() as s:
+        with pytest.raises(hpc.HpcError):
+            s.apply_tuning([{"level": 2, "ranks_per_task": 96, "tasks_per_block": 4}])
+        reqs = []
+        for k, (label, x, a, ml) in enumerate(inputs):
+            if k == 2:
+                s.apply_tuning([{"level": 1, "edges_per_block": 64},
+                                {"level": 2, "ranks_per_task": 256, "tasks_per_block": 8},
+                                {"level": 4, "ranks_per_task": 1024, "tasks_per_block": 2}])
+            reqs.append((label, x, a, ml, s.submit(x, a, ml, copy=True)))
+        for label, x, a, ml, r in reqs:
+            assert s.wait(r, 120.0), f"{label} did not finish"
+            adj, sep, info = s.result(r, mc=16)
+            cor, ref, recs = _oracle(orc, x, a, ml)
+            mc = ref.mc()
+            assert np.array_equal(adj, ref.adj()), f"{label}: adjacency differs"
+            assert np.array_equal(sep[:, :, :mc], ref.sepsets()), f"{label}: sepsets differ"
+            assert (sep[:, :, mc:] == -1).all()
+            assert info["tests"] == sum(o["tests"] for o in recs), label

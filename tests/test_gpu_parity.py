@@ -220,3 +220,36 @@ def test_autotune_picks_shapes_and_keeps_results(hpc, orc):
         assert np.array_equal(dev.adj(), ref.adj())
         assert np.array_equal(dev.sepsets(ref.mc()), ref.sepsets())
         assert [s["tests"] for s in st] == [r["tests"] for r in recs]
+
+
+def test_applied_tuning_any_shape_keeps_results(hpc, orc):
+    """Launch shapes handed over with hpc_apply_tuning -- the extremes of the
+    accepted ranges included -- leave adjacency, sepsets and test counts
+    unchanged; an invalid entry is rejected before anything is applied."""
+    x, n, alpha, _ = workloads.sem("dense")
+    p = x.shape[0]
+    ref = orc.Skeleton(orc.corr(x), n, alpha, max_level=4)
+    recs = ref.run(4)
+    shapes = [
+        [{"level": 1, "edges_per_block": 1}] +
+        [{"level": l, "ranks_per_task": 64, "tasks_per_block": 1} for l in (2, 3, 4)],
+        [{"level": 1, "edges_per_block": 4096}] +
+        [{"level": l, "ranks_per_task": 1 << 20, "tasks_per_block": 1024} for l in (2, 3, 4)],
+        [{"level": 3, "ranks_per_task": 192, "tasks_per_block": 3}],
+    ]
+    with hpc.DeviceSkeleton(p, n, alpha, max_level=4) as dev:
+        dev.pmcc(x)
+        for bad in ({"level": 0, "edges_per_block": 8},
+                    {"level": 2, "ranks_per_task": 100, "tasks_per_block": 4},
+                    {"level": 2, "ranks_per_task": 128, "tasks_per_block": 0},
+                    {"level": 1, "edges_per_block": 5000}):
+            with pytest.raises(hpc.HpcError):
+                dev.apply_tuning([{"level": 1, "edges_per_block": 16}, bad])
+        for k, entries in enumerate(shapes):
+            print(f"[apply_tuning] shape set {k}: {entries}", flush=True)
+            dev.apply_tuning(entries)
+            dev.reset()
+            st = dev.run(4)
+            assert np.array_equal(dev.adj(), ref.adj()), k
+            assert np.array_equal(dev.sepsets(ref.mc()), ref.sepsets()), k
+            assert [s["tests"] for s in st] == [r["tests"] for r in recs], k
