@@ -35,7 +35,9 @@ __host__ __device__ __forceinline__ double pcor_l1(double x1, double x2, double 
 // A (row-major, overwritten) through a symmetric cyclic Jacobi
 // eigen-decomposition and the pseudoinverse with Armadillo's default
 // tolerance m * max|lambda| * eps (cpuInd.cpp:35-36).  Taken only when the
-// Cholesky form below meets a non-positive pivot.
+// Cholesky form below meets a non-positive pivot.  The rotation loops stay
+// rolled (the matrices live in private memory): the search kernels call this
+// rarely, and an unrolled form would set their register count.
 template <int M>
 __host__ __device__ __noinline__ double pcor_pinv(double (&A)[M][M]) {
   double V[M][M];
@@ -48,7 +50,9 @@ __host__ __device__ __noinline__ double pcor_pinv(double (&A)[M][M]) {
       for (int b = a + 1; b < M; ++b) off = __builtin_fma(A[a][b], A[a][b], off);
     }
     if (!(off > 1e-34 * dia)) break;
+#pragma unroll 1
     for (int pp = 0; pp < M - 1; ++pp) {
+#pragma unroll 1
       for (int qq = pp + 1; qq < M; ++qq) {
         double apq = A[pp][qq];
         if (apq == 0.0) continue;
@@ -58,6 +62,7 @@ __host__ __device__ __noinline__ double pcor_pinv(double (&A)[M][M]) {
         if (theta < 0.0) t = -t;
         double c = 1.0 / __builtin_sqrt(__builtin_fma(t, t, 1.0));
         double s = t * c;
+#pragma unroll 1
         for (int k = 0; k < M; ++k) {
           if (k == pp || k == qq) continue;
           double akp = A[k][pp], akq = A[k][qq];
@@ -69,6 +74,7 @@ __host__ __device__ __noinline__ double pcor_pinv(double (&A)[M][M]) {
         A[pp][pp] = __builtin_fma(-t, apq, app);
         A[qq][qq] = __builtin_fma(t, apq, aqq);
         A[pp][qq] = 0.0; A[qq][pp] = 0.0;
+#pragma unroll 1
         for (int k = 0; k < M; ++k) {
           double vkp = V[k][pp], vkq = V[k][qq];
           V[k][pp] = __builtin_fma(c, vkp, -(s * vkq));
@@ -327,6 +333,23 @@ __host__ __device__ __forceinline__ void unrank(const unsigned long long *tab, i
     q[i] = v;
     rank = rank + tab[(size_t)(lo + 1) * kcols + k] - top;
     v0 = v + 1;
+  }
+}
+
+// Lexicographic successor of the L-subset q of {0..nn-1}: the rightmost
+// member that can grow grows by one and the members after it follow it
+// consecutively, so next_comb(unrank(r)) == unrank(r + 1).  The last subset
+// is left unchanged.
+template <int L>
+__host__ __device__ __forceinline__ void next_comb(int (&q)[L], int nn) {
+  int piv = -1;
+#pragma unroll
+  for (int i = L - 1; i >= 0; --i)
+    if (piv < 0 && q[i] < nn - L + i) piv = i;
+#pragma unroll
+  for (int i = 0; i < L; ++i) {
+    if (i == piv) q[i] += 1;
+    else if (piv >= 0 && i > piv) q[i] = q[i - 1] + 1;
   }
 }
 

@@ -30,6 +30,53 @@ namespace hpc {
 namespace {
 
 constexpr int kWavesPerBlock = 4;
+// consecutive ranks per lane in the LDS-staged level >= 2 search
+constexpr int kRun = 4;
+
+// Binomial table staged in LDS, addressed as LDS (32-bit addresses, ds_read)
+// rather than through generic pointers.
+typedef __attribute__((address_space(3))) const unsigned long long lds_u64;
+
+// unrank() (the rank-th L-subset of {0..nn-1} in lexicographic order, L >= 2)
+// on a table in LDS with kcols columns: a binary search per member but the
+// last, which is the remaining rank past the previous member.
+template <int L>
+__device__ __forceinline__ void unrank_lds(const lds_u64 *tab, int kcols, unsigned long long rank,
+                                           int nn, int (&q)[L]) {
+  int v0 = 0;
+#pragma unroll
+  for (int i = 0; i < L - 1; ++i) {
+    const int k = L - i;
+    const unsigned long long top = tab[(nn - v0) * kcols + k];
+    const unsigned long long X = top - rank;
+    int lo = k - 1, hi = nn - 1 - v0;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tab[mid * kcols + k] < X) lo = mid; else hi = mid - 1;
+    }
+    const int v = nn - 1 - lo;
+    q[i] = v;
+    rank = rank + tab[(lo + 1) * kcols + k] - top;
+    v0 = v + 1;
+  }
+  q[L - 1] = v0 + (int)rank;
+}
+
+// Uncertain (edge, rank) pairs of a fast instance, one per lane where `unc`
+// holds, appended to the recheck queue.  Wave-uniform call.
+__device__ __forceinline__ void queue_rank(const Layout &L, bool unc, long long e,
+                                           unsigned long long rank, int lane) {
+  const unsigned long long m = __ballot(unc);
+  if (!m) return;
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd(&L.ctr->n_rchk, (unsigned long long)__popcll(m));
+  base = __shfl(base, 0, 64);
+  if (unc) {
+    const unsigned long long slot = base + __popcll(m & ((1ull << lane) - 1ull));
+    if (slot < L.rchk_cap) L.rchk[slot] = Recheck{(unsigned long long)e, rank};
+    else L.ctr->overflow = 1;
+  }
+}
 
 // Lanes of a fast instance's test chunk whose decision is uncertain and whose
 // rank lies below the chunk's lowest certain pass (`certain_passes`) are
@@ -340,47 +387,69 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
         const unsigned long long r_end = min(total, r_begin + (unsigned long long)rpt);
         const double a = Mi[y];
         const double *Mj = M + (size_t)y * D;  // row j
-        for (unsigned long long r0 = r_begin; r0 < r_end; r0 += 64) {
-          const unsigned long long rank = r0 + lane;
-          bool pass = false, unc = false;
-          if (rank < r_end) {
-            int q[LV], pos[LV];
-            unrank<LV>(bt, LV + 1, rank, nn, q);
+        // lane t tests the kRun consecutive ranks rb = r0 + kRun * t ..: one
+        // unrank, then lexicographic successors; the chunk's lowest passing
+        // rank is the lowest passing lane's first passing step
+        for (unsigned long long r0 = r_begin; r0 < r_end; r0 += 64 * kRun) {
+          const unsigned long long rb = r0 + (unsigned long long)lane * kRun;
+          int first = kRun;       // this lane's first certain pass
+          unsigned unc_bits = 0;  // its uncertain steps
+          int q[LV];
 #pragma unroll
-            for (int c = 0; c < LV; ++c) pos[c] = q[c] < y ? q[c] : q[c] + 1;
-            double x[LV], yv[LV], C[LV][LV];
-            bool anynan = __builtin_isnan(a);
+          for (int c = 0; c < LV; ++c) q[c] = c;
+          if (rb < r_end) unrank_lds<LV>((const lds_u64 *)bt, LV + 1, rb, nn, q);
 #pragma unroll
-            for (int c = 0; c < LV; ++c) {
-              x[c] = Mi[pos[c]];
-              yv[c] = Mj[pos[c]];
-              anynan |= __builtin_isnan(x[c]) || __builtin_isnan(yv[c]);
-            }
+          for (int st = 0; st < kRun; ++st) {
+            if (rb + st < r_end && first == kRun) {
+              int pos[LV];
 #pragma unroll
-            for (int c = 0; c < LV; ++c)
+              for (int c = 0; c < LV; ++c) pos[c] = q[c] < y ? q[c] : q[c] + 1;
+              double x[LV], yv[LV], C[LV][LV];
+              bool anynan = __builtin_isnan(a);
 #pragma unroll
-              for (int r = c + 1; r < LV; ++r) {
-                C[r][c] = M[(size_t)pos[c] * D + pos[r]];
-                anynan |= __builtin_isnan(C[r][c]);
+              for (int c = 0; c < LV; ++c) {
+                x[c] = Mi[pos[c]];
+                yv[c] = Mj[pos[c]];
+                anynan |= __builtin_isnan(x[c]) || __builtin_isnan(yv[c]);
               }
-            if (!anynan) {
-              bool fb;
-              double r;
-              if (fast) {
-                double sens;
-                r = pcor_canon_sens<LV>(a, x, yv, C, fb, sens);
-                unc = fb || !fast_certain(r, r_crit, sens, In.eps);
-              } else {
-                r = pcor_canon<LV>(a, x, yv, C, fb);
+#pragma unroll
+              for (int c = 0; c < LV; ++c)
+#pragma unroll
+                for (int r = c + 1; r < LV; ++r) {
+                  C[r][c] = M[pos[c] * D + pos[r]];
+                  anynan |= __builtin_isnan(C[r][c]);
+                }
+              if (!anynan) {
+                bool fb, unc = false;
+                double r;
+                if (fast) {
+                  double sens;
+                  r = pcor_canon_sens<LV>(a, x, yv, C, fb, sens);
+                  unc = fb || !fast_certain(r, r_crit, sens, In.eps);
+                } else {
+                  r = pcor_canon<LV>(a, x, yv, C, fb);
+                }
+                if (removes(r, r_crit) && !unc) first = st;
+                if (unc) unc_bits |= 1u << st;
+                F += fb ? 1 : 0;
               }
-              pass = removes(r, r_crit) && !unc;
-              F += fb ? 1 : 0;
             }
+            if (st + 1 < kRun) next_comb<LV>(q, nn);
           }
-          const unsigned long long m = __ballot(pass);
-          if (fast) queue_uncertain(L, unc, m, e, r0, lane);
+          const unsigned long long m = __ballot(first < kRun);
+          unsigned long long best = ~0ull;
           if (m) {
-            if (lane == 0) atomicMin(&fpu[e], r0 + (unsigned long long)(__ffsll((long long)m) - 1));
+            const int t = __ffsll((long long)m) - 1;
+            best = r0 + (unsigned long long)t * kRun + (unsigned long long)__shfl(first, t, 64);
+          }
+          if (fast) {
+            // uncertain tests below the chunk's lowest certain pass
+#pragma unroll
+            for (int st = 0; st < kRun; ++st)
+              queue_rank(L, ((unc_bits >> st) & 1u) && rb + st < best, e, rb + st, lane);
+          }
+          if (m) {
+            if (lane == 0) atomicMin(&fpu[e], best);
             break;
           }
         }
@@ -533,16 +602,29 @@ __global__ __launch_bounds__(256) void commit_kernel(Layout L, long long n_edges
     out[0] = min(li, lj);
     out[1] = max(li, lj);
     out[2] = level;
-    int val = 0;
+    // the members of the rank-th level-subset, as unrank() finds them (a
+    // binary search per member over the binomials; the last member is the
+    // remaining rank past the previous one)
+    const unsigned long long *tab = L.binom;
+    int v0 = 0;
     for (int m = 0; m < level; ++m) {
-      for (;;) {
-        const unsigned long long c = binom(L.binom, kBinomK, nn - 1 - val, level - 1 - m);
-        if (rank < c) break;
-        rank -= c;
-        ++val;
+      const int km = level - m;
+      int v;
+      if (km == 1) {
+        v = v0 + (int)rank;
+      } else {
+        const unsigned long long top = tab[(size_t)(nn - v0) * kBinomK + km];
+        const unsigned long long X = top - rank;
+        int lo = km - 1, hi = nn - 1 - v0;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (tab[(size_t)mid * kBinomK + km] < X) lo = mid; else hi = mid - 1;
+        }
+        v = nn - 1 - lo;
+        rank = rank + tab[(size_t)(lo + 1) * kBinomK + km] - top;
       }
-      const int q = val++;
-      out[3 + m] = L.nbr[base + (q < y ? q : q + 1)] - s0;
+      out[3 + m] = L.nbr[base + (v < y ? v : v + 1)] - s0;
+      v0 = v + 1;
     }
     R = 1;
   }
