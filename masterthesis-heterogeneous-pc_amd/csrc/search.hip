@@ -32,6 +32,9 @@ namespace {
 constexpr int kWavesPerBlock = 4;
 // consecutive ranks per lane in the LDS-staged level >= 2 search
 constexpr int kRun = 4;
+// wavefronts per workgroup of the removal kernel (one record reservation and
+// one set of counter atomics per workgroup)
+constexpr int kCommitWaves = 16;
 
 // Binomial table staged in LDS, addressed as LDS (32-bit addresses, ds_read)
 // rather than through generic pointers.
@@ -519,7 +522,7 @@ __device__ __forceinline__ int find_pos(const int32_t *row, int len, int node) {
 // wavefront reserves the space of all its records with one atomic when it
 // holds one instance; the record order is scheduling-dependent, the records
 // are not.
-__global__ __launch_bounds__(256) void commit_kernel(Layout L, long long n_edges) {
+__global__ __launch_bounds__(64 * kCommitWaves) void commit_kernel(Layout L, long long n_edges) {
   // an overflowed recheck queue leaves the level undecided: the host redoes it
   if (L.ctr->overflow) return;
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -569,22 +572,54 @@ __global__ __launch_bounds__(256) void commit_kernel(Layout L, long long n_edges
       }
     }
   }
-  // the wave's instance, if it holds only one
+  // the wave's instance, if it holds only one (-1: no edge), and the
+  // workgroup's, if all its waves hold the same one
   int kk = k;
 #pragma unroll
   for (int s = 32; s > 0; s >>= 1) kk = max(kk, __shfl_xor(kk, s, 64));
   const bool uniform = __all(k == kk || k < 0);
-  // record space
-  const unsigned long long need = rm ? (unsigned long long)(3 + level) : 0ull;
-  unsigned long long off = 0;
-  if (uniform) {
-    unsigned long long incl = need;
+  // sk: each wave's instance for the record reservation (written once, before
+  // the barrier that every wave passes); sc: for the counts
+  __shared__ int sk[kCommitWaves], sc[kCommitWaves];
+  __shared__ unsigned long long sT[kCommitWaves], sU[kCommitWaves], sE[kCommitWaves],
+      sR[kCommitWaves], sW[kCommitWaves];
+  __shared__ unsigned long long s_base;
+  if (lane == 0) sk[wv] = uniform ? kk : -2;
+  __syncthreads();
+  int K0 = -1;
+  bool bu = true;
 #pragma unroll
-    for (int s = 1; s < 64; s <<= 1) {
-      const unsigned long long v = __shfl_up(incl, s, 64);
-      if (lane >= s) incl += v;
+  for (int w = 0; w < kCommitWaves; ++w) {
+    const int v = sk[w];
+    if (v == -2 || (v >= 0 && K0 >= 0 && v != K0)) bu = false;
+    if (v >= 0 && K0 < 0) K0 = v;
+  }
+  // record space: one reservation per workgroup when it holds one instance,
+  // else per wave or per edge
+  const unsigned long long need = rm ? (unsigned long long)(3 + level) : 0ull;
+  unsigned long long incl = need;
+#pragma unroll
+  for (int s = 1; s < 64; s <<= 1) {
+    const unsigned long long v = __shfl_up(incl, s, 64);
+    if (lane >= s) incl += v;
+  }
+  const unsigned long long tot = __shfl(incl, 63, 64);
+  unsigned long long off = 0;
+  if (bu) {
+    if (lane == 63) sW[wv] = tot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long sum = 0;
+      for (int w = 0; w < kCommitWaves; ++w) {
+        const unsigned long long t = sW[w];
+        sW[w] = sum;  // exclusive prefix over the waves
+        sum += t;
+      }
+      s_base = (sum && K0 >= 0) ? atomicAdd(&L.ictr[K0].rec_words, sum) : 0ull;
     }
-    const unsigned long long tot = __shfl(incl, 63, 64);
+    __syncthreads();
+    off = s_base + sW[wv] + incl - need;
+  } else if (uniform) {
     unsigned long long wbase = 0;
     if (lane == 63 && tot) wbase = atomicAdd(&L.ictr[kk].rec_words, tot);
     wbase = __shfl(wbase, 63, 64);
@@ -631,8 +666,6 @@ __global__ __launch_bounds__(256) void commit_kernel(Layout L, long long n_edges
   // per-instance counts: reduced per wave when the wave holds one instance,
   // then merged across the workgroup, so a one-instance launch issues one
   // atomic per counter and workgroup
-  __shared__ int sk[4];
-  __shared__ unsigned long long sT[4], sU[4], sE[4], sR[4];
   if (uniform) {
 #pragma unroll
     for (int s = 32; s > 0; s >>= 1) {
@@ -642,7 +675,7 @@ __global__ __launch_bounds__(256) void commit_kernel(Layout L, long long n_edges
       R += __shfl_xor(R, s, 64);
     }
     if (lane == 0) {
-      sk[wv] = (E | R) ? kk : -1;
+      sc[wv] = (E | R) ? kk : -1;
       sT[wv] = T; sU[wv] = U; sE[wv] = E; sR[wv] = R;
     }
   } else {
@@ -652,14 +685,15 @@ __global__ __launch_bounds__(256) void commit_kernel(Layout L, long long n_edges
       atomicAdd(&L.ictr[k].edges, E);
       if (R) atomicAdd(&L.ictr[k].removed, R);
     }
-    if (lane == 0) sk[wv] = -1;
+    if (lane == 0) sc[wv] = -1;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
+    // runs of waves with the same instance share one set of atomics
     int cur = -1;
     unsigned long long t = 0, u = 0, ed = 0, rmv = 0;
-    for (int w = 0; w <= 4; ++w) {
-      const int kw = w < 4 ? sk[w] : -2;
+    for (int w = 0; w <= kCommitWaves; ++w) {
+      const int kw = w < kCommitWaves ? sc[w] : -2;
       if (kw == -1) continue;
       if (kw != cur && cur >= 0) {
         atomicAdd(&L.ictr[cur].tests, t);
@@ -777,7 +811,8 @@ void launch_search_ln(hipStream_t s, int level, const Layout &L, int rpt,
 
 void launch_commit(hipStream_t s, const Layout &L, int64_t n_edges) {
   if (n_edges <= 0) return;
-  hipLaunchKernelGGL(commit_kernel, dim3((unsigned)((n_edges + 255) / 256)), dim3(256), 0, s, L,
+  const int tpb = 64 * kCommitWaves;
+  hipLaunchKernelGGL(commit_kernel, dim3((unsigned)((n_edges + tpb - 1) / tpb)), dim3(tpb), 0, s, L,
                      (long long)n_edges);
 }
 
