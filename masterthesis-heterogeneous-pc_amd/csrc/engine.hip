@@ -325,6 +325,13 @@ int engine_set_corr(Engine &e, int id, const double *h_cor) {
   h.have_cor = true;
   h.fast = false;
   e.h_cor_valid = false;
+  h.cor_nan = 0;
+  const size_t pp = (size_t)h.p * h.p;
+  for (size_t t = 0; t < pp; ++t)
+    if (h_cor[t] != h_cor[t]) {
+      h.cor_nan = 1;
+      break;
+    }
   return HPC_OK;
 }
 
@@ -482,7 +489,7 @@ int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &
     I.start = start;
     I.p = h.p; I.n = h.n; I.npad = h.npad; I.W = h.W;
     I.level = levels[k];
-    I.flags = kActive;
+    I.flags = kActive | (h.cor_nan != 0 ? kCorNaN : 0);
     h.start = start;
     std::fill(e.h_row_inst.begin() + start, e.h_row_inst.begin() + start + h.p, k);
     start += h.p;
@@ -601,7 +608,10 @@ int engine_corr(Engine &e, float *ms_moments, float *ms_corr) {
     HPC_HIPCHK(hipGetLastError());
   }
   HPC_HIPCHK(hipEventRecord(e.ev[2], e.stream));
-  for (int id : e.packed) e.inst[id].have_cor = true;
+  for (int id : e.packed) {
+    e.inst[id].have_cor = true;
+    e.inst[id].cor_nan = -1;
+  }
   e.h_cor_valid = false;
   if (any_fast_at(e, -1)) {
     // instances with NaN cells took the exact masked kernel; the float
@@ -677,6 +687,7 @@ int engine_level0(Engine &e, float *ms) {
     if (e.h_inst[k].level != 0) continue;
     h.undirected -= (int64_t)e.h_ictr[k].removed;
     if (e.h_ictr[k].removed) h.compacted = false;
+    h.cor_nan = e.h_ictr[k].has_nan ? 1 : 0;
   }
   return HPC_OK;
 }
@@ -967,12 +978,15 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
     }
     HPC_HIPCHK(hipGetLastError());
   }
-  HPC_HIPCHK(hipEventRecord(e.ev[4], e.stream));
-  if (any1) launch_commit(e.stream, L, e.n_edges);
-  HPC_HIPCHK(hipGetLastError());
-  HPC_HIPCHK(hipEventRecord(e.ev[5], e.stream));
-  rc = read_counters(e);
-  if (rc) return rc;
+  if (any1) {
+    HPC_HIPCHK(hipEventRecord(e.ev[4], e.stream));
+    launch_commit(e.stream, L, e.n_edges);
+    HPC_HIPCHK(hipGetLastError());
+    HPC_HIPCHK(hipEventRecord(e.ev[5], e.stream));
+    // (a level-0-only step has its counts from the read above)
+    rc = read_counters(e);
+    if (rc) return rc;
+  }
   if (e.h_ctr->overflow) {
     // a recheck queue overflowed: level 0 and the removal were skipped, so
     // the step left no trace; exact correlations for the fast instances, then
@@ -985,7 +999,7 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
   const double wall =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   const double ms_l0 = elapsed(e.ev[0], e.ev[1]), ms_c = elapsed(e.ev[1], e.ev[2]);
-  const double ms_m = elapsed(e.ev[4], e.ev[5]);
+  const double ms_m = any1 ? elapsed(e.ev[4], e.ev[5]) : 0.0;
   double ms_s = 0.0;
   for (int k = 0; k < npairs; ++k) ms_s += elapsed(e.sev[2 * k], e.sev[2 * k + 1]);
   for (int k = 0; k < K; ++k) {
@@ -1010,6 +1024,7 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
       s.edges = (int64_t)c.edges;
       s.removed = (int64_t)c.removed;
       h.undirected -= s.removed;
+      if (lev[k] == 0) h.cor_nan = c.has_nan ? 1 : 0;
       s.edges_left = h.undirected;
       if (K == 1) {
         s.pinv_fallbacks = (int64_t)e.h_ctr->pinv_fallbacks;

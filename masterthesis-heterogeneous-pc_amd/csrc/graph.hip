@@ -72,6 +72,7 @@ __global__ __launch_bounds__(64 * kL0Waves) void level0_kernel(Layout L,
   unsigned long long colword = 0ull;   // bits r: pair (64I + r, j) kept
   unsigned long long rowword = 0ull;   // captured by lane r: row 64I + r, word J
   unsigned long long tested = 0ull, removed = 0ull;
+  bool nan_seen = false;  // any pair of the tile, present or not
   const int r0 = wv * kL0Rows;
 #pragma unroll
   for (int rr = 0; rr < kL0Rows; ++rr) {
@@ -82,7 +83,11 @@ __global__ __launch_bounds__(64 * kL0Waves) void level0_kernel(Layout L,
       const unsigned long long aw = In.adj[(size_t)i * W + J];
       const bool valid = (j < p) && (diag ? (lane < r) : true);
       present = valid && ((aw >> lane) & 1ull);
-      if (present) keep = !removes(In.cor[(size_t)i * p + j], In.r_crit);
+      if (valid) {
+        const double c = In.cor[(size_t)i * p + j];
+        nan_seen |= __builtin_isnan(c);
+        if (present) keep = !removes(c, In.r_crit);
+      }
     }
     const unsigned long long kb = __ballot(keep);
     tested += __popcll(__ballot(present));
@@ -93,6 +98,9 @@ __global__ __launch_bounds__(64 * kL0Waves) void level0_kernel(Layout L,
   if (lane >= r0 && lane < r0 + kL0Rows) s_row[lane] = rowword;
   s_col[wv][lane] = colword;
   if (lane == 0) { s_cnt[wv][0] = tested; s_cnt[wv][1] = removed; }
+  // the matrix is symmetric: the lower triangle decides whether tests at
+  // later levels must check their inputs for NaN
+  if (__any(nan_seen) && lane == 0) atomicOr(&L.ictr[k].has_nan, 1);
   __syncthreads();
   if (wv != 0) return;
   unsigned long long cw = 0ull;

@@ -59,11 +59,13 @@ struct InstCounters {
   unsigned long long removed;    // undirected edges removed
   unsigned long long rec_words;  // sepset record words appended
   int max_deg;                   // largest row length at level start
-  int has_nan;                   // observations hold NaN cells (correlation phase)
+  int has_nan;                   // observations hold NaN cells (correlation phase);
+                                 // the correlation matrix holds NaN (level 0)
 };
 
 enum InstFlags {
   kActive = 1,   // takes part in this phase
+  kCorNaN = 2,   // the correlation matrix may hold NaN: tests check their inputs
 };
 
 // Device-side descriptor of one instance.
@@ -180,6 +182,9 @@ struct InstHost {
   int corr_mode = 0;       // HPC_CORR_* of the next correlation
   bool fast = false;       // cor holds fast values (mode F32 and no NaN cells)
   bool have_x = false, have_cor = false;
+  // NaN in the correlation matrix: -1 unknown (tests check their inputs),
+  // 0 none, 1 some; found by level 0, which reads every pair, or on upload
+  int cor_nan = -1;
   bool drop_obs = false;   // free observations and z-scores once correlated
   // level state
   int start = 0;           // global id of local variable 0 in the last pack

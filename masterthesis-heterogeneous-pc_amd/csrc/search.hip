@@ -180,6 +180,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_lds_kernel(
       const double r_crit = In.r_crit;
       const int nn = deg - 1;
       const bool fast = FAST && In.fast;
+      const bool chk = In.flags & kCorNaN;  // inputs may be NaN
       for (long long e = e0 + wave; e < seg_end; e += kWavesPerBlock) {
         const int y = (int)(e - base);
         const int lj = ids[y];
@@ -193,7 +194,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_lds_kernel(
             const int pos = q < y ? q : q + 1;
             const double x2 = Mi[pos];
             const double x3 = cj[ids[pos]];
-            if (!(__builtin_isnan(x1) || __builtin_isnan(x2) || __builtin_isnan(x3))) {
+            if (!(chk && (__builtin_isnan(x1) || __builtin_isnan(x2) || __builtin_isnan(x3)))) {
               const double r = pcor_l1(x1, x2, x3);
               pass = removes(r, r_crit);
               if (fast) {
@@ -378,6 +379,9 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
       __syncthreads();
       const double r_crit = In.r_crit;
       const bool fast = FAST && In.fast;
+      // NaN inputs keep the edge (cpuInd.cpp:23-25,31-33); checked only when
+      // the instance's correlation matrix may hold NaN
+      const bool chk = In.flags & kCorNaN;
       const int nn = deg - 1;
       const unsigned long long total = binom(bt, LV + 1, nn, LV);
       const double *Mi = M + (size_t)deg * D;  // row i
@@ -408,20 +412,26 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
 #pragma unroll
               for (int c = 0; c < LV; ++c) pos[c] = q[c] < y ? q[c] : q[c] + 1;
               double x[LV], yv[LV], C[LV][LV];
-              bool anynan = __builtin_isnan(a);
 #pragma unroll
               for (int c = 0; c < LV; ++c) {
                 x[c] = Mi[pos[c]];
                 yv[c] = Mj[pos[c]];
-                anynan |= __builtin_isnan(x[c]) || __builtin_isnan(yv[c]);
               }
 #pragma unroll
               for (int c = 0; c < LV; ++c)
 #pragma unroll
-                for (int r = c + 1; r < LV; ++r) {
-                  C[r][c] = M[pos[c] * D + pos[r]];
-                  anynan |= __builtin_isnan(C[r][c]);
-                }
+                for (int r = c + 1; r < LV; ++r) C[r][c] = M[pos[c] * D + pos[r]];
+              bool anynan = false;
+              if (chk) {
+                anynan = __builtin_isnan(a);
+#pragma unroll
+                for (int c = 0; c < LV; ++c)
+                  anynan |= __builtin_isnan(x[c]) || __builtin_isnan(yv[c]);
+#pragma unroll
+                for (int c = 0; c < LV; ++c)
+#pragma unroll
+                  for (int r = c + 1; r < LV; ++r) anynan |= __builtin_isnan(C[r][c]);
+              }
               if (!anynan) {
                 bool fb, unc = false;
                 double r;
