@@ -182,6 +182,7 @@ void engine_free(Engine &e) {
   dfree(e.d_task_off); dfree(e.d_binom); dfree(e.d_prefix); dfree(e.d_ctr);
   e.d_ictr = nullptr;
   dfree(e.d_scatter_idx); dfree(e.d_scatter_val); dfree(e.d_rchk); dfree(e.d_pairs);
+  dfree(e.d_blk);
   if (e.h_ctr) (void)hipHostFree(e.h_ctr);
   e.h_ctr = e.h_ctr_dev = nullptr;
   e.h_ictr = nullptr;
@@ -392,7 +393,12 @@ static int ensure_inst_cap(Engine &e, size_t K) {
 
 // The packed instance table to the device: a kernel argument when it is
 // small, else a copy of e.h_inst (the next pack waits for it).
-static int upload_table(Engine &e) {
+static size_t counter_words(size_t K) {
+  return (K ? kIctrOffset + K * sizeof(InstCounters) : sizeof(Counters)) / 8;
+}
+
+// zero: the counter block of the K packed instances is cleared too
+static int upload_table(Engine &e, bool zero = false) {
   const int K = (int)e.h_inst.size();
   if (K == 0) return HPC_OK;
   if (K <= kTableArg) {
@@ -400,6 +406,10 @@ static int upload_table(Engine &e) {
     memset(&a, 0, sizeof(a));
     for (int k = 0; k < K; ++k) a.inst[k] = e.h_inst[k];
     a.K = K;
+    if (zero) {
+      a.zero = reinterpret_cast<unsigned long long *>(e.d_ctr);
+      a.zero_words = (int)counter_words(K);
+    }
     launch_table(e.stream, e.d_inst, a);
     HPC_HIPCHK(hipGetLastError());
     return HPC_OK;
@@ -407,6 +417,7 @@ static int upload_table(Engine &e) {
   HPC_HIPCHK(hipMemcpyAsync(e.d_inst, e.h_inst.data(), K * sizeof(Inst), hipMemcpyHostToDevice,
                             e.stream));
   e.host_upload_pending = true;
+  if (zero) HPC_HIPCHK(hipMemsetAsync(e.d_ctr, 0, counter_words(K) * 8, e.stream));
   return HPC_OK;
 }
 
@@ -414,7 +425,7 @@ static int ensure_row_cap(Engine &e, size_t rows) {
   if (rows <= e.cap_rows && e.d_row_inst) return HPC_OK;
   const size_t c = std::max<size_t>(rows + rows / 4, 64);
   dfree(e.d_row_inst); dfree(e.d_deg); dfree(e.d_rowptr); dfree(e.d_rowmask);
-  dfree(e.d_row_tasks); dfree(e.d_task_off);
+  dfree(e.d_row_tasks); dfree(e.d_task_off); dfree(e.d_blk);
   e.cap_rows = 0;
   e.row_inst_valid = false;
   int rc = dalloc((void **)&e.d_row_inst, c * sizeof(int32_t));
@@ -423,8 +434,11 @@ static int ensure_row_cap(Engine &e, size_t rows) {
   if (!rc) rc = dalloc((void **)&e.d_rowmask, c * sizeof(int32_t));
   if (!rc) rc = dalloc((void **)&e.d_row_tasks, c * sizeof(long long));
   if (!rc) rc = dalloc((void **)&e.d_task_off, c * sizeof(long long));
-  if (!rc) e.cap_rows = c;
-  return rc;
+  const size_t nb = (size_t)compact_fused_blocks((int)c);
+  if (!rc) rc = dalloc((void **)&e.d_blk, 2 * nb * sizeof(unsigned long long));
+  if (rc) return rc;
+  e.cap_rows = c;
+  return HPC_OK;
 }
 
 static int ensure_edge_cap(Engine &e, size_t edges) {
@@ -451,7 +465,8 @@ static int ensure_recheck_cap(Engine &e) {
   return HPC_OK;
 }
 
-int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &levels) {
+int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &levels,
+                bool zero_ctr) {
   HPC_HIPCHK(hipSetDevice(e.device));
   // a previous phase's upload may still read these host vectors
   if (e.host_upload_pending) {
@@ -497,7 +512,7 @@ int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &
   e.packed = ids;
   e.p_total = p_total;
   if (K > 0) {
-    rc = upload_table(e);
+    rc = upload_table(e, zero_ctr);
     if (rc) return rc;
     // the row map changes only with the set of instances
     if (!e.row_inst_valid || e.h_row_inst != e.h_row_inst_dev) {
@@ -549,18 +564,13 @@ static float elapsed(hipEvent_t a, hipEvent_t b) {
 }
 
 // Device counters and the packed instances' counters into the host mirror,
-// and a wait until all work so far on the stream is done.  A one-workgroup
-// kernel stores them into fine-grained host memory and then a sequence
-// number; the host spins on that number, which comes back sooner than a
-// copy plus a stream synchronisation.  The spin asks the stream every few
-// hundred polls, so a failed launch or a stream error ends it.
-static int read_counters(Engine &e) {
-  const size_t K = e.packed.size();
-  const size_t bytes = K ? kIctrOffset + K * sizeof(InstCounters) : sizeof(Counters);
-  const unsigned long long seq = ++e.seq;
-  launch_publish(e.stream, reinterpret_cast<const unsigned long long *>(e.d_ctr), bytes / 8,
-                 reinterpret_cast<unsigned long long *>(e.h_ctr_dev), e.d_seq, seq);
-  HPC_HIPCHK(hipGetLastError());
+// and a wait until all work so far on the stream is done.  A kernel stores
+// them into fine-grained host memory and then a sequence number; the host
+// spins on that number, which comes back sooner than a copy plus a stream
+// synchronisation.  The spin asks the stream every few hundred polls, so a
+// failed launch or a stream error ends it.
+// Waits until the publication numbered `seq` has arrived in host memory.
+static int wait_published(Engine &e, unsigned long long seq) {
   for (unsigned it = 1;; ++it) {
     if (__atomic_load_n(e.h_seq, __ATOMIC_ACQUIRE) == seq) break;
     if ((it & 255u) == 0) {
@@ -576,6 +586,25 @@ static int read_counters(Engine &e) {
   }
   e.host_upload_pending = false;  // everything before the publication is done
   return HPC_OK;
+}
+
+// Publication arguments for a kernel that publishes the counters itself.
+static Publish next_publish(Engine &e) {
+  Publish pub;
+  pub.dst = reinterpret_cast<unsigned long long *>(e.h_ctr_dev);
+  pub.flag = e.d_seq;
+  pub.seq = ++e.seq;
+  pub.words = (int)counter_words(e.packed.size());
+  pub.pad_ = 0;
+  return pub;
+}
+
+static int read_counters(Engine &e) {
+  const Publish pub = next_publish(e);
+  launch_publish(e.stream, reinterpret_cast<const unsigned long long *>(e.d_ctr), pub.words,
+                 pub.dst, pub.flag, pub.seq);
+  HPC_HIPCHK(hipGetLastError());
+  return wait_published(e, pub.seq);
 }
 
 int engine_read_inst_counters(Engine &e) { return read_counters(e); }
@@ -805,7 +834,9 @@ int engine_commit(Engine &e, float *ms) {
   }
   Layout L = layout(e);
   HPC_HIPCHK(hipEventRecord(e.ev[6], e.stream));
-  launch_commit(e.stream, L, e.n_edges);
+  Publish none;
+  memset(&none, 0, sizeof(none));
+  launch_commit(e.stream, L, e.n_edges, none);
   HPC_HIPCHK(hipGetLastError());
   HPC_HIPCHK(hipEventRecord(e.ev[7], e.stream));
   int rc = engine_read_inst_counters(e);
@@ -879,8 +910,7 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
     rc = ensure_rec(e, e.inst[act[k]], lev[k]);
     if (rc) return rc;
   }
-  rc = engine_pack(e, act, lev);
-  if (!rc) rc = zero_counters(e);
+  rc = engine_pack(e, act, lev, /*zero_ctr=*/true);
   if (rc) return rc;
   // observations and z-scores are only read by the correlation, which the
   // pack above has waited for; blocks freed here may go to an upload on
@@ -902,6 +932,11 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
   Layout L = layout(e);
   bool any0 = false, any1 = false;
   for (int l : lev) { any0 |= l == 0; any1 |= l >= 1; }
+  // every instance at one level >= 1: one compaction kernel that also plans
+  // the level's tasks and publishes the counters
+  int lvl_u = lev[0];
+  for (int l : lev) if (l != lvl_u) lvl_u = -1;
+  const bool fused = lvl_u >= 1 && lvl_u <= kMaxLevel;
   HPC_HIPCHK(hipEventRecord(e.ev[0], e.stream));
   if (any0 && any_fast_at(e, 0)) launch_refine_l0(e.stream, L);
   if (any0) {
@@ -910,11 +945,22 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
   }
   HPC_HIPCHK(hipGetLastError());
   HPC_HIPCHK(hipEventRecord(e.ev[1], e.stream));
-  if (any1) launch_compact(e.stream, L);
-  HPC_HIPCHK(hipGetLastError());
-  HPC_HIPCHK(hipEventRecord(e.ev[2], e.stream));
-  rc = read_counters(e);
-  if (rc) return rc;
+  if (fused) {
+    const Publish pub = next_publish(e);
+    launch_compact_fused(e.stream, L, lvl_u, engine_rpt(e, lvl_u), e.d_blk, e.d_row_tasks,
+                         e.d_task_off, e.cap_edges, pub);
+    HPC_HIPCHK(hipGetLastError());
+    HPC_HIPCHK(hipEventRecord(e.ev[2], e.stream));
+    // the totals arrive while the fill kernel runs
+    rc = wait_published(e, pub.seq);
+    if (rc) return rc;
+  } else {
+    if (any1) launch_compact(e.stream, L);
+    HPC_HIPCHK(hipGetLastError());
+    HPC_HIPCHK(hipEventRecord(e.ev[2], e.stream));
+    rc = read_counters(e);
+    if (rc) return rc;
+  }
   e.n_edges = any1 ? (int64_t)e.h_ctr->n_edges : 0;
   e.max_deg = any1 ? e.h_ctr->max_deg : 0;
   L.max_deg = e.max_deg;
@@ -967,7 +1013,9 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
       ++npairs;
     }
     for (int l : levels_ge2) {
-      launch_task_plan(e.stream, l, L, engine_rpt(e, l), nullptr, e.d_row_tasks, e.d_task_off);
+      // (the fused compaction planned the tasks of its level)
+      if (!fused)
+        launch_task_plan(e.stream, l, L, engine_rpt(e, l), nullptr, e.d_row_tasks, e.d_task_off);
       HPC_HIPCHK(hipEventRecord(e.sev[2 * npairs], e.stream));
       const bool fl = any_fast_at(e, l);
       launch_search_ln(e.stream, l, L, engine_rpt(e, l), e.d_row_tasks, e.d_task_off, fl,
@@ -980,11 +1028,17 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
   }
   if (any1) {
     HPC_HIPCHK(hipEventRecord(e.ev[4], e.stream));
-    launch_commit(e.stream, L, e.n_edges);
-    HPC_HIPCHK(hipGetLastError());
-    HPC_HIPCHK(hipEventRecord(e.ev[5], e.stream));
-    // (a level-0-only step has its counts from the read above)
-    rc = read_counters(e);
+    if (e.n_edges > 0) {
+      // the removal kernel publishes the counters itself
+      const Publish pub = next_publish(e);
+      launch_commit(e.stream, L, e.n_edges, pub);
+      HPC_HIPCHK(hipGetLastError());
+      HPC_HIPCHK(hipEventRecord(e.ev[5], e.stream));
+      rc = wait_published(e, pub.seq);
+    } else {
+      HPC_HIPCHK(hipEventRecord(e.ev[5], e.stream));
+      rc = read_counters(e);
+    }
     if (rc) return rc;
   }
   if (e.h_ctr->overflow) {

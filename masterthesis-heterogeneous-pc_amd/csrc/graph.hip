@@ -236,6 +236,191 @@ __global__ __launch_bounds__(256) void fill_kernel(Layout L) {
 __global__ void table_kernel(Inst *__restrict__ dst, TableArg arg) {
   const int k = threadIdx.x;
   if (k < arg.K) dst[k] = arg.inst[k];
+  for (int w = k; w < arg.zero_words; w += blockDim.x) arg.zero[w] = 0ull;
+}
+
+// ---- two-kernel compaction ---------------------------------------------------
+// Count: row lengths (compact.cu:80-88), task counts deg * ceil(C(deg-1, l) /
+// rpt) and per-workgroup sums; the last workgroup to finish scans the sums,
+// writes the totals and publishes the counters.  Fill: row pointers, the task
+// plan and the neighbour lists of each workgroup's rows.  Nothing waits on
+// another workgroup.
+constexpr int kCompWaves = 4;
+constexpr int kCompRowsPerWave = 4;
+constexpr int kCompRows = kCompWaves * kCompRowsPerWave;
+
+__device__ __forceinline__ unsigned long long ld_agent(const unsigned long long *v) {
+  return __hip_atomic_load(v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// blk[2b], blk[2b + 1]: edges and tasks of workgroup b; then, from the last
+// workgroup, their exclusive prefixes
+__global__ __launch_bounds__(64 * kCompWaves) void compact_count_kernel(
+    Layout L, int level, int rpt, int nblk, unsigned long long *blk,
+    long long *__restrict__ row_tasks, Publish pub) {
+  __shared__ int s_deg[kCompRows], s_k[kCompRows];
+  __shared__ long long s_tasks[kCompRows];
+  __shared__ int s_last;
+  __shared__ unsigned long long s_part[2 * 64 * kCompWaves];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int b = blockIdx.x;
+  const int g0 = b * kCompRows;
+#pragma unroll
+  for (int rr = 0; rr < kCompRowsPerWave; ++rr) {
+    const int r = wv * kCompRowsPerWave + rr;
+    const int g = g0 + r;
+    int c = 0, k = -1;
+    long long tk = 0;
+    if (g < L.p_total) {
+      k = L.row_inst[g];
+      const Inst &In = L.inst[k];
+      const bool on = (In.flags & kActive) && In.level >= 1;
+      if (on) {
+        const unsigned long long *row = In.adj + (size_t)(g - In.start) * In.W;
+        for (int w = lane; w < In.W; w += 64) c += __popcll(row[w]);
+      }
+#pragma unroll
+      for (int sh = 32; sh > 0; sh >>= 1) c += __shfl_xor(c, sh, 64);
+      if (on && level >= 2 && In.level == level && c > 0) {
+        const unsigned long long total = binom(L.binom, kBinomK, c - 1, level);
+        if (total > 0) tk = (long long)c * (long long)((total + rpt - 1) / rpt);
+      }
+    }
+    if (lane == 0) {
+      s_deg[r] = c;
+      s_k[r] = c ? k : -1;
+      s_tasks[r] = tk;
+      if (g < L.p_total) {
+        L.deg[g] = c;
+        row_tasks[g] = tk;
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    unsigned long long E = 0, T = 0;
+    int mx = 0, cur = -1, m = 0;
+    for (int r = 0; r < kCompRows; ++r) {
+      E += (unsigned long long)s_deg[r];
+      T += (unsigned long long)s_tasks[r];
+      mx = max(mx, s_deg[r]);
+      // per-instance maxima over runs of rows of one instance
+      const int kr = s_k[r];
+      if (kr < 0) continue;
+      if (kr != cur && cur >= 0) { atomicMax(&L.ictr[cur].max_deg, m); m = 0; }
+      cur = kr;
+      m = max(m, s_deg[r]);
+    }
+    if (cur >= 0) atomicMax(&L.ictr[cur].max_deg, m);
+    if (mx) atomicMax(&L.ctr->max_deg, mx);
+    blk[2 * (size_t)b] = E;
+    blk[2 * (size_t)b + 1] = T;
+    __threadfence();
+    s_last = atomicAdd(&L.ctr->done, 1u) == (unsigned)(nblk - 1);
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  // the last workgroup: exclusive scan of the workgroup sums
+  const int nt = blockDim.x;
+  const int per = (nblk + nt - 1) / nt;
+  const int lo = tid * per, hi = min(nblk, lo + per);
+  unsigned long long se = 0, st = 0;
+  for (int q = lo; q < hi; ++q) {
+    se += ld_agent(&blk[2 * (size_t)q]);
+    st += ld_agent(&blk[2 * (size_t)q + 1]);
+  }
+  s_part[tid] = se;
+  s_part[nt + tid] = st;
+  __syncthreads();
+  for (int off = 1; off < nt; off <<= 1) {
+    const unsigned long long ve = tid >= off ? s_part[tid - off] : 0ull;
+    const unsigned long long vt = tid >= off ? s_part[nt + tid - off] : 0ull;
+    __syncthreads();
+    s_part[tid] += ve;
+    s_part[nt + tid] += vt;
+    __syncthreads();
+  }
+  unsigned long long re = s_part[tid] - se, rt = s_part[nt + tid] - st;
+  for (int q = lo; q < hi; ++q) {
+    const unsigned long long e = ld_agent(&blk[2 * (size_t)q]);
+    const unsigned long long t = ld_agent(&blk[2 * (size_t)q + 1]);
+    blk[2 * (size_t)q] = re;
+    blk[2 * (size_t)q + 1] = rt;
+    re += e;
+    rt += t;
+  }
+  if (tid == nt - 1) {
+    L.rowptr[L.p_total] = (int32_t)s_part[tid];
+    L.ctr->n_edges = s_part[tid];
+    L.ctr->n_tasks = s_part[nt + tid];
+  }
+  __threadfence();
+  __syncthreads();
+  if (!pub.flag) return;
+  const unsigned long long *src = reinterpret_cast<const unsigned long long *>(L.ctr);
+  for (int w = tid; w < pub.words; w += nt) pub.dst[w] = ld_agent(&src[w]);
+  __threadfence_system();
+  __syncthreads();
+  if (tid == 0)
+    __hip_atomic_store(pub.flag, pub.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__global__ __launch_bounds__(64 * kCompWaves) void compact_fill_kernel(
+    Layout L, const unsigned long long *__restrict__ blk, const long long *__restrict__ row_tasks,
+    long long *__restrict__ task_off, unsigned long long edge_cap) {
+  __shared__ int s_deg[kCompRows];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int b = blockIdx.x;
+  const int g0 = b * kCompRows;
+  const unsigned long long exe = blk[2 * (size_t)b], ext = blk[2 * (size_t)b + 1];
+  if (tid < kCompRows) s_deg[tid] = g0 + tid < L.p_total ? L.deg[g0 + tid] : 0;
+  __syncthreads();
+  if (tid < kCompRows) {
+    const int g = g0 + tid;
+    if (g < L.p_total) {
+      unsigned long long pe = 0, pt = 0;
+      for (int r = 0; r < tid; ++r) {
+        pe += (unsigned long long)s_deg[r];
+        pt += (unsigned long long)row_tasks[g0 + r];
+      }
+      L.rowptr[g] = (int32_t)(exe + pe);
+      task_off[g] = (long long)(ext + pt);
+    }
+  }
+#pragma unroll 1
+  for (int rr = 0; rr < kCompRowsPerWave; ++rr) {
+    const int r = wv * kCompRowsPerWave + rr;
+    const int g = g0 + r;
+    if (g >= L.p_total || s_deg[r] == 0) continue;
+    unsigned long long base = exe;
+    for (int q = 0; q < r; ++q) base += (unsigned long long)s_deg[q];
+    if (base + (unsigned long long)s_deg[r] > edge_cap) continue;  // never with a true prefix
+    const Inst &In = L.inst[L.row_inst[g]];
+    const unsigned long long *row = In.adj + (size_t)(g - In.start) * In.W;
+    int wbase = (int)base;
+    for (int w0 = 0; w0 < In.W; w0 += 64) {
+      const int w = w0 + lane;
+      unsigned long long word = w < In.W ? row[w] : 0ull;
+      const int c = __popcll(word);
+      int incl = c;
+#pragma unroll
+      for (int sh = 1; sh < 64; sh <<= 1) {
+        const int v = __shfl_up(incl, sh, 64);
+        if (lane >= sh) incl += v;
+      }
+      int off = wbase + incl - c;
+      while (word) {
+        const int bit = __ffsll((long long)word) - 1;
+        word &= word - 1ull;
+        L.nbr[off] = In.start + w * 64 + bit;
+        L.erow[off] = g;
+        L.fp[off] = -1;
+        ++off;
+      }
+      wbase += __shfl(incl, 63, 64);
+    }
+  }
 }
 
 // One workgroup: the words, a system-scope fence per thread, then the flag.
@@ -255,6 +440,19 @@ __global__ __launch_bounds__(256) void publish_kernel(const unsigned long long *
 void launch_publish(hipStream_t s, const unsigned long long *src, size_t words,
                     unsigned long long *dst, unsigned long long *flag, unsigned long long seq) {
   hipLaunchKernelGGL(publish_kernel, dim3(1), dim3(256), 0, s, src, words, dst, flag, seq);
+}
+
+int compact_fused_blocks(int p_total) { return (p_total + kCompRows - 1) / kCompRows; }
+
+void launch_compact_fused(hipStream_t s, const Layout &L, int level, int rpt,
+                          unsigned long long *blk, long long *row_tasks, long long *task_off,
+                          size_t edge_cap, const Publish &pub) {
+  const int nblk = compact_fused_blocks(L.p_total);
+  hipLaunchKernelGGL(compact_count_kernel, dim3(nblk), dim3(64 * kCompWaves), 0, s, L, level, rpt,
+                     nblk, blk, row_tasks, pub);
+  hipLaunchKernelGGL(compact_fill_kernel, dim3(nblk), dim3(64 * kCompWaves), 0, s, L,
+                     (const unsigned long long *)blk, (const long long *)row_tasks, task_off,
+                     (unsigned long long)edge_cap);
 }
 
 void launch_table(hipStream_t s, Inst *dst, const TableArg &arg) {

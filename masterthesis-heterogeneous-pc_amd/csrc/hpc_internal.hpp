@@ -42,6 +42,8 @@ struct Counters {
   unsigned long long n_pairs;         // level-0 pairs queued for an exact correlation
   int max_deg;                        // largest row length of the snapshot
   int overflow;                       // a recheck queue overflowed: the phase is void
+  unsigned int done;                  // compaction workgroups finished (the last scans)
+  unsigned int done2;                 // removal workgroups finished (the last publishes)
 };
 
 // A test whose fast-correlation statistic lies within its error bound of the
@@ -136,6 +138,8 @@ void launch_adj_init(hipStream_t s, unsigned long long *adj, int p, int W);
 constexpr int kTableArg = 8;
 struct TableArg {
   Inst inst[kTableArg];
+  unsigned long long *zero;  // words cleared by the same launch (or null)
+  int zero_words;
   int K;
 };
 void launch_table(hipStream_t s, Inst *dst, const TableArg &arg);
@@ -147,6 +151,26 @@ void launch_publish(hipStream_t s, const unsigned long long *src, size_t words,
 void launch_level0(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst,
                    long long *d_prefix, std::vector<long long> &h_prefix);
 void launch_compact(hipStream_t s, const Layout &L);
+// Counter publication by the last workgroup of a kernel: copies `words`
+// 8-byte words from the counter block to fine-grained host memory `dst`,
+// then stores `seq` to *flag (device-visible pointers).
+struct Publish {
+  unsigned long long *dst;
+  unsigned long long *flag;
+  unsigned long long seq;
+  int words;
+  int pad_;
+};
+// Compaction of every active instance (all at `level` >= 1) in two kernels
+// that also plan the level's tasks (level >= 2, `rpt` ranks per task) and
+// publish the counters (the first kernel's last workgroup, once the totals
+// are known): row lengths and per-workgroup sums, then row pointers, task
+// offsets and the neighbour lists with their first passing ranks reset.
+// blk: 2 words per workgroup (compact_fused_blocks).
+void launch_compact_fused(hipStream_t s, const Layout &L, int level, int rpt,
+                          unsigned long long *blk, long long *row_tasks, long long *task_off,
+                          size_t edge_cap, const Publish &pub);
+int compact_fused_blocks(int p_total);
 // `fast`: the launch holds an instance with fast correlations (selects the
 // variant with uncertainty bounds and the recheck queue)
 // epb: ordered edges per workgroup of the LDS-staged form
@@ -159,7 +183,8 @@ void launch_task_plan(hipStream_t s, int level, const Layout &L, int rpt,
 // tpb: tasks per workgroup of the LDS-staged form
 void launch_search_ln(hipStream_t s, int level, const Layout &L, int rpt,
                       const long long *row_tasks, const long long *task_off, bool fast, int tpb);
-void launch_commit(hipStream_t s, const Layout &L, int64_t n_edges);
+// With pub.flag set, the last workgroup to finish publishes the counters.
+void launch_commit(hipStream_t s, const Layout &L, int64_t n_edges, const Publish &pub);
 void launch_scatter_fp(hipStream_t s, const int64_t *idx, const int64_t *val,
                        long long count, int64_t *fp);
 
@@ -234,6 +259,8 @@ struct Engine {
   int32_t *d_rowmask = nullptr;
   int64_t *d_fp = nullptr;
   long long *d_row_tasks = nullptr, *d_task_off = nullptr;
+  // per-workgroup sums of the two-kernel compaction
+  unsigned long long *d_blk = nullptr;
   unsigned long long *d_binom = nullptr;
   long long *d_prefix = nullptr;
   // device counters and the packed instances' counters in one block (and
@@ -300,7 +327,9 @@ int engine_set_corr(Engine &e, int id, const double *h_cor);
 int engine_reset(Engine &e, int id);
 // Instance table of `ids` with the given per-instance levels; every phase
 // below works on the packed set.
-int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &levels);
+// zero_ctr: the same launch clears the counter block (small tables)
+int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &levels,
+                bool zero_ctr = false);
 int engine_corr(Engine &e, float *ms_moments, float *ms_corr);
 // Before packing an instance for its correlation: the fast flag from its
 // correlation mode, and its float z-score block.

@@ -532,9 +532,11 @@ __device__ __forceinline__ int find_pos(const int32_t *row, int len, int node) {
 // wavefront reserves the space of all its records with one atomic when it
 // holds one instance; the record order is scheduling-dependent, the records
 // are not.
-__global__ __launch_bounds__(64 * kCommitWaves) void commit_kernel(Layout L, long long n_edges) {
-  // an overflowed recheck queue leaves the level undecided: the host redoes it
-  if (L.ctr->overflow) return;
+__global__ __launch_bounds__(64 * kCommitWaves) void commit_kernel(Layout L, long long n_edges,
+                                                                 Publish pub) {
+  // an overflowed recheck queue leaves the level undecided: the host redoes
+  // it (the workgroups still take part in the publication below)
+  const bool live = !L.ctr->overflow;
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   int k = -1, level = 0;
@@ -542,7 +544,7 @@ __global__ __launch_bounds__(64 * kCommitWaves) void commit_kernel(Layout L, lon
   bool rm = false;
   int i = 0, j = 0, y = 0, base = 0, nn = 0;
   unsigned long long rank = 0;
-  if (e < n_edges) {
+  if (live && e < n_edges) {
     i = L.erow[e];
     k = L.row_inst[i];
     level = L.inst[k].level;
@@ -717,6 +719,25 @@ __global__ __launch_bounds__(64 * kCommitWaves) void commit_kernel(Layout L, lon
       t += sT[w]; u += sU[w]; ed += sE[w]; rmv += sR[w];
     }
   }
+  if (!pub.flag) return;
+  // the last workgroup to finish publishes the counters
+  __shared__ int s_last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    s_last = atomicAdd(&L.ctr->done2, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (s_last) {
+    __threadfence();
+    const unsigned long long *src = reinterpret_cast<const unsigned long long *>(L.ctr);
+    for (int w = threadIdx.x; w < pub.words; w += blockDim.x)
+      pub.dst[w] = __hip_atomic_load(&src[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0)
+      __hip_atomic_store(pub.flag, pub.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
 
 __global__ void scatter_kernel(const int64_t *__restrict__ idx, const int64_t *__restrict__ val,
@@ -819,11 +840,11 @@ void launch_search_ln(hipStream_t s, int level, const Layout &L, int rpt,
   }
 }
 
-void launch_commit(hipStream_t s, const Layout &L, int64_t n_edges) {
+void launch_commit(hipStream_t s, const Layout &L, int64_t n_edges, const Publish &pub) {
   if (n_edges <= 0) return;
   const int tpb = 64 * kCommitWaves;
   hipLaunchKernelGGL(commit_kernel, dim3((unsigned)((n_edges + tpb - 1) / tpb)), dim3(tpb), 0, s, L,
-                     (long long)n_edges);
+                     (long long)n_edges, pub);
 }
 
 void launch_scatter_fp(hipStream_t s, const int64_t *idx, const int64_t *val, long long count,
