@@ -181,34 +181,58 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_lds_kernel(
       const int nn = deg - 1;
       const bool fast = FAST && In.fast;
       const bool chk = In.flags & kCorNaN;  // inputs may be NaN
-      for (long long e = e0 + wave; e < seg_end; e += kWavesPerBlock) {
-        const int y = (int)(e - base);
-        const int lj = ids[y];
-        const double x1 = Mi[y];
-        const double *cj = In.cor + (size_t)lj * p;
-        long long found = -1;
-        for (int q0 = 0; q0 < nn; q0 += 64) {
-          const int q = q0 + lane;
-          bool pass = false, unc = false;
-          if (q < nn) {
-            const int pos = q < y ? q : q + 1;
-            const double x2 = Mi[pos];
-            const double x3 = cj[ids[pos]];
-            if (!(chk && (__builtin_isnan(x1) || __builtin_isnan(x2) || __builtin_isnan(x3)))) {
-              const double r = pcor_l1(x1, x2, x3);
-              pass = removes(r, r_crit);
-              if (fast) {
-                const double xa[1] = {x2}, ya[1] = {x3}, Cz[1][1] = {{1.0}};
-                unc = !fast_certain(r, r_crit, pcor_sensitivity<1>(x1, xa, ya, Cz), In.eps);
-                pass &= !unc;
-              }
-            }
-          }
-          const unsigned long long m = __ballot(pass);
-          if (fast) queue_uncertain(L, unc, m, e, (unsigned long long)q0, lane);
-          if (m) { found = q0 + __ffsll((long long)m) - 1; break; }
+      // A wave takes 4 edges: first ranks 0..15 of all 4 (16 lanes each),
+      // then 64-rank chunks of each edge that has not passed yet, so an
+      // early pass costs 16 gathers instead of 64.
+      auto test = [&](int y, double x1, const double *cj, int q, bool &unc) -> bool {
+        unc = false;
+        const int pos = q < y ? q : q + 1;
+        const double x2 = Mi[pos];
+        const double x3 = cj[ids[pos]];
+        if (chk && (__builtin_isnan(x1) || __builtin_isnan(x2) || __builtin_isnan(x3))) return false;
+        const double r = pcor_l1(x1, x2, x3);
+        bool pass = removes(r, r_crit);
+        if (fast) {
+          const double xa[1] = {x2}, ya[1] = {x3}, Cz[1][1] = {{1.0}};
+          unc = !fast_certain(r, r_crit, pcor_sensitivity<1>(x1, xa, ya, Cz), In.eps);
+          pass &= !unc;
         }
-        if (lane == 0) L.fp[e] = found;
+        return pass;
+      };
+      for (long long eb = e0 + 4 * wave; eb < seg_end; eb += 4 * kWavesPerBlock) {
+        const int grp = lane >> 4, t = lane & 15;
+        const long long eg = eb + grp;
+        const bool live = eg < seg_end;
+        const int yg = live ? (int)(eg - base) : 0;
+        const double *cjg = In.cor + (size_t)ids[yg] * p;
+        bool pass = false, unc = false;
+        if (live && t < nn) pass = test(yg, Mi[yg], cjg, t, unc);
+        const unsigned long long m = __ballot(pass);
+        const unsigned gm = (unsigned)(m >> (16 * grp)) & 0xFFFFu;
+        // this group's uncertain tests below its first certain pass
+        if (fast) queue_rank(L, unc && (gm == 0u || t < __ffs(gm) - 1), eg, (unsigned long long)t, lane);
+        long long found = gm ? (long long)(__ffs(gm) - 1) : -1;
+        if (live && t == 0 && (found >= 0 || nn <= 16)) L.fp[eg] = found;
+        // the edges still open, one after the other with 64-rank chunks
+        const unsigned long long open = __ballot(live && t == 0 && found < 0 && nn > 16);
+#pragma unroll 1
+        for (int g = 0; g < 4; ++g) {
+          if (!((open >> (16 * g)) & 1ull)) continue;
+          const long long e = eb + g;
+          const int y = (int)(e - base);
+          const double x1 = Mi[y];
+          const double *cj = In.cor + (size_t)ids[y] * p;
+          long long f = -1;
+          for (int q0 = 16; q0 < nn; q0 += 64) {
+            const int q = q0 + lane;
+            bool ps = false, un = false;
+            if (q < nn) ps = test(y, x1, cj, q, un);
+            const unsigned long long mm = __ballot(ps);
+            if (fast) queue_uncertain(L, un, mm, e, (unsigned long long)q0, lane);
+            if (mm) { f = q0 + __ffsll((long long)mm) - 1; break; }
+          }
+          if (lane == 0) L.fp[e] = f;
+        }
       }
     }
     e0 = seg_end;
