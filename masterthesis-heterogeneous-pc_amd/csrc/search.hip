@@ -30,8 +30,11 @@ namespace hpc {
 namespace {
 
 constexpr int kWavesPerBlock = 4;
-// consecutive ranks per lane in the LDS-staged level >= 2 search
-constexpr int kRun = 4;
+// consecutive ranks per lane in the LDS-staged level >= 2 search: longer
+// runs share one unrank among more tests, shorter ones waste fewer lanes
+// on the short rank ranges of low levels
+template <int LV>
+constexpr int run_len() { return LV >= 4 ? 8 : 4; }
 // wavefronts per workgroup of the removal kernel (one record reservation and
 // one set of counter atomics per workgroup)
 constexpr int kCommitWaves = 16;
@@ -372,6 +375,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
   double *M = reinterpret_cast<double *>(smem + (size_t)brows * (LV + 1));  // dmax x dmax
   int *ids = reinterpret_cast<int *>(M + (size_t)dmax * dmax);     // dmax local ids
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int kRun = run_len<LV>();
   for (int t = tid; t < brows * (LV + 1); t += blockDim.x) {
     const int m = t / (LV + 1), k = t % (LV + 1);
     bt[t] = L.binom[(size_t)m * kBinomK + k];
