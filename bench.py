@@ -386,6 +386,15 @@ def main() -> int:
             "traffic": traffic.get("ci_tests_traffic_bytes_per_step"),
             "algorithmic_bytes_per_step": test_bytes,
             "traffic_source": traffic.get("source")}
+    if roof["traffic"] and k_test > 0:
+        # what the kernels actually move through HBM, at the measured time
+        rate = roof["traffic"] / (k_test * 1e-3) / 1e9
+        roof["hbm_traffic_GBs"] = rate
+        roof["hbm_traffic_frac"] = rate / HBM_PEAK_GBS
+        roof["note"] = ("achieved counts algorithmic bytes; the level >= 2 kernels read their "
+                        "operands from an LDS-staged row block, so HBM traffic is %.0f%% of them "
+                        "and frac can exceed 1; those kernels are bound by FP64 VALU issue"
+                        % (100.0 * roof["traffic"] / max(test_bytes, 1)))
     corr_roof = {"kernel": "corr_tile_kernel (fp64, canonical summation order)", "bound": "fp64",
                  "achieved": corr_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                  "frac": corr_tflops / FP64_PEAK_TFLOPS,
