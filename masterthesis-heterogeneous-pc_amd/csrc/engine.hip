@@ -184,8 +184,14 @@ void engine_free(Engine &e) {
   dfree(e.d_scatter_idx); dfree(e.d_scatter_val); dfree(e.d_rchk); dfree(e.d_pairs);
   dfree(e.d_blk);
   if (e.h_ctr) (void)hipHostFree(e.h_ctr);
-  e.h_ctr = e.h_ctr_dev = nullptr;
+  if (e.h_ctr2) (void)hipHostFree(e.h_ctr2);
+  e.h_ctr = e.h_ctr_dev = e.h_ctr2 = e.h_ctr2_dev = nullptr;
   e.h_ictr = nullptr;
+  if (e.lev_ev_ready)
+    for (auto &row : e.lev_ev)
+      for (auto &ev : row)
+        if (ev) (void)hipEventDestroy(ev);
+  e.lev_ev_ready = false;
   if (e.h_seq) (void)hipHostFree(e.h_seq);
   e.h_seq = e.d_seq = nullptr;
   for (auto &ev : e.ev)
@@ -367,7 +373,8 @@ static int ensure_inst_cap(Engine &e, size_t K) {
   const size_t c = std::max<size_t>(K + K / 4, 8);
   dfree(e.d_inst); dfree(e.d_ctr); dfree(e.d_prefix);
   if (e.h_ctr) (void)hipHostFree(e.h_ctr);
-  e.h_ctr = e.h_ctr_dev = nullptr;
+  if (e.h_ctr2) (void)hipHostFree(e.h_ctr2);
+  e.h_ctr = e.h_ctr_dev = e.h_ctr2 = e.h_ctr2_dev = nullptr;
   e.d_ictr = e.h_ictr = nullptr;
   e.cap_inst = 0;
   const size_t block = kIctrOffset + c * sizeof(InstCounters);
@@ -380,7 +387,13 @@ static int ensure_inst_cap(Engine &e, size_t K) {
     e.h_ctr = nullptr;
     rc = hpc_fail(HPC_ENOMEM, "hipHostMalloc failed");
   }
-  if (!rc && hipHostGetDevicePointer((void **)&e.h_ctr_dev, e.h_ctr, 0) != hipSuccess)
+  if (!rc && hipHostMalloc((void **)&e.h_ctr2, block, hipHostMallocMapped | hipHostMallocCoherent) !=
+                 hipSuccess) {
+    e.h_ctr2 = nullptr;
+    rc = hpc_fail(HPC_ENOMEM, "hipHostMalloc failed");
+  }
+  if (!rc && (hipHostGetDevicePointer((void **)&e.h_ctr_dev, e.h_ctr, 0) != hipSuccess ||
+              hipHostGetDevicePointer((void **)&e.h_ctr2_dev, e.h_ctr2, 0) != hipSuccess))
     rc = hpc_fail(HPC_EHIP, "hipHostGetDevicePointer failed");
   if (rc) return rc;
   e.d_ictr = reinterpret_cast<InstCounters *>(reinterpret_cast<char *>(e.d_ctr) + kIctrOffset);
@@ -595,7 +608,7 @@ static Publish next_publish(Engine &e) {
   pub.flag = e.d_seq;
   pub.seq = ++e.seq;
   pub.words = (int)counter_words(e.packed.size());
-  pub.pad_ = 0;
+  pub.advance = 0;
   return pub;
 }
 
@@ -696,7 +709,7 @@ int engine_level0(Engine &e, float *ms) {
   HPC_HIPCHK(hipEventRecord(e.ev[3], e.stream));
   if (any_fast_at(e, 0)) launch_refine_l0(e.stream, L);
   launch_level0(e.stream, L, e.h_inst, e.d_prefix + 2 * (L.K + 1), e.h_prefix);
-  e.host_upload_pending = true;
+  if (L.K > 1) e.host_upload_pending = true;  // the tile offsets
   HPC_HIPCHK(hipGetLastError());
   HPC_HIPCHK(hipEventRecord(e.ev[4], e.stream));
   rc = read_counters(e);
@@ -873,6 +886,193 @@ size_t engine_footprint(int p, int n, int max_level) {
   return b;
 }
 
+// One instance on exact correlations, levels h.next_level .. its cap, with
+// one host wait per level.  Per level l >= 1 the stream holds: compaction
+// (publishes n_edges / max_deg into h_ctr), [host wait], CI tests, removal.
+// The removal publishes its counts into h_ctr2, clears the counter block and
+// advances the device table to level l + 1 and its record base, so the next
+// compaction is queued right behind it; the host reads the removal's counts
+// at the next wait (publications arrive in stream order).  Level 0 publishes
+// into h_ctr2 the same way.  Results and per-level records equal
+// engine_step's.
+int engine_run_single(Engine &e, int id) {
+  HPC_HIPCHK(hipSetDevice(e.device));
+  InstHost &h = e.inst[id];
+  if (!h.have_cor) return hpc_fail(HPC_ESTATE, "no correlation matrix");
+  if (h.fast) return hpc_fail(HPC_ESTATE, "the one-wait level loop needs exact correlations");
+  if (!e.lev_ev_ready) {
+    for (auto &row : e.lev_ev)
+      for (auto &ev : row) HPC_HIPCHK(hipEventCreate(&ev));
+    e.lev_ev_ready = true;
+  }
+  using Clock = std::chrono::steady_clock;
+  const std::vector<int> ids{id};
+  const InstCounters *i2 =
+      reinterpret_cast<const InstCounters *>(reinterpret_cast<const char *>(e.h_ctr2) + kIctrOffset);
+  auto publish2 = [&]() {
+    Publish pb = next_publish(e);
+    pb.dst = reinterpret_cast<unsigned long long *>(e.h_ctr2_dev);
+    return pb;
+  };
+  // the level whose counts sit in h_ctr2 (or will, once the next wait ends)
+  int pend = -1;
+  unsigned long long pend_seq = 0;
+  Clock::time_point pend_t0 = Clock::now();
+  auto finalize = [&](int lv, int max_deg, const Clock::time_point &t_end) {
+    const InstCounters &c = i2[0];
+    hpc_level_stats st;
+    memset(&st, 0, sizeof(st));
+    st.level = lv;
+    st.max_deg = lv == 0 ? h.p : max_deg;
+    st.tests = (int64_t)c.tests;
+    st.tests_max = (int64_t)c.tests_max;
+    st.edges = (int64_t)c.edges;
+    st.removed = (int64_t)c.removed;
+    h.undirected -= st.removed;
+    st.edges_left = h.undirected;
+    st.pinv_fallbacks = (int64_t)e.h_ctr2->pinv_fallbacks;
+    st.rechecks = 0;
+    hipEvent_t *ev = e.lev_ev[lv];
+    if (lv == 0) {
+      st.ms_test = elapsed(ev[4], ev[5]);
+      h.cor_nan = c.has_nan ? 1 : 0;
+    } else {
+      st.ms_compact = elapsed(ev[0], ev[1]);
+      st.ms_test = elapsed(ev[4], ev[5]);
+      st.ms_commit = elapsed(ev[2], ev[3]);
+      h.rec_used += (size_t)c.rec_words;
+      engine_note_level(e, id, lv);
+    }
+    if (st.removed) h.compacted = false;
+    st.ms_wall = std::chrono::duration<double, std::milli>(t_end - pend_t0).count();
+    h.hist.push_back(st);
+  };
+  int pend_max_deg = 0;
+  int l = h.next_level;
+  int rc = HPC_OK;
+  if (l == 0) {
+    pend_t0 = Clock::now();
+    rc = engine_pack(e, ids, {0}, /*zero_ctr=*/true);
+    if (rc) return rc;
+    Layout L = layout(e);
+    HPC_HIPCHK(hipEventRecord(e.lev_ev[0][4], e.stream));
+    launch_level0(e.stream, L, e.h_inst, e.d_prefix + 2 * (L.K + 1), e.h_prefix);
+    HPC_HIPCHK(hipGetLastError());
+    HPC_HIPCHK(hipEventRecord(e.lev_ev[0][5], e.stream));
+    const Publish pb = publish2();
+    launch_publish(e.stream, reinterpret_cast<const unsigned long long *>(e.d_ctr), pb.words,
+                   pb.dst, pb.flag, pb.seq);
+    HPC_HIPCHK(hipGetLastError());
+    pend = 0;
+    pend_seq = pb.seq;
+    pend_max_deg = h.p;
+    l = 1;
+    h.next_level = 1;
+    if (h.max_level == 0) {
+      rc = wait_published(e, pend_seq);
+      if (rc) return rc;
+      finalize(0, pend_max_deg, Clock::now());
+      h.finished = true;
+      return HPC_OK;
+    }
+    // level 1's table on clear counters (level 0's are in h_ctr2)
+    e.h_inst[0].level = 1;
+    rc = upload_table(e, /*zero=*/true);
+    if (rc) return rc;
+  } else {
+    // the device table starts at level l
+    rc = engine_pack(e, ids, {l}, /*zero_ctr=*/true);
+    if (rc) return rc;
+  }
+  for (;;) {
+    if (l > kMaxLevel) break;  // the step loop takes levels the kernels do not reach
+    const Clock::time_point t_lv = Clock::now();
+    Layout L = layout(e);
+    HPC_HIPCHK(hipEventRecord(e.lev_ev[l][0], e.stream));
+    const Publish pa = next_publish(e);
+    launch_compact_fused(e.stream, L, l, engine_rpt(e, l), e.d_blk, e.d_row_tasks, e.d_task_off,
+                         e.cap_edges, pa);
+    HPC_HIPCHK(hipGetLastError());
+    HPC_HIPCHK(hipEventRecord(e.lev_ev[l][1], e.stream));
+    rc = wait_published(e, pa.seq);
+    if (rc) return rc;
+    if (pend >= 0) {
+      finalize(pend, pend_max_deg, t_lv);
+      pend = -1;
+    }
+    pend_t0 = t_lv;
+    e.n_edges = (int64_t)e.h_ctr->n_edges;
+    e.max_deg = e.h_ctr->max_deg;
+    h.max_deg = e.h_ictr[0].max_deg;
+    h.compacted = true;
+    h.ever_compacted = true;
+    const bool testable = h.max_deg - 1 >= l;
+    if (!testable && h.max_level < 0) {
+      h.finished = true;  // nothing left to test: no record (as engine_step)
+      return HPC_OK;
+    }
+    // record space, the NaN flag of level 1 on: the table when it changed
+    rc = ensure_rec(e, h, l);
+    if (rc) return rc;
+    Inst &I = e.h_inst[0];
+    const int flags = kActive | (h.cor_nan != 0 ? kCorNaN : 0);
+    const bool retable = I.rec != h.rec || I.flags != flags;
+    I.level = l;
+    I.rec = h.rec;
+    I.rec_base = h.rec_used;
+    I.flags = flags;
+    if (retable) {
+      rc = upload_table(e, /*zero=*/false);
+      if (rc) return rc;
+    }
+    L = layout(e);
+    L.max_deg = e.max_deg;
+    HPC_HIPCHK(hipEventRecord(e.lev_ev[l][4], e.stream));
+    if (e.n_edges > 0) {
+      if (l == 1)
+        launch_search_l1(e.stream, L, e.n_edges, nullptr, false, engine_epb(e));
+      else
+        launch_search_ln(e.stream, l, L, engine_rpt(e, l), e.d_row_tasks, e.d_task_off, false,
+                         engine_tpb(e, l));
+      HPC_HIPCHK(hipGetLastError());
+    }
+    HPC_HIPCHK(hipEventRecord(e.lev_ev[l][5], e.stream));
+    HPC_HIPCHK(hipEventRecord(e.lev_ev[l][2], e.stream));
+    const bool last = h.max_level >= 0 && l + 1 > h.max_level;
+    Publish pb = publish2();
+    if (e.n_edges > 0) {
+      // publishes, clears the counters, advances level and record base
+      pb.advance = 1;
+      launch_commit(e.stream, L, e.n_edges, pb);
+      HPC_HIPCHK(hipGetLastError());
+    } else {
+      launch_publish(e.stream, reinterpret_cast<const unsigned long long *>(e.d_ctr), pb.words,
+                     pb.dst, pb.flag, pb.seq);
+      HPC_HIPCHK(hipGetLastError());
+      if (!last) {
+        I.level = l + 1;
+        rc = upload_table(e, /*zero=*/true);
+        if (rc) return rc;
+      }
+    }
+    HPC_HIPCHK(hipEventRecord(e.lev_ev[l][3], e.stream));
+    pend = l;
+    pend_seq = pb.seq;
+    pend_max_deg = h.max_deg;
+    ++l;
+    h.next_level = l;
+    e.h_inst[0].level = l;  // the device table's level after the removal
+    if (last) break;
+  }
+  if (pend >= 0) {
+    rc = wait_published(e, pend_seq);
+    if (rc) return rc;
+    finalize(pend, pend_max_deg, Clock::now());
+  }
+  if (h.max_level >= 0 && h.next_level > h.max_level) h.finished = true;
+  return HPC_OK;
+}
+
 int engine_step(Engine &e, const std::vector<int> &ids) {
   HPC_HIPCHK(hipSetDevice(e.device));
   int rc = 0;
@@ -941,7 +1141,7 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
   if (any0 && any_fast_at(e, 0)) launch_refine_l0(e.stream, L);
   if (any0) {
     launch_level0(e.stream, L, e.h_inst, e.d_prefix + 2 * (K + 1), e.h_prefix);
-    e.host_upload_pending = true;
+    if (K > 1) e.host_upload_pending = true;  // the tile offsets
   }
   HPC_HIPCHK(hipGetLastError());
   HPC_HIPCHK(hipEventRecord(e.ev[1], e.stream));

@@ -48,20 +48,23 @@ __global__ void adj_init_kernel(unsigned long long *adj, int p, int W) {
 constexpr int kL0Waves = 8;
 constexpr int kL0Rows = 64 / kL0Waves;
 
+// prefix: per-instance tile offsets (K + 1), or null for one instance with
+// `tiles` tiles
 __global__ __launch_bounds__(64 * kL0Waves) void level0_kernel(Layout L,
-                                                              const long long *__restrict__ prefix) {
+                                                              const long long *__restrict__ prefix,
+                                                              long long tiles) {
   __shared__ unsigned long long s_row[64];
   __shared__ unsigned long long s_col[kL0Waves][64];
   __shared__ unsigned long long s_cnt[kL0Waves][2];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const long long t = blockIdx.x;
-  if (t >= prefix[L.K]) return;
+  if (t >= (prefix ? prefix[L.K] : tiles)) return;
   // an overflowed refinement queue leaves level 0 undecided: the host redoes it
   if (L.ctr->overflow) return;
-  const int k = find_inst(prefix, L.K, t);
+  const int k = prefix ? find_inst(prefix, L.K, t) : 0;
   const Inst In = L.inst[k];
   const int p = In.p, W = In.W;
-  const long long tl = t - prefix[k];
+  const long long tl = prefix ? t - prefix[k] : t;
   // lower block triangle, row-major: tile tl -> (I, J), J <= I
   int I = (int)((sqrt(8.0 * (double)tl + 1.0) - 1.0) * 0.5);
   while ((long long)I * (I + 1) / 2 > tl) --I;
@@ -474,10 +477,16 @@ void launch_level0(hipStream_t s, const Layout &L, const std::vector<Inst> &host
     h_prefix[k + 1] = h_prefix[k] + (on ? (long long)I.W * (I.W + 1) / 2 : 0);
   }
   if (h_prefix[K] == 0) return;
+  if (K == 1) {
+    // one instance: no offsets to upload
+    hipLaunchKernelGGL(level0_kernel, dim3((unsigned)h_prefix[K]), dim3(64 * kL0Waves), 0, s, L,
+                       (const long long *)nullptr, h_prefix[K]);
+    return;
+  }
   (void)hipMemcpyAsync(d_prefix, h_prefix.data(), h_prefix.size() * sizeof(long long),
                        hipMemcpyHostToDevice, s);
   hipLaunchKernelGGL(level0_kernel, dim3((unsigned)h_prefix[K]), dim3(64 * kL0Waves), 0, s, L,
-                     d_prefix);
+                     (const long long *)d_prefix, h_prefix[K]);
 }
 
 void launch_compact(hipStream_t s, const Layout &L) {

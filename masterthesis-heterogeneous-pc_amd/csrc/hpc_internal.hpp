@@ -159,7 +159,9 @@ struct Publish {
   unsigned long long *flag;
   unsigned long long seq;
   int words;
-  int pad_;
+  // removal kernel of a one-instance run: after publishing, clear the counter
+  // block and advance the instance's level and record base on the device
+  int advance;
 };
 // Compaction of every active instance (all at `level` >= 1) in two kernels
 // that also plan the level's tasks (level >= 2, `rpt` ranks per task) and
@@ -268,6 +270,12 @@ struct Engine {
   // d_ctr at the start, d_ictr at kIctrOffset
   Counters *d_ctr = nullptr, *h_ctr = nullptr;
   Counters *h_ctr_dev = nullptr;  // h_ctr as the device addresses it
+  // a second host mirror: a one-instance run's removal publishes there while
+  // the next level's compaction publishes into h_ctr
+  Counters *h_ctr2 = nullptr, *h_ctr2_dev = nullptr;
+  // per-level events of a one-instance run: compaction, search, removal
+  hipEvent_t lev_ev[kMaxLevel + 1][6] = {};
+  bool lev_ev_ready = false;
   // publication flag (fine-grained host memory): the last sequence number a
   // counter publication wrote, its device address, and the next number
   unsigned long long *h_seq = nullptr, *d_seq = nullptr;
@@ -361,6 +369,9 @@ int engine_epb(const Engine &e);
 // (results do not depend on the shape), keeps the fastest per level in the
 // engine and leaves the instance reset.
 int engine_autotune(Engine &e, int id, int max_level, std::vector<hpc_tune_entry> &out);
+// hpc_run's level loop for one instance on exact correlations (levels from
+// h.next_level to its cap), with one host wait per level.
+int engine_run_single(Engine &e, int id);
 // Per-level launch shapes (entries as hpc_autotune reports them): the check
 // alone, and check-then-apply (nothing is applied unless every entry is valid).
 int engine_check_tuning(const hpc_tune_entry *t, int count);

@@ -759,12 +759,25 @@ __global__ __launch_bounds__(64 * kCommitWaves) void commit_kernel(Layout L, lon
   if (s_last) {
     __threadfence();
     const unsigned long long *src = reinterpret_cast<const unsigned long long *>(L.ctr);
+    unsigned long long words_rec = 0;
+    if (pub.advance && threadIdx.x == 0)
+      words_rec = __hip_atomic_load(&L.ictr[0].rec_words, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (int w = threadIdx.x; w < pub.words; w += blockDim.x)
       pub.dst[w] = __hip_atomic_load(&src[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __threadfence_system();
     __syncthreads();
     if (threadIdx.x == 0)
       __hip_atomic_store(pub.flag, pub.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (pub.advance) {
+      // the next level of the (single) instance starts from clear counters
+      unsigned long long *blk = reinterpret_cast<unsigned long long *>(L.ctr);
+      for (int w = threadIdx.x; w < pub.words; w += blockDim.x) blk[w] = 0ull;
+      if (threadIdx.x == 0) {
+        Inst *In = const_cast<Inst *>(L.inst);
+        In[0].level += 1;
+        In[0].rec_base += words_rec;
+      }
+    }
   }
 }
 
