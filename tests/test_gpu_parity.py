@@ -253,3 +253,35 @@ def test_applied_tuning_any_shape_keeps_results(hpc, orc):
             assert np.array_equal(dev.adj(), ref.adj()), k
             assert np.array_equal(dev.sepsets(ref.mc()), ref.sepsets()), k
             assert [s["tests"] for s in st] == [r["tests"] for r in recs], k
+
+
+def test_run_caps_and_mixed_entry_points(hpc, orc):
+    """hpc_run's one-wait level loop stops exactly at level caps 0, 1, 2 and
+    4, and a run after levels taken one by one (hpc_level) gives the same
+    skeleton as a fresh run."""
+    x, n, alpha, _ = workloads.sem("small")
+    p = x.shape[0]
+    cor = orc.corr(x)
+    with hpc.DeviceSkeleton(p, n, alpha, max_level=4) as dev:
+        dev.pmcc(x)
+        for cap in (0, 1, 2, 4):
+            print(f"[run] cap {cap}", flush=True)
+            ref = orc.Skeleton(cor, n, alpha, max_level=cap)
+            recs = ref.run(cap)
+            dev.reset()
+            st = dev.run(cap)
+            assert [s["level"] for s in st] == list(range(cap + 1)), (cap, st)
+            assert np.array_equal(dev.adj(), ref.adj()), cap
+            assert np.array_equal(dev.sepsets(ref.mc()), ref.sepsets()), cap
+            assert [s["tests"] for s in st] == [r["tests"] for r in recs], cap
+            assert [s["removed"] for s in st] == [r["removed"] for r in recs], cap
+        # levels 0 and 1 one by one, then a full run from scratch
+        dev.reset()
+        dev.level(0)
+        dev.level(1)
+        dev.reset()
+        st = dev.run(4)
+        ref = orc.Skeleton(cor, n, alpha, max_level=4)
+        recs = ref.run(4)
+        assert np.array_equal(dev.adj(), ref.adj())
+        assert [s["tests"] for s in st] == [r["tests"] for r in recs]
