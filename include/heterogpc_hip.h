@@ -122,7 +122,7 @@ int hpc_set_tuning(hpc_state *st, int param, int value);
  * 1..max_level once, timing each level's CI-test kernel on that level's
  * snapshot for every candidate launch shape -- level 1: ordered edges per
  * workgroup in {8..128}; levels >= 2: conditioning sets per task in
- * {256..4096} x tasks per workgroup in {4..32} -- and keeps the fastest per
+ * {256..4096} x tasks per workgroup in {4..128} -- and keeps the fastest per
  * level for this state's later runs.  Results do not depend on the shape.
  * The state is left reset (adjacency and sepsets back to the start). */
 typedef struct hpc_tune_entry {

@@ -1377,7 +1377,7 @@ int engine_autotune(Engine &e, int id, int max_level, std::vector<hpc_tune_entry
       for (int epb : {8, 16, 32, 64, 128}) cand.push_back({0, epb});
     } else {
       for (int rpt : {256, 512, 1024, 2048, 4096})
-        for (int tpb : {4, 8, 16, 32}) cand.push_back({rpt, tpb});
+        for (int tpb : {4, 8, 16, 32, 64, 128}) cand.push_back({rpt, tpb});
     }
     double best = 1e300;
     std::pair<int, int> pick = cand[0];

@@ -211,7 +211,7 @@ def test_autotune_picks_shapes_and_keeps_results(hpc, orc):
         assert entries[0]["edges_per_block"] in (8, 16, 32, 64, 128)
         for t in entries[1:]:
             assert t["ranks_per_task"] in (256, 512, 1024, 2048, 4096)
-            assert t["tasks_per_block"] in (4, 8, 16, 32)
+            assert t["tasks_per_block"] in (4, 8, 16, 32, 64, 128)
             assert 0 < t["ms_best"] <= t["ms_default"]
         assert dev.adj().sum() == p * (p - 1)  # reset
         st = dev.run(4)
