@@ -356,9 +356,9 @@ int engine_read_inst_counters(Engine &e);
 // Sepset width of an instance's exports covers `level`.
 void engine_note_level(Engine &e, int id, int level);
 // One level for every id (correlation first for those without one): level 0,
-// compaction, the searches of every level present and the commit, with
-// three host synchronisations for the whole set.  Finished instances are
-// marked; their per-level records go to InstHost::hist.
+// compaction, the searches of every level present and the commit, with two
+// host waits for published counters for the whole set.  Finished instances
+// are marked; their per-level records go to InstHost::hist.
 int engine_step(Engine &e, const std::vector<int> &ids);
 // Launch shapes of a level's CI-test kernel.
 int engine_rpt(const Engine &e, int level);

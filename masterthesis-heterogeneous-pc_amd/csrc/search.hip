@@ -5,14 +5,16 @@
 // Search: for every ordered edge (i, j) of the level-start snapshot, the
 // lowest rank whose conditioning set S (an l-subset of N(i)\{j} in
 // lexicographic order of positions, cuda_util.cu:34-50 and
-// gpuRowTests.cu:128-142) gives p >= alpha.  A 64-wide wavefront tests 64
-// consecutive ranks at a time, lane t rank base + t, and a ballot picks the
-// lowest passing lane.  Level 1 runs one wavefront per edge.  Levels >= 2
-// cut each edge's ranks into tasks of `ranks_per_task` (a multiple of 64) so
-// that edges with many sets spread over many wavefronts; a task skips ranks
-// above a pass another task of its edge already found, and the lowest rank
-// wins through atomicMin.  Edges do not cut each other's searches short, so
-// the result is independent of scheduling.
+// gpuRowTests.cu:128-142) gives p >= alpha.  A 64-wide wavefront tests a
+// chunk of consecutive ranks at a time and a ballot picks the lowest passing
+// one.  Level 1: a wavefront takes four edges, ranks 0..15 of each first,
+// then 64 at a time for the edges still open.  Levels >= 2 cut each edge's
+// ranks into tasks of `ranks_per_task` (a multiple of 64) so that edges with
+// many sets spread over many wavefronts; within a task each lane tests a run
+// of consecutive ranks.  A task skips ranks above a pass another task of its
+// edge already found, and the lowest rank wins through atomicMin.  Edges do
+// not cut each other's searches short, so the result is independent of
+// scheduling.
 //
 // Commit: edge i-j (i < j) is removed when either side found a set; the
 // sepset is the lower-id row's lowest passing set, else the higher-id row's.
