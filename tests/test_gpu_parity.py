@@ -285,3 +285,26 @@ def test_run_caps_and_mixed_entry_points(hpc, orc):
         recs = ref.run(4)
         assert np.array_equal(dev.adj(), ref.adj())
         assert [s["tests"] for s in st] == [r["tests"] for r in recs]
+
+
+@pytest.mark.parametrize("mode", ["exact", "f32"])
+def test_benchmark_workload_parity(hpc, orc, mode):
+    """The p=2000 benchmark workload (n=5000, degree 3, alpha 0.05, levels
+    0..4) through hpc_run: the oracle's correlation (exact mode), adjacency,
+    sepsets and per-level test counts, in the exact and the FP32-MFMA
+    correlation modes."""
+    x, n, alpha, ml = workloads.sem("c2k")
+    p = x.shape[0]
+    cor = orc.corr(x)
+    ref = orc.Skeleton(cor, n, alpha, max_level=ml)
+    recs = ref.run(ml)
+    with hpc.DeviceSkeleton(p, n, alpha, max_level=ml) as dev:
+        print(f"[c2k] mode {mode}", flush=True)
+        dev.pmcc(x, mode=0 if mode == "exact" else 1)  # HPC_CORR_CANON / HPC_CORR_F32
+        if mode == "exact":
+            assert np.array_equal(dev.cor(), cor)
+        st = dev.run(ml)
+        assert np.array_equal(dev.adj(), ref.adj())
+        assert np.array_equal(dev.sepsets(ref.mc()), ref.sepsets())
+        assert [s["tests"] for s in st] == [r["tests"] for r in recs]
+        assert [s["removed"] for s in st] == [r["removed"] for r in recs]
