@@ -392,11 +392,10 @@ extern "C" int hpc_run(hpc_state *st, int max_level, hpc_level_stats *stats, int
   InstHost &h = H(st);
   if (!h.have_cor) return fail(HPC_ESTATE, "no correlation matrix");
   // levels 0..maxLevel (skeleton.cpp:172) from the current adjacency; with no
-  // cap, until no row has enough neighbours.  On exact correlations the
-  // one-instance loop queues each level's compaction behind the previous
-  // removal and waits once per level; fast correlations (their recheck
-  // queues may void a level) and levels beyond the device maximum take the
-  // engine's step, which waits twice.
+  // cap, until no row has enough neighbours.  The one-instance loop queues
+  // each level's compaction behind the previous removal and waits once per
+  // level; a level voided by a recheck-queue overflow and levels beyond the
+  // device maximum continue in the engine's step, which waits twice.
   const int saved = h.max_level;
   if (max_level != -2) h.max_level = max_level;
   h.next_level = 0;
@@ -404,7 +403,7 @@ extern "C" int hpc_run(hpc_state *st, int max_level, hpc_level_stats *stats, int
   h.error = 0;
   const size_t first = h.hist.size();
   int rc = HPC_OK;
-  if (!h.fast) rc = engine_run_single(st->eng, st->id);
+  rc = engine_run_single(st->eng, st->id);
   while (!h.finished && !rc) rc = engine_step(st->eng, {st->id});
   if (!rc && h.error) rc = fail(h.error, "level %d exceeds the device maximum %d", h.next_level - 1,
                                kMaxLevel);

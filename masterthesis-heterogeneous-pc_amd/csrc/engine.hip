@@ -886,20 +886,22 @@ size_t engine_footprint(int p, int n, int max_level) {
   return b;
 }
 
-// One instance on exact correlations, levels h.next_level .. its cap, with
-// one host wait per level.  Per level l >= 1 the stream holds: compaction
+// One instance, levels h.next_level .. its cap, with one host wait per
+// level.  Per level l >= 1 the stream holds: compaction
 // (publishes n_edges / max_deg into h_ctr), [host wait], CI tests, removal.
 // The removal publishes its counts into h_ctr2, clears the counter block and
 // advances the device table to level l + 1 and its record base, so the next
 // compaction is queued right behind it; the host reads the removal's counts
 // at the next wait (publications arrive in stream order).  Level 0 publishes
 // into h_ctr2 the same way.  Results and per-level records equal
-// engine_step's.
+// engine_step's.  With fast correlations a level whose recheck queue
+// overflowed did nothing (level 0 and the removal skip themselves); the next
+// wait finds it, the correlations are made exact and the loop returns with
+// h.next_level at that level for engine_step to continue.
 int engine_run_single(Engine &e, int id) {
   HPC_HIPCHK(hipSetDevice(e.device));
   InstHost &h = e.inst[id];
   if (!h.have_cor) return hpc_fail(HPC_ESTATE, "no correlation matrix");
-  if (h.fast) return hpc_fail(HPC_ESTATE, "the one-wait level loop needs exact correlations");
   if (!e.lev_ev_ready) {
     for (auto &row : e.lev_ev)
       for (auto &ev : row) HPC_HIPCHK(hipEventCreate(&ev));
@@ -931,7 +933,8 @@ int engine_run_single(Engine &e, int id) {
     h.undirected -= st.removed;
     st.edges_left = h.undirected;
     st.pinv_fallbacks = (int64_t)e.h_ctr2->pinv_fallbacks;
-    st.rechecks = 0;
+    st.rechecks = (int64_t)e.h_ctr2->rechecks;
+    e.rechecks = e.h_ctr2->rechecks;
     hipEvent_t *ev = e.lev_ev[lv];
     if (lv == 0) {
       st.ms_test = elapsed(ev[4], ev[5]);
@@ -948,6 +951,15 @@ int engine_run_single(Engine &e, int id) {
     h.hist.push_back(st);
   };
   int pend_max_deg = 0;
+  // a voided level (recheck queue overflow): exact correlations, and the step
+  // loop takes over at that level
+  auto voided = [&](int lv) -> int {
+    int r = engine_exact_from_z(e, ids);
+    if (r) return r;
+    h.next_level = lv;
+    h.finished = false;
+    return HPC_OK;
+  };
   int l = h.next_level;
   int rc = HPC_OK;
   if (l == 0) {
@@ -956,6 +968,7 @@ int engine_run_single(Engine &e, int id) {
     if (rc) return rc;
     Layout L = layout(e);
     HPC_HIPCHK(hipEventRecord(e.lev_ev[0][4], e.stream));
+    if (h.fast) launch_refine_l0(e.stream, L);
     launch_level0(e.stream, L, e.h_inst, e.d_prefix + 2 * (L.K + 1), e.h_prefix);
     HPC_HIPCHK(hipGetLastError());
     HPC_HIPCHK(hipEventRecord(e.lev_ev[0][5], e.stream));
@@ -971,6 +984,7 @@ int engine_run_single(Engine &e, int id) {
     if (h.max_level == 0) {
       rc = wait_published(e, pend_seq);
       if (rc) return rc;
+      if (e.h_ctr2->overflow) return voided(0);
       finalize(0, pend_max_deg, Clock::now());
       h.finished = true;
       return HPC_OK;
@@ -997,6 +1011,7 @@ int engine_run_single(Engine &e, int id) {
     rc = wait_published(e, pa.seq);
     if (rc) return rc;
     if (pend >= 0) {
+      if (e.h_ctr2->overflow) return voided(pend);
       finalize(pend, pend_max_deg, t_lv);
       pend = -1;
     }
@@ -1030,10 +1045,12 @@ int engine_run_single(Engine &e, int id) {
     HPC_HIPCHK(hipEventRecord(e.lev_ev[l][4], e.stream));
     if (e.n_edges > 0) {
       if (l == 1)
-        launch_search_l1(e.stream, L, e.n_edges, nullptr, false, engine_epb(e));
+        launch_search_l1(e.stream, L, e.n_edges, nullptr, h.fast, engine_epb(e));
       else
-        launch_search_ln(e.stream, l, L, engine_rpt(e, l), e.d_row_tasks, e.d_task_off, false,
+        launch_search_ln(e.stream, l, L, engine_rpt(e, l), e.d_row_tasks, e.d_task_off, h.fast,
                          engine_tpb(e, l));
+      // exact re-runs of the tests near the threshold (fast correlations)
+      if (h.fast) launch_recheck(e.stream, l, L);
       HPC_HIPCHK(hipGetLastError());
     }
     HPC_HIPCHK(hipEventRecord(e.lev_ev[l][5], e.stream));
@@ -1067,6 +1084,7 @@ int engine_run_single(Engine &e, int id) {
   if (pend >= 0) {
     rc = wait_published(e, pend_seq);
     if (rc) return rc;
+    if (e.h_ctr2->overflow) return voided(pend);
     finalize(pend, pend_max_deg, Clock::now());
   }
   if (h.max_level >= 0 && h.next_level > h.max_level) h.finished = true;

@@ -125,3 +125,28 @@ def test_f32_batch_and_server_mixed_modes(hpc, orc):
             got_adj, got_sep, _ = s.result(r, mc=sep.shape[2])
             assert np.array_equal(got_adj, adj)
             assert np.array_equal(got_sep, sep)
+
+
+@pytest.mark.parametrize("queue", [0, 1])
+def test_f32_run_rechecks_and_overflow(hpc, orc, queue):
+    """hpc_run in the FP32 mode on data with many tests near the threshold:
+    rechecks run inside the one-wait level loop, and with a one-entry queue
+    the voided level is redone on exact correlations; the skeleton and the
+    per-level test counts are the oracle's either way."""
+    x, n, alpha, _ = _near_threshold(hpc, p=200, seed=7)
+    p = x.shape[0]
+    exact = orc.corr(x)
+    ref = orc.Skeleton(exact, n, alpha, max_level=4)
+    recs = ref.run(4)
+    with hpc.DeviceSkeleton(p, n, alpha, max_level=4) as dev:
+        if queue:
+            dev.set_recheck_queue(queue)
+        print(f"[f32 run] queue={queue}", flush=True)
+        dev.pmcc(x, F32)
+        st = dev.run(4)
+        assert np.array_equal(dev.adj(), ref.adj())
+        assert np.array_equal(dev.sepsets(ref.mc()), ref.sepsets())
+        assert [s["tests"] for s in st] == [r["tests"] for r in recs]
+        assert [s["level"] for s in st] == list(range(len(recs)))
+        if not queue:
+            assert sum(s["rechecks"] for s in st) > 0
