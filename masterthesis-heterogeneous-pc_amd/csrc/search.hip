@@ -762,7 +762,7 @@ __global__ __launch_bounds__(64 * kCommitWaves) void commit_kernel(Layout L, lon
     __threadfence();
     const unsigned long long *src = reinterpret_cast<const unsigned long long *>(L.ctr);
     unsigned long long words_rec = 0;
-    if (pub.advance && threadIdx.x == 0)
+    if (pub.advance && L.K == 1 && threadIdx.x == 0)
       words_rec = __hip_atomic_load(&L.ictr[0].rec_words, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (int w = threadIdx.x; w < pub.words; w += blockDim.x)
       pub.dst[w] = __hip_atomic_load(&src[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -770,7 +770,7 @@ __global__ __launch_bounds__(64 * kCommitWaves) void commit_kernel(Layout L, lon
     __syncthreads();
     if (threadIdx.x == 0)
       __hip_atomic_store(pub.flag, pub.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-    if (pub.advance) {
+    if (pub.advance && L.K == 1) {
       // the next level of the (single) instance starts from clear counters
       unsigned long long *blk = reinterpret_cast<unsigned long long *>(L.ctr);
       for (int w = threadIdx.x; w < pub.words; w += blockDim.x) blk[w] = 0ull;
