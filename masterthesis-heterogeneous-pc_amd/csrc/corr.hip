@@ -466,14 +466,15 @@ void launch_moments(hipStream_t s, const Layout &L) {
 void launch_corr(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst,
                  long long *d_prefix, std::vector<long long> &h_prefix) {
   const int K = L.K;
-  // two prefix arrays in one upload: tiled slots, then masked blocks, both
-  // over every active instance; the moments kernel's NaN flag decides which
-  // kernel computes an instance
+  // two prefix arrays in one upload: tiled slots over the active exact
+  // instances, then masked blocks over every active instance; the moments
+  // kernel's NaN flag decides which kernel computes an instance (fast ones
+  // without NaN cells go to the MFMA kernel)
   h_prefix.assign(2 * (K + 1), 0);
   for (int k = 0; k < K; ++k) {
     const Inst &I = host_inst[k];
     const bool on = I.flags & kActive;
-    h_prefix[k + 1] = h_prefix[k] + (on ? tiled_slots(I.p) : 0);
+    h_prefix[k + 1] = h_prefix[k] + (on && !I.fast ? tiled_slots(I.p) : 0);
     h_prefix[K + 1 + k + 1] = h_prefix[K + 1 + k] + (on ? masked_blocks(I.p) : 0);
   }
   (void)hipMemcpyAsync(d_prefix, h_prefix.data(), h_prefix.size() * sizeof(long long),

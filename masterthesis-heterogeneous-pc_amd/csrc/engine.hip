@@ -149,8 +149,11 @@ int engine_init(Engine &e, int device) {
   e.device = device;
   HPC_HIPCHK(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
   HPC_HIPCHK(hipStreamCreateWithFlags(&e.copy_stream, hipStreamNonBlocking));
-  for (auto &ev : e.ev) HPC_HIPCHK(hipEventCreate(&ev));
-  for (auto &ev : e.sev) HPC_HIPCHK(hipEventCreate(&ev));
+  // timing events only: without the system-scope fence a recorded event
+  // costs no cache writeback between the kernels it separates (counters reach
+  // the host through the kernels' own system-scope releases)
+  for (auto &ev : e.ev) HPC_HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableSystemFence));
+  for (auto &ev : e.sev) HPC_HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableSystemFence));
   HPC_HIPCHK(hipEventCreateWithFlags(&e.fp_ready, hipEventDisableTiming));
   HPC_HIPCHK(hipHostMalloc((void **)&e.h_seq, 64, hipHostMallocMapped | hipHostMallocCoherent));
   HPC_HIPCHK(hipHostGetDevicePointer((void **)&e.d_seq, e.h_seq, 0));
@@ -181,8 +184,8 @@ void engine_free(Engine &e) {
   dfree(e.d_nbr); dfree(e.d_erow); dfree(e.d_rowmask); dfree(e.d_fp); dfree(e.d_row_tasks);
   dfree(e.d_task_off); dfree(e.d_binom); dfree(e.d_prefix); dfree(e.d_ctr);
   e.d_ictr = nullptr;
-  dfree(e.d_scatter_idx); dfree(e.d_scatter_val); dfree(e.d_rchk); dfree(e.d_pairs);
-  dfree(e.d_blk);
+  dfree(e.d_scatter_idx); dfree(e.d_scatter_val); dfree(e.d_rchk);
+  dfree(e.d_blk); dfree(e.d_l0slot);
   if (e.h_ctr) (void)hipHostFree(e.h_ctr);
   if (e.h_ctr2) (void)hipHostFree(e.h_ctr2);
   e.h_ctr = e.h_ctr_dev = e.h_ctr2 = e.h_ctr2_dev = nullptr;
@@ -371,7 +374,7 @@ static int ensure_rec(Engine &e, InstHost &h, int level) {
 static int ensure_inst_cap(Engine &e, size_t K) {
   if (K <= e.cap_inst && e.d_inst) return HPC_OK;
   const size_t c = std::max<size_t>(K + K / 4, 8);
-  dfree(e.d_inst); dfree(e.d_ctr); dfree(e.d_prefix);
+  dfree(e.d_inst); dfree(e.d_ctr); dfree(e.d_prefix); dfree(e.d_l0slot);
   if (e.h_ctr) (void)hipHostFree(e.h_ctr);
   if (e.h_ctr2) (void)hipHostFree(e.h_ctr2);
   e.h_ctr = e.h_ctr_dev = e.h_ctr2 = e.h_ctr2_dev = nullptr;
@@ -381,6 +384,8 @@ static int ensure_inst_cap(Engine &e, size_t K) {
   int rc = dalloc((void **)&e.d_inst, c * sizeof(Inst));
   if (!rc) rc = dalloc((void **)&e.d_ctr, block);
   if (!rc) rc = dalloc((void **)&e.d_prefix, 4 * (c + 1) * sizeof(long long));
+  const size_t slot_bytes = c * kL0Slots * kL0SlotWords * sizeof(unsigned long long);
+  if (!rc) rc = dalloc((void **)&e.d_l0slot, slot_bytes);
   // fine-grained, so the publication kernel's stores reach host memory
   if (!rc && hipHostMalloc((void **)&e.h_ctr, block, hipHostMallocMapped | hipHostMallocCoherent) !=
                  hipSuccess) {
@@ -400,6 +405,7 @@ static int ensure_inst_cap(Engine &e, size_t K) {
   e.h_ictr = reinterpret_cast<InstCounters *>(reinterpret_cast<char *>(e.h_ctr) + kIctrOffset);
   // a fresh block holds no counts
   HPC_HIPCHK(hipMemset(e.d_ctr, 0, block));
+  HPC_HIPCHK(hipMemset(e.d_l0slot, 0, slot_bytes));
   e.cap_inst = c;
   return HPC_OK;
 }
@@ -448,7 +454,7 @@ static int ensure_row_cap(Engine &e, size_t rows) {
   if (!rc) rc = dalloc((void **)&e.d_row_tasks, c * sizeof(long long));
   if (!rc) rc = dalloc((void **)&e.d_task_off, c * sizeof(long long));
   const size_t nb = (size_t)compact_fused_blocks((int)c);
-  if (!rc) rc = dalloc((void **)&e.d_blk, 2 * nb * sizeof(unsigned long long));
+  if (!rc) rc = dalloc((void **)&e.d_blk, 3 * nb * sizeof(unsigned long long));
   if (rc) return rc;
   e.cap_rows = c;
   return HPC_OK;
@@ -470,11 +476,10 @@ static int ensure_edge_cap(Engine &e, size_t edges) {
 // phase and the host redoes it with exact correlations).
 static int ensure_recheck_cap(Engine &e) {
   const size_t cap = (size_t)1 << 20;
-  if (e.d_rchk && e.d_pairs) return HPC_OK;
+  if (e.d_rchk) return HPC_OK;
   int rc = dalloc((void **)&e.d_rchk, cap * sizeof(Recheck));
-  if (!rc) rc = dalloc((void **)&e.d_pairs, cap * sizeof(int2));
   if (rc) return rc;
-  e.cap_rchk = e.cap_pairs = cap;
+  e.cap_rchk = cap;
   return HPC_OK;
 }
 
@@ -563,8 +568,7 @@ static Layout layout(Engine &e) {
   L.ctr = e.d_ctr;
   L.rchk = e.d_rchk;
   L.rchk_cap = e.rchk_limit ? std::min(e.rchk_limit, e.cap_rchk) : e.cap_rchk;
-  L.pairs = e.d_pairs;
-  L.pairs_cap = e.rchk_limit ? std::min(e.rchk_limit, e.cap_pairs) : e.cap_pairs;
+  L.l0slot = e.d_l0slot;
   return L;
 }
 
@@ -707,22 +711,12 @@ int engine_level0(Engine &e, float *ms) {
   if (rc) return rc;
   Layout L = layout(e);
   HPC_HIPCHK(hipEventRecord(e.ev[3], e.stream));
-  if (any_fast_at(e, 0)) launch_refine_l0(e.stream, L);
   launch_level0(e.stream, L, e.h_inst, e.d_prefix + 2 * (L.K + 1), e.h_prefix);
   if (L.K > 1) e.host_upload_pending = true;  // the tile offsets
   HPC_HIPCHK(hipGetLastError());
   HPC_HIPCHK(hipEventRecord(e.ev[4], e.stream));
   rc = read_counters(e);
   if (rc) return rc;
-  if (e.h_ctr->overflow) {
-    // too many pairs near the threshold: exact correlations, then level 0 again
-    const std::vector<int> ids = e.packed;
-    const std::vector<int> lv(ids.size(), 0);
-    rc = engine_exact_from_z(e, ids);
-    if (!rc) rc = engine_pack(e, ids, lv);
-    if (rc) return rc;
-    return engine_level0(e, ms);
-  }
   if (ms) *ms = elapsed(e.ev[3], e.ev[4]);
   for (size_t k = 0; k < e.packed.size(); ++k) {
     InstHost &h = e.inst[e.packed[k]];
@@ -895,7 +889,7 @@ size_t engine_footprint(int p, int n, int max_level) {
 // at the next wait (publications arrive in stream order).  Level 0 publishes
 // into h_ctr2 the same way.  Results and per-level records equal
 // engine_step's.  With fast correlations a level whose recheck queue
-// overflowed did nothing (level 0 and the removal skip themselves); the next
+// overflowed did nothing (the removal skips itself); the next
 // wait finds it, the correlations are made exact and the loop returns with
 // h.next_level at that level for engine_step to continue.
 int engine_run_single(Engine &e, int id) {
@@ -904,11 +898,16 @@ int engine_run_single(Engine &e, int id) {
   if (!h.have_cor) return hpc_fail(HPC_ESTATE, "no correlation matrix");
   if (!e.lev_ev_ready) {
     for (auto &row : e.lev_ev)
-      for (auto &ev : row) HPC_HIPCHK(hipEventCreate(&ev));
+      for (auto &ev : row) HPC_HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableSystemFence));
     e.lev_ev_ready = true;
   }
   using Clock = std::chrono::steady_clock;
   const std::vector<int> ids{id};
+  // Each event marker between two queued kernels delays the second by a few
+  // microseconds, so one marker serves both neighbours: the search's end
+  // starts the removal, and a removal's end starts the next compaction.
+  hipEvent_t comp_start[kMaxLevel + 1] = {};
+  hipEvent_t prev_commit_end = nullptr;
   const InstCounters *i2 =
       reinterpret_cast<const InstCounters *>(reinterpret_cast<const char *>(e.h_ctr2) + kIctrOffset);
   auto publish2 = [&]() {
@@ -940,9 +939,9 @@ int engine_run_single(Engine &e, int id) {
       st.ms_test = elapsed(ev[4], ev[5]);
       h.cor_nan = c.has_nan ? 1 : 0;
     } else {
-      st.ms_compact = elapsed(ev[0], ev[1]);
+      st.ms_compact = elapsed(comp_start[lv], ev[1]);
       st.ms_test = elapsed(ev[4], ev[5]);
-      st.ms_commit = elapsed(ev[2], ev[3]);
+      st.ms_commit = elapsed(ev[5], ev[3]);
       h.rec_used += (size_t)c.rec_words;
       engine_note_level(e, id, lv);
     }
@@ -968,7 +967,6 @@ int engine_run_single(Engine &e, int id) {
     if (rc) return rc;
     Layout L = layout(e);
     HPC_HIPCHK(hipEventRecord(e.lev_ev[0][4], e.stream));
-    if (h.fast) launch_refine_l0(e.stream, L);
     launch_level0(e.stream, L, e.h_inst, e.d_prefix + 2 * (L.K + 1), e.h_prefix);
     HPC_HIPCHK(hipGetLastError());
     HPC_HIPCHK(hipEventRecord(e.lev_ev[0][5], e.stream));
@@ -984,7 +982,6 @@ int engine_run_single(Engine &e, int id) {
     if (h.max_level == 0) {
       rc = wait_published(e, pend_seq);
       if (rc) return rc;
-      if (e.h_ctr2->overflow) return voided(0);
       finalize(0, pend_max_deg, Clock::now());
       h.finished = true;
       return HPC_OK;
@@ -1002,7 +999,13 @@ int engine_run_single(Engine &e, int id) {
     if (l > kMaxLevel) break;  // the step loop takes levels the kernels do not reach
     const Clock::time_point t_lv = Clock::now();
     Layout L = layout(e);
-    HPC_HIPCHK(hipEventRecord(e.lev_ev[l][0], e.stream));
+    if (prev_commit_end) {
+      comp_start[l] = prev_commit_end;
+    } else {
+      HPC_HIPCHK(hipEventRecord(e.lev_ev[l][0], e.stream));
+      comp_start[l] = e.lev_ev[l][0];
+    }
+    prev_commit_end = nullptr;
     const Publish pa = next_publish(e);
     launch_compact_fused(e.stream, L, l, engine_rpt(e, l), e.d_blk, e.d_row_tasks, e.d_task_off,
                          e.cap_edges, pa);
@@ -1049,12 +1052,12 @@ int engine_run_single(Engine &e, int id) {
       else
         launch_search_ln(e.stream, l, L, engine_rpt(e, l), e.d_row_tasks, e.d_task_off, h.fast,
                          engine_tpb(e, l));
-      // exact re-runs of the tests near the threshold (fast correlations)
-      if (h.fast) launch_recheck(e.stream, l, L);
+      // exact re-runs of the tests near the threshold (fast correlations);
+      // the removal below clears the queue with the other counters
+      if (h.fast) launch_recheck(e.stream, l, L, /*clear_queue=*/false);
       HPC_HIPCHK(hipGetLastError());
     }
     HPC_HIPCHK(hipEventRecord(e.lev_ev[l][5], e.stream));
-    HPC_HIPCHK(hipEventRecord(e.lev_ev[l][2], e.stream));
     const bool last = h.max_level >= 0 && l + 1 > h.max_level;
     Publish pb = publish2();
     if (e.n_edges > 0) {
@@ -1073,6 +1076,7 @@ int engine_run_single(Engine &e, int id) {
       }
     }
     HPC_HIPCHK(hipEventRecord(e.lev_ev[l][3], e.stream));
+    prev_commit_end = e.lev_ev[l][3];
     pend = l;
     pend_seq = pb.seq;
     pend_max_deg = h.max_deg;
@@ -1156,7 +1160,6 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
   for (int l : lev) if (l != lvl_u) lvl_u = -1;
   const bool fused = lvl_u >= 1 && lvl_u <= kMaxLevel;
   HPC_HIPCHK(hipEventRecord(e.ev[0], e.stream));
-  if (any0 && any_fast_at(e, 0)) launch_refine_l0(e.stream, L);
   if (any0) {
     launch_level0(e.stream, L, e.h_inst, e.d_prefix + 2 * (K + 1), e.h_prefix);
     if (K > 1) e.host_upload_pending = true;  // the tile offsets
@@ -1260,9 +1263,16 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
     if (rc) return rc;
   }
   if (e.h_ctr->overflow) {
-    // a recheck queue overflowed: level 0 and the removal were skipped, so
-    // the step left no trace; exact correlations for the fast instances, then
-    // the same step again
+    // a recheck queue overflowed and the removal skipped itself.  Level 0 of
+    // the step's level-0 instances did run: their adjacency goes back to the
+    // complete graph level 0 starts from.  Then exact correlations for the
+    // fast instances and the same step again.
+    for (int k = 0; k < K; ++k)
+      if (lev[k] == 0) {
+        const InstHost &h = e.inst[act[k]];
+        launch_adj_init(e.stream, h.adj, h.p, h.W);
+      }
+    HPC_HIPCHK(hipGetLastError());
     rc = engine_exact_from_z(e, act);
     if (rc) return rc;
     return engine_step(e, ids);

@@ -7,6 +7,7 @@
 // matrix (state.cu:21-38).
 #pragma clang fp contract(off)
 
+#include "exact_dot.hpp"
 #include "hpc_internal.hpp"
 #include "pc_math.hpp"
 
@@ -45,6 +46,13 @@ __global__ void adj_init_kernel(unsigned long long *adj, int p, int W) {
 // pairs is row i's word J, and lane c collects bit r of row (64J + c)'s word
 // I.  The column words of the wavefronts are OR-ed in LDS; each adjacency
 // word has exactly one writer, so no atomics are needed.
+//
+// An instance with fast (MFMA) correlations first replaces every entry of the
+// row whose fast value lies within eps of the threshold by the exact one
+// (exact_cor, the whole wavefront per entry; both triangles are written), so
+// level 0 decides on exact values.  Level 0 reads only the lower triangle and
+// each of its entries belongs to one tile row, so the writes race with no
+// read.
 constexpr int kL0Waves = 8;
 constexpr int kL0Rows = 64 / kL0Waves;
 
@@ -59,8 +67,6 @@ __global__ __launch_bounds__(64 * kL0Waves) void level0_kernel(Layout L,
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const long long t = blockIdx.x;
   if (t >= (prefix ? prefix[L.K] : tiles)) return;
-  // an overflowed refinement queue leaves level 0 undecided: the host redoes it
-  if (L.ctr->overflow) return;
   const int k = prefix ? find_inst(prefix, L.K, t) : 0;
   const Inst In = L.inst[k];
   const int p = In.p, W = In.W;
@@ -86,8 +92,24 @@ __global__ __launch_bounds__(64 * kL0Waves) void level0_kernel(Layout L,
       const unsigned long long aw = In.adj[(size_t)i * W + J];
       const bool valid = (j < p) && (diag ? (lane < r) : true);
       present = valid && ((aw >> lane) & 1ull);
+      double c = valid ? In.cor[(size_t)i * p + j] : 0.0;
+      if (In.fast) {
+        const double re = __builtin_fmin(kCutThr, __builtin_fabs(c));
+        unsigned long long m =
+            __ballot(valid && !__builtin_isnan(c) && __builtin_fabs(re - In.r_crit) <= In.eps);
+        while (m) {
+          const int bit = __ffsll((long long)m) - 1;
+          m &= m - 1ull;
+          const int jt = J * 64 + bit;
+          const double ex = exact_cor(In, i, jt, lane);
+          if (lane == bit) c = ex;
+          if (lane == 0) {
+            In.cor[(size_t)i * p + jt] = ex;
+            In.cor[(size_t)jt * p + i] = ex;
+          }
+        }
+      }
       if (valid) {
-        const double c = In.cor[(size_t)i * p + j];
         nan_seen |= __builtin_isnan(c);
         if (present) keep = !removes(c, In.r_crit);
       }
@@ -122,11 +144,38 @@ __global__ __launch_bounds__(64 * kL0Waves) void level0_kernel(Layout L,
 #pragma unroll
     for (int w = 0; w < kL0Waves; ++w) { te += s_cnt[w][0]; re += s_cnt[w][1]; }
     if (te) {
-      atomicAdd(&L.ictr[k].tests, te);
-      atomicAdd(&L.ictr[k].tests_max, te);
-      atomicAdd(&L.ictr[k].edges, te);
-      atomicAdd(&L.ictr[k].removed, re);
+      unsigned long long *slot =
+          L.l0slot + ((size_t)k * kL0Slots + (blockIdx.x & (kL0Slots - 1))) * kL0SlotWords;
+      atomicAdd(&slot[0], te);
+      atomicAdd(&slot[1], re);
     }
+  }
+}
+
+// Folds the level-0 slots of instance blockIdx.x into its counters and
+// clears them.  Tests, test bound and searched edges all equal the pairs
+// tested at level 0.
+__global__ __launch_bounds__(64) void level0_fold_kernel(Layout L) {
+  const int k = blockIdx.x, lane = threadIdx.x;
+  unsigned long long te = 0, re = 0;
+  if (lane < kL0Slots) {
+    unsigned long long *slot = L.l0slot + ((size_t)k * kL0Slots + lane) * kL0SlotWords;
+    te = slot[0];
+    re = slot[1];
+    slot[0] = 0ull;
+    slot[1] = 0ull;
+  }
+#pragma unroll
+  for (int sh = 32; sh > 0; sh >>= 1) {
+    te += __shfl_xor(te, sh, 64);
+    re += __shfl_xor(re, sh, 64);
+  }
+  if (lane == 0 && te) {
+    InstCounters &c = L.ictr[k];
+    c.tests += te;
+    c.tests_max += te;
+    c.edges += te;
+    c.removed += re;
   }
 }
 
@@ -257,13 +306,16 @@ __device__ __forceinline__ unsigned long long ld_agent(const unsigned long long 
 }
 
 // blk[2b], blk[2b + 1]: edges and tasks of workgroup b; then, from the last
-// workgroup, their exclusive prefixes
+// workgroup, their exclusive prefixes.  blk[2 nblk + b]: the longest row of
+// workgroup b, reduced by the last workgroup (per-instance maxima of several
+// instances still take one atomic per workgroup and instance).
 __global__ __launch_bounds__(64 * kCompWaves) void compact_count_kernel(
     Layout L, int level, int rpt, int nblk, unsigned long long *blk,
     long long *__restrict__ row_tasks, Publish pub) {
   __shared__ int s_deg[kCompRows], s_k[kCompRows];
   __shared__ long long s_tasks[kCompRows];
   __shared__ int s_last;
+  __shared__ int s_mx[kCompWaves];
   __shared__ unsigned long long s_part[2 * 64 * kCompWaves];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int b = blockIdx.x;
@@ -309,15 +361,15 @@ __global__ __launch_bounds__(64 * kCompWaves) void compact_count_kernel(
       mx = max(mx, s_deg[r]);
       // per-instance maxima over runs of rows of one instance
       const int kr = s_k[r];
-      if (kr < 0) continue;
+      if (kr < 0 || L.K == 1) continue;
       if (kr != cur && cur >= 0) { atomicMax(&L.ictr[cur].max_deg, m); m = 0; }
       cur = kr;
       m = max(m, s_deg[r]);
     }
     if (cur >= 0) atomicMax(&L.ictr[cur].max_deg, m);
-    if (mx) atomicMax(&L.ctr->max_deg, mx);
     blk[2 * (size_t)b] = E;
     blk[2 * (size_t)b + 1] = T;
+    blk[2 * (size_t)nblk + b] = (unsigned long long)mx;
     __threadfence();
     s_last = atomicAdd(&L.ctr->done, 1u) == (unsigned)(nblk - 1);
   }
@@ -329,10 +381,15 @@ __global__ __launch_bounds__(64 * kCompWaves) void compact_count_kernel(
   const int per = (nblk + nt - 1) / nt;
   const int lo = tid * per, hi = min(nblk, lo + per);
   unsigned long long se = 0, st = 0;
+  int mloc = 0;
   for (int q = lo; q < hi; ++q) {
     se += ld_agent(&blk[2 * (size_t)q]);
     st += ld_agent(&blk[2 * (size_t)q + 1]);
+    mloc = max(mloc, (int)ld_agent(&blk[2 * (size_t)nblk + q]));
   }
+#pragma unroll
+  for (int sh = 32; sh > 0; sh >>= 1) mloc = max(mloc, __shfl_xor(mloc, sh, 64));
+  if (lane == 0) s_mx[wv] = mloc;
   s_part[tid] = se;
   s_part[nt + tid] = st;
   __syncthreads();
@@ -357,6 +414,11 @@ __global__ __launch_bounds__(64 * kCompWaves) void compact_count_kernel(
     L.rowptr[L.p_total] = (int32_t)s_part[tid];
     L.ctr->n_edges = s_part[tid];
     L.ctr->n_tasks = s_part[nt + tid];
+    int mx = 0;
+#pragma unroll
+    for (int w = 0; w < kCompWaves; ++w) mx = max(mx, s_mx[w]);
+    L.ctr->max_deg = max(L.ctr->max_deg, mx);
+    if (L.K == 1) L.ictr[0].max_deg = max(L.ictr[0].max_deg, mx);
   }
   __threadfence();
   __syncthreads();
@@ -481,12 +543,13 @@ void launch_level0(hipStream_t s, const Layout &L, const std::vector<Inst> &host
     // one instance: no offsets to upload
     hipLaunchKernelGGL(level0_kernel, dim3((unsigned)h_prefix[K]), dim3(64 * kL0Waves), 0, s, L,
                        (const long long *)nullptr, h_prefix[K]);
-    return;
+  } else {
+    (void)hipMemcpyAsync(d_prefix, h_prefix.data(), h_prefix.size() * sizeof(long long),
+                         hipMemcpyHostToDevice, s);
+    hipLaunchKernelGGL(level0_kernel, dim3((unsigned)h_prefix[K]), dim3(64 * kL0Waves), 0, s, L,
+                       (const long long *)d_prefix, h_prefix[K]);
   }
-  (void)hipMemcpyAsync(d_prefix, h_prefix.data(), h_prefix.size() * sizeof(long long),
-                       hipMemcpyHostToDevice, s);
-  hipLaunchKernelGGL(level0_kernel, dim3((unsigned)h_prefix[K]), dim3(64 * kL0Waves), 0, s, L,
-                     (const long long *)d_prefix, h_prefix[K]);
+  hipLaunchKernelGGL(level0_fold_kernel, dim3((unsigned)K), dim3(64), 0, s, L);
 }
 
 void launch_compact(hipStream_t s, const Layout &L) {

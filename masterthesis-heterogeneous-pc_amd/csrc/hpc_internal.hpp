@@ -39,7 +39,6 @@ struct Counters {
   unsigned long long pinv_fallbacks;  // tests that took the pinv form
   unsigned long long n_rchk;          // queued exact rechecks of the current search
   unsigned long long rechecks;        // exact rechecks run in this phase
-  unsigned long long n_pairs;         // level-0 pairs queued for an exact correlation
   int max_deg;                        // largest row length of the snapshot
   int overflow;                       // a recheck queue overflowed: the phase is void
   unsigned int done;                  // compaction workgroups finished (the last scans)
@@ -52,6 +51,12 @@ struct Recheck {
   unsigned long long edge;
   unsigned long long rank;
 };
+
+// Level-0 counts are first added into kL0Slots slots per instance, each in its
+// own 128-byte line (same-line atomics from thousands of workgroups
+// serialise), then folded into InstCounters by one small kernel.
+constexpr int kL0Slots = 32;
+constexpr int kL0SlotWords = 16;
 
 // Per-instance counters of one level.
 struct InstCounters {
@@ -111,8 +116,7 @@ struct Layout {
   Counters *ctr;
   Recheck *rchk;                // recheck queue of a search
   unsigned long long rchk_cap;
-  int2 *pairs;                  // level-0 pairs (global ids) for exact correlations
-  unsigned long long pairs_cap;
+  unsigned long long *l0slot;   // [K][kL0Slots][kL0SlotWords] level-0 partial counts, zero between uses
 };
 
 // ---- launchers (each defined in the .hip file that owns the kernel) ------
@@ -126,12 +130,9 @@ void launch_corr(hipStream_t s, const Layout &L, const std::vector<Inst> &host_i
 // left to launch_corr's masked kernel).
 void launch_corr_fast(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst,
                       long long *d_prefix, std::vector<long long> &h_prefix);
-// Exact (canonical-order) correlations of the level-0 pairs of fast instances
-// that lie within eps of the threshold, written into cor.
-void launch_refine_l0(hipStream_t s, const Layout &L);
 // Exact re-runs of the queued uncertain tests of `level`; then the queue is
-// emptied.
-void launch_recheck(hipStream_t s, int level, const Layout &L);
+// emptied, unless the caller's next kernel clears the counters anyway.
+void launch_recheck(hipStream_t s, int level, const Layout &L, bool clear_queue = true);
 void launch_adj_init(hipStream_t s, unsigned long long *adj, int p, int W);
 // Instance tables of up to kTableArg entries travel as a kernel argument (one
 // launch instead of a host-to-device copy); larger ones are copied.
@@ -263,6 +264,7 @@ struct Engine {
   long long *d_row_tasks = nullptr, *d_task_off = nullptr;
   // per-workgroup sums of the two-kernel compaction
   unsigned long long *d_blk = nullptr;
+  unsigned long long *d_l0slot = nullptr;  // [cap_inst][kL0Slots][kL0SlotWords]
   unsigned long long *d_binom = nullptr;
   long long *d_prefix = nullptr;
   // device counters and the packed instances' counters in one block (and
@@ -312,8 +314,7 @@ struct Engine {
   size_t scatter_cap = 0;
   // recheck queues of fast instances
   Recheck *d_rchk = nullptr;
-  int2 *d_pairs = nullptr;
-  size_t cap_rchk = 0, cap_pairs = 0;
+  size_t cap_rchk = 0;
   size_t rchk_limit = 0;  // queue length in use (0 = capacity); a tuning knob
   unsigned long long rechecks = 0;  // exact rechecks of the last step or level
 };

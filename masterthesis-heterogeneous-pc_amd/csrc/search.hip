@@ -334,10 +334,10 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_kernel(
         if (!anynan) {
           bool fb;
           double r;
-          if (fast) {
+          if constexpr (FAST) {
             double sens;
             r = pcor_canon_sens<LV>(a, x, yv, C, fb, sens);
-            unc = fb || !fast_certain(r, r_crit, sens, In.eps);
+            unc = fast && (fb || !fast_certain(r, r_crit, sens, In.eps));
           } else {
             r = pcor_canon<LV>(a, x, yv, C, fb);
           }
@@ -465,10 +465,14 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
               if (!anynan) {
                 bool fb, unc = false;
                 double r;
-                if (fast) {
+                // the fast instantiation runs one straight-line path for
+                // every instance (a branch on In.fast kept both paths'
+                // registers live and halved occupancy); exact instances of a
+                // mixed launch get the same statistic and ignore the bound
+                if constexpr (FAST) {
                   double sens;
                   r = pcor_canon_sens<LV>(a, x, yv, C, fb, sens);
-                  unc = fb || !fast_certain(r, r_crit, sens, In.eps);
+                  unc = fast && (fb || !fast_certain(r, r_crit, sens, In.eps));
                 } else {
                   r = pcor_canon<LV>(a, x, yv, C, fb);
                 }
