@@ -904,8 +904,9 @@ int engine_run_single(Engine &e, int id) {
   using Clock = std::chrono::steady_clock;
   const std::vector<int> ids{id};
   // Each event marker between two queued kernels delays the second by a few
-  // microseconds, so one marker serves both neighbours: the search's end
-  // starts the removal, and a removal's end starts the next compaction.
+  // microseconds, so one marker serves both neighbours: the search's start
+  // ends the compaction, the search's end starts the removal, and a
+  // removal's end starts the next compaction.
   hipEvent_t comp_start[kMaxLevel + 1] = {};
   hipEvent_t prev_commit_end = nullptr;
   const InstCounters *i2 =
@@ -919,6 +920,7 @@ int engine_run_single(Engine &e, int id) {
   int pend = -1;
   unsigned long long pend_seq = 0;
   Clock::time_point pend_t0 = Clock::now();
+  std::vector<std::pair<size_t, int>> timed;  // (history entry, level)
   auto finalize = [&](int lv, int max_deg, const Clock::time_point &t_end) {
     const InstCounters &c = i2[0];
     hpc_level_stats st;
@@ -934,19 +936,17 @@ int engine_run_single(Engine &e, int id) {
     st.pinv_fallbacks = (int64_t)e.h_ctr2->pinv_fallbacks;
     st.rechecks = (int64_t)e.h_ctr2->rechecks;
     e.rechecks = e.h_ctr2->rechecks;
-    hipEvent_t *ev = e.lev_ev[lv];
     if (lv == 0) {
-      st.ms_test = elapsed(ev[4], ev[5]);
       h.cor_nan = c.has_nan ? 1 : 0;
     } else {
-      st.ms_compact = elapsed(comp_start[lv], ev[1]);
-      st.ms_test = elapsed(ev[4], ev[5]);
-      st.ms_commit = elapsed(ev[5], ev[3]);
       h.rec_used += (size_t)c.rec_words;
       engine_note_level(e, id, lv);
     }
     if (st.removed) h.compacted = false;
     st.ms_wall = std::chrono::duration<double, std::milli>(t_end - pend_t0).count();
+    // kernel times once the run is over: a level's counts arrive from inside
+    // its removal kernel, before the events after it have completed
+    timed.push_back({h.hist.size(), lv});
     h.hist.push_back(st);
   };
   int pend_max_deg = 0;
@@ -959,140 +959,184 @@ int engine_run_single(Engine &e, int id) {
     h.finished = false;
     return HPC_OK;
   };
-  int l = h.next_level;
-  int rc = HPC_OK;
-  if (l == 0) {
-    pend_t0 = Clock::now();
-    rc = engine_pack(e, ids, {0}, /*zero_ctr=*/true);
-    if (rc) return rc;
-    Layout L = layout(e);
-    HPC_HIPCHK(hipEventRecord(e.lev_ev[0][4], e.stream));
-    launch_level0(e.stream, L, e.h_inst, e.d_prefix + 2 * (L.K + 1), e.h_prefix);
-    HPC_HIPCHK(hipGetLastError());
-    HPC_HIPCHK(hipEventRecord(e.lev_ev[0][5], e.stream));
-    const Publish pb = publish2();
-    launch_publish(e.stream, reinterpret_cast<const unsigned long long *>(e.d_ctr), pb.words,
-                   pb.dst, pb.flag, pb.seq);
-    HPC_HIPCHK(hipGetLastError());
-    pend = 0;
-    pend_seq = pb.seq;
-    pend_max_deg = h.p;
-    l = 1;
-    h.next_level = 1;
-    if (h.max_level == 0) {
-      rc = wait_published(e, pend_seq);
+  auto levels = [&]() -> int {
+    int l = h.next_level;
+    int rc = HPC_OK;
+    if (l == 0) {
+      pend_t0 = Clock::now();
+      rc = engine_pack(e, ids, {0}, /*zero_ctr=*/true);
       if (rc) return rc;
-      finalize(0, pend_max_deg, Clock::now());
-      h.finished = true;
-      return HPC_OK;
-    }
-    // level 1's table on clear counters (level 0's are in h_ctr2)
-    e.h_inst[0].level = 1;
-    rc = upload_table(e, /*zero=*/true);
-    if (rc) return rc;
-  } else {
-    // the device table starts at level l
-    rc = engine_pack(e, ids, {l}, /*zero_ctr=*/true);
-    if (rc) return rc;
-  }
-  for (;;) {
-    if (l > kMaxLevel) break;  // the step loop takes levels the kernels do not reach
-    const Clock::time_point t_lv = Clock::now();
-    Layout L = layout(e);
-    if (prev_commit_end) {
-      comp_start[l] = prev_commit_end;
-    } else {
-      HPC_HIPCHK(hipEventRecord(e.lev_ev[l][0], e.stream));
-      comp_start[l] = e.lev_ev[l][0];
-    }
-    prev_commit_end = nullptr;
-    const Publish pa = next_publish(e);
-    launch_compact_fused(e.stream, L, l, engine_rpt(e, l), e.d_blk, e.d_row_tasks, e.d_task_off,
-                         e.cap_edges, pa);
-    HPC_HIPCHK(hipGetLastError());
-    HPC_HIPCHK(hipEventRecord(e.lev_ev[l][1], e.stream));
-    rc = wait_published(e, pa.seq);
-    if (rc) return rc;
-    if (pend >= 0) {
-      if (e.h_ctr2->overflow) return voided(pend);
-      finalize(pend, pend_max_deg, t_lv);
-      pend = -1;
-    }
-    pend_t0 = t_lv;
-    e.n_edges = (int64_t)e.h_ctr->n_edges;
-    e.max_deg = e.h_ctr->max_deg;
-    h.max_deg = e.h_ictr[0].max_deg;
-    h.compacted = true;
-    h.ever_compacted = true;
-    const bool testable = h.max_deg - 1 >= l;
-    if (!testable && h.max_level < 0) {
-      h.finished = true;  // nothing left to test: no record (as engine_step)
-      return HPC_OK;
-    }
-    // record space, the NaN flag of level 1 on: the table when it changed
-    rc = ensure_rec(e, h, l);
-    if (rc) return rc;
-    Inst &I = e.h_inst[0];
-    const int flags = kActive | (h.cor_nan != 0 ? kCorNaN : 0);
-    const bool retable = I.rec != h.rec || I.flags != flags;
-    I.level = l;
-    I.rec = h.rec;
-    I.rec_base = h.rec_used;
-    I.flags = flags;
-    if (retable) {
-      rc = upload_table(e, /*zero=*/false);
-      if (rc) return rc;
-    }
-    L = layout(e);
-    L.max_deg = e.max_deg;
-    HPC_HIPCHK(hipEventRecord(e.lev_ev[l][4], e.stream));
-    if (e.n_edges > 0) {
-      if (l == 1)
-        launch_search_l1(e.stream, L, e.n_edges, nullptr, h.fast, engine_epb(e));
-      else
-        launch_search_ln(e.stream, l, L, engine_rpt(e, l), e.d_row_tasks, e.d_task_off, h.fast,
-                         engine_tpb(e, l));
-      // exact re-runs of the tests near the threshold (fast correlations);
-      // the removal below clears the queue with the other counters
-      if (h.fast) launch_recheck(e.stream, l, L, /*clear_queue=*/false);
+      Layout L = layout(e);
+      HPC_HIPCHK(hipEventRecord(e.lev_ev[0][4], e.stream));
+      launch_level0(e.stream, L, e.h_inst, e.d_prefix + 2 * (L.K + 1), e.h_prefix);
       HPC_HIPCHK(hipGetLastError());
-    }
-    HPC_HIPCHK(hipEventRecord(e.lev_ev[l][5], e.stream));
-    const bool last = h.max_level >= 0 && l + 1 > h.max_level;
-    Publish pb = publish2();
-    if (e.n_edges > 0) {
-      // publishes, clears the counters, advances level and record base
-      pb.advance = 1;
-      launch_commit(e.stream, L, e.n_edges, pb);
-      HPC_HIPCHK(hipGetLastError());
-    } else {
+      HPC_HIPCHK(hipEventRecord(e.lev_ev[0][5], e.stream));
+      prev_commit_end = e.lev_ev[0][5];  // level 1's compaction starts here
+      const Publish pb = publish2();
       launch_publish(e.stream, reinterpret_cast<const unsigned long long *>(e.d_ctr), pb.words,
                      pb.dst, pb.flag, pb.seq);
       HPC_HIPCHK(hipGetLastError());
-      if (!last) {
-        I.level = l + 1;
-        rc = upload_table(e, /*zero=*/true);
+      pend = 0;
+      pend_seq = pb.seq;
+      pend_max_deg = h.p;
+      l = 1;
+      h.next_level = 1;
+      if (h.max_level == 0) {
+        rc = wait_published(e, pend_seq);
+        if (rc) return rc;
+        finalize(0, pend_max_deg, Clock::now());
+        h.finished = true;
+        return HPC_OK;
+      }
+      // level 1's table on clear counters (level 0's are in h_ctr2)
+      e.h_inst[0].level = 1;
+      rc = upload_table(e, /*zero=*/true);
+      if (rc) return rc;
+    } else {
+      // the device table starts at level l
+      rc = engine_pack(e, ids, {l}, /*zero_ctr=*/true);
+      if (rc) return rc;
+    }
+    // the previous removal counted this level (its publication, already waited
+    // for, holds the level's sizes) and its fill kernel is queued
+    bool counted = false;
+    for (;;) {
+      if (l > kMaxLevel) break;  // the step loop takes levels the kernels do not reach
+      const Clock::time_point t_lv = Clock::now();
+      Layout L = layout(e);
+      if (counted) {
+        e.n_edges = (int64_t)e.h_ctr2->n_edges;
+        e.max_deg = e.h_ctr2->max_deg;
+        h.max_deg = i2[0].max_deg;
+        counted = false;
+      } else {
+        if (prev_commit_end) {
+          comp_start[l] = prev_commit_end;
+        } else {
+          HPC_HIPCHK(hipEventRecord(e.lev_ev[l][0], e.stream));
+          comp_start[l] = e.lev_ev[l][0];
+        }
+        prev_commit_end = nullptr;
+        const Publish pa = next_publish(e);
+        launch_compact_fused(e.stream, L, l, engine_rpt(e, l), e.d_blk, e.d_row_tasks,
+                             e.d_task_off, e.cap_edges, pa);
+        HPC_HIPCHK(hipGetLastError());
+        rc = wait_published(e, pa.seq);
+        if (rc) return rc;
+        if (pend >= 0) {
+          if (e.h_ctr2->overflow) return voided(pend);
+          finalize(pend, pend_max_deg, t_lv);
+          pend = -1;
+        }
+        e.n_edges = (int64_t)e.h_ctr->n_edges;
+        e.max_deg = e.h_ctr->max_deg;
+        h.max_deg = e.h_ictr[0].max_deg;
+      }
+      pend_t0 = t_lv;
+      h.compacted = true;
+      h.ever_compacted = true;
+      const bool testable = h.max_deg - 1 >= l;
+      if (!testable && h.max_level < 0) {
+        h.finished = true;  // nothing left to test: no record (as engine_step)
+        return HPC_OK;
+      }
+      // record space, the NaN flag of level 1 on: the table when it changed
+      rc = ensure_rec(e, h, l);
+      if (rc) return rc;
+      Inst &I = e.h_inst[0];
+      const int flags = kActive | (h.cor_nan != 0 ? kCorNaN : 0);
+      const bool retable = I.rec != h.rec || I.flags != flags;
+      I.level = l;
+      I.rec = h.rec;
+      I.rec_base = h.rec_used;
+      I.flags = flags;
+      if (retable) {
+        rc = upload_table(e, /*zero=*/false);
         if (rc) return rc;
       }
+      L = layout(e);
+      L.max_deg = e.max_deg;
+      HPC_HIPCHK(hipEventRecord(e.lev_ev[l][4], e.stream));
+      if (e.n_edges > 0) {
+        if (l == 1)
+          launch_search_l1(e.stream, L, e.n_edges, nullptr, h.fast, engine_epb(e));
+        else
+          launch_search_ln(e.stream, l, L, engine_rpt(e, l), e.d_row_tasks, e.d_task_off, h.fast,
+                           engine_tpb(e, l));
+        // exact re-runs of the tests near the threshold (fast correlations);
+        // the removal below clears the queue with the other counters
+        if (h.fast) launch_recheck(e.stream, l, L, /*clear_queue=*/false);
+        HPC_HIPCHK(hipGetLastError());
+      }
+      HPC_HIPCHK(hipEventRecord(e.lev_ev[l][5], e.stream));
+      const bool last = h.max_level >= 0 && l + 1 > h.max_level;
+      // the removal also counts the next level when there is one to run.  Not
+      // after level 1: it removes most edges, and their row-length updates
+      // (device atomics on a few cache lines) cost more than the separate
+      // count kernel
+      const bool count_next = e.n_edges > 0 && !last && l + 1 <= kMaxLevel && l >= 2;
+      Publish pb = publish2();
+      if (e.n_edges > 0) {
+        // publishes, clears the counters, advances level and record base
+        pb.advance = 1;
+        const NextCount nc{e.d_blk, e.d_row_tasks, l + 1, count_next ? engine_rpt(e, l + 1) : 0,
+                           count_next ? 1 : 0};
+        launch_commit(e.stream, L, e.n_edges, pb, nc);
+        HPC_HIPCHK(hipGetLastError());
+      } else {
+        launch_publish(e.stream, reinterpret_cast<const unsigned long long *>(e.d_ctr), pb.words,
+                       pb.dst, pb.flag, pb.seq);
+        HPC_HIPCHK(hipGetLastError());
+        if (!last) {
+          I.level = l + 1;
+          rc = upload_table(e, /*zero=*/true);
+          if (rc) return rc;
+        }
+      }
+      HPC_HIPCHK(hipEventRecord(e.lev_ev[l][3], e.stream));
+      prev_commit_end = e.lev_ev[l][3];
+      if (count_next) {
+        // the next level's neighbour lists while the host takes the counts
+        comp_start[l + 1] = e.lev_ev[l][3];
+        prev_commit_end = nullptr;
+        launch_compact_fill(e.stream, L, e.d_blk, e.d_row_tasks, e.d_task_off, e.cap_edges);
+        HPC_HIPCHK(hipGetLastError());
+        rc = wait_published(e, pb.seq);
+        if (rc) return rc;
+        if (e.h_ctr2->overflow) return voided(l);
+        finalize(l, h.max_deg, Clock::now());
+        counted = true;
+      } else {
+        pend = l;
+        pend_seq = pb.seq;
+        pend_max_deg = h.max_deg;
+      }
+      ++l;
+      h.next_level = l;
+      e.h_inst[0].level = l;  // the device table's level after the removal
+      if (last) break;
     }
-    HPC_HIPCHK(hipEventRecord(e.lev_ev[l][3], e.stream));
-    prev_commit_end = e.lev_ev[l][3];
-    pend = l;
-    pend_seq = pb.seq;
-    pend_max_deg = h.max_deg;
-    ++l;
-    h.next_level = l;
-    e.h_inst[0].level = l;  // the device table's level after the removal
-    if (last) break;
+    if (pend >= 0) {
+      rc = wait_published(e, pend_seq);
+      if (rc) return rc;
+      if (e.h_ctr2->overflow) return voided(pend);
+      finalize(pend, pend_max_deg, Clock::now());
+    }
+    if (h.max_level >= 0 && h.next_level > h.max_level) h.finished = true;
+    return HPC_OK;
+  };
+  const int rc_run = levels();
+  for (const auto &t : timed) {
+    hpc_level_stats &st = h.hist[t.first];
+    hipEvent_t *ev = e.lev_ev[t.second];
+    st.ms_test = elapsed(ev[4], ev[5]);
+    if (t.second > 0) {
+      // the compaction runs up to the search's start event
+      st.ms_compact = elapsed(comp_start[t.second], ev[4]);
+      st.ms_commit = elapsed(ev[5], ev[3]);
+    }
   }
-  if (pend >= 0) {
-    rc = wait_published(e, pend_seq);
-    if (rc) return rc;
-    if (e.h_ctr2->overflow) return voided(pend);
-    finalize(pend, pend_max_deg, Clock::now());
-  }
-  if (h.max_level >= 0 && h.next_level > h.max_level) h.finished = true;
-  return HPC_OK;
+  return rc_run;
 }
 
 int engine_step(Engine &e, const std::vector<int> &ids) {

@@ -299,7 +299,7 @@ __global__ void table_kernel(Inst *__restrict__ dst, TableArg arg) {
 // another workgroup.
 constexpr int kCompWaves = 4;
 constexpr int kCompRowsPerWave = 4;
-constexpr int kCompRows = kCompWaves * kCompRowsPerWave;
+static_assert(kCompRows == kCompWaves * kCompRowsPerWave, "compaction rows per workgroup");
 
 __device__ __forceinline__ unsigned long long ld_agent(const unsigned long long *v) {
   return __hip_atomic_load(v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -320,23 +320,48 @@ __global__ __launch_bounds__(64 * kCompWaves) void compact_count_kernel(
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int b = blockIdx.x;
   const int g0 = b * kCompRows;
+  // the wave's rows side by side, so their word loads are in flight together
+  // (a one-instance launch needs no row -> instance lookup)
+  int kq[kCompRowsPerWave], Wq[kCompRowsPerWave], cq[kCompRowsPerWave];
+  bool onq[kCompRowsPerWave];
+  const unsigned long long *rowq[kCompRowsPerWave];
+  int Wmax = 0;
+#pragma unroll
+  for (int rr = 0; rr < kCompRowsPerWave; ++rr) {
+    const int g = g0 + wv * kCompRowsPerWave + rr;
+    kq[rr] = -1;
+    onq[rr] = false;
+    Wq[rr] = 0;
+    cq[rr] = 0;
+    rowq[rr] = nullptr;
+    if (g < L.p_total) {
+      kq[rr] = L.K == 1 ? 0 : L.row_inst[g];
+      const Inst &In = L.inst[kq[rr]];
+      onq[rr] = (In.flags & kActive) && In.level >= 1;
+      if (onq[rr]) {
+        rowq[rr] = In.adj + (size_t)(g - In.start) * In.W;
+        Wq[rr] = In.W;
+        Wmax = max(Wmax, In.W);
+      }
+    }
+  }
+  for (int w0 = 0; w0 < Wmax; w0 += 64) {
+    const int w = w0 + lane;
+#pragma unroll
+    for (int rr = 0; rr < kCompRowsPerWave; ++rr)
+      if (w < Wq[rr]) cq[rr] += __popcll(rowq[rr][w]);
+  }
 #pragma unroll
   for (int rr = 0; rr < kCompRowsPerWave; ++rr) {
     const int r = wv * kCompRowsPerWave + rr;
     const int g = g0 + r;
-    int c = 0, k = -1;
+    int c = cq[rr];
+    const int k = kq[rr];
     long long tk = 0;
     if (g < L.p_total) {
-      k = L.row_inst[g];
-      const Inst &In = L.inst[k];
-      const bool on = (In.flags & kActive) && In.level >= 1;
-      if (on) {
-        const unsigned long long *row = In.adj + (size_t)(g - In.start) * In.W;
-        for (int w = lane; w < In.W; w += 64) c += __popcll(row[w]);
-      }
 #pragma unroll
       for (int sh = 32; sh > 0; sh >>= 1) c += __shfl_xor(c, sh, 64);
-      if (on && level >= 2 && In.level == level && c > 0) {
+      if (onq[rr] && level >= 2 && L.inst[k].level == level && c > 0) {
         const unsigned long long total = binom(L.binom, kBinomK, c - 1, level);
         if (total > 0) tk = (long long)c * (long long)((total + rpt - 1) / rpt);
       }
@@ -518,6 +543,13 @@ void launch_compact_fused(hipStream_t s, const Layout &L, int level, int rpt,
   hipLaunchKernelGGL(compact_fill_kernel, dim3(nblk), dim3(64 * kCompWaves), 0, s, L,
                      (const unsigned long long *)blk, (const long long *)row_tasks, task_off,
                      (unsigned long long)edge_cap);
+}
+
+void launch_compact_fill(hipStream_t s, const Layout &L, const unsigned long long *blk,
+                         const long long *row_tasks, long long *task_off, size_t edge_cap) {
+  const int nblk = compact_fused_blocks(L.p_total);
+  hipLaunchKernelGGL(compact_fill_kernel, dim3(nblk), dim3(64 * kCompWaves), 0, s, L, blk,
+                     row_tasks, task_off, (unsigned long long)edge_cap);
 }
 
 void launch_table(hipStream_t s, Inst *dst, const TableArg &arg) {

@@ -169,11 +169,31 @@ struct Publish {
 // publish the counters (the first kernel's last workgroup, once the totals
 // are known): row lengths and per-workgroup sums, then row pointers, task
 // offsets and the neighbour lists with their first passing ranks reset.
-// blk: 2 words per workgroup (compact_fused_blocks).
+// blk: 3 words per workgroup (compact_fused_blocks).
 void launch_compact_fused(hipStream_t s, const Layout &L, int level, int rpt,
                           unsigned long long *blk, long long *row_tasks, long long *task_off,
                           size_t edge_cap, const Publish &pub);
+// Rows per compaction workgroup.
+constexpr int kCompRows = 16;
 int compact_fused_blocks(int p_total);
+// The compaction's second kernel alone, for counts that the removal kernel
+// produced (NextCount).
+void launch_compact_fill(hipStream_t s, const Layout &L, const unsigned long long *blk,
+                         const long long *row_tasks, long long *task_off, size_t edge_cap);
+// Removal kernel of a one-instance run that also counts the next level: each
+// removal lowers both row lengths (L.deg), and the last workgroup turns the
+// row lengths into what the compaction's first kernel computes (task plan of
+// `level` with `rpt`, per-workgroup sums and their prefixes in blk, totals
+// and longest row in the counters) before it publishes, so its publication
+// carries the next level's sizes and launch_compact_fill completes the
+// compaction.
+struct NextCount {
+  unsigned long long *blk;
+  long long *row_tasks;
+  int level;
+  int rpt;
+  int on;
+};
 // `fast`: the launch holds an instance with fast correlations (selects the
 // variant with uncertainty bounds and the recheck queue)
 // epb: ordered edges per workgroup of the LDS-staged form
@@ -187,7 +207,8 @@ void launch_task_plan(hipStream_t s, int level, const Layout &L, int rpt,
 void launch_search_ln(hipStream_t s, int level, const Layout &L, int rpt,
                       const long long *row_tasks, const long long *task_off, bool fast, int tpb);
 // With pub.flag set, the last workgroup to finish publishes the counters.
-void launch_commit(hipStream_t s, const Layout &L, int64_t n_edges, const Publish &pub);
+void launch_commit(hipStream_t s, const Layout &L, int64_t n_edges, const Publish &pub,
+                   const NextCount &next = NextCount{nullptr, nullptr, 0, 0, 0});
 void launch_scatter_fp(hipStream_t s, const int64_t *idx, const int64_t *val,
                        long long count, int64_t *fp);
 

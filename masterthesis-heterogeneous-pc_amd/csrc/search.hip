@@ -560,14 +560,117 @@ __device__ __forceinline__ int find_pos(const int32_t *row, int len, int node) {
   return lo;
 }
 
+// The next level's counts from the row lengths the removals left (one
+// workgroup, the removal kernel's last): per row the task count of nc.level,
+// per compaction workgroup the sums of row lengths and tasks, replaced by
+// their exclusive prefixes, and the totals and longest row in the counters.
+// The same values compact_count_kernel derives from the adjacency.  Returns
+// (on every thread) the totals and the longest row.
+__device__ void next_count(const Layout &L, const NextCount &nc, const unsigned long long *tab,
+                           unsigned long long &te, unsigned long long &tt, int &tmax) {
+  __shared__ unsigned long long s_we[kCommitWaves], s_wt[kCommitWaves];
+  __shared__ int s_wm[kCommitWaves];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nt = blockDim.x;
+  const int nblk = (L.p_total + kCompRows - 1) / kCompRows;
+  const int per = (nblk + nt - 1) / nt;
+  const int lo = min(nblk, tid * per), hi = min(nblk, lo + per);
+  unsigned long long se = 0, st = 0;
+  int mx = 0;
+  for (int b = lo; b < hi; ++b) {
+    int c[kCompRows];
+#pragma unroll
+    for (int r = 0; r < kCompRows; ++r) {
+      const int g = b * kCompRows + r;
+      c[r] = g < L.p_total ? __hip_atomic_load(&L.deg[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                           : 0;
+    }
+    unsigned long long eb = 0, tb = 0;
+#pragma unroll
+    for (int r = 0; r < kCompRows; ++r) {
+      const int g = b * kCompRows + r;
+      long long tk = 0;
+      if (nc.level >= 2 && c[r] > 0) {
+        const unsigned long long total = binom(tab, kBinomK, c[r] - 1, nc.level);
+        if (total > 0) tk = (long long)c[r] * (long long)((total + nc.rpt - 1) / nc.rpt);
+      }
+      if (g < L.p_total) nc.row_tasks[g] = tk;
+      eb += (unsigned long long)c[r];
+      tb += (unsigned long long)tk;
+      mx = max(mx, c[r]);
+    }
+    nc.blk[2 * (size_t)b] = eb;
+    nc.blk[2 * (size_t)b + 1] = tb;
+    se += eb;
+    st += tb;
+  }
+  // exclusive prefixes over the threads' block ranges (thread order = block
+  // order): a scan within each wave, then the waves' totals
+  unsigned long long ie = se, it = st;
+#pragma unroll
+  for (int s = 1; s < 64; s <<= 1) {
+    const unsigned long long ve = __shfl_up(ie, s, 64), vt = __shfl_up(it, s, 64);
+    if (lane >= s) {
+      ie += ve;
+      it += vt;
+    }
+  }
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) mx = max(mx, __shfl_xor(mx, s, 64));
+  if (lane == 63) {
+    s_we[wv] = ie;
+    s_wt[wv] = it;
+  }
+  if (lane == 0) s_wm[wv] = mx;
+  __syncthreads();
+  unsigned long long oe = 0, ot = 0, ae = 0, at = 0;
+  int M = 0;
+  for (int w = 0; w < (nt >> 6); ++w) {
+    if (w < wv) {
+      oe += s_we[w];
+      ot += s_wt[w];
+    }
+    ae += s_we[w];
+    at += s_wt[w];
+    M = max(M, s_wm[w]);
+  }
+  unsigned long long re = oe + ie - se, rt = ot + it - st;
+  for (int b = lo; b < hi; ++b) {
+    const unsigned long long e = nc.blk[2 * (size_t)b], t = nc.blk[2 * (size_t)b + 1];
+    nc.blk[2 * (size_t)b] = re;
+    nc.blk[2 * (size_t)b + 1] = rt;
+    re += e;
+    rt += t;
+  }
+  if (tid == 0) {
+    L.rowptr[L.p_total] = (int32_t)ae;
+    L.ctr->n_edges = ae;
+    L.ctr->n_tasks = at;
+    L.ctr->max_deg = M;
+    L.ictr[0].max_deg = M;
+  }
+  te = ae;
+  tt = at;
+  tmax = M;
+  __threadfence();
+  __syncthreads();
+}
+
 // Removal and sepset recording (cpuUtil.cpp:23-48).  Each removed edge at
 // level l appends a record of 3 + l words to its instance's record buffer:
 // i, j (local ids, i < j), l, then the members in ascending order.  A
 // wavefront reserves the space of all its records with one atomic when it
 // holds one instance; the record order is scheduling-dependent, the records
 // are not.
+//
+// The rows 0..tab_rows-1 of the binomial table (every row a test of the
+// level can reach: tab_rows = longest row + 1) are staged in LDS, so the
+// unranking of a record's set does its dependent lookups there; tab_rows = 0
+// reads the global table.
 __global__ __launch_bounds__(64 * kCommitWaves) void commit_kernel(Layout L, long long n_edges,
-                                                                 Publish pub) {
+                                                                 Publish pub, int tab_rows,
+                                                                 NextCount nc) {
+  extern __shared__ unsigned long long s_tab[];
+  for (int t = threadIdx.x; t < tab_rows * kBinomK; t += blockDim.x) s_tab[t] = L.binom[t];
   // an overflowed recheck queue leaves the level undecided: the host redoes
   // it (the workgroups still take part in the publication below)
   const bool live = !L.ctr->overflow;
@@ -575,25 +678,21 @@ __global__ __launch_bounds__(64 * kCommitWaves) void commit_kernel(Layout L, lon
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   int k = -1, level = 0;
   unsigned long long T = 0, U = 0, E = 0, R = 0;
-  bool rm = false;
-  int i = 0, j = 0, y = 0, base = 0, nn = 0;
+  bool rm = false, passed = false;
+  int i = 0, j = 0, y = 0, base = 0, nn = 0, nni = 0;
   unsigned long long rank = 0;
   if (live && e < n_edges) {
     i = L.erow[e];
-    k = L.row_inst[i];
+    k = L.K == 1 ? 0 : L.row_inst[i];
     level = L.inst[k].level;
     // levels the binomial table covers (the host never packs a higher one)
     if (level >= 1 && level <= kMaxLevel) {
       j = L.nbr[e];
       const int bi = L.rowptr[i];
-      const int nni = L.rowptr[i + 1] - bi - 1;
-      const unsigned long long total = binom(L.binom, kBinomK, nni, level);
+      nni = L.rowptr[i + 1] - bi - 1;
       const long long fi = L.fp[e];
-      if (total > 0) {
-        T = fi >= 0 ? (unsigned long long)fi + 1 : total;
-        U = total;
-        E = 1;
-      }
+      E = 1;  // until the total (read after the staging barrier) says otherwise
+      if (fi >= 0) T = (unsigned long long)fi + 1;
       // the lower-id row's lowest passing set wins, else the higher-id
       // row's: row i records when it passed and either it is the lower id or
       // row j did not pass (the only case that needs the reverse edge)
@@ -612,8 +711,7 @@ __global__ __launch_bounds__(64 * kCommitWaves) void commit_kernel(Layout L, lon
           base = bi;
           rank = (unsigned long long)fi;
           nn = nni;
-          // a rank valid for this level and row (so the unranking below ends)
-          rm = rank < total;
+          passed = true;
         }
       }
     }
@@ -631,7 +729,20 @@ __global__ __launch_bounds__(64 * kCommitWaves) void commit_kernel(Layout L, lon
       sR[kCommitWaves], sW[kCommitWaves];
   __shared__ unsigned long long s_base;
   if (lane == 0) sk[wv] = uniform ? kk : -2;
-  __syncthreads();
+  __syncthreads();  // also: the staged binomial rows are in place
+  const unsigned long long *tab = tab_rows > 0 ? s_tab : L.binom;
+  if (E) {
+    const unsigned long long total = binom(tab, kBinomK, nni, level);
+    if (total > 0) {
+      if (!T) T = total;
+      U = total;
+    } else {
+      T = 0;
+      E = 0;
+    }
+    // a rank valid for this level and row (so the unranking below ends)
+    rm = passed && rank < total;
+  }
   int K0 = -1;
   bool bu = true;
 #pragma unroll
@@ -679,6 +790,10 @@ __global__ __launch_bounds__(64 * kCommitWaves) void commit_kernel(Layout L, lon
     const int li = i - s0, lj = j - s0;
     atomicAnd(&In.adj[(size_t)li * W + (lj >> 6)], ~(1ull << (lj & 63)));
     atomicAnd(&In.adj[(size_t)lj * W + (li >> 6)], ~(1ull << (li & 63)));
+    if (nc.on) {
+      atomicSub(&L.deg[i], 1);
+      atomicSub(&L.deg[j], 1);
+    }
     int32_t *out = In.rec + In.rec_base + off;
     out[0] = min(li, lj);
     out[1] = max(li, lj);
@@ -686,7 +801,6 @@ __global__ __launch_bounds__(64 * kCommitWaves) void commit_kernel(Layout L, lon
     // the members of the rank-th level-subset, as unrank() finds them (a
     // binary search per member over the binomials; the last member is the
     // remaining rank past the previous one)
-    const unsigned long long *tab = L.binom;
     int v0 = 0;
     for (int m = 0; m < level; ++m) {
       const int km = level - m;
@@ -764,6 +878,9 @@ __global__ __launch_bounds__(64 * kCommitWaves) void commit_kernel(Layout L, lon
   __syncthreads();
   if (s_last) {
     __threadfence();
+    unsigned long long nte = 0, ntt = 0;
+    int nmax = 0;
+    if (nc.on) next_count(L, nc, tab, nte, ntt, nmax);
     const unsigned long long *src = reinterpret_cast<const unsigned long long *>(L.ctr);
     unsigned long long words_rec = 0;
     if (pub.advance && L.K == 1 && threadIdx.x == 0)
@@ -778,10 +895,18 @@ __global__ __launch_bounds__(64 * kCommitWaves) void commit_kernel(Layout L, lon
       // the next level of the (single) instance starts from clear counters
       unsigned long long *blk = reinterpret_cast<unsigned long long *>(L.ctr);
       for (int w = threadIdx.x; w < pub.words; w += blockDim.x) blk[w] = 0ull;
+      __syncthreads();
       if (threadIdx.x == 0) {
         Inst *In = const_cast<Inst *>(L.inst);
         In[0].level += 1;
         In[0].rec_base += words_rec;
+        if (nc.on) {
+          // the next level's sizes stay for its search
+          L.ctr->n_edges = nte;
+          L.ctr->n_tasks = ntt;
+          L.ctr->max_deg = nmax;
+          L.ictr[0].max_deg = nmax;
+        }
       }
     }
   }
@@ -887,11 +1012,16 @@ void launch_search_ln(hipStream_t s, int level, const Layout &L, int rpt,
   }
 }
 
-void launch_commit(hipStream_t s, const Layout &L, int64_t n_edges, const Publish &pub) {
+void launch_commit(hipStream_t s, const Layout &L, int64_t n_edges, const Publish &pub,
+                   const NextCount &next) {
   if (n_edges <= 0) return;
   const int tpb = 64 * kCommitWaves;
-  hipLaunchKernelGGL(commit_kernel, dim3((unsigned)((n_edges + tpb - 1) / tpb)), dim3(tpb), 0, s, L,
-                     (long long)n_edges, pub);
+  // binomial rows 0..max_deg in LDS when they fit in 48 KB
+  int tab_rows = L.max_deg + 1;
+  if ((size_t)tab_rows * kBinomK * sizeof(unsigned long long) > 48 * 1024) tab_rows = 0;
+  hipLaunchKernelGGL(commit_kernel, dim3((unsigned)((n_edges + tpb - 1) / tpb)), dim3(tpb),
+                     (size_t)tab_rows * kBinomK * sizeof(unsigned long long), s, L,
+                     (long long)n_edges, pub, tab_rows, next);
 }
 
 void launch_scatter_fp(hipStream_t s, const int64_t *idx, const int64_t *val, long long count,
