@@ -153,19 +153,31 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_kernel(
   }
 }
 
-// Level 1 with row i's correlations staged in LDS: workgroup b takes the
-// ordered edges [b * epb, (b + 1) * epb) (CSR order, so grouped by row),
-// copies ids and cor[i][N(i)] of each row it meets into LDS and gathers only
+// Level 1 with row i's correlations staged in LDS: copies ids and
+// cor[i][N(i)] of each row a workgroup meets into LDS and gathers only
 // cor[j][k] per test.  Same tests, order and arithmetic as search_l1_kernel.
+//
+// Every test of edge (i, j) gathers from row j, so the edges are split by the
+// class of j (local id mod 8) over the eight XCDs: workgroup b (on XCD b mod
+// 8 under round-robin dispatch) takes the edges of class b mod 8 among the
+// ordered edges [c * 8 epb, (c + 1) * 8 epb), c = b / 8 (CSR order, grouped
+// by row).  An XCD then gathers from an eighth of the rows, which its L2 can
+// hold, instead of from all of them.
+constexpr int kL1Classes = 8;
+
 template <bool FAST>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_lds_kernel(
     Layout L, long long n_edges, const int32_t *__restrict__ row_mask, int epb) {
   extern __shared__ unsigned long long smem[];
   double *Mi = reinterpret_cast<double *>(smem);                 // max_deg
   int *ids = reinterpret_cast<int *>(Mi + L.max_deg);             // max_deg
+  int *sel = ids + L.max_deg;                                     // max_deg: the class's edges
+  __shared__ int s_nsel;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  long long e0 = (long long)blockIdx.x * epb;
-  const long long e1 = min(n_edges, e0 + epb);
+  const int cls = blockIdx.x % kL1Classes;
+  const long long span = (long long)kL1Classes * epb;
+  long long e0 = (long long)(blockIdx.x / kL1Classes) * span;
+  const long long e1 = min(n_edges, e0 + span);
   while (e0 < e1) {
     const int i = L.erow[e0];
     const int base = L.rowptr[i];
@@ -182,6 +194,22 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_lds_kernel(
         Mi[a] = In.cor[(size_t)li * p + lk];
       }
       __syncthreads();
+      // positions y (edges base + y) of the segment whose neighbour is in
+      // this workgroup's class, in order
+      if (wave == 0) {
+        const int y_lo = (int)(e0 - base), y_hi = (int)(seg_end - base);
+        int cnt = 0;
+        for (int y0 = y_lo; y0 < y_hi; y0 += 64) {
+          const int y = y0 + lane;
+          const bool mine = y < y_hi && (ids[y] % kL1Classes) == cls;
+          const unsigned long long m = __ballot(mine);
+          if (mine) sel[cnt + __popcll(m & ((1ull << lane) - 1ull))] = y;
+          cnt += __popcll(m);
+        }
+        if (lane == 0) s_nsel = cnt;
+      }
+      __syncthreads();
+      const int nsel = s_nsel;
       const double r_crit = In.r_crit;
       const int nn = deg - 1;
       const bool fast = FAST && In.fast;
@@ -204,11 +232,11 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_lds_kernel(
         }
         return pass;
       };
-      for (long long eb = e0 + 4 * wave; eb < seg_end; eb += 4 * kWavesPerBlock) {
+      for (int kb = 4 * wave; kb < nsel; kb += 4 * kWavesPerBlock) {
         const int grp = lane >> 4, t = lane & 15;
-        const long long eg = eb + grp;
-        const bool live = eg < seg_end;
-        const int yg = live ? (int)(eg - base) : 0;
+        const bool live = kb + grp < nsel;
+        const int yg = live ? sel[kb + grp] : 0;
+        const long long eg = (long long)base + yg;
         const double *cjg = In.cor + (size_t)ids[yg] * p;
         bool pass = false, unc = false;
         if (live && t < nn) pass = test(yg, Mi[yg], cjg, t, unc);
@@ -223,8 +251,8 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_lds_kernel(
 #pragma unroll 1
         for (int g = 0; g < 4; ++g) {
           if (!((open >> (16 * g)) & 1ull)) continue;
-          const long long e = eb + g;
-          const int y = (int)(e - base);
+          const int y = sel[kb + g];
+          const long long e = (long long)base + y;
           const double x1 = Mi[y];
           const double *cj = In.cor + (size_t)ids[y] * p;
           long long f = -1;
@@ -961,9 +989,10 @@ void launch_ln(hipStream_t s, const Layout &L, int rpt, const long long *row_tas
 void launch_search_l1(hipStream_t s, const Layout &L, int64_t n_edges, const int32_t *row_mask,
                       bool fast, int epb) {
   if (n_edges <= 0) return;
-  const size_t staged = (size_t)L.max_deg * 12;
+  const size_t staged = (size_t)L.max_deg * 16;
   if (staged <= kLdsLimit) {
-    const long long blocks = (n_edges + epb - 1) / epb;
+    const long long span = (long long)kL1Classes * epb;
+    const long long blocks = kL1Classes * ((n_edges + span - 1) / span);
     if (fast)
       hipLaunchKernelGGL(search_l1_lds_kernel<true>, dim3((unsigned)blocks),
                          dim3(64 * kWavesPerBlock), staged, s, L, (long long)n_edges, row_mask, epb);
