@@ -345,6 +345,10 @@ def main() -> int:
     dev.corr()
     tuned = dev.autotune(max_level if max_level >= 0 else 16)
 
+    # The timed steps run the level loop as a caller does, without the
+    # per-level kernel-time events (each event marker between two kernels
+    # delays the second by a few microseconds).
+    dev.set_kernel_timing(False)
     for _ in range(args.warmup):
         run_step(dev)
     heterogpc.load().hpc_device_synchronize(0)
@@ -353,15 +357,24 @@ def main() -> int:
     heterogpc.load().hpc_device_synchronize(0)
     wall = time.perf_counter() - t0
 
-    last = steps[-1]["levels"]
-    tests_total = sum(lv["tests"] for lv in last)
+    tests_total = sum(lv["tests"] for lv in steps[-1]["levels"])
     levels_ms = sum(s["levels_ms"] for s in steps) / len(steps)
     e2e_ms = wall * 1e3 / len(steps)
     value = tests_total / (levels_ms * 1e-3)
 
-    # per-kernel device time, averaged over steps
+    # Kernel times (roofline, per-level figures): the same number of steps
+    # again with the events on, after one untimed step.
+    dev.set_kernel_timing(True)
+    run_step(dev)
+    ksteps = [run_step(dev) for _ in range(args.steps)]
+    last = ksteps[-1]["levels"]
+    if sum(lv["tests"] for lv in last) != tests_total:
+        raise RuntimeError("the kernel-timing pass counted different tests")
+    levels_ms_events = sum(s["levels_ms"] for s in ksteps) / len(ksteps)
+
+    # per-kernel device time, averaged over the kernel-timing steps
     def avg(fn):
-        return sum(fn(s) for s in steps) / len(steps)
+        return sum(fn(s) for s in ksteps) / len(ksteps)
 
     k_corr = avg(lambda s: s["ms_corr"])
     k_mom = avg(lambda s: s["ms_moments"])
@@ -420,6 +433,11 @@ def main() -> int:
                    "p": p, "n": n, "alpha": alpha, "max_level": max_level},
         "skeleton_ms_levels": levels_ms,
         "skeleton_ms_e2e": e2e_ms,
+        "kernel_timing_pass": {
+            "note": "value, ms_per_step and skeleton_ms_* come from the timed steps, run "
+                    "without per-level HIP events; kernel_ms, levels[].ms_* and roofline from "
+                    "a second pass of the same steps with the events on",
+            "skeleton_ms_levels_with_events": levels_ms_events},
         "tests_total": tests_total,
         "levels": [{k: lv[k] for k in ("level", "max_deg", "tests", "tests_max", "edges",
                                        "removed", "edges_left", "ms_compact", "ms_test",

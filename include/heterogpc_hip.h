@@ -112,9 +112,15 @@ int hpc_max_cond_size(const hpc_state *st);
 enum {
   HPC_TUNE_RANKS_PER_TASK = 1, /* conditioning sets per search task at levels
                                   >= 2: a multiple of 64, default 512 */
-  HPC_TUNE_RECHECK_QUEUE = 2   /* HPC_CORR_F32: length of the exact-recheck
+  HPC_TUNE_RECHECK_QUEUE = 2,  /* HPC_CORR_F32: length of the exact-recheck
                                   queues (0 = 1 Mi); a level whose queue
                                   overflows is redone on exact correlations */
+  HPC_TUNE_KERNEL_TIMING = 3   /* 1 (default): hpc_run records device-time
+                                  events around each level's kernels (the
+                                  ms_compact / ms_test / ms_commit fields);
+                                  0: no events (each marker between two
+                                  kernels delays the second by a few
+                                  microseconds) and those fields stay 0 */
 };
 int hpc_set_tuning(hpc_state *st, int param, int value);
 

@@ -179,6 +179,10 @@ extern "C" int hpc_set_tuning(hpc_state *st, int param, int value) {
       if (value < 0) return fail(HPC_EINVAL, "recheck queue length must be >= 0");
       st->eng.rchk_limit = (size_t)value;
       return HPC_OK;
+    case HPC_TUNE_KERNEL_TIMING:
+      if (value != 0 && value != 1) return fail(HPC_EINVAL, "kernel timing must be 0 or 1");
+      st->eng.kernel_timing = value != 0;
+      return HPC_OK;
     default:
       return fail(HPC_EINVAL, "unknown tuning parameter %d", param);
   }

@@ -908,6 +908,10 @@ int engine_run_single(Engine &e, int id) {
   // ends the compaction, the search's end starts the removal, and a
   // removal's end starts the next compaction.
   hipEvent_t comp_start[kMaxLevel + 1] = {};
+  // the level-loop events, unless kernel timing is off
+  auto mark = [&](hipEvent_t ev) -> hipError_t {
+    return e.kernel_timing ? hipEventRecord(ev, e.stream) : hipSuccess;
+  };
   hipEvent_t prev_commit_end = nullptr;
   const InstCounters *i2 =
       reinterpret_cast<const InstCounters *>(reinterpret_cast<const char *>(e.h_ctr2) + kIctrOffset);
@@ -967,10 +971,10 @@ int engine_run_single(Engine &e, int id) {
       rc = engine_pack(e, ids, {0}, /*zero_ctr=*/true);
       if (rc) return rc;
       Layout L = layout(e);
-      HPC_HIPCHK(hipEventRecord(e.lev_ev[0][4], e.stream));
+      HPC_HIPCHK(mark(e.lev_ev[0][4]));
       launch_level0(e.stream, L, e.h_inst, e.d_prefix + 2 * (L.K + 1), e.h_prefix);
       HPC_HIPCHK(hipGetLastError());
-      HPC_HIPCHK(hipEventRecord(e.lev_ev[0][5], e.stream));
+      HPC_HIPCHK(mark(e.lev_ev[0][5]));
       prev_commit_end = e.lev_ev[0][5];  // level 1's compaction starts here
       const Publish pb = publish2();
       launch_publish(e.stream, reinterpret_cast<const unsigned long long *>(e.d_ctr), pb.words,
@@ -1013,7 +1017,7 @@ int engine_run_single(Engine &e, int id) {
         if (prev_commit_end) {
           comp_start[l] = prev_commit_end;
         } else {
-          HPC_HIPCHK(hipEventRecord(e.lev_ev[l][0], e.stream));
+          HPC_HIPCHK(mark(e.lev_ev[l][0]));
           comp_start[l] = e.lev_ev[l][0];
         }
         prev_commit_end = nullptr;
@@ -1056,7 +1060,7 @@ int engine_run_single(Engine &e, int id) {
       }
       L = layout(e);
       L.max_deg = e.max_deg;
-      HPC_HIPCHK(hipEventRecord(e.lev_ev[l][4], e.stream));
+      HPC_HIPCHK(mark(e.lev_ev[l][4]));
       if (e.n_edges > 0) {
         if (l == 1)
           launch_search_l1(e.stream, L, e.n_edges, nullptr, h.fast, engine_epb(e));
@@ -1068,7 +1072,7 @@ int engine_run_single(Engine &e, int id) {
         if (h.fast) launch_recheck(e.stream, l, L, /*clear_queue=*/false);
         HPC_HIPCHK(hipGetLastError());
       }
-      HPC_HIPCHK(hipEventRecord(e.lev_ev[l][5], e.stream));
+      HPC_HIPCHK(mark(e.lev_ev[l][5]));
       const bool last = h.max_level >= 0 && l + 1 > h.max_level;
       // the removal also counts the next level when there is one to run.  Not
       // after level 1: it removes most edges, and their row-length updates
@@ -1093,7 +1097,7 @@ int engine_run_single(Engine &e, int id) {
           if (rc) return rc;
         }
       }
-      HPC_HIPCHK(hipEventRecord(e.lev_ev[l][3], e.stream));
+      HPC_HIPCHK(mark(e.lev_ev[l][3]));
       prev_commit_end = e.lev_ev[l][3];
       if (count_next) {
         // the next level's neighbour lists while the host takes the counts
@@ -1126,6 +1130,7 @@ int engine_run_single(Engine &e, int id) {
     return HPC_OK;
   };
   const int rc_run = levels();
+  if (!e.kernel_timing) timed.clear();
   for (const auto &t : timed) {
     hpc_level_stats &st = h.hist[t.first];
     hipEvent_t *ev = e.lev_ev[t.second];

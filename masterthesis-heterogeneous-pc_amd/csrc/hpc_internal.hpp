@@ -337,6 +337,7 @@ struct Engine {
   Recheck *d_rchk = nullptr;
   size_t cap_rchk = 0;
   size_t rchk_limit = 0;  // queue length in use (0 = capacity); a tuning knob
+  bool kernel_timing = true;  // level-loop device-time events (HPC_TUNE_KERNEL_TIMING)
   unsigned long long rechecks = 0;  // exact rechecks of the last step or level
 };
 

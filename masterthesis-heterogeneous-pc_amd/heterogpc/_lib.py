@@ -17,6 +17,7 @@ HPC_CORR_CANON = 0
 HPC_CORR_F32 = 1
 HPC_TUNE_RANKS_PER_TASK = 1
 HPC_TUNE_RECHECK_QUEUE = 2
+HPC_TUNE_KERNEL_TIMING = 3
 
 
 class LevelStats(C.Structure):

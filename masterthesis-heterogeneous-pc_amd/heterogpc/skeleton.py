@@ -125,6 +125,12 @@ class DeviceSkeleton:
         """Length of the HPC_CORR_F32 recheck queues (0 = full)."""
         check(self.lib.hpc_set_tuning(self.h, 2, int(value)))
 
+    def set_kernel_timing(self, on: bool) -> None:
+        """Device-time events around each level's kernels in run() (the
+        ms_compact / ms_test / ms_commit fields); off, the level loop runs
+        without them and those fields are 0."""
+        check(self.lib.hpc_set_tuning(self.h, 3, 1 if on else 0))
+
     # ---- levels ----------------------------------------------------------
     def reset(self) -> None:
         check(self.lib.hpc_reset(self.h))

@@ -287,6 +287,30 @@ def test_run_caps_and_mixed_entry_points(hpc, orc):
         assert [s["tests"] for s in st] == [r["tests"] for r in recs]
 
 
+def test_kernel_timing_off_keeps_results(hpc, orc):
+    """hpc_run without the per-level kernel-time events (HPC_TUNE_KERNEL_TIMING
+    = 0): the oracle's adjacency, sepsets and per-level counts, and device
+    times of 0; switched back on, the times return."""
+    x, n, alpha, _ = workloads.sem("dense")
+    p = x.shape[0]
+    cor = orc.corr(x)
+    ref = orc.Skeleton(cor, n, alpha, max_level=4)
+    recs = ref.run(4)
+    with hpc.DeviceSkeleton(p, n, alpha, max_level=4) as dev:
+        dev.pmcc(x)
+        for on in (False, True):
+            print(f"[timing] on={on}", flush=True)
+            dev.set_kernel_timing(on)
+            dev.reset()
+            st = dev.run(4)
+            assert np.array_equal(dev.adj(), ref.adj()), on
+            assert np.array_equal(dev.sepsets(ref.mc()), ref.sepsets()), on
+            assert [s["tests"] for s in st] == [r["tests"] for r in recs], on
+            assert [s["removed"] for s in st] == [r["removed"] for r in recs], on
+            ms = sum(s["ms_test"] for s in st)
+            assert (ms > 0) if on else (ms == 0), (on, ms)
+
+
 @pytest.mark.parametrize("mode", ["exact", "f32"])
 def test_benchmark_workload_parity(hpc, orc, mode):
     """The p=2000 benchmark workload (n=5000, degree 3, alpha 0.05, levels
