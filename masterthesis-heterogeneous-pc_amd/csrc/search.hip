@@ -591,48 +591,55 @@ __device__ __forceinline__ int find_pos(const int32_t *row, int len, int node) {
 // The next level's counts from the row lengths the removals left (one
 // workgroup, the removal kernel's last): per row the task count of nc.level,
 // per compaction workgroup the sums of row lengths and tasks, replaced by
-// their exclusive prefixes, and the totals and longest row in the counters.
-// The same values compact_count_kernel derives from the adjacency.  Returns
-// (on every thread) the totals and the longest row.
+// their exclusive prefixes, and the edge total in rowptr[p_total].  The same
+// values compact_count_kernel derives from the adjacency.  Returns (on every
+// thread) the totals and the longest row, which the caller publishes; what
+// is written here is read by the next kernel only, so no fence is needed.
+// One row per thread and pass: a compaction workgroup's rows are an aligned
+// group of lanes, summed with shuffles (the code stays small: this runs once
+// per launch, from a cold instruction cache).
 __device__ void next_count(const Layout &L, const NextCount &nc, const unsigned long long *tab,
                            unsigned long long &te, unsigned long long &tt, int &tmax) {
   __shared__ unsigned long long s_we[kCommitWaves], s_wt[kCommitWaves];
   __shared__ int s_wm[kCommitWaves];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nt = blockDim.x;
   const int nblk = (L.p_total + kCompRows - 1) / kCompRows;
+  int mx = 0;
+  for (int g0 = 0; g0 < L.p_total; g0 += nt) {
+    const int g = g0 + tid;
+    int c = 0;
+    long long tk = 0;
+    if (g < L.p_total) {
+      c = __hip_atomic_load(&L.deg[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (nc.level >= 2 && c > 0) {
+        const unsigned long long total = binom(tab, kBinomK, c - 1, nc.level);
+        if (total > 0) tk = (long long)c * (long long)((total + nc.rpt - 1) / nc.rpt);
+      }
+      nc.row_tasks[g] = tk;
+    }
+    mx = max(mx, c);
+    unsigned long long e = (unsigned long long)c, t = (unsigned long long)tk;
+#pragma unroll
+    for (int sh = 1; sh < kCompRows; sh <<= 1) {
+      e += __shfl_xor(e, sh, 64);
+      t += __shfl_xor(t, sh, 64);
+    }
+    if (tid % kCompRows == 0 && g < L.p_total) {
+      const size_t b = (size_t)(g / kCompRows);
+      nc.blk[2 * b] = e;
+      nc.blk[2 * b + 1] = t;
+    }
+  }
+  __syncthreads();
+  // exclusive prefixes over the threads' block ranges (thread order = block
+  // order): a scan within each wave, then the waves' totals
   const int per = (nblk + nt - 1) / nt;
   const int lo = min(nblk, tid * per), hi = min(nblk, lo + per);
   unsigned long long se = 0, st = 0;
-  int mx = 0;
   for (int b = lo; b < hi; ++b) {
-    int c[kCompRows];
-#pragma unroll
-    for (int r = 0; r < kCompRows; ++r) {
-      const int g = b * kCompRows + r;
-      c[r] = g < L.p_total ? __hip_atomic_load(&L.deg[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                           : 0;
-    }
-    unsigned long long eb = 0, tb = 0;
-#pragma unroll
-    for (int r = 0; r < kCompRows; ++r) {
-      const int g = b * kCompRows + r;
-      long long tk = 0;
-      if (nc.level >= 2 && c[r] > 0) {
-        const unsigned long long total = binom(tab, kBinomK, c[r] - 1, nc.level);
-        if (total > 0) tk = (long long)c[r] * (long long)((total + nc.rpt - 1) / nc.rpt);
-      }
-      if (g < L.p_total) nc.row_tasks[g] = tk;
-      eb += (unsigned long long)c[r];
-      tb += (unsigned long long)tk;
-      mx = max(mx, c[r]);
-    }
-    nc.blk[2 * (size_t)b] = eb;
-    nc.blk[2 * (size_t)b + 1] = tb;
-    se += eb;
-    st += tb;
+    se += nc.blk[2 * (size_t)b];
+    st += nc.blk[2 * (size_t)b + 1];
   }
-  // exclusive prefixes over the threads' block ranges (thread order = block
-  // order): a scan within each wave, then the waves' totals
   unsigned long long ie = se, it = st;
 #pragma unroll
   for (int s = 1; s < 64; s <<= 1) {
@@ -669,18 +676,28 @@ __device__ void next_count(const Layout &L, const NextCount &nc, const unsigned 
     re += e;
     rt += t;
   }
-  if (tid == 0) {
-    L.rowptr[L.p_total] = (int32_t)ae;
-    L.ctr->n_edges = ae;
-    L.ctr->n_tasks = at;
-    L.ctr->max_deg = M;
-    L.ictr[0].max_deg = M;
-  }
+  if (tid == 0) L.rowptr[L.p_total] = (int32_t)ae;
   te = ae;
   tt = at;
   tmax = M;
-  __threadfence();
-  __syncthreads();
+}
+
+// A published counter word with the next level's sizes in place (the device
+// counters get them after the clear that follows the publication).
+__device__ __forceinline__ unsigned long long with_next(int w, unsigned long long v,
+                                                        unsigned long long te, unsigned long long tt,
+                                                        int tmax) {
+  auto put32 = [](unsigned long long word, size_t byte, int x) {
+    const int sh = (int)(byte & 4) * 8;
+    return (word & ~(0xffffffffull << sh)) | ((unsigned long long)(unsigned)x << sh);
+  };
+  constexpr size_t o_md = offsetof(Counters, max_deg);
+  constexpr size_t o_imd = kIctrOffset + offsetof(InstCounters, max_deg);
+  if (w == (int)(offsetof(Counters, n_edges) / 8)) return te;
+  if (w == (int)(offsetof(Counters, n_tasks) / 8)) return tt;
+  if (w == (int)(o_md / 8)) v = put32(v, o_md, tmax);
+  if (w == (int)(o_imd / 8)) v = put32(v, o_imd, tmax);
+  return v;
 }
 
 // Removal and sepset recording (cpuUtil.cpp:23-48).  Each removed edge at
@@ -913,8 +930,11 @@ __global__ __launch_bounds__(64 * kCommitWaves) void commit_kernel(Layout L, lon
     unsigned long long words_rec = 0;
     if (pub.advance && L.K == 1 && threadIdx.x == 0)
       words_rec = __hip_atomic_load(&L.ictr[0].rec_words, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (int w = threadIdx.x; w < pub.words; w += blockDim.x)
-      pub.dst[w] = __hip_atomic_load(&src[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int w = threadIdx.x; w < pub.words; w += blockDim.x) {
+      unsigned long long v = __hip_atomic_load(&src[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (nc.on) v = with_next(w, v, nte, ntt, nmax);
+      pub.dst[w] = v;
+    }
     __threadfence_system();
     __syncthreads();
     if (threadIdx.x == 0)
