@@ -972,14 +972,14 @@ int engine_run_single(Engine &e, int id) {
       if (rc) return rc;
       Layout L = layout(e);
       HPC_HIPCHK(mark(e.lev_ev[0][4]));
-      launch_level0(e.stream, L, e.h_inst, e.d_prefix + 2 * (L.K + 1), e.h_prefix);
+      // the fold kernel publishes level 0's counts, clears the counters and
+      // sets the device table to level 1
+      Publish pb = publish2();
+      pb.advance = 1;
+      launch_level0(e.stream, L, e.h_inst, e.d_prefix + 2 * (L.K + 1), e.h_prefix, &pb);
       HPC_HIPCHK(hipGetLastError());
       HPC_HIPCHK(mark(e.lev_ev[0][5]));
       prev_commit_end = e.lev_ev[0][5];  // level 1's compaction starts here
-      const Publish pb = publish2();
-      launch_publish(e.stream, reinterpret_cast<const unsigned long long *>(e.d_ctr), pb.words,
-                     pb.dst, pb.flag, pb.seq);
-      HPC_HIPCHK(hipGetLastError());
       pend = 0;
       pend_seq = pb.seq;
       pend_max_deg = h.p;
@@ -992,10 +992,9 @@ int engine_run_single(Engine &e, int id) {
         h.finished = true;
         return HPC_OK;
       }
-      // level 1's table on clear counters (level 0's are in h_ctr2)
+      // the device table is at level 1 on clear counters (level 0's are in
+      // h_ctr2): the fold kernel saw to both
       e.h_inst[0].level = 1;
-      rc = upload_table(e, /*zero=*/true);
-      if (rc) return rc;
     } else {
       // the device table starts at level l
       rc = engine_pack(e, ids, {l}, /*zero_ctr=*/true);

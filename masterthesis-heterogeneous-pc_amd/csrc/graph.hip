@@ -154,8 +154,10 @@ __global__ __launch_bounds__(64 * kL0Waves) void level0_kernel(Layout L,
 
 // Folds the level-0 slots of instance blockIdx.x into its counters and
 // clears them.  Tests, test bound and searched edges all equal the pairs
-// tested at level 0.
-__global__ __launch_bounds__(64) void level0_fold_kernel(Layout L) {
+// tested at level 0.  With pub.flag set (one instance, one workgroup) lane 0
+// then publishes the counter block, clears it and sets the device table to
+// level 1, as the removal kernel does at the end of a level.
+__global__ __launch_bounds__(64) void level0_fold_kernel(Layout L, Publish pub) {
   const int k = blockIdx.x, lane = threadIdx.x;
   unsigned long long te = 0, re = 0;
   if (lane < kL0Slots) {
@@ -176,6 +178,15 @@ __global__ __launch_bounds__(64) void level0_fold_kernel(Layout L) {
     c.tests_max += te;
     c.edges += te;
     c.removed += re;
+  }
+  if (!pub.flag || lane != 0) return;
+  unsigned long long *blk = reinterpret_cast<unsigned long long *>(L.ctr);
+  for (int w = 0; w < pub.words; ++w) pub.dst[w] = blk[w];
+  __threadfence_system();
+  __hip_atomic_store(pub.flag, pub.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  if (pub.advance) {
+    for (int w = 0; w < pub.words; ++w) blk[w] = 0ull;
+    const_cast<Inst *>(L.inst)[0].level = 1;
   }
 }
 
@@ -562,7 +573,7 @@ void launch_adj_init(hipStream_t s, unsigned long long *adj, int p, int W) {
 }
 
 void launch_level0(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst,
-                   long long *d_prefix, std::vector<long long> &h_prefix) {
+                   long long *d_prefix, std::vector<long long> &h_prefix, const Publish *pub) {
   const int K = L.K;
   h_prefix.assign(K + 1, 0);
   for (int k = 0; k < K; ++k) {
@@ -570,7 +581,13 @@ void launch_level0(hipStream_t s, const Layout &L, const std::vector<Inst> &host
     const bool on = (I.flags & kActive) && I.level == 0;
     h_prefix[k + 1] = h_prefix[k] + (on ? (long long)I.W * (I.W + 1) / 2 : 0);
   }
-  if (h_prefix[K] == 0) return;
+  Publish none{};
+  const Publish fold_pub = (pub && K == 1) ? *pub : none;
+  if (h_prefix[K] == 0) {
+    // nothing to test; a requested publication still happens
+    if (fold_pub.flag) hipLaunchKernelGGL(level0_fold_kernel, dim3(1), dim3(64), 0, s, L, fold_pub);
+    return;
+  }
   if (K == 1) {
     // one instance: no offsets to upload
     hipLaunchKernelGGL(level0_kernel, dim3((unsigned)h_prefix[K]), dim3(64 * kL0Waves), 0, s, L,
@@ -581,7 +598,7 @@ void launch_level0(hipStream_t s, const Layout &L, const std::vector<Inst> &host
     hipLaunchKernelGGL(level0_kernel, dim3((unsigned)h_prefix[K]), dim3(64 * kL0Waves), 0, s, L,
                        (const long long *)d_prefix, h_prefix[K]);
   }
-  hipLaunchKernelGGL(level0_fold_kernel, dim3((unsigned)K), dim3(64), 0, s, L);
+  hipLaunchKernelGGL(level0_fold_kernel, dim3((unsigned)K), dim3(64), 0, s, L, fold_pub);
 }
 
 void launch_compact(hipStream_t s, const Layout &L) {

@@ -149,8 +149,6 @@ void launch_table(hipStream_t s, Inst *dst, const TableArg &arg);
 // words once it sees the flag.
 void launch_publish(hipStream_t s, const unsigned long long *src, size_t words,
                     unsigned long long *dst, unsigned long long *flag, unsigned long long seq);
-void launch_level0(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst,
-                   long long *d_prefix, std::vector<long long> &h_prefix);
 void launch_compact(hipStream_t s, const Layout &L);
 // Counter publication by the last workgroup of a kernel: copies `words`
 // 8-byte words from the counter block to fine-grained host memory `dst`,
@@ -164,6 +162,12 @@ struct Publish {
   // block and advance the instance's level and record base on the device
   int advance;
 };
+// Level 0 of the packed instances at level 0.  With `pub` (one instance),
+// the kernel that folds the counts also publishes them, clears the counter
+// block and sets the device table to level 1.
+void launch_level0(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst,
+                   long long *d_prefix, std::vector<long long> &h_prefix,
+                   const Publish *pub = nullptr);
 // Compaction of every active instance (all at `level` >= 1) in two kernels
 // that also plan the level's tasks (level >= 2, `rpt` ranks per task) and
 // publish the counters (the first kernel's last workgroup, once the totals
