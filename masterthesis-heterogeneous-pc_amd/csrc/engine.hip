@@ -351,9 +351,14 @@ void engine_note_level(Engine &e, int id, int level) {
 }
 
 // Room for the worst case of one level: every remaining edge removed, each
-// with a record of 3 + level words; contents kept.
+// with a record of 3 + level words; contents kept.  With a level cap the
+// room covers every remaining level at once (an edge is removed once, its
+// record at most 3 + cap words), so the buffer grows once per run rather
+// than between a level's kernels.
 static int ensure_rec(Engine &e, InstHost &h, int level) {
-  const size_t need = h.rec_used + (size_t)std::max<int64_t>(h.undirected, 0) * (3 + level);
+  const int deepest =
+      h.max_level >= 0 ? std::min(std::max(h.max_level, level), kMaxLevel) : level;
+  const size_t need = h.rec_used + (size_t)std::max<int64_t>(h.undirected, 0) * (3 + deepest);
   if (need <= h.rec_cap && h.rec) return HPC_OK;
   const size_t cap = std::max(need, 2 * h.rec_cap);
   int32_t *nr = nullptr;
@@ -1003,6 +1008,8 @@ int engine_run_single(Engine &e, int id) {
     // the previous removal counted this level (its publication, already waited
     // for, holds the level's sizes) and its fill kernel is queued
     bool counted = false;
+    // this level's search is already in the stream (queued before the wait)
+    bool searched = false;
     for (;;) {
       if (l > kMaxLevel) break;  // the step loop takes levels the kernels do not reach
       const Clock::time_point t_lv = Clock::now();
@@ -1059,8 +1066,8 @@ int engine_run_single(Engine &e, int id) {
       }
       L = layout(e);
       L.max_deg = e.max_deg;
-      HPC_HIPCHK(mark(e.lev_ev[l][4]));
-      if (e.n_edges > 0) {
+      if (!searched) HPC_HIPCHK(mark(e.lev_ev[l][4]));
+      if (e.n_edges > 0 && !searched) {
         if (l == 1)
           launch_search_l1(e.stream, L, e.n_edges, nullptr, h.fast, engine_epb(e));
         else
@@ -1071,7 +1078,8 @@ int engine_run_single(Engine &e, int id) {
         if (h.fast) launch_recheck(e.stream, l, L, /*clear_queue=*/false);
         HPC_HIPCHK(hipGetLastError());
       }
-      HPC_HIPCHK(mark(e.lev_ev[l][5]));
+      if (!searched) HPC_HIPCHK(mark(e.lev_ev[l][5]));
+      searched = false;
       const bool last = h.max_level >= 0 && l + 1 > h.max_level;
       // the removal also counts the next level when there is one to run.  Not
       // after level 1: it removes most edges, and their row-length updates
@@ -1104,6 +1112,18 @@ int engine_run_single(Engine &e, int id) {
         prev_commit_end = nullptr;
         launch_compact_fill(e.stream, L, e.d_blk, e.d_row_tasks, e.d_task_off, e.cap_edges);
         HPC_HIPCHK(hipGetLastError());
+        // Exact runs with a level cap (the next level runs whatever its
+        // sizes): the next level's search goes into the stream before the
+        // wait.  It reads its task count on the device, and rows only shrink,
+        // so this level's longest row bounds its LDS block.
+        if (!h.fast && h.max_level >= 0) {
+          HPC_HIPCHK(mark(e.lev_ev[l + 1][4]));
+          launch_search_ln(e.stream, l + 1, L, engine_rpt(e, l + 1), e.d_row_tasks, e.d_task_off,
+                           false, engine_tpb(e, l + 1));
+          HPC_HIPCHK(hipGetLastError());
+          HPC_HIPCHK(mark(e.lev_ev[l + 1][5]));
+          searched = true;
+        }
         rc = wait_published(e, pb.seq);
         if (rc) return rc;
         if (e.h_ctr2->overflow) return voided(l);
