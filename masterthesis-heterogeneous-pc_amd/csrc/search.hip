@@ -453,11 +453,15 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
         const unsigned long long r_end = min(total, r_begin + (unsigned long long)rpt);
         const double a = Mi[y];
         const double *Mj = M + (size_t)y * D;  // row j
-        // lane t tests the kRun consecutive ranks rb = r0 + kRun * t ..: one
+        // lane t tests the kr consecutive ranks rb = r0 + kr * t ..: one
         // unrank, then lexicographic successors; the chunk's lowest passing
-        // rank is the lowest passing lane's first passing step
-        for (unsigned long long r0 = r_begin; r0 < r_end; r0 += 64 * kRun) {
-          const unsigned long long rb = r0 + (unsigned long long)lane * kRun;
+        // rank is the lowest passing lane's first passing step.  kr = kRun,
+        // or fewer when the task's range is shorter than a chunk, so its
+        // ranks spread over all 64 lanes
+        const unsigned long long span = r_end > r_begin ? r_end - r_begin : 0ull;
+        const int kr = span >= 64ull * kRun ? kRun : max(1, (int)((span + 63ull) / 64ull));
+        for (unsigned long long r0 = r_begin; r0 < r_end; r0 += 64ull * kr) {
+          const unsigned long long rb = r0 + (unsigned long long)lane * kr;
           int first = kRun;       // this lane's first certain pass
           unsigned unc_bits = 0;  // its uncertain steps
           int q[LV];
@@ -466,7 +470,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
           if (rb < r_end) unrank_lds<LV>((const lds_u64 *)bt, LV + 1, rb, nn, q);
 #pragma unroll
           for (int st = 0; st < kRun; ++st) {
-            if (rb + st < r_end && first == kRun) {
+            if (st < kr && rb + st < r_end && first == kRun) {
               int pos[LV];
 #pragma unroll
               for (int c = 0; c < LV; ++c) pos[c] = q[c] < y ? q[c] : q[c] + 1;
@@ -510,13 +514,13 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
                 F += fb ? 1 : 0;
               }
             }
-            if (st + 1 < kRun) next_comb<LV>(q, nn);
+            if (st + 1 < kr) next_comb<LV>(q, nn);
           }
           const unsigned long long m = __ballot(first < kRun);
           unsigned long long best = ~0ull;
           if (m) {
             const int t = __ffsll((long long)m) - 1;
-            best = r0 + (unsigned long long)t * kRun + (unsigned long long)__shfl(first, t, 64);
+            best = r0 + (unsigned long long)t * kr + (unsigned long long)__shfl(first, t, 64);
           }
           if (fast) {
             // uncertain tests below the chunk's lowest certain pass
