@@ -33,11 +33,10 @@ namespace {
 
 constexpr int kWavesPerBlock = 4;
 // consecutive ranks per lane in the LDS-staged level >= 2 search: longer
-// runs share one unrank among more tests, shorter ones waste fewer lanes
-// on the short rank ranges of low levels (a level-2 edge of a 17-neighbour
-// row has 120 sets: 2 per lane keep 60 of 64 lanes busy)
+// runs share one unrank among more tests (a task shorter than a chunk
+// shortens them, so its ranks still spread over every lane)
 template <int LV>
-constexpr int run_len() { return LV >= 4 ? 8 : (LV == 3 ? 4 : 2); }
+constexpr int run_len() { return LV >= 4 ? 8 : 4; }
 // wavefronts per workgroup of the removal kernel (one record reservation and
 // one set of counter atomics per workgroup)
 constexpr int kCommitWaves = 16;
