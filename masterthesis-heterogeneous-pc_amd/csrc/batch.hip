@@ -400,8 +400,18 @@ static void server_loop(hpc_server *s) {
       s->peak_bytes = std::max(s->peak_bytes, s->in_use);
     }
     if (ids.empty()) continue;
+    // A lone request with nothing queued or uploading runs its remaining
+    // levels in the one-instance loop (one wait per level, no table kernel).
+    // A request that arrives meanwhile waits for it, as it would for a step.
+    bool lone = false;
+    if (ids.size() == 1) {
+      std::lock_guard<std::mutex> lk(s->mu);
+      const InstHost &h = s->eng.inst[ids[0]];
+      lone = s->queue.empty() && s->uploading == 0 && s->up_done.empty() && h.have_cor &&
+             h.next_level >= 1 && !h.finished;
+    }
     auto t0 = Clock::now();
-    const int rc = engine_step(s->eng, ids);
+    const int rc = lone ? engine_run_single(s->eng, ids[0]) : engine_step(s->eng, ids);
     const double ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
     const std::string msg = rc ? hpc_last_error() : "";
     // only the worker writes `running`, so it reads it without the lock
