@@ -886,17 +886,21 @@ size_t engine_footprint(int p, int n, int max_level) {
 }
 
 // One instance, levels h.next_level .. its cap, with one host wait per
-// level.  Per level l >= 1 the stream holds: compaction
-// (publishes n_edges / max_deg into h_ctr), [host wait], CI tests, removal.
-// The removal publishes its counts into h_ctr2, clears the counter block and
-// advances the device table to level l + 1 and its record base, so the next
-// compaction is queued right behind it; the host reads the removal's counts
-// at the next wait (publications arrive in stream order).  Level 0 publishes
-// into h_ctr2 the same way.  Results and per-level records equal
-// engine_step's.  With fast correlations a level whose recheck queue
-// overflowed did nothing (the removal skips itself); the next
-// wait finds it, the correlations are made exact and the loop returns with
-// h.next_level at that level for engine_step to continue.
+// level.  Level 0's fold kernel and every removal publish their counts into
+// h_ctr2, clear the counter block and advance the device table (level,
+// record base), so no table kernel runs between levels.  Per level l >= 1:
+//  - l = 1 (or after a level with no edges): compaction, publishing
+//    n_edges / max_deg into h_ctr; [host wait]; CI tests; removal, with the
+//    next compaction queued right behind it (its counts are read at the next
+//    wait: publications arrive in stream order);
+//  - l >= 2: CI tests; a removal that also counts level l + 1 and publishes
+//    those sizes with its counts; the fill kernel; in an exact capped run
+//    also level l + 1's search; [host wait].
+// Results and per-level records equal engine_step's.  Kernel times are read
+// after the run.  With fast correlations a level whose recheck queue
+// overflowed did nothing (the removal skips itself); the next wait finds it,
+// the correlations are made exact and the loop returns with h.next_level at
+// that level for engine_step to continue.
 int engine_run_single(Engine &e, int id) {
   HPC_HIPCHK(hipSetDevice(e.device));
   InstHost &h = e.inst[id];
@@ -968,6 +972,7 @@ int engine_run_single(Engine &e, int id) {
     h.finished = false;
     return HPC_OK;
   };
+  // the level loop proper; its kernel times are filled in once it returns
   auto levels = [&]() -> int {
     int l = h.next_level;
     int rc = HPC_OK;
