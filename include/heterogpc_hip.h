@@ -195,7 +195,9 @@ int hpc_level(hpc_state *st, int level, hpc_level_stats *stats);
 /* SkeletonCalculator::run's level loop (skeleton.cpp:172-176): levels
  * 0..max_level (the state's max_level when max_level == -2; until no row
  * has enough neighbours when max_level == -1).  stats[0..cap) receives one
- * record per level; *levels_run the number of levels that ran. */
+ * record per level; *levels_run the number of levels that ran.  With no
+ * cap, the level found to have no testable row ends the loop after its
+ * compaction and gets no record. */
 int hpc_run(hpc_state *st, int max_level, hpc_level_stats *stats, int cap,
             int *levels_run);
 

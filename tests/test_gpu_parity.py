@@ -287,6 +287,32 @@ def test_run_caps_and_mixed_entry_points(hpc, orc):
         assert [s["tests"] for s in st] == [r["tests"] for r in recs]
 
 
+@pytest.mark.parametrize("name,cap", [("deep", -1), ("dense", -1), ("deep", 6)])
+def test_run_unbounded_and_deep_cap(hpc, orc, name, cap):
+    """hpc_run to the end (no level cap: the loop stops when no row has
+    enough neighbours) and with a cap of 6 (the next level's search queued
+    before each wait): the oracle's adjacency, sepsets and per-level counts."""
+    x, n, alpha, _ = workloads.sem(name)
+    p = x.shape[0]
+    cor = orc.corr(x)
+    ref = orc.Skeleton(cor, n, alpha, max_level=cap)
+    recs = ref.run(cap)
+    if cap < 0:
+        # the oracle records the level that finds no testable row; hpc_run
+        # stops at its compaction without a record
+        assert recs[-1]["tests"] == 0 and recs[-1]["max_deg"] - 1 < len(recs) - 1
+        recs = recs[:-1]
+    print(f"[run] {name} cap={cap}: {len(recs)} levels", flush=True)
+    with hpc.DeviceSkeleton(p, n, alpha, max_level=cap) as dev:
+        dev.pmcc(x)
+        st = dev.run(cap)
+        assert [s["level"] for s in st] == list(range(len(recs)))
+        assert [s["tests"] for s in st] == [r["tests"] for r in recs]
+        assert [s["removed"] for s in st] == [r["removed"] for r in recs]
+        assert np.array_equal(dev.adj(), ref.adj())
+        assert np.array_equal(dev.sepsets(ref.mc()), ref.sepsets())
+
+
 def test_kernel_timing_off_keeps_results(hpc, orc):
     """hpc_run without the per-level kernel-time events (HPC_TUNE_KERNEL_TIMING
     = 0): the oracle's adjacency, sepsets and per-level counts, and device
