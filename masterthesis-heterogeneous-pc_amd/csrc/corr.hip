@@ -22,6 +22,34 @@ __device__ __forceinline__ double tree64(double v) {
   return v;
 }
 
+// All 64 accumulators of a wavefront at once, with tree64's additions: v[a]
+// holds lane t's partial sum of accumulator a; on return v[0] of lane t holds
+// accumulator t's total.  Each stage s = 32, 16, ..., 1 adds the partial sums
+// of lanes t and t + s of an accumulator, as tree64 does, but in whichever of
+// the two lanes keeps that accumulator (bit s of the lane picks the upper or
+// lower half of the values left); addition is commutative, so the sum has the
+// same bits.  63 exchanges per lane instead of tree64's 6 per accumulator.
+template <int H>
+__device__ __forceinline__ void halve64(double *v, int lane) {
+  const bool up = (lane & H) != 0;
+#pragma unroll
+  for (int a = 0; a < H; ++a) {
+    const double lo = v[a], hi = v[a + H];
+    const double keep = up ? hi : lo;
+    const double send = up ? lo : hi;
+    v[a] = keep + __shfl_xor(send, H, 64);
+  }
+}
+
+__device__ __forceinline__ void tree64_all(double (&v)[64], int lane) {
+  halve64<32>(v, lane);
+  halve64<16>(v, lane);
+  halve64<8>(v, lane);
+  halve64<4>(v, lane);
+  halve64<2>(v, lane);
+  halve64<1>(v, lane);
+}
+
 __device__ __forceinline__ int tree64i(int v) {
 #pragma unroll
   for (int s = 32; s > 0; s >>= 1) v = v + __shfl_down(v, s, 64);
@@ -220,11 +248,9 @@ __global__ __launch_bounds__(256) void corr_tile_kernel(Layout L, const long lon
   const double *gb0 = I.z + (size_t)min(b0 + lr0, p - 1) * npad + lc;
   const double *gb1 = I.z + (size_t)min(b0 + lr1, p - 1) * npad + lc;
   const int nsteps = npad >> 6;
-  double acc[8][8];
+  double acc[64];  // acc[8 u + v]: pair (wa * 8 + u, wb * 8 + v) of the tile
 #pragma unroll
-  for (int u = 0; u < 8; ++u)
-#pragma unroll
-    for (int v = 0; v < 8; ++v) acc[u][v] = 0.0;
+  for (int a = 0; a < 64; ++a) acc[a] = 0.0;
   // Step s of the samples lives in LDS buffer s % 3.  While a step's FMAs
   // run, the wave's next operands are read from LDS and the global loads of
   // the step after next are in flight; the loop body covers two steps with
@@ -240,7 +266,7 @@ __global__ __launch_bounds__(256) void corr_tile_kernel(Layout L, const long lon
 #pragma unroll
     for (int u = 0; u < 8; ++u)
 #pragma unroll
-      for (int v = 0; v < 8; ++v) acc[u][v] = __builtin_fma(za[u], zb[v], acc[u][v]);
+      for (int v = 0; v < 8; ++v) acc[8 * u + v] = __builtin_fma(za[u], zb[v], acc[8 * u + v]);
   };
   const int last = nsteps - 1;
 #define HPC_LOAD(st, A0, A1, B0, B1)                                       \
@@ -294,18 +320,14 @@ __global__ __launch_bounds__(256) void corr_tile_kernel(Layout L, const long lon
 #undef HPC_LOAD
 #undef HPC_STORE
   const double div = (double)((float)I.n - 1.f);
-#pragma unroll
-  for (int u = 0; u < 8; ++u)
-#pragma unroll
-    for (int v = 0; v < 8; ++v) {
-      double r = tree64(acc[u][v]);
-      const int a = a0 + wa * 8 + u, b = b0 + wb * 8 + v;
-      if (lane == 0 && a < p && b < p) {
-        r = r / div;
-        I.cor[(size_t)a * p + b] = r;
-        I.cor[(size_t)b * p + a] = r;
-      }
-    }
+  // lane t: the total of pair t of the wave's 8 x 8 sub-tile
+  tree64_all(acc, lane);
+  const int a = a0 + wa * 8 + (lane >> 3), b = b0 + wb * 8 + (lane & 7);
+  if (a < p && b < p) {
+    const double r = acc[0] / div;
+    I.cor[(size_t)a * p + b] = r;
+    I.cor[(size_t)b * p + a] = r;
+  }
 }
 
 // ---------------------------------------------------------------------------
