@@ -69,7 +69,11 @@ __device__ __forceinline__ int find_inst(const long long *prefix, int K, long lo
 // One wavefront per variable: mean over non-NaN samples, sd rounded through
 // float (corHelper.cu:75), z-scores (x - mean) / sd padded with zeros to npad,
 // NaN mask.  A variable with NaN cells flags its instance (ictr.has_nan),
-// which selects the masked pairwise kernel for it.
+// which selects the masked pairwise kernel for it.  With J > 0 (every
+// instance's n <= 64 J) lane t loads its samples t, t + 64, ... once into
+// registers and the three passes run from there; J = 0 re-reads the row for
+// each pass.  Both do the same operations in the same order.
+template <int J>
 __global__ __launch_bounds__(256) void moments_kernel(Layout L) {
   const int lane = threadIdx.x & 63;
   const int g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -79,22 +83,49 @@ __global__ __launch_bounds__(256) void moments_kernel(Layout L) {
   if (!(I.flags & kActive)) return;
   const int a = g - I.start, n = I.n, npad = I.npad;
   const double *xa = I.x + (size_t)a * n;
+  double xv[J > 0 ? J : 1];
+  if constexpr (J > 0) {
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int k = lane + 64 * j;
+      xv[j] = k < n ? xa[k] : 0.0;
+    }
+  }
   double s = 0.0;
   int c = 0;
-  for (int k = lane; k < n; k += 64) {
-    double v = xa[k];
-    if (!__builtin_isnan(v)) { s = s + v; c += 1; }
+  if constexpr (J > 0) {
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const double v = xv[j];
+      if (lane + 64 * j < n && !__builtin_isnan(v)) { s = s + v; c += 1; }
+    }
+  } else {
+    for (int k = lane; k < n; k += 64) {
+      double v = xa[k];
+      if (!__builtin_isnan(v)) { s = s + v; c += 1; }
+    }
   }
   s = tree64(s);
   c = tree64i(c);
   if (lane == 0 && c < n) atomicOr(&L.ictr[kin].has_nan, 1);
   const double mu = __shfl(s, 0, 64) / (double)__shfl(c, 0, 64);
   double q = 0.0;
-  for (int k = lane; k < n; k += 64) {
-    double v = xa[k];
-    if (!__builtin_isnan(v)) {
-      double d = v - mu;
-      q = __builtin_fma(d, d, q);
+  if constexpr (J > 0) {
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const double v = xv[j];
+      if (lane + 64 * j < n && !__builtin_isnan(v)) {
+        double d = v - mu;
+        q = __builtin_fma(d, d, q);
+      }
+    }
+  } else {
+    for (int k = lane; k < n; k += 64) {
+      double v = xa[k];
+      if (!__builtin_isnan(v)) {
+        double d = v - mu;
+        q = __builtin_fma(d, d, q);
+      }
     }
   }
   q = tree64(q);
@@ -107,12 +138,20 @@ __global__ __launch_bounds__(256) void moments_kernel(Layout L) {
   double *za = I.z + (size_t)a * npad;
   uint8_t *na = I.nanm + (size_t)a * npad;
   float *z32 = I.z32 ? I.z32 + (size_t)a * npad : nullptr;
-  for (int k = lane; k < npad; k += 64) {
-    double v = k < n ? xa[k] : 0.0;
+  auto put = [&](int k, double v) {
     const double zk = k < n ? (v - mu) / sd : 0.0;
     za[k] = zk;
     na[k] = (k < n && __builtin_isnan(v)) ? 1 : 0;
     if (z32) z32[k] = (float)zk;
+  };
+  if constexpr (J > 0) {
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int k = lane + 64 * j;
+      if (k < npad) put(k, xv[j]);
+    }
+  } else {
+    for (int k = lane; k < npad; k += 64) put(k, k < n ? xa[k] : 0.0);
   }
 }
 
@@ -480,9 +519,19 @@ long long masked_blocks(int p) {
 
 }  // namespace
 
-void launch_moments(hipStream_t s, const Layout &L) {
+void launch_moments(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst) {
   const int wpb = 4;
-  hipLaunchKernelGGL(moments_kernel, dim3((L.p_total + wpb - 1) / wpb), dim3(64 * wpb), 0, s, L);
+  const dim3 grid((L.p_total + wpb - 1) / wpb), block(64 * wpb);
+  // the samples in registers when every active instance's row fits
+  int max_n = 0;
+  for (const Inst &I : host_inst)
+    if (I.flags & kActive) max_n = std::max(max_n, I.n);
+  if (max_n <= 64 * 32)
+    hipLaunchKernelGGL(moments_kernel<32>, grid, block, 0, s, L);
+  else if (max_n <= 64 * 80)
+    hipLaunchKernelGGL(moments_kernel<80>, grid, block, 0, s, L);
+  else
+    hipLaunchKernelGGL(moments_kernel<0>, grid, block, 0, s, L);
 }
 
 void launch_corr(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst,

@@ -647,7 +647,7 @@ int engine_corr(Engine &e, float *ms_moments, float *ms_corr) {
   if (rc) return rc;
   Layout L = layout(e);
   HPC_HIPCHK(hipEventRecord(e.ev[0], e.stream));
-  launch_moments(e.stream, L);
+  launch_moments(e.stream, L, e.h_inst);
   HPC_HIPCHK(hipGetLastError());
   HPC_HIPCHK(hipEventRecord(e.ev[1], e.stream));
   launch_corr(e.stream, L, e.h_inst, e.d_prefix, e.h_prefix);

@@ -120,7 +120,7 @@ struct Layout {
 };
 
 // ---- launchers (each defined in the .hip file that owns the kernel) ------
-void launch_moments(hipStream_t s, const Layout &L);
+void launch_moments(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst);
 // pairwise sums: tiled kernel for NaN-free instances, masked kernel for the
 // others; prefix arrays hold per-instance block counts (device and host)
 void launch_corr(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst,
