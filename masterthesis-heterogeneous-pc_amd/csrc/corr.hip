@@ -338,10 +338,11 @@ __global__ __launch_bounds__(256) void corr_tile_kernel(Layout L, const long lon
     }                                                                      \
   } while (0)
   double2 pa0, pa1, pb0, pb1, qa0, qa1, qb0, qb1;
+  // steps 0 and 1 in flight together
   HPC_LOAD(0, pa0, pa1, pb0, pb1);
+  HPC_LOAD(min(1, last), qa0, qa1, qb0, qb1);
   HPC_STORE(0, pa0, pa1, pb0, pb1);
-  HPC_LOAD(min(1, last), pa0, pa1, pb0, pb1);
-  HPC_STORE(1, pa0, pa1, pb0, pb1);
+  HPC_STORE(1, qa0, qa1, qb0, qb1);
   HPC_LOAD(min(2, last), pa0, pa1, pb0, pb1);
   __syncthreads();
   lds_read(0, za, zb);
