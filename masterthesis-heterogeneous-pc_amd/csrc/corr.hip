@@ -29,25 +29,39 @@ __device__ __forceinline__ double tree64(double v) {
 // the two lanes keeps that accumulator (bit s of the lane picks the upper or
 // lower half of the values left); addition is commutative, so the sum has the
 // same bits.  63 exchanges per lane instead of tree64's 6 per accumulator.
-template <int H>
-__device__ __forceinline__ void halve64(double *v, int lane) {
-  const bool up = (lane & H) != 0;
+// One stage: 2 H values per lane in, H out; the partner is lane ^ S.
+template <int H, int S>
+__device__ __forceinline__ void halve(double *v, int lane) {
+  const bool up = (lane & S) != 0;
 #pragma unroll
   for (int a = 0; a < H; ++a) {
     const double lo = v[a], hi = v[a + H];
     const double keep = up ? hi : lo;
     const double send = up ? lo : hi;
-    v[a] = keep + __shfl_xor(send, H, 64);
+    v[a] = keep + __shfl_xor(send, S, 64);
   }
 }
 
 __device__ __forceinline__ void tree64_all(double (&v)[64], int lane) {
-  halve64<32>(v, lane);
-  halve64<16>(v, lane);
-  halve64<8>(v, lane);
-  halve64<4>(v, lane);
-  halve64<2>(v, lane);
-  halve64<1>(v, lane);
+  halve<32, 32>(v, lane);
+  halve<16, 16>(v, lane);
+  halve<8, 8>(v, lane);
+  halve<4, 4>(v, lane);
+  halve<2, 2>(v, lane);
+  halve<1, 1>(v, lane);
+}
+
+// The same for 16 accumulators: stages 32, 16, 8, 4 halve the values kept,
+// stages 2 and 1 add the one left in both lanes of a pair.  Lane t ends with
+// the total of accumulator (t >> 2) & 15 (its four lanes t & ~3 .. t | 3 all
+// hold it).
+__device__ __forceinline__ void tree64_16(double (&v)[16], int lane) {
+  halve<8, 32>(v, lane);
+  halve<4, 16>(v, lane);
+  halve<2, 8>(v, lane);
+  halve<1, 4>(v, lane);
+  v[0] = v[0] + __shfl_xor(v[0], 2, 64);
+  v[0] = v[0] + __shfl_xor(v[0], 1, 64);
 }
 
 __device__ __forceinline__ int tree64i(int v) {
@@ -179,11 +193,10 @@ __device__ __forceinline__ void corr_masked_block(Layout L, int k, long long rem
     ia[u] = min(a0 + u, p - 1);
     ib[u] = min(b0 + u, p - 1);
   }
-  double acc[TB][TB];
+  static_assert(TB * TB == 16, "tree64_16 reduces 16 accumulators");
+  double acc[TB * TB];  // acc[TB u + v]: pair (a0 + u, b0 + v)
 #pragma unroll
-  for (int u = 0; u < TB; ++u)
-#pragma unroll
-    for (int v = 0; v < TB; ++v) acc[u][v] = 0.0;
+  for (int a = 0; a < TB * TB; ++a) acc[a] = 0.0;
   for (int kk = lane; kk < npad; kk += 64) {
     double za[TB], zb[TB];
     bool na[TB], nbm[TB];
@@ -201,20 +214,17 @@ __device__ __forceinline__ void corr_masked_block(Layout L, int k, long long rem
     for (int u = 0; u < TB; ++u)
 #pragma unroll
       for (int v = 0; v < TB; ++v)
-        if (!(na[u] || nbm[v])) acc[u][v] = __builtin_fma(za[u], zb[v], acc[u][v]);
+        if (!(na[u] || nbm[v])) acc[TB * u + v] = __builtin_fma(za[u], zb[v], acc[TB * u + v]);
   }
   const double div = (double)((float)I.n - 1.f);
-#pragma unroll
-  for (int u = 0; u < TB; ++u)
-#pragma unroll
-    for (int v = 0; v < TB; ++v) {
-      double r = tree64(acc[u][v]);
-      if (lane == 0 && a0 + u < p && b0 + v < p) {
-        r = r / div;
-        I.cor[(size_t)(a0 + u) * p + (b0 + v)] = r;
-        I.cor[(size_t)(b0 + v) * p + (a0 + u)] = r;
-      }
-    }
+  tree64_16(acc, lane);
+  const int idx = (lane >> 2) & 15;
+  const int a = a0 + idx / TB, b = b0 + idx % TB;
+  if ((lane & 3) == 0 && a < p && b < p) {
+    const double r = acc[0] / div;
+    I.cor[(size_t)a * p + b] = r;
+    I.cor[(size_t)b * p + a] = r;
+  }
 }
 
 __global__ __launch_bounds__(256) void corr_canon_kernel(Layout L, const long long *__restrict__ prefix) {
