@@ -247,7 +247,7 @@ __global__ __launch_bounds__(256) void corr_canon_kernel(Layout L, const long lo
 // Tiled form for NaN-free instances: a 256-thread workgroup owns a 16 x 16
 // tile of variable pairs; each wavefront an 8 x 8 sub-tile (64 accumulators
 // per lane, lane t = residue class t).  Samples stream through LDS 64 at a
-// time, double buffered, so each z element is read from L2 once per
+// time in a four-slot ring, so each z element is read from L2 once per
 // workgroup.  Tiles are visited in 8 x 8 super-tiles of each instance's
 // upper triangle; consecutive logical indices go to one XCD (workgroup id
 // mod 8), so every XCD walks a contiguous range of super-tiles and keeps the
@@ -259,8 +259,8 @@ constexpr int kXcds = 8;
 
 __global__ __launch_bounds__(256) void corr_tile_kernel(Layout L, const long long *__restrict__ prefix,
                                                         long long per_xcd) {
-  __shared__ double sa[3][CT][64];
-  __shared__ double sb[3][CT][64];
+  __shared__ double sa[4][CT][64];
+  __shared__ double sb[4][CT][64];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wa = wave >> 1, wb = wave & 1;
@@ -300,10 +300,14 @@ __global__ __launch_bounds__(256) void corr_tile_kernel(Layout L, const long lon
   double acc[64];  // acc[8 u + v]: pair (wa * 8 + u, wb * 8 + v) of the tile
 #pragma unroll
   for (int a = 0; a < 64; ++a) acc[a] = 0.0;
-  // Step s of the samples lives in LDS buffer s % 3.  While a step's FMAs
-  // run, the wave's next operands are read from LDS and the global loads of
-  // the step after next are in flight; the loop body covers two steps with
-  // two sets of load registers, so no register copy waits on a global load.
+  // Step s of the samples lives in LDS slot s % 4.  While a step's FMAs run,
+  // the wave's next operands are read from LDS, step s + 3 is stored from
+  // registers and the global loads of step s + 4 are in flight; the loop body
+  // covers two steps with two sets of load registers.  One barrier per two
+  // steps suffices: a slot is rewritten (at step s + 1 for the data of step
+  // s + 4) only after the barrier that follows every read of its old data (at
+  // step s - 1), and a slot is read (at step s - 1) only after the barrier
+  // that follows its store (at step s - 3).
   double za[8], zb[8], zn[8], zm[8];
   auto lds_read = [&](int buf, double (&xa)[8], double (&xb)[8]) {
 #pragma unroll
@@ -334,36 +338,39 @@ __global__ __launch_bounds__(256) void corr_tile_kernel(Layout L, const long lon
     *reinterpret_cast<double2 *>(&sb[buf][lr1][lc]) = B1;                  \
   } while (0)
   // one step: FMAs on (za, zb) = step m while reading step m + 1 into
-  // (zn, zm) and storing step m + 2 from the given registers
-#define HPC_STEP(m, A0, A1, B0, B1)                                        \
+  // (zn, zm) and storing step m + 3 from the given registers
+#define HPC_STEP(m, A0, A1, B0, B1, SYNC)                                  \
   do {                                                                     \
-    lds_read(((m) + 1) % 3, zn, zm);                                       \
+    lds_read(((m) + 1) & 3, zn, zm);                                       \
     fmas();                                                                \
     __builtin_amdgcn_sched_barrier(0);                                     \
-    HPC_STORE(((m) + 2) % 3, A0, A1, B0, B1);                              \
-    __syncthreads();                                                       \
+    HPC_STORE(((m) + 3) & 3, A0, A1, B0, B1);                              \
+    if (SYNC) __syncthreads();                                             \
     _Pragma("unroll") for (int u_ = 0; u_ < 8; ++u_) {                     \
       za[u_] = zn[u_];                                                     \
       zb[u_] = zm[u_];                                                     \
     }                                                                      \
   } while (0)
   double2 pa0, pa1, pb0, pb1, qa0, qa1, qb0, qb1;
-  // steps 0 and 1 in flight together
+  // steps 0 and 1 in flight together, then step 2; step 3 in registers
   HPC_LOAD(0, pa0, pa1, pb0, pb1);
   HPC_LOAD(min(1, last), qa0, qa1, qb0, qb1);
   HPC_STORE(0, pa0, pa1, pb0, pb1);
   HPC_STORE(1, qa0, qa1, qb0, qb1);
   HPC_LOAD(min(2, last), pa0, pa1, pb0, pb1);
+  HPC_STORE(2, pa0, pa1, pb0, pb1);
+  HPC_LOAD(min(3, last), pa0, pa1, pb0, pb1);
   __syncthreads();
   lds_read(0, za, zb);
+  __syncthreads();  // slot 0 is rewritten at step 1
   int m = 0;
   for (; m + 1 < nsteps; m += 2) {
-    // the p registers hold step m + 2
-    HPC_LOAD(min(m + 3, last), qa0, qa1, qb0, qb1);
-    HPC_STEP(m, pa0, pa1, pb0, pb1);
-    // the q registers hold step m + 3
-    HPC_LOAD(min(m + 4, last), pa0, pa1, pb0, pb1);
-    HPC_STEP(m + 1, qa0, qa1, qb0, qb1);
+    // the p registers hold step m + 3
+    HPC_LOAD(min(m + 4, last), qa0, qa1, qb0, qb1);
+    HPC_STEP(m, pa0, pa1, pb0, pb1, false);
+    // the q registers hold step m + 4
+    HPC_LOAD(min(m + 5, last), pa0, pa1, pb0, pb1);
+    HPC_STEP(m + 1, qa0, qa1, qb0, qb1, true);
   }
   if (m < nsteps) fmas();  // odd step count: the last step
 #undef HPC_STEP
