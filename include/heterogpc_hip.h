@@ -115,12 +115,21 @@ enum {
   HPC_TUNE_RECHECK_QUEUE = 2,  /* HPC_CORR_F32: length of the exact-recheck
                                   queues (0 = 1 Mi); a level whose queue
                                   overflows is redone on exact correlations */
-  HPC_TUNE_KERNEL_TIMING = 3   /* 1 (default): hpc_run records device-time
+  HPC_TUNE_KERNEL_TIMING = 3,  /* 1 (default): hpc_run records device-time
                                   events around each level's kernels (the
                                   ms_compact / ms_test / ms_commit fields);
                                   0: no events (each marker between two
                                   kernels delays the second by a few
                                   microseconds) and those fields stay 0 */
+  HPC_TUNE_LDS_LIMIT = 4,      /* bytes of LDS per workgroup for staging a
+                                  row's neighbourhood in the CI-test kernels,
+                                  1..65536 (0 = 65536, the default).  A row
+                                  whose staging needs more is searched by a
+                                  gather kernel that reads the correlation
+                                  matrix; results do not depend on it */
+  HPC_TUNE_WAIT_LIMIT = 5      /* seconds the host waits for the device's
+                                  counts of a level before the call fails with
+                                  HPC_EHIP (0 = no limit; default 900) */
 };
 int hpc_set_tuning(hpc_state *st, int param, int value);
 
@@ -229,6 +238,12 @@ typedef struct hpc_batch hpc_batch;
 
 /* Device bytes one instance of (p, n, max_level) needs at most. */
 int hpc_footprint(uint64_t p, int n, int max_level, uint64_t *bytes);
+/* Whether instances with these p can share one batch or server step (host
+ * only): HPC_EINVAL for a p outside [2, 46340], HPC_ENOMEM when their ordered
+ * edges, the sum of p (p - 1), exceed 2^31 - 1 (one step indexes them with
+ * int32).  hpc_batch_add refuses an instance that breaks it; the server
+ * admits a request only while it holds. */
+int hpc_pack_check(const uint64_t *p, int count);
 int hpc_batch_create(int device, hpc_batch **out);
 int hpc_batch_destroy(hpc_batch *b);
 /* New instance on observations x ([p][n], as hpc_set_observations); max_level
@@ -250,6 +265,8 @@ int hpc_batch_get_adj(hpc_batch *b, int id, int32_t *adj);
 int hpc_batch_get_sepsets(hpc_batch *b, int id, int32_t *sep, int mc);
 int hpc_batch_get_sepset_list(hpc_batch *b, int id, int32_t *out, int64_t cap,
                               int64_t *words);
+/* HPC_TUNE_RANKS_PER_TASK, HPC_TUNE_LDS_LIMIT or HPC_TUNE_WAIT_LIMIT for
+ * every instance. */
 int hpc_batch_set_tuning(hpc_batch *b, int param, int value);
 /* hpc_apply_tuning for every instance of the batch. */
 int hpc_batch_apply_tuning(hpc_batch *b, const hpc_tune_entry *entries, int count);
@@ -311,9 +328,34 @@ int hpc_server_set_corr_mode(hpc_server *s, int mode);
 /* hpc_apply_tuning for the server's later steps (taken up by the worker
  * before its next step). */
 int hpc_server_apply_tuning(hpc_server *s, const hpc_tune_entry *entries, int count);
+/* HPC_TUNE_RANKS_PER_TASK, HPC_TUNE_LDS_LIMIT or HPC_TUNE_WAIT_LIMIT for the
+ * server's later steps (taken up by the worker before its next step). */
+int hpc_server_set_tuning(hpc_server *s, int param, int value);
 /* Drops a finished request's results. */
 int hpc_server_release(hpc_server *s, int64_t req);
 int hpc_server_stats(hpc_server *s, hpc_server_info *info);
+
+/* ---- fault injection (tests of failure isolation) -------------------------
+ * No counterpart in the reference.  The host side of the instance's (or the
+ * request's) given level fails with HPC_ENOMEM, as an allocation failure
+ * would, before any of its kernels are queued; device work is untouched.
+ * The instance stops with that error (hpc_batch_status, hpc_server_result)
+ * while the others run on.  level < 0 clears it. */
+int hpc_batch_inject_failure(hpc_batch *b, int id, int level);
+/* For a request not yet admitted, or one not yet submitted: request ids
+ * count up from 1 in submission order. */
+int hpc_server_inject_failure(hpc_server *s, int64_t req, int level);
+/* The search plan of one row (host only): *tasks receives the tasks a row of
+ * deg neighbours gets at `level` with ranks_per_task conditioning sets per
+ * task, deg * ceil(C(deg - 1, level) / ranks_per_task), from the same
+ * saturating binomial table and overflow-checked arithmetic the device
+ * plans with; HPC_ELEVEL when C(deg - 1, level) does not fit 64 bits or the
+ * count exceeds 2^46 (such a level fails with HPC_ELEVEL). */
+int hpc_plan_tasks(uint64_t deg, int level, int ranks_per_task, int64_t *tasks);
+/* The host wait for device counts on its own (host only): spins until *word
+ * equals value, failing with HPC_EHIP after limit_s seconds (<= 0: no
+ * limit), as a level's wait does under HPC_TUNE_WAIT_LIMIT. */
+int hpc_wait_word(const uint64_t *word, uint64_t value, double limit_s);
 
 #ifdef __cplusplus
 }

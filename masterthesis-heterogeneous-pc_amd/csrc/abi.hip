@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "hpc_internal.hpp"
+#include "pc_math.hpp"
 
 using namespace hpc;
 
@@ -50,6 +51,29 @@ extern "C" int hpc_device_synchronize(int device) {
 }
 
 extern "C" double hpc_fisher_r_crit(int n, double alpha) { return fisher_r_crit(n, alpha); }
+
+extern "C" int hpc_plan_tasks(uint64_t deg, int level, int ranks_per_task, int64_t *tasks) {
+  if (!tasks) return fail(HPC_EINVAL, "tasks is NULL");
+  if (deg < 1 || deg > (uint64_t)kMaxP) return fail(HPC_EINVAL, "row length %llu out of range",
+                                                    (unsigned long long)deg);
+  if (level < 2 || level > kMaxLevel) return fail(HPC_ELEVEL, "level %d outside 2..%d", level, kMaxLevel);
+  if (ranks_per_task < 64 || ranks_per_task % 64 != 0 || ranks_per_task > (1 << 24))
+    return fail(HPC_EINVAL, "ranks per task must be a multiple of 64 in [64, 2^24]");
+  const std::vector<unsigned long long> tab = binom_table((size_t)deg);
+  const long long t = row_task_count(binom(tab.data(), kBinomK, (int)deg - 1, level), (int)deg,
+                                     ranks_per_task);
+  if (t < 0)
+    return fail(HPC_ELEVEL, "level %d: a row of %llu neighbours is beyond what one search plans",
+                level, (unsigned long long)deg);
+  *tasks = (int64_t)t;
+  return HPC_OK;
+}
+
+extern "C" int hpc_wait_word(const uint64_t *word, uint64_t value, double limit_s) {
+  if (!word) return fail(HPC_EINVAL, "word is NULL");
+  return spin_wait_word(reinterpret_cast<const unsigned long long *>(word),
+                        (unsigned long long)value, limit_s);
+}
 
 // ---------------------------------------------------------------------------
 // synthetic SEM
@@ -138,8 +162,8 @@ extern "C" int hpc_state_create(uint64_t p64, int observations, double alpha, in
                                 int device, hpc_state **out) {
   if (!out) return fail(HPC_EINVAL, "out is NULL");
   *out = nullptr;
-  if (p64 < 2 || p64 > 46340)
-    return fail(HPC_EINVAL, "p = %llu outside [2, 46340]", (unsigned long long)p64);
+  if (p64 < 2 || p64 > (uint64_t)kMaxP)
+    return fail(HPC_EINVAL, "p = %llu outside [2, %d]", (unsigned long long)p64, kMaxP);
   if (observations < 1) return fail(HPC_EINVAL, "observations must be >= 1");
   hpc_state *st = new hpc_state();
   int rc = engine_init(st->eng, device);
@@ -182,6 +206,15 @@ extern "C" int hpc_set_tuning(hpc_state *st, int param, int value) {
     case HPC_TUNE_KERNEL_TIMING:
       if (value != 0 && value != 1) return fail(HPC_EINVAL, "kernel timing must be 0 or 1");
       st->eng.kernel_timing = value != 0;
+      return HPC_OK;
+    case HPC_TUNE_LDS_LIMIT:
+      if (value < 0 || (size_t)value > kSearchLdsLimit)
+        return fail(HPC_EINVAL, "LDS limit must be in [0, %zu] bytes", kSearchLdsLimit);
+      st->eng.lds_limit = (size_t)value;
+      return HPC_OK;
+    case HPC_TUNE_WAIT_LIMIT:
+      if (value < 0) return fail(HPC_EINVAL, "wait limit must be >= 0 seconds");
+      st->eng.wait_limit_s = (double)value;
       return HPC_OK;
     default:
       return fail(HPC_EINVAL, "unknown tuning parameter %d", param);

@@ -42,7 +42,24 @@ static constexpr size_t kSlabBytes = (size_t)512 << 20;
 
 extern "C" int hpc_footprint(uint64_t p, int n, int max_level, uint64_t *bytes) {
   if (!bytes || p < 2 || n < 1) return hpc_fail(HPC_EINVAL, "bad argument");
+  if (p > (uint64_t)kMaxP) return hpc_fail(HPC_EINVAL, "p = %llu outside [2, %d]",
+                                           (unsigned long long)p, kMaxP);
   *bytes = engine_footprint((int)p, n, max_level);
+  return HPC_OK;
+}
+
+extern "C" int hpc_pack_check(const uint64_t *p, int count) {
+  if (count < 0 || (count > 0 && !p)) return hpc_fail(HPC_EINVAL, "bad argument");
+  int64_t edges = 0;
+  for (int k = 0; k < count; ++k) {
+    if (p[k] < 2 || p[k] > (uint64_t)kMaxP)
+      return hpc_fail(HPC_EINVAL, "p = %llu outside [2, %d]", (unsigned long long)p[k], kMaxP);
+    edges += (int64_t)p[k] * (int64_t)(p[k] - 1);
+    if (edges > kMaxPackEdges)
+      return hpc_fail(HPC_ENOMEM,
+                      "%d instances hold more than %lld ordered edges, what one step indexes",
+                      k + 1, (long long)kMaxPackEdges);
+  }
   return HPC_OK;
 }
 
@@ -74,7 +91,16 @@ static int check_id(hpc_batch *b, int id) {
 extern "C" int hpc_batch_add(hpc_batch *b, uint64_t p, int n, double alpha, int max_level,
                              const double *x, int *id) {
   if (!b || !x || !id) return hpc_fail(HPC_EINVAL, "NULL argument");
-  int rc = engine_add(b->eng, (int)p, n, alpha, max_level, id);
+  if (p < 2 || p > (uint64_t)kMaxP)
+    return hpc_fail(HPC_EINVAL, "p = %llu outside [2, %d]", (unsigned long long)p, kMaxP);
+  // every unfinished instance is packed into one step, whose neighbour lists
+  // are indexed with int32 row pointers
+  std::vector<uint64_t> ps{p};
+  for (const InstHost &h : b->eng.inst)
+    if (h.used && !h.finished) ps.push_back((uint64_t)h.p);
+  int rc = hpc_pack_check(ps.data(), (int)ps.size());
+  if (rc) return rc;
+  rc = engine_add(b->eng, (int)p, n, alpha, max_level, id);
   if (rc) return rc;
   rc = engine_set_observations(b->eng, *id, x);
   if (rc) { engine_remove(b->eng, *id); return rc; }
@@ -181,11 +207,44 @@ extern "C" int hpc_batch_set_corr_mode(hpc_batch *b, int id, int mode) {
   return HPC_OK;
 }
 
+// The tuning knobs a batch or a server takes: checked, then applied.
+static int check_engine_tuning(int param, int value) {
+  switch (param) {
+    case HPC_TUNE_RANKS_PER_TASK:
+      if (value < 64 || value % 64 != 0 || value > (1 << 24))
+        return hpc_fail(HPC_EINVAL, "ranks per task must be a multiple of 64 in [64, 2^24]");
+      return HPC_OK;
+    case HPC_TUNE_LDS_LIMIT:
+      if (value < 0 || (size_t)value > kSearchLdsLimit)
+        return hpc_fail(HPC_EINVAL, "LDS limit must be in [0, %zu] bytes", kSearchLdsLimit);
+      return HPC_OK;
+    case HPC_TUNE_WAIT_LIMIT:
+      if (value < 0) return hpc_fail(HPC_EINVAL, "wait limit must be >= 0 seconds");
+      return HPC_OK;
+    default:
+      return hpc_fail(HPC_EINVAL, "tuning parameter %d does not apply here", param);
+  }
+}
+
+static int set_engine_tuning(Engine &e, int param, int value) {
+  const int rc = check_engine_tuning(param, value);
+  if (rc) return rc;
+  if (param == HPC_TUNE_RANKS_PER_TASK) e.ranks_per_task = value;
+  else if (param == HPC_TUNE_LDS_LIMIT) e.lds_limit = (size_t)value;
+  else e.wait_limit_s = (double)value;
+  return HPC_OK;
+}
+
 extern "C" int hpc_batch_set_tuning(hpc_batch *b, int param, int value) {
   if (!b) return hpc_fail(HPC_EINVAL, "batch is NULL");
-  if (param != HPC_TUNE_RANKS_PER_TASK || value < 64 || value % 64 != 0 || value > (1 << 24))
-    return hpc_fail(HPC_EINVAL, "bad tuning parameter");
-  b->eng.ranks_per_task = value;
+  const int rc = set_engine_tuning(b->eng, param, value);
+  return rc;
+}
+
+extern "C" int hpc_batch_inject_failure(hpc_batch *b, int id, int level) {
+  int rc = check_id(b, id);
+  if (rc) return rc;
+  b->eng.inst[id].inject_level = level < 0 ? -1 : level;
   return HPC_OK;
 }
 
@@ -222,6 +281,7 @@ struct Request {
   int mc = 0;
   int levels = 0;
   int64_t tests = 0;
+  int inject_level = -1;  // hpc_server_inject_failure
 };
 
 }  // namespace
@@ -229,6 +289,9 @@ struct Request {
 struct hpc_server {
   Engine eng;
   size_t budget = 0, in_use = 0;
+  // ordered edges p(p - 1) of the admitted requests: one step packs them all
+  // and indexes its neighbour lists with int32 row pointers
+  int64_t edges_in_use = 0;
   int max_active = 0;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -238,9 +301,11 @@ struct hpc_server {
   int64_t next_id = 1;
   int64_t submits = 0;
   int corr_mode = HPC_CORR_CANON;  // of requests admitted from now on
-  // launch shapes for the worker to apply before its next step
+  // launch shapes and knobs for the worker to apply before its next step
   std::vector<hpc_tune_entry> tune;
+  std::vector<std::pair<int, int>> knobs;
   bool tune_pending = false;
+  std::map<int64_t, int> inject;  // failures for requests not yet submitted
   bool stop = false;
   std::thread worker;
   // observation uploads: the uploader thread copies admitted requests' data
@@ -268,6 +333,7 @@ static int server_collect(hpc_server *s, Request &r) {
   for (auto &l : h.hist) r.tests += l.tests;
   r.mc = h.mc;
   int rc = h.error;
+  if (rc && r.message.empty() && !h.message.empty()) r.message = h.message;
   if (!rc) rc = fetch_adj_bits(s->eng, r.slot, r.bits);
   if (!rc) rc = fetch_records(s->eng, r.slot, r.rec);
   if (rc && r.message.empty()) r.message = hpc_last_error();
@@ -276,12 +342,15 @@ static int server_collect(hpc_server *s, Request &r) {
   return rc;
 }
 
+static int64_t request_edges(const Request &r) { return (int64_t)r.p * (r.p - 1); }
+
 static void fail_request(hpc_server *s, Request &r, int rc, const std::string &msg) {
   r.error = rc;
   r.message = msg;
   r.state = 3;
   r.t_done = Clock::now();
   s->in_use -= r.bytes;
+  s->edges_in_use -= request_edges(r);
 }
 
 static void uploader_loop(hpc_server *s) {
@@ -328,9 +397,11 @@ static void server_loop(hpc_server *s) {
         auto &r = s->queue.front();
         if (active >= s->max_active) break;
         if (used + r->bytes > s->budget && active > 0) break;
+        if (s->edges_in_use + request_edges(*r) > kMaxPackEdges && active > 0) break;
         used += r->bytes;
         ++active;
         s->in_use += r->bytes;
+        s->edges_in_use += request_edges(*r);
         admit.push_back(r);
         s->queue.pop_front();
       }
@@ -342,6 +413,7 @@ static void server_loop(hpc_server *s) {
       int rc = engine_add(s->eng, r->p, r->n, r->alpha, r->max_level, &slot);
       if (!rc) {
         s->eng.inst[slot].corr_mode = r->corr_mode;
+        s->eng.inst[slot].inject_level = r->inject_level;
         rc = engine_alloc_observations(s->eng, slot, &dst);
         if (rc) engine_remove(s->eng, slot);
       }
@@ -391,8 +463,12 @@ static void server_loop(hpc_server *s) {
     {
       std::lock_guard<std::mutex> lk(s->mu);
       if (s->tune_pending) {
-        // checked by hpc_server_apply_tuning
-        (void)engine_apply_tuning(s->eng, s->tune.data(), (int)s->tune.size());
+        // checked by hpc_server_apply_tuning and hpc_server_set_tuning
+        if (!s->tune.empty())
+          (void)engine_apply_tuning(s->eng, s->tune.data(), (int)s->tune.size());
+        for (const auto &kv : s->knobs) (void)set_engine_tuning(s->eng, kv.first, kv.second);
+        s->tune.clear();
+        s->knobs.clear();
         s->tune_pending = false;
       }
       for (auto &r : s->running) ids.push_back(r->slot);
@@ -433,6 +509,7 @@ static void server_loop(hpc_server *s) {
     for (size_t k = 0; k < done.size(); ++k) {
       Request &r = *done[k];
       s->in_use -= r.bytes;
+      s->edges_in_use -= request_edges(r);
       r.error = codes[k];
       r.t_done = now;
       r.state = codes[k] ? 3 : 2;
@@ -492,7 +569,7 @@ extern "C" int hpc_server_destroy(hpc_server *s) {
 static int server_submit(hpc_server *s, uint64_t p, int n, double alpha, int max_level,
                          const double *x, bool borrow, int64_t *req) {
   if (!s || !x || !req) return hpc_fail(HPC_EINVAL, "NULL argument");
-  if (p < 2 || p > 46340 || n < 1) return hpc_fail(HPC_EINVAL, "bad request shape");
+  if (p < 2 || p > (uint64_t)kMaxP || n < 1) return hpc_fail(HPC_EINVAL, "bad request shape");
   auto r = std::make_shared<Request>();
   r->p = (int)p;
   r->n = n;
@@ -513,6 +590,11 @@ static int server_submit(hpc_server *s, uint64_t p, int n, double alpha, int max
     if (r->bytes > s->budget)
       return hpc_fail(HPC_ENOMEM, "request needs %zu bytes, budget is %zu", r->bytes, s->budget);
     r->id = s->next_id++;
+    auto inj = s->inject.find(r->id);
+    if (inj != s->inject.end()) {
+      r->inject_level = inj->second;
+      s->inject.erase(inj);
+    }
     s->all[r->id] = r;
     s->queue.push_back(r);
     ++s->submits;
@@ -609,6 +691,31 @@ extern "C" int hpc_server_apply_tuning(hpc_server *s, const hpc_tune_entry *entr
   std::lock_guard<std::mutex> lk(s->mu);
   s->tune.insert(s->tune.end(), entries, entries + count);
   s->tune_pending = true;
+  return HPC_OK;
+}
+
+extern "C" int hpc_server_set_tuning(hpc_server *s, int param, int value) {
+  if (!s) return hpc_fail(HPC_EINVAL, "server is NULL");
+  const int rc = check_engine_tuning(param, value);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->knobs.push_back({param, value});
+  s->tune_pending = true;
+  return HPC_OK;
+}
+
+extern "C" int hpc_server_inject_failure(hpc_server *s, int64_t req, int level) {
+  if (!s) return hpc_fail(HPC_EINVAL, "server is NULL");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (req < s->next_id) {
+    auto it = s->all.find(req);
+    if (it == s->all.end()) return hpc_fail(HPC_EINVAL, "no request %lld", (long long)req);
+    if (it->second->state != 0)
+      return hpc_fail(HPC_ESTATE, "request %lld is already admitted", (long long)req);
+    it->second->inject_level = level < 0 ? -1 : level;
+  } else {
+    s->inject[req] = level < 0 ? -1 : level;  // taken up when it is submitted
+  }
   return HPC_OK;
 }
 

@@ -11,6 +11,7 @@
 #include <chrono>
 
 #include "hpc_internal.hpp"
+#include "pc_math.hpp"
 
 namespace hpc {
 
@@ -38,6 +39,10 @@ double fisher_r_crit(int n, double alpha) {
   double out;
   memcpy(&out, &a, 8);
   return out;
+}
+
+int spin_wait_word(const unsigned long long *word, unsigned long long value, double limit_s) {
+  return spin_wait(word, value, limit_s, []() -> int { return 0; });
 }
 
 static int dalloc(void **ptr, size_t bytes) {
@@ -208,20 +213,26 @@ void engine_free(Engine &e) {
   e.stream = e.copy_stream = nullptr;
 }
 
-// saturating C(m, k), k <= kMaxLevel, for m < rows
-static int ensure_binom(Engine &e, int rows) {
-  if ((size_t)rows <= e.cap_binom) return HPC_OK;
-  const size_t R = std::max<size_t>(rows, 64);
-  std::vector<unsigned long long> tab(R * kBinomK, 0ull);
-  for (size_t nn = 0; nn < R; ++nn) {
+std::vector<unsigned long long> binom_table(size_t rows) {
+  std::vector<unsigned long long> tab(rows * kBinomK, 0ull);
+  for (size_t nn = 0; nn < rows; ++nn) {
     tab[nn * kBinomK] = 1ull;
     for (int k = 1; k < kBinomK && (size_t)k <= nn; ++k) {
       unsigned long long a = tab[(nn - 1) * kBinomK + k - 1];
       unsigned long long b = ((size_t)k <= nn - 1) ? tab[(nn - 1) * kBinomK + k] : 0ull;
       unsigned long long s = a + b;
-      tab[nn * kBinomK + k] = (s < a) ? ~0ull : s;
+      // Pascal's rule; a saturated operand or a carry saturates the sum
+      tab[nn * kBinomK + k] = (s < a || a == ~0ull || b == ~0ull) ? ~0ull : s;
     }
   }
+  return tab;
+}
+
+// saturating C(m, k), k <= kMaxLevel, for m < rows
+static int ensure_binom(Engine &e, int rows) {
+  if ((size_t)rows <= e.cap_binom) return HPC_OK;
+  const size_t R = std::max<size_t>(rows, 64);
+  std::vector<unsigned long long> tab = binom_table(R);
   dfree(e.d_binom);
   int rc = dalloc((void **)&e.d_binom, tab.size() * sizeof(tab[0]));
   if (rc) return rc;
@@ -238,7 +249,7 @@ static int initial_mc(int max_level) {
 }
 
 int engine_add(Engine &e, int p, int n, double alpha, int max_level, int *id) {
-  if (p < 2 || p > 46340) return hpc_fail(HPC_EINVAL, "p = %d outside [2, 46340]", p);
+  if (p < 2 || p > kMaxP) return hpc_fail(HPC_EINVAL, "p = %d outside [2, %d]", p, kMaxP);
   if (n < 1) return hpc_fail(HPC_EINVAL, "observations must be >= 1");
   HPC_HIPCHK(hipSetDevice(e.device));
   int slot = -1;
@@ -348,6 +359,15 @@ int engine_set_corr(Engine &e, int id, const double *h_cor) {
 void engine_note_level(Engine &e, int id, int level) {
   InstHost &h = e.inst[id];
   if (level > h.mc) h.mc = std::min(level, kMaxLevel);
+}
+
+// Host-side preparation of an instance's level: room for its sepset records
+// (ensure_rec), and the failure a test asked for (hpc_batch_inject_failure).
+static int ensure_rec(Engine &e, InstHost &h, int level);
+static int prepare_level(Engine &e, InstHost &h, int level) {
+  if (h.inject_level >= 0 && h.inject_level == level)
+    return hpc_fail(HPC_ENOMEM, "injected host failure at level %d", level);
+  return level >= 1 ? ensure_rec(e, h, level) : HPC_OK;
 }
 
 // Room for the worst case of one level: every remaining edge removed, each
@@ -504,6 +524,9 @@ int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &
     pmax = std::max(pmax, e.inst[id].p);
     edges += (size_t)e.inst[id].p * (e.inst[id].p - 1);
   }
+  if (edges > (size_t)kMaxPackEdges)
+    return hpc_fail(HPC_ENOMEM, "%zu ordered edges in one step, more than int32 row pointers index",
+                    edges);
   int rc = ensure_inst_cap(e, (size_t)K);
   if (!rc) rc = ensure_row_cap(e, (size_t)p_total + 1);
   if (!rc) rc = ensure_edge_cap(e, edges + 1);
@@ -592,20 +615,20 @@ static float elapsed(hipEvent_t a, hipEvent_t b) {
 // synchronisation.  The spin asks the stream every few hundred polls, so a
 // failed launch or a stream error ends it.
 // Waits until the publication numbered `seq` has arrived in host memory.
+// A publication that has not arrived after the engine's wait limit ends the
+// wait with HPC_EHIP (a kernel that never completes would otherwise hold the
+// caller, and a server's worker, for good).
 static int wait_published(Engine &e, unsigned long long seq) {
-  for (unsigned it = 1;; ++it) {
-    if (__atomic_load_n(e.h_seq, __ATOMIC_ACQUIRE) == seq) break;
-    if ((it & 255u) == 0) {
-      const hipError_t q = hipStreamQuery(e.stream);
-      if (q == hipSuccess) {
-        if (__atomic_load_n(e.h_seq, __ATOMIC_ACQUIRE) == seq) break;
-        return hpc_fail(HPC_EHIP, "counter publication %llu did not arrive", seq);
-      }
-      if (q != hipErrorNotReady)
-        return hpc_fail(HPC_EHIP, "HIP error: %s", hipGetErrorString(q));
-    }
-    __builtin_ia32_pause();
-  }
+  const hipStream_t stream = e.stream;
+  const int rc = spin_wait(e.h_seq, seq, e.wait_limit_s, [stream, seq]() -> int {
+    const hipError_t q = hipStreamQuery(stream);
+    if (q == hipSuccess) return 1;  // idle: the word is final
+    if (q != hipErrorNotReady)
+      return hpc_fail(HPC_EHIP, "HIP error while waiting for publication %llu: %s", seq,
+                      hipGetErrorString(q));
+    return 0;
+  });
+  if (rc) return rc;
   e.host_upload_pending = false;  // everything before the publication is done
   return HPC_OK;
 }
@@ -764,6 +787,13 @@ int engine_compact(Engine &e, float *ms) {
 int engine_search(Engine &e, int level, const int32_t *rows, int nrows, float *ms) {
   HPC_HIPCHK(hipSetDevice(e.device));
   e.rechecks = 0;
+  for (size_t k = 0; k < e.packed.size(); ++k) {
+    const InstHost &h = e.inst[e.packed[k]];
+    if (e.h_inst[k].level == level && !engine_plannable(e, h.max_deg, level))
+      return hpc_fail(HPC_ELEVEL,
+                      "level %d: C(%d, %d) conditioning sets per edge is beyond what one search "
+                      "plans", level, h.max_deg - 1, level);
+  }
   Layout L = layout(e);
   const int32_t *mask = nullptr;
   if (rows) {
@@ -787,10 +817,11 @@ int engine_search(Engine &e, int level, const int32_t *rows, int nrows, float *m
   HPC_HIPCHK(hipEventRecord(e.ev[5], e.stream));
   const bool fast = any && any_fast_at(e, level);
   if (any) {
-    if (level == 1) launch_search_l1(e.stream, L, e.n_edges, mask, fast, engine_epb(e));
+    if (level == 1) launch_search_l1(e.stream, L, e.n_edges, mask, fast, engine_epb(e),
+                                     search_lds_limit(e.lds_limit));
     else
       launch_search_ln(e.stream, level, L, engine_rpt(e, level), e.d_row_tasks, e.d_task_off, fast,
-                       engine_tpb(e, level));
+                       engine_tpb(e, level), search_lds_limit(e.lds_limit));
     if (fast) launch_recheck(e.stream, level, L);
   }
   HPC_HIPCHK(hipGetLastError());
@@ -1055,8 +1086,12 @@ int engine_run_single(Engine &e, int id) {
         h.finished = true;  // nothing left to test: no record (as engine_step)
         return HPC_OK;
       }
+      if (testable && !engine_plannable(e, h.max_deg, l))
+        return hpc_fail(HPC_ELEVEL,
+                        "level %d: C(%d, %d) conditioning sets per edge is beyond what one search "
+                        "plans", l, h.max_deg - 1, l);
       // record space, the NaN flag of level 1 on: the table when it changed
-      rc = ensure_rec(e, h, l);
+      rc = prepare_level(e, h, l);
       if (rc) return rc;
       Inst &I = e.h_inst[0];
       const int flags = kActive | (h.cor_nan != 0 ? kCorNaN : 0);
@@ -1074,10 +1109,11 @@ int engine_run_single(Engine &e, int id) {
       if (!searched) HPC_HIPCHK(mark(e.lev_ev[l][4]));
       if (e.n_edges > 0 && !searched) {
         if (l == 1)
-          launch_search_l1(e.stream, L, e.n_edges, nullptr, h.fast, engine_epb(e));
+          launch_search_l1(e.stream, L, e.n_edges, nullptr, h.fast, engine_epb(e),
+                           search_lds_limit(e.lds_limit));
         else
           launch_search_ln(e.stream, l, L, engine_rpt(e, l), e.d_row_tasks, e.d_task_off, h.fast,
-                           engine_tpb(e, l));
+                           engine_tpb(e, l), search_lds_limit(e.lds_limit));
         // exact re-runs of the tests near the threshold (fast correlations);
         // the removal below clears the queue with the other counters
         if (h.fast) launch_recheck(e.stream, l, L, /*clear_queue=*/false);
@@ -1124,7 +1160,7 @@ int engine_run_single(Engine &e, int id) {
         if (!h.fast && h.max_level >= 0) {
           HPC_HIPCHK(mark(e.lev_ev[l + 1][4]));
           launch_search_ln(e.stream, l + 1, L, engine_rpt(e, l + 1), e.d_row_tasks, e.d_task_off,
-                           false, engine_tpb(e, l + 1));
+                           false, engine_tpb(e, l + 1), search_lds_limit(e.lds_limit));
           HPC_HIPCHK(hipGetLastError());
           HPC_HIPCHK(mark(e.lev_ev[l + 1][5]));
           searched = true;
@@ -1180,11 +1216,25 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
     need.push_back(id);
     zero.push_back(0);
   }
+  // an instance whose host-side preparation fails (an allocation) stops
+  // alone with its error; the others go on
+  auto drop = [&](int id, int code) {
+    InstHost &h = e.inst[id];
+    h.error = code;
+    h.message = hpc_last_error();
+    h.finished = true;
+  };
   if (!need.empty()) {
+    std::vector<int> ok;
     for (int id : need) {
       rc = engine_prepare_corr(e, id);
-      if (rc) return rc;
+      if (rc) drop(id, rc);
+      else ok.push_back(id);
     }
+    need.swap(ok);
+    zero.assign(need.size(), 0);
+  }
+  if (!need.empty()) {
     rc = engine_pack(e, need, zero);
     if (!rc) rc = engine_corr(e, nullptr, nullptr);
     if (rc) return rc;
@@ -1200,11 +1250,19 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
   auto t0 = std::chrono::steady_clock::now();
   // room for each level's sepset records before the table is packed (the
   // worst case depends only on the edges left, known from the last step)
-  for (size_t k = 0; k < act.size(); ++k) {
-    if (lev[k] < 1 || lev[k] > kMaxLevel) continue;
-    rc = ensure_rec(e, e.inst[act[k]], lev[k]);
-    if (rc) return rc;
+  for (size_t k = 0; k < act.size();) {
+    if (lev[k] >= 0 && lev[k] <= kMaxLevel) {
+      rc = prepare_level(e, e.inst[act[k]], lev[k]);
+      if (rc) {
+        drop(act[k], rc);
+        act.erase(act.begin() + (long)k);
+        lev.erase(lev.begin() + (long)k);
+        continue;
+      }
+    }
+    ++k;
   }
+  if (act.empty()) return HPC_OK;
   rc = engine_pack(e, act, lev, /*zero_ctr=*/true);
   if (rc) return rc;
   // observations and z-scores are only read by the correlation, which the
@@ -1271,9 +1329,14 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
     h.compacted = true;
     h.ever_compacted = true;
     const bool testable = h.max_deg - 1 >= lev[k];
-    const bool beyond = lev[k] > kMaxLevel;
+    // beyond the kernels' levels, or more conditioning sets than a plan holds
+    const bool beyond = lev[k] > kMaxLevel || (testable && !engine_plannable(e, h.max_deg, lev[k]));
     if ((!testable && h.max_level < 0) || beyond) {
-      if (testable) h.error = HPC_ELEVEL;
+      if (testable) {
+        h.error = HPC_ELEVEL;
+        h.message = "level " + std::to_string(lev[k]) + ": C(" + std::to_string(h.max_deg - 1) +
+                    ", level) conditioning sets per edge is beyond what one search plans";
+      }
       skip[k] = (testable || h.max_level < 0) ? 1 : 2;
       e.h_inst[k].level = -1;
       retable = true;
@@ -1301,7 +1364,8 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
     if (run1) {
       HPC_HIPCHK(hipEventRecord(e.sev[2 * npairs], e.stream));
       const bool f1 = any_fast_at(e, 1);
-      launch_search_l1(e.stream, L, e.n_edges, nullptr, f1, engine_epb(e));
+      launch_search_l1(e.stream, L, e.n_edges, nullptr, f1, engine_epb(e),
+                       search_lds_limit(e.lds_limit));
       if (f1) launch_recheck(e.stream, 1, L);
       HPC_HIPCHK(hipEventRecord(e.sev[2 * npairs + 1], e.stream));
       ++npairs;
@@ -1313,7 +1377,7 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
       HPC_HIPCHK(hipEventRecord(e.sev[2 * npairs], e.stream));
       const bool fl = any_fast_at(e, l);
       launch_search_ln(e.stream, l, L, engine_rpt(e, l), e.d_row_tasks, e.d_task_off, fl,
-                       engine_tpb(e, l));
+                       engine_tpb(e, l), search_lds_limit(e.lds_limit));
       if (fl) launch_recheck(e.stream, l, L);
       HPC_HIPCHK(hipEventRecord(e.sev[2 * npairs + 1], e.stream));
       ++npairs;
@@ -1403,6 +1467,14 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
       h.finished = true;
   }
   return HPC_OK;
+}
+
+bool engine_plannable(const Engine &e, int max_deg, int level) {
+  if (level < 2 || max_deg - 1 < level) return true;
+  const size_t row = (size_t)(max_deg - 1);
+  if (row >= e.cap_binom) return false;
+  const unsigned long long total = e.h_binom[row * kBinomK + level];
+  return row_task_count(total, max_deg, engine_rpt(e, level)) >= 0;
 }
 
 int engine_rpt(const Engine &e, int level) {

@@ -373,8 +373,8 @@ __global__ __launch_bounds__(64 * kCompWaves) void compact_count_kernel(
 #pragma unroll
       for (int sh = 32; sh > 0; sh >>= 1) c += __shfl_xor(c, sh, 64);
       if (onq[rr] && level >= 2 && L.inst[k].level == level && c > 0) {
-        const unsigned long long total = binom(L.binom, kBinomK, c - 1, level);
-        if (total > 0) tk = (long long)c * (long long)((total + rpt - 1) / rpt);
+        // a row that cannot be planned gets no tasks; the host fails its level
+        tk = max(0ll, row_task_count(binom(L.binom, kBinomK, c - 1, level), c, rpt));
       }
     }
     if (lane == 0) {

@@ -171,6 +171,11 @@ extern "C" int hpc_search_level_host(hpc_state *st, int level, const int32_t *ro
   }
   std::stable_sort(order.begin(), order.end(),
                    [](const std::pair<int, int> &a, const std::pair<int, int> &b) { return a.first > b.first; });
+  // the longest row's conditioning sets per edge (saturated: beyond any run)
+  if (!order.empty() && level >= 2 && order[0].first - 1 >= level &&
+      binom(snap.binom, kBinomK, order[0].first - 1, level) == ~0ull)
+    return hpc_fail(HPC_ELEVEL, "level %d: C(%d, %d) conditioning sets per edge saturates",
+                    level, order[0].first - 1, level);
   std::vector<std::pair<int, int>> items;
   for (auto &o : order)
     for (int y = 0; y < o.first; ++y) items.push_back({o.second, y});

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <chrono>
 #include <map>
 #include <string>
 #include <utility>
@@ -20,12 +21,26 @@
 
 #include "../../include/heterogpc_hip.h"
 
+// Error reporting shared by the ABI translation units.
+int hpc_fail(int code, const char *fmt, ...);
+
 namespace hpc {
 
 // Bound on |fast correlation - exact correlation| (HPC_CORR_F32): 64-sample
 // FP32 accumulation (64 + 8 roundings of 2^-24), the float rounding of both
 // operands, and 1% for the float-rounded standard deviation.
 constexpr double kFastEps = 74.0 * 5.9604644775390625e-8 * 1.01;
+
+// Largest p of an instance: p (p - 1) ordered edges still fit int32
+// (the reference's level-0 index caps p the same way, gpuRowTests.cu:12).
+constexpr int kMaxP = 46340;
+// Ordered edges of all instances packed into one step: the neighbour lists
+// are indexed with int32 row pointers.
+constexpr int64_t kMaxPackEdges = 2147483647;
+
+// Default longest host wait for a publication of device counts, in seconds
+// (HPC_TUNE_WAIT_LIMIT).
+constexpr double kWaitLimitS = 900.0;
 
 // Largest level the device kernels accept (conditioning-set size).
 constexpr int kMaxLevel = 16;
@@ -201,15 +216,23 @@ struct NextCount {
 // `fast`: the launch holds an instance with fast correlations (selects the
 // variant with uncertainty bounds and the recheck queue)
 // epb: ordered edges per workgroup of the LDS-staged form
+// lds_limit: LDS bytes per workgroup for staging a row (search_lds_limit);
+// a row whose staging needs more is searched by a gather kernel, launched
+// after the staged one only when such a row exists
 void launch_search_l1(hipStream_t s, const Layout &L, int64_t n_edges,
-                      const int32_t *row_mask, bool fast, int epb);
+                      const int32_t *row_mask, bool fast, int epb, size_t lds_limit);
+// Default (and largest) LDS bytes per workgroup for staging a row.
+constexpr size_t kSearchLdsLimit = 64 * 1024;
+// The staging limit in effect for a requested one (0 = the default).
+size_t search_lds_limit(size_t requested);
 void launch_task_plan(hipStream_t s, int level, const Layout &L, int rpt,
                       const int32_t *row_mask, long long *row_tasks, long long *task_off);
 // The plan's per-row task counts and exclusive offsets define the tasks; the
 // kernels read the task count from the device counters.
 // tpb: tasks per workgroup of the LDS-staged form
 void launch_search_ln(hipStream_t s, int level, const Layout &L, int rpt,
-                      const long long *row_tasks, const long long *task_off, bool fast, int tpb);
+                      const long long *row_tasks, const long long *task_off, bool fast, int tpb,
+                      size_t lds_limit);
 // With pub.flag set, the last workgroup to finish publishes the counters.
 void launch_commit(hipStream_t s, const Layout &L, int64_t n_edges, const Publish &pub,
                    const NextCount &next = NextCount{nullptr, nullptr, 0, 0, 0});
@@ -249,6 +272,8 @@ struct InstHost {
   int64_t undirected = 0;
   size_t bytes = 0;        // device footprint
   int error = 0;           // first error of a batched run (HPC_E*), 0 = none
+  std::string message;     // its description
+  int inject_level = -1;   // test hook: the host side of this level fails
   std::vector<hpc_level_stats> hist;  // one record per level run
 };
 
@@ -341,11 +366,44 @@ struct Engine {
   Recheck *d_rchk = nullptr;
   size_t cap_rchk = 0;
   size_t rchk_limit = 0;  // queue length in use (0 = capacity); a tuning knob
+  size_t lds_limit = 0;   // LDS bytes for staging a search row (0 = default); a tuning knob
   bool kernel_timing = true;  // level-loop device-time events (HPC_TUNE_KERNEL_TIMING)
+  double wait_limit_s = kWaitLimitS;  // longest wait for a publication (HPC_TUNE_WAIT_LIMIT)
   unsigned long long rechecks = 0;  // exact rechecks of the last step or level
 };
 
 double fisher_r_crit(int n, double alpha);
+// Saturating C(m, k) for m < rows, k <= kMaxLevel (row-major, kBinomK
+// columns): 2^64 - 1 stands for every value that does not fit.
+std::vector<unsigned long long> binom_table(size_t rows);
+
+// Spins until *word == value (acquire loads).  Every 256 spins it calls
+// poll(): a negative return ends the wait with that error, 1 says that the
+// word will not change any more (one last look decides), 0 goes on.  After
+// limit_s seconds (<= 0: no limit) the wait ends with HPC_EHIP.
+template <class Poll>
+int spin_wait(const unsigned long long *word, unsigned long long value, double limit_s, Poll poll) {
+  const auto t0 = std::chrono::steady_clock::now();
+  for (unsigned it = 1;; ++it) {
+    if (__atomic_load_n(word, __ATOMIC_ACQUIRE) == value) return HPC_OK;
+    if ((it & 255u) == 0) {
+      const int q = poll();
+      if (q < 0) return q;
+      if (q > 0) {
+        if (__atomic_load_n(word, __ATOMIC_ACQUIRE) == value) return HPC_OK;
+        return hpc_fail(HPC_EHIP, "publication %llu did not arrive (the stream is idle)", value);
+      }
+      if (limit_s > 0.0 &&
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > limit_s)
+        return hpc_fail(HPC_EHIP,
+                        "publication %llu did not arrive within %.3f s (HPC_TUNE_WAIT_LIMIT)",
+                        value, limit_s);
+    }
+    __builtin_ia32_pause();
+  }
+}
+// spin_wait with nothing to poll (host words only).
+int spin_wait_word(const unsigned long long *word, unsigned long long value, double limit_s);
 int engine_init(Engine &e, int device);
 void engine_free(Engine &e);
 // New instance with device blocks and the initial adjacency; returns its id.
@@ -387,6 +445,12 @@ void engine_note_level(Engine &e, int id, int level);
 // host waits for published counters for the whole set.  Finished instances
 // are marked; their per-level records go to InstHost::hist.
 int engine_step(Engine &e, const std::vector<int> &ids);
+// Whether a level can be planned for an instance whose longest row has
+// max_deg neighbours: C(max_deg - 1, level) did not saturate and the row's
+// task count stays within kMaxRowTasks (row_task_count; monotone in the row
+// length).  Kernels give rows that cannot be planned no tasks; the host fails
+// the level with HPC_ELEVEL.
+bool engine_plannable(const Engine &e, int max_deg, int level);
 // Launch shapes of a level's CI-test kernel.
 int engine_rpt(const Engine &e, int level);
 int engine_tpb(const Engine &e, int level);

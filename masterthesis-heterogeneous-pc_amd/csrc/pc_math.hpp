@@ -307,6 +307,25 @@ __host__ __device__ __forceinline__ unsigned long long binom(const unsigned long
   return tab[(size_t)nn * kcols + l];
 }
 
+// Largest task count of one row.  A row's tasks, and their prefix sums over
+// up to 46340 rows, then stay far below 2^63; a row beyond it would hold more
+// than 2^46 tasks of at least 64 tests, more than any run completes.
+constexpr long long kMaxRowTasks = 1ll << 46;
+
+// Search tasks of a row of length deg at a level with `total` = C(deg - 1,
+// level) conditioning sets per ordered edge (saturated at 2^64 - 1 by the
+// binomial table) and rpt ranks per task: deg * ceil(total / rpt).  -1 when
+// total saturated or the count exceeds kMaxRowTasks: the level cannot be
+// planned.  Monotone in deg, so the longest row decides for all.
+__host__ __device__ __forceinline__ long long row_task_count(unsigned long long total, int deg,
+                                                             int rpt) {
+  if (total == 0ull || deg <= 0) return 0;
+  if (total == ~0ull) return -1;
+  const unsigned long long per = (total - 1ull) / (unsigned long long)rpt + 1ull;
+  if (per > (unsigned long long)kMaxRowTasks / (unsigned long long)deg) return -1;
+  return (long long)deg * (long long)per;
+}
+
 // ithCombination (cuda_util.cu:34-50), 0-based: positions q[0] < ... < q[L-1]
 // of the rank-th L-subset of {0..nn-1} in lexicographic order.
 //

@@ -25,6 +25,8 @@
 // indexed with local ids (global - start).
 #pragma clang fp contract(off)
 
+#include <algorithm>
+
 #include "hpc_internal.hpp"
 #include "pc_math.hpp"
 
@@ -105,51 +107,68 @@ __device__ __forceinline__ void queue_uncertain(const Layout &L, bool unc,
   }
 }
 
-// Level 1 (testRowL1, gpuRowTests.cu:29-100; cpuRowTests.cpp:23-56).
+// Level 1 (testRowL1, gpuRowTests.cu:29-100; cpuRowTests.cpp:23-56) for the
+// rows longer than `cap` neighbours, whose ids and correlations do not fit
+// the LDS-staged kernel below: the operands are gathered from the
+// correlation matrix.  Workgroups take blocks of `epb` consecutive ordered
+// edges (grid-stride); a block's segment on a row that the staged kernel
+// takes is skipped whole, and the four wavefronts share the edges of a long
+// row's segment, one edge at a time.  Same tests, order and arithmetic as
+// the staged kernel.
 template <bool FAST>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_kernel(
-    Layout L, long long n_edges, const int32_t *__restrict__ row_mask) {
-  const int lane = threadIdx.x & 63;
-  const long long wave0 = (long long)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-  const long long nw = (long long)gridDim.x * kWavesPerBlock;
-  for (long long e = wave0; e < n_edges; e += nw) {
-    const int i = L.erow[e];
-    if (row_mask && !row_mask[i]) continue;
-    const Inst &In = L.inst[L.row_inst[i]];
-    if (In.level != 1) continue;
-    const int base = L.rowptr[i];
-    const int nn = L.rowptr[i + 1] - base - 1;
-    const int y = (int)(e - base);
-    if (nn < 1) continue;
-    const int p = In.p, s0 = In.start;
-    const double *cor = In.cor;
-    const double r_crit = In.r_crit;
-    const bool fast = FAST && In.fast;
-    const int li = i - s0, lj = L.nbr[e] - s0;
-    const double x1 = cor[(size_t)li * p + lj];
-    long long found = -1;
-    for (int q0 = 0; q0 < nn; q0 += 64) {
-      const int q = q0 + lane;
-      bool pass = false, unc = false;
-      if (q < nn) {
-        const int lk = L.nbr[base + (q < y ? q : q + 1)] - s0;
-        const double x2 = cor[(size_t)li * p + lk];
-        const double x3 = cor[(size_t)lj * p + lk];
-        if (!(__builtin_isnan(x1) || __builtin_isnan(x2) || __builtin_isnan(x3))) {
-          const double r = pcor_l1(x1, x2, x3);
-          pass = removes(r, r_crit);
-          if (fast) {
-            const double xa[1] = {x2}, ya[1] = {x3}, Cz[1][1] = {{1.0}};
-            unc = !fast_certain(r, r_crit, pcor_sensitivity<1>(x1, xa, ya, Cz), In.eps);
-            pass &= !unc;
+    Layout L, long long n_edges, const int32_t *__restrict__ row_mask, int cap, int epb) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (long long b0 = (long long)blockIdx.x * epb; b0 < n_edges;
+       b0 += (long long)gridDim.x * epb) {
+    const long long b1 = min(n_edges, b0 + epb);
+    long long e0 = b0;
+    while (e0 < b1) {
+      const int i = L.erow[e0];
+      const int base = L.rowptr[i];
+      const int deg = L.rowptr[i + 1] - base;
+      const long long seg_end = min(b1, (long long)base + deg);
+      const Inst &In = L.inst[L.row_inst[i]];
+      // uniform across the workgroup
+      if (deg > cap && In.level == 1 && (!row_mask || row_mask[i])) {
+        const int nn = deg - 1;
+        const int p = In.p, s0 = In.start;
+        const double *cor = In.cor;
+        const double r_crit = In.r_crit;
+        const bool fast = FAST && In.fast;
+        const bool chk = In.flags & kCorNaN;  // inputs may be NaN
+        const int li = i - s0;
+        for (long long e = e0 + wave; e < seg_end; e += kWavesPerBlock) {
+          const int y = (int)(e - base);
+          const int lj = L.nbr[e] - s0;
+          const double x1 = cor[(size_t)li * p + lj];
+          long long found = -1;
+          for (int q0 = 0; q0 < nn; q0 += 64) {
+            const int q = q0 + lane;
+            bool pass = false, unc = false;
+            if (q < nn) {
+              const int lk = L.nbr[base + (q < y ? q : q + 1)] - s0;
+              const double x2 = cor[(size_t)li * p + lk];
+              const double x3 = cor[(size_t)lj * p + lk];
+              if (!(chk && (__builtin_isnan(x1) || __builtin_isnan(x2) || __builtin_isnan(x3)))) {
+                const double r = pcor_l1(x1, x2, x3);
+                pass = removes(r, r_crit);
+                if (fast) {
+                  const double xa[1] = {x2}, ya[1] = {x3}, Cz[1][1] = {{1.0}};
+                  unc = !fast_certain(r, r_crit, pcor_sensitivity<1>(x1, xa, ya, Cz), In.eps);
+                  pass &= !unc;
+                }
+              }
+            }
+            const unsigned long long m = __ballot(pass);
+            if (fast) queue_uncertain(L, unc, m, e, (unsigned long long)q0, lane);
+            if (m) { found = q0 + __ffsll((long long)m) - 1; break; }
           }
+          if (lane == 0) L.fp[e] = found;
         }
       }
-      const unsigned long long m = __ballot(pass);
-      if (fast) queue_uncertain(L, unc, m, e, (unsigned long long)q0, lane);
-      if (m) { found = q0 + __ffsll((long long)m) - 1; break; }
+      e0 = seg_end;
     }
-    if (lane == 0) L.fp[e] = found;
   }
 }
 
@@ -165,13 +184,16 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_kernel(
 // hold, instead of from all of them.
 constexpr int kL1Classes = 8;
 
+//
+// Rows longer than `cap` neighbours (the staged capacity) are left to
+// search_l1_kernel.
 template <bool FAST>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_lds_kernel(
-    Layout L, long long n_edges, const int32_t *__restrict__ row_mask, int epb) {
+    Layout L, long long n_edges, const int32_t *__restrict__ row_mask, int epb, int cap) {
   extern __shared__ unsigned long long smem[];
-  double *Mi = reinterpret_cast<double *>(smem);                 // max_deg
-  int *ids = reinterpret_cast<int *>(Mi + L.max_deg);             // max_deg
-  int *sel = ids + L.max_deg;                                     // max_deg: the class's edges
+  double *Mi = reinterpret_cast<double *>(smem);                 // cap
+  int *ids = reinterpret_cast<int *>(Mi + cap);                   // cap
+  int *sel = ids + cap;                                           // cap: the class's edges
   __shared__ int s_nsel;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int cls = blockIdx.x % kL1Classes;
@@ -185,7 +207,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_lds_kernel(
     const long long seg_end = min(e1, (long long)base + deg);
     const Inst &In = L.inst[L.row_inst[i]];
     // uniform across the workgroup: the row is skipped or staged by all
-    if (In.level == 1 && (!row_mask || row_mask[i]) && deg >= 2) {
+    if (In.level == 1 && (!row_mask || row_mask[i]) && deg >= 2 && deg <= cap) {
       const int p = In.p, s0 = In.start, li = i - s0;
       __syncthreads();  // the previous row is done with the staged values
       for (int a = tid; a < deg; a += blockDim.x) {
@@ -287,14 +309,19 @@ __device__ __forceinline__ int task_row(const long long *__restrict__ task_off, 
 }
 
 // Level L >= 2 (testRowLN<>, gpuRowTests.cu:102-205; the general-level form of
-// cpuWorkstealingTests.cpp:39-112), gathering operands from the correlation
-// matrix (rows too long for the LDS form).  Each wavefront takes tasks
-// t = wave, wave + waves, ... < n_tasks (read on the device); task t covers
-// ranks [chunk * rpt, (chunk + 1) * rpt) of its edge.
+// cpuWorkstealingTests.cpp:39-112) for the rows whose neighbourhood block
+// does not fit the LDS-staged kernel below (row length + 1 > dmin), gathering
+// the operands from the correlation matrix.  Workgroups take blocks of tpb
+// consecutive tasks (grid-stride; the task count is read on the device); a
+// block's segment on a row that the staged kernel takes is skipped whole,
+// and the four wavefronts share the tasks of a long row's segment.  Task t
+// covers ranks [chunk * rpt, (chunk + 1) * rpt) of its edge, 64 at a time,
+// one rank per lane.  Same ranks, arithmetic and orientation of every
+// correlation read as the staged kernel.
 template <int LV, bool FAST>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_kernel(
     Layout L, int rpt, const long long *__restrict__ task_off,
-    const long long *__restrict__ row_tasks, int lds_binom) {
+    const long long *__restrict__ row_tasks, int tpb, int dmin, int lds_binom) {
   // C(m, k) for m <= max_deg, k <= LV staged in LDS (columns 0..LV)
   extern __shared__ unsigned long long s_binom[];
   const unsigned long long *tab = L.binom;
@@ -310,75 +337,95 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_kernel(
     kc = LV + 1;
   }
   const long long n_tasks = (long long)L.ctr->n_tasks;
-  const int lane = threadIdx.x & 63;
-  const long long wave0 = (long long)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-  const long long nw = (long long)gridDim.x * kWavesPerBlock;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   unsigned long long F = 0;
   unsigned long long *fpu = reinterpret_cast<unsigned long long *>(L.fp);
-  for (long long t = wave0; t < n_tasks; t += nw) {
-    const int i = task_row(task_off, L.p_total, t);
-    const int base = L.rowptr[i];
-    const int deg = L.rowptr[i + 1] - base;
-    const long long local = t - task_off[i];
-    const long long nch = row_tasks[i] / deg;
-    const long long e = base + local / nch;
-    const unsigned long long r_begin = (unsigned long long)(local % nch) * (unsigned long long)rpt;
-    // a lower rank of this edge already passed: nothing here can win
-    if (__atomic_load_n(&fpu[e], __ATOMIC_RELAXED) < r_begin) continue;
-    const Inst &In = L.inst[L.row_inst[i]];
-    const int p = In.p, s0 = In.start;
-    const double *cor = In.cor;
-    const double r_crit = In.r_crit;
-    const bool fast = FAST && In.fast;
-    const int nn = deg - 1;
-    const int y = (int)(e - base);
-    const unsigned long long total = binom(tab, kc, nn, LV);
-    const unsigned long long r_end = min(total, r_begin + (unsigned long long)rpt);
-    const int li = i - s0, lj = L.nbr[e] - s0;
-    const double a = cor[(size_t)li * p + lj];
-    for (unsigned long long r0 = r_begin; r0 < r_end; r0 += 64) {
-      const unsigned long long rank = r0 + lane;
-      bool pass = false, unc = false;
-      if (rank < r_end) {
-        int q[LV], S[LV];
-        unrank<LV>(tab, kc, rank, nn, q);
+  for (long long blk = blockIdx.x; blk * tpb < n_tasks; blk += gridDim.x) {
+    long long t0 = blk * tpb;
+    const long long t1 = min(n_tasks, t0 + tpb);
+    while (t0 < t1) {
+      // the segment of this block's tasks on row i (uniform across the block)
+      const int i = task_row(task_off, L.p_total, t0);
+      const long long off = task_off[i];
+      const long long seg_end = min(t1, off + row_tasks[i]);
+      const int base = L.rowptr[i];
+      const int deg = L.rowptr[i + 1] - base;
+      if (deg + 1 <= dmin) {  // the staged kernel's row
+        t0 = seg_end;
+        continue;
+      }
+      const Inst &In = L.inst[L.row_inst[i]];
+      const int p = In.p, s0 = In.start;
+      const double *cor = In.cor;
+      const double r_crit = In.r_crit;
+      const bool fast = FAST && In.fast;
+      const bool chk = In.flags & kCorNaN;  // inputs may be NaN
+      const int nn = deg - 1;
+      const long long nch = row_tasks[i] / deg;
+      const unsigned long long total = binom(tab, kc, nn, LV);
+      const int li = i - s0;
+      for (long long t = t0 + wave; t < seg_end; t += kWavesPerBlock) {
+        const long long local = t - off;
+        const int y = (int)(local / nch);
+        const long long e = base + y;
+        const unsigned long long r_begin = (unsigned long long)(local % nch) * (unsigned long long)rpt;
+        // a lower rank of this edge already passed: nothing here can win
+        if (__atomic_load_n(&fpu[e], __ATOMIC_RELAXED) < r_begin) continue;
+        const unsigned long long r_end = min(total, r_begin + (unsigned long long)rpt);
+        const int lj = L.nbr[e] - s0;
+        const double a = cor[(size_t)li * p + lj];
+        for (unsigned long long r0 = r_begin; r0 < r_end; r0 += 64) {
+          const unsigned long long rank = r0 + lane;
+          bool pass = false, unc = false;
+          if (rank < r_end) {
+            int q[LV], S[LV];
+            unrank<LV>(tab, kc, rank, nn, q);
 #pragma unroll
-        for (int c = 0; c < LV; ++c) S[c] = L.nbr[base + (q[c] < y ? q[c] : q[c] + 1)] - s0;
-        double x[LV], yv[LV], C[LV][LV];
-        bool anynan = __builtin_isnan(a);
+            for (int c = 0; c < LV; ++c) S[c] = L.nbr[base + (q[c] < y ? q[c] : q[c] + 1)] - s0;
+            double x[LV], yv[LV], C[LV][LV];
 #pragma unroll
-        for (int c = 0; c < LV; ++c) {
-          x[c] = cor[(size_t)li * p + S[c]];
-          yv[c] = cor[(size_t)lj * p + S[c]];
-          anynan |= __builtin_isnan(x[c]) || __builtin_isnan(yv[c]);
-        }
+            for (int c = 0; c < LV; ++c) {
+              x[c] = cor[(size_t)li * p + S[c]];
+              yv[c] = cor[(size_t)lj * p + S[c]];
+            }
 #pragma unroll
-        for (int c = 0; c < LV; ++c)
+            for (int c = 0; c < LV; ++c)
 #pragma unroll
-          for (int r = c + 1; r < LV; ++r) {
-            C[r][c] = cor[(size_t)S[c] * p + S[r]];
-            anynan |= __builtin_isnan(C[r][c]);
+              for (int r = c + 1; r < LV; ++r) C[r][c] = cor[(size_t)S[c] * p + S[r]];
+            bool anynan = false;
+            if (chk) {
+              anynan = __builtin_isnan(a);
+#pragma unroll
+              for (int c = 0; c < LV; ++c)
+                anynan |= __builtin_isnan(x[c]) || __builtin_isnan(yv[c]);
+#pragma unroll
+              for (int c = 0; c < LV; ++c)
+#pragma unroll
+                for (int r = c + 1; r < LV; ++r) anynan |= __builtin_isnan(C[r][c]);
+            }
+            if (!anynan) {
+              bool fb;
+              double r;
+              if constexpr (FAST) {
+                double sens;
+                r = pcor_canon_sens<LV>(a, x, yv, C, fb, sens);
+                unc = fast && (fb || !fast_certain(r, r_crit, sens, In.eps));
+              } else {
+                r = pcor_canon<LV>(a, x, yv, C, fb);
+              }
+              pass = removes(r, r_crit) && !unc;
+              F += fb ? 1 : 0;
+            }
           }
-        if (!anynan) {
-          bool fb;
-          double r;
-          if constexpr (FAST) {
-            double sens;
-            r = pcor_canon_sens<LV>(a, x, yv, C, fb, sens);
-            unc = fast && (fb || !fast_certain(r, r_crit, sens, In.eps));
-          } else {
-            r = pcor_canon<LV>(a, x, yv, C, fb);
+          const unsigned long long m = __ballot(pass);
+          if (fast) queue_uncertain(L, unc, m, e, r0, lane);
+          if (m) {
+            if (lane == 0) atomicMin(&fpu[e], r0 + (unsigned long long)(__ffsll((long long)m) - 1));
+            break;
           }
-          pass = removes(r, r_crit) && !unc;
-          F += fb ? 1 : 0;
         }
       }
-      const unsigned long long m = __ballot(pass);
-      if (fast) queue_uncertain(L, unc, m, e, r0, lane);
-      if (m) {
-        if (lane == 0) atomicMin(&fpu[e], r0 + (unsigned long long)(__ffsll((long long)m) - 1));
-        break;
-      }
+      t0 = seg_end;
     }
   }
 #pragma unroll
@@ -392,16 +439,17 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_kernel(
 // correlation matrix into LDS once -- position a < deg is neighbour N(i)[a],
 // position deg is i -- and runs the row's tasks against it.  Workgroups take
 // blocks of tpb consecutive tasks (grid-stride; the task count is read on
-// the device), one staging per row a block meets.  Same ranks, same
-// arithmetic and the same orientation of every correlation read as
-// search_ln_kernel.
+// the device), one staging per row a block meets.  Rows whose block is larger
+// than dmax x dmax are left to search_ln_kernel.  Same ranks, same arithmetic
+// and the same orientation of every correlation read as search_ln_kernel.
 template <int LV, bool FAST>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
     Layout L, int rpt, const long long *__restrict__ task_off,
     const long long *__restrict__ row_tasks, int tpb, int dmax) {
   extern __shared__ unsigned long long smem[];
-  unsigned long long *bt = smem;                                   // (max_deg+1) x (LV+1)
-  const int brows = L.max_deg + 1;
+  // binomial rows 0..dmax-1 (a staged row's tests reach row deg - 1 < dmax)
+  unsigned long long *bt = smem;                                   // dmax x (LV+1)
+  const int brows = dmax;
   double *M = reinterpret_cast<double *>(smem + (size_t)brows * (LV + 1));  // dmax x dmax
   int *ids = reinterpret_cast<int *>(M + (size_t)dmax * dmax);     // dmax local ids
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -421,11 +469,15 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
       const int i = task_row(task_off, L.p_total, t0);
       const long long off = task_off[i];
       const long long seg_end = min(t1, off + row_tasks[i]);
-      const Inst &In = L.inst[L.row_inst[i]];
-      const int p = In.p, s0 = In.start;
       const int base = L.rowptr[i];
       const int deg = L.rowptr[i + 1] - base;
       const int D = deg + 1;
+      if (D > dmax) {  // search_ln_kernel's row
+        t0 = seg_end;
+        continue;
+      }
+      const Inst &In = L.inst[L.row_inst[i]];
+      const int p = In.p, s0 = In.start;
       const long long nch = row_tasks[i] / deg;
       __syncthreads();  // the previous segment is done with M (and bt is staged)
       for (int a = tid; a < D; a += blockDim.x) ids[a] = (a < deg ? L.nbr[base + a] : i) - s0;
@@ -541,8 +593,8 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
   if (lane == 0 && F) atomicAdd(&L.ctr->pinv_fallbacks, F);
 }
 
-// Tasks per row: deg * ceil(C(deg - 1, level) / rpt) for rows of instances at
-// this level (and in the row mask).
+// Tasks per row: deg * ceil(C(deg - 1, level) / rpt) (row_task_count) for rows
+// of instances at this level (and in the row mask).
 __global__ void task_count_kernel(Layout L, int level, int rpt, const int32_t *__restrict__ row_mask,
                                   long long *__restrict__ row_tasks) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -551,8 +603,8 @@ __global__ void task_count_kernel(Layout L, int level, int rpt, const int32_t *_
   const int deg = L.rowptr[g + 1] - L.rowptr[g];
   long long n = 0;
   if ((In.flags & kActive) && In.level == level && (!row_mask || row_mask[g]) && deg > 0) {
-    const unsigned long long total = binom(L.binom, kBinomK, deg - 1, level);
-    if (total > 0) n = (long long)deg * (long long)((total + rpt - 1) / rpt);
+    // a row that cannot be planned gets no tasks; the host fails its level
+    n = max(0ll, row_task_count(binom(L.binom, kBinomK, deg - 1, level), deg, rpt));
   }
   row_tasks[g] = n;
 }
@@ -616,8 +668,8 @@ __device__ void next_count(const Layout &L, const NextCount &nc, const unsigned 
     if (g < L.p_total) {
       c = __hip_atomic_load(&L.deg[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (nc.level >= 2 && c > 0) {
-        const unsigned long long total = binom(tab, kBinomK, c - 1, nc.level);
-        if (total > 0) tk = (long long)c * (long long)((total + nc.rpt - 1) / nc.rpt);
+        // a row that cannot be planned gets no tasks; the host fails its level
+        tk = max(0ll, row_task_count(binom(tab, kBinomK, c - 1, nc.level), c, nc.rpt));
       }
       nc.row_tasks[g] = tk;
     }
@@ -970,69 +1022,88 @@ __global__ void scatter_kernel(const int64_t *__restrict__ idx, const int64_t *_
   if (t < count) fp[idx[t]] = val[t];
 }
 
-// LDS bytes of search_ln_lds_kernel: binomials, the neighbourhood block and
-// its ids.
-size_t lds_bytes(int max_deg, int lv) {
-  const size_t d = (size_t)max_deg + 1;
+// LDS bytes of search_ln_lds_kernel for blocks of side d (row length + 1):
+// binomial rows 0..d-1, the d x d neighbourhood block and its ids.
+size_t lds_bytes(int d_in, int lv) {
+  const size_t d = (size_t)d_in;
   return d * (lv + 1) * 8 + d * d * 8 + ((d * 4 + 7) & ~(size_t)7);
 }
 
-constexpr size_t kLdsLimit = 64 * 1024;
+// Largest block side whose staging fits `limit` bytes of LDS.
+int lds_side(size_t limit, int lv) {
+  int d = 0;
+  while (lds_bytes(d + 1, lv) <= limit) ++d;
+  return d;
+}
 
 // Resident workgroups of the grid-stride search kernels (the task count is
 // only known on the device): enough to fill all 256 CUs several times.
 constexpr int kSearchBlocks = 256 * 8;
 
+// Ordered edges per workgroup block of the level-1 gather kernel.
+constexpr int kL1GatherEpb = 64;
+
+// Rows whose block fits `limit` bytes of LDS go to the staged kernel, longer
+// ones to the gather kernel (launched only when such a row exists).  Both
+// read the same task plan; each skips the other's rows.
 template <int LV, bool FAST>
 void launch_ln_as(hipStream_t s, const Layout &L, int rpt, const long long *row_tasks,
-                  const long long *task_off, int tpb) {
-  const size_t staged = lds_bytes(L.max_deg, LV);
-  if (staged <= kLdsLimit) {
+                  const long long *task_off, int tpb, size_t limit) {
+  const int dmax = std::min(L.max_deg + 1, lds_side(limit, LV));
+  // a testable row has deg - 1 >= LV, so a block side of at least LV + 2
+  if (dmax >= LV + 2)
     hipLaunchKernelGGL((search_ln_lds_kernel<LV, FAST>), dim3(kSearchBlocks),
-                       dim3(64 * kWavesPerBlock), staged, s, L, rpt, task_off, row_tasks,
-                       tpb, L.max_deg + 1);
-    return;
-  }
+                       dim3(64 * kWavesPerBlock), lds_bytes(dmax, LV), s, L, rpt, task_off,
+                       row_tasks, tpb, dmax);
+  if (L.max_deg + 1 <= dmax) return;
   const size_t lds = (size_t)(L.max_deg + 1) * (LV + 1) * sizeof(unsigned long long);
   const int use = lds <= 48 * 1024;
   hipLaunchKernelGGL((search_ln_kernel<LV, FAST>), dim3(kSearchBlocks), dim3(64 * kWavesPerBlock),
-                     use ? lds : 0, s, L, rpt, task_off, row_tasks, use);
+                     use ? lds : 0, s, L, rpt, task_off, row_tasks, tpb, dmax, use);
 }
 
 // The fast-correlation variant (uncertainty bounds, recheck queue) is only
 // instantiated into launches that hold a fast instance.
 template <int LV>
 void launch_ln(hipStream_t s, const Layout &L, int rpt, const long long *row_tasks,
-               const long long *task_off, bool fast, int tpb) {
-  if (fast) launch_ln_as<LV, true>(s, L, rpt, row_tasks, task_off, tpb);
-  else launch_ln_as<LV, false>(s, L, rpt, row_tasks, task_off, tpb);
+               const long long *task_off, bool fast, int tpb, size_t limit) {
+  if (fast) launch_ln_as<LV, true>(s, L, rpt, row_tasks, task_off, tpb, limit);
+  else launch_ln_as<LV, false>(s, L, rpt, row_tasks, task_off, tpb, limit);
 }
 
 }  // namespace
 
+size_t search_lds_limit(size_t requested) {
+  return requested == 0 || requested > kSearchLdsLimit ? kSearchLdsLimit : requested;
+}
+
 void launch_search_l1(hipStream_t s, const Layout &L, int64_t n_edges, const int32_t *row_mask,
-                      bool fast, int epb) {
+                      bool fast, int epb, size_t limit) {
   if (n_edges <= 0) return;
-  const size_t staged = (size_t)L.max_deg * 16;
-  if (staged <= kLdsLimit) {
+  // rows of up to `cap` neighbours are staged (16 bytes per neighbour)
+  const int cap = (int)std::min<size_t>((size_t)L.max_deg, limit / 16);
+  if (cap >= 2) {
     const long long span = (long long)kL1Classes * epb;
     const long long blocks = kL1Classes * ((n_edges + span - 1) / span);
+    const size_t staged = (size_t)cap * 16;
     if (fast)
       hipLaunchKernelGGL(search_l1_lds_kernel<true>, dim3((unsigned)blocks),
-                         dim3(64 * kWavesPerBlock), staged, s, L, (long long)n_edges, row_mask, epb);
+                         dim3(64 * kWavesPerBlock), staged, s, L, (long long)n_edges, row_mask,
+                         epb, cap);
     else
       hipLaunchKernelGGL(search_l1_lds_kernel<false>, dim3((unsigned)blocks),
-                         dim3(64 * kWavesPerBlock), staged, s, L, (long long)n_edges, row_mask, epb);
-    return;
+                         dim3(64 * kWavesPerBlock), staged, s, L, (long long)n_edges, row_mask,
+                         epb, cap);
   }
-  long long blocks = (n_edges + kWavesPerBlock - 1) / kWavesPerBlock;
+  if (L.max_deg <= cap) return;
+  long long blocks = (n_edges + kL1GatherEpb - 1) / kL1GatherEpb;
   if (blocks > 16384) blocks = 16384;
   if (fast)
     hipLaunchKernelGGL(search_l1_kernel<true>, dim3((unsigned)blocks), dim3(64 * kWavesPerBlock), 0,
-                       s, L, (long long)n_edges, row_mask);
+                       s, L, (long long)n_edges, row_mask, cap, kL1GatherEpb);
   else
     hipLaunchKernelGGL(search_l1_kernel<false>, dim3((unsigned)blocks), dim3(64 * kWavesPerBlock),
-                       0, s, L, (long long)n_edges, row_mask);
+                       0, s, L, (long long)n_edges, row_mask, cap, kL1GatherEpb);
 }
 
 void launch_task_plan(hipStream_t s, int level, const Layout &L, int rpt,
@@ -1044,23 +1115,24 @@ void launch_task_plan(hipStream_t s, int level, const Layout &L, int rpt,
 }
 
 void launch_search_ln(hipStream_t s, int level, const Layout &L, int rpt,
-                      const long long *row_tasks, const long long *task_off, bool fast, int tpb) {
+                      const long long *row_tasks, const long long *task_off, bool fast, int tpb,
+                      size_t limit) {
   switch (level) {
-    case 2: launch_ln<2>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
-    case 3: launch_ln<3>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
-    case 4: launch_ln<4>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
-    case 5: launch_ln<5>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
-    case 6: launch_ln<6>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
-    case 7: launch_ln<7>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
-    case 8: launch_ln<8>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
-    case 9: launch_ln<9>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
-    case 10: launch_ln<10>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
-    case 11: launch_ln<11>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
-    case 12: launch_ln<12>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
-    case 13: launch_ln<13>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
-    case 14: launch_ln<14>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
-    case 15: launch_ln<15>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
-    case 16: launch_ln<16>(s, L, rpt, row_tasks, task_off, fast, tpb); break;
+    case 2: launch_ln<2>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
+    case 3: launch_ln<3>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
+    case 4: launch_ln<4>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
+    case 5: launch_ln<5>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
+    case 6: launch_ln<6>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
+    case 7: launch_ln<7>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
+    case 8: launch_ln<8>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
+    case 9: launch_ln<9>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
+    case 10: launch_ln<10>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
+    case 11: launch_ln<11>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
+    case 12: launch_ln<12>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
+    case 13: launch_ln<13>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
+    case 14: launch_ln<14>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
+    case 15: launch_ln<15>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
+    case 16: launch_ln<16>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
     default: break;  // rejected by the caller (kMaxLevel)
   }
 }

@@ -13,11 +13,18 @@ LIB_PATH = os.path.join(PKG_ROOT, "libheterogpc_hip.so")
 HEADER = os.path.join(os.path.dirname(PKG_ROOT), "include", "heterogpc_hip.h")
 
 HPC_OK = 0
+HPC_EINVAL = -1
+HPC_EHIP = -2
+HPC_ENOMEM = -3
+HPC_ESTATE = -4
+HPC_ELEVEL = -5
 HPC_CORR_CANON = 0
 HPC_CORR_F32 = 1
 HPC_TUNE_RANKS_PER_TASK = 1
 HPC_TUNE_RECHECK_QUEUE = 2
 HPC_TUNE_KERNEL_TIMING = 3
+HPC_TUNE_LDS_LIMIT = 4
+HPC_TUNE_WAIT_LIMIT = 5
 
 
 class LevelStats(C.Structure):
@@ -153,6 +160,12 @@ SIGNATURES = {
     "hpc_server_sepset_list": (_i, [_vp, _i64, _vp, _i64, C.POINTER(_i64)]),
     "hpc_server_release": (_i, [_vp, _i64]),
     "hpc_server_stats": (_i, [_vp, C.POINTER(ServerInfo)]),
+    "hpc_server_set_tuning": (_i, [_vp, _i, _i]),
+    "hpc_pack_check": (_i, [_vp, _i]),
+    "hpc_batch_inject_failure": (_i, [_vp, _i, _i]),
+    "hpc_server_inject_failure": (_i, [_vp, _i64, _i]),
+    "hpc_plan_tasks": (_i, [_u64, _i, _i, C.POINTER(_i64)]),
+    "hpc_wait_word": (_i, [_vp, _u64, _d]),
 }
 
 _lib = None

@@ -14,6 +14,21 @@ def _ptr(a: np.ndarray) -> int:
     return a.ctypes.data
 
 
+def pack_check(ps) -> int:
+    """0 when instances of these p can share one batch or server step, else
+    the HPC_E* code (hpc_pack_check)."""
+    arr = np.ascontiguousarray(ps, dtype=np.uint64)
+    return load().hpc_pack_check(_ptr(arr) if len(arr) else None, len(arr))
+
+
+def plan_tasks(deg: int, level: int, ranks_per_task: int = 512) -> int:
+    """Search tasks of a row of `deg` neighbours at `level` (hpc_plan_tasks);
+    raises HpcError when the level cannot be planned."""
+    out = C.c_int64()
+    check(load().hpc_plan_tasks(int(deg), int(level), int(ranks_per_task), C.byref(out)))
+    return int(out.value)
+
+
 def footprint(p: int, n: int, max_level: int) -> int:
     """Device bytes one instance of (p, n, max_level) needs at most."""
     out = C.c_uint64()
@@ -114,6 +129,16 @@ class Batch:
     def set_ranks_per_task(self, value: int) -> None:
         check(self.lib.hpc_batch_set_tuning(self.h, 1, int(value)))
 
+    def set_lds_limit(self, nbytes: int) -> None:
+        """LDS bytes per workgroup for staging a row in the CI-test kernels
+        (0 = the 64 KiB default); rows that need more take the gather
+        kernels."""
+        check(self.lib.hpc_batch_set_tuning(self.h, 4, int(nbytes)))
+
+    def inject_failure(self, iid: int, level: int) -> None:
+        """Test hook: the host side of instance iid's `level` fails."""
+        check(self.lib.hpc_batch_inject_failure(self.h, int(iid), int(level)))
+
     def apply_tuning(self, entries: list[dict]) -> None:
         """Launch shapes (as DeviceSkeleton.autotune() returns them)."""
         arr, n = tune_array(entries)
@@ -199,6 +224,15 @@ class Server:
             self.shape.pop(req, None)
             self._borrowed.pop(req, None)
         return adj, sep, {name: getattr(info, name) for name, _ in info._fields_}
+
+    def set_tuning(self, param: int, value: int) -> None:
+        """A knob (HPC_TUNE_*) for the server's later steps."""
+        check(self.lib.hpc_server_set_tuning(self.h, int(param), int(value)))
+
+    def inject_failure(self, req: int, level: int) -> None:
+        """Test hook: the host side of request req's `level` fails (req may
+        be an id not yet submitted; ids count up from 1)."""
+        check(self.lib.hpc_server_inject_failure(self.h, int(req), int(level)))
 
     def apply_tuning(self, entries: list[dict]) -> None:
         """Launch shapes for the server's later steps."""

@@ -125,6 +125,12 @@ class DeviceSkeleton:
         """Length of the HPC_CORR_F32 recheck queues (0 = full)."""
         check(self.lib.hpc_set_tuning(self.h, 2, int(value)))
 
+    def set_lds_limit(self, nbytes: int) -> None:
+        """LDS bytes per workgroup for staging a row in the CI-test kernels
+        (0 = the 64 KiB default); rows that need more take the gather
+        kernels."""
+        check(self.lib.hpc_set_tuning(self.h, 4, int(nbytes)))
+
     def set_kernel_timing(self, on: bool) -> None:
         """Device-time events around each level's kernels in run() (the
         ms_compact / ms_test / ms_commit fields); off, the level loop runs

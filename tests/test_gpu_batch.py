@@ -54,11 +54,17 @@ def _oracle(orc, x, alpha, ml):
     return cor, ref, recs
 
 
-def _check_instance(b, iid, label, orc, x, alpha, ml, single=None):
+def _check_instance(b, iid, label, orc, x, alpha, ml, single=None, fast=False):
+    """fast: the instance's correlation is the FP32-MFMA one (within its
+    bound of the oracle's); everything else must still be identical."""
     print(f"[batch] {label}: p={x.shape[0]} n={x.shape[1]} alpha={alpha} max_level={ml}",
           flush=True)
     cor, ref, recs = _oracle(orc, x, alpha, ml)
-    assert np.array_equal(b.cor(iid), cor, equal_nan=True), f"{label}: correlation differs"
+    if fast:
+        off = ~np.eye(x.shape[0], dtype=bool)
+        assert np.nanmax(np.abs(b.cor(iid) - cor)[off]) <= 74.0 * 2.0 ** -24 * 1.01, label
+    else:
+        assert np.array_equal(b.cor(iid), cor, equal_nan=True), f"{label}: correlation differs"
     assert np.array_equal(b.adj(iid), ref.adj()), f"{label}: adjacency differs"
     mc = ref.mc()
     so = ref.sepsets()

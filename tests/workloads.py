@@ -38,6 +38,21 @@ def sem(name: str):
     return x, n, alpha, ml
 
 
+def latent(p: int = 150, block: int = 110, n: int = 1000, lo: float = 0.2, hi: float = 0.6,
+           seed: int = 5) -> np.ndarray:
+    """Dense neighbourhoods: the first `block` of p variables load on one
+    latent common factor (loadings +-U[lo, hi]), which is not observed, so no
+    conditioning set separates them and their level-2 rows hold about
+    `block` neighbours (past what a workgroup stages in LDS); the other
+    variables are independent N(0,1).  FP32-rounded, variable-major."""
+    rng = np.random.default_rng(seed)
+    f = rng.standard_normal(n)
+    x = rng.standard_normal((p, n))
+    lam = rng.uniform(lo, hi, block) * rng.choice([-1.0, 1.0], block)
+    x[:block] += lam[:, None] * f[None, :]
+    return np.ascontiguousarray(x.astype(np.float32).astype(np.float64))
+
+
 def cooling_house() -> np.ndarray:
     """data/cooling_house.csv as variable-major [6][10000]."""
     X = np.loadtxt(os.path.join(GOLDEN, "cooling_house.csv"), delimiter=",")
