@@ -168,6 +168,20 @@ int hpc_corr(hpc_state *st, int mode);
 /* Device time of the last hpc_corr: moments (mean, sd, z-scores) and the
  * pairwise sums (waits for them to finish). */
 int hpc_corr_ms(hpc_state *st, double *ms_moments, double *ms_corr);
+/* hpc_corr's exact (HPC_CORR_CANON) correlation without allocating: no
+ * counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  x: device observations [p][n] (as hpc_set_observations
+ * takes them on the host); cor: device output, p x p row-major, both
+ * triangles, bit-identical to hpc_corr's.  The kernels' working memory (means,
+ * standard deviations, z-scores, NaN mask, instance table) is one static
+ * 64 MiB device array per device, so 9 p ceil(n/64) 64 + 20 p bytes plus a
+ * few KiB must fit it (p = 1000: n <= 7400); larger shapes fail with
+ * HPC_EINVAL.  Queues its kernels on `stream` (a hipStream_t; NULL = the
+ * default stream) and returns without waiting.  Calls that overlap on one
+ * device share the array and race; calls in one stream's order do not. */
+int hpc_corr_noalloc(int device, const double *x, uint64_t p, int n, double *cor,
+                     void *stream);
 /* add_correlation_matrix (src/independence/skeleton.cpp:60-65): a given
  * p x p correlation matrix. */
 int hpc_set_corr(hpc_state *st, const double *h_cor);

@@ -24,7 +24,7 @@ CLI = os.path.join(PKG_DIR, "bin", "heterogpc")
 HOST_SOURCES = ["state.cpp", "io.cpp", "executors.cpp", "balancer.cpp", "skeleton.cpp", "main.cpp"]
 
 SOURCES = ["abi.hip", "engine.hip", "corr.hip", "graph.hip", "search.hip", "hostshare.hip",
-           "batch.hip", "recheck.hip"]
+           "batch.hip", "recheck.hip", "noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [

@@ -524,6 +524,8 @@ long long mfma_tiles(int p) {
   return nt * (nt + 1) / 2;
 }
 
+}  // namespace
+
 long long tiled_slots(int p) {
   const int nt = (p + CT - 1) / CT;
   const int ns = (nt + ST - 1) / ST;
@@ -535,15 +537,17 @@ long long masked_blocks(int p) {
   return nb * (nb + 1) / 2;
 }
 
-}  // namespace
-
 void launch_moments(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst) {
-  const int wpb = 4;
-  const dim3 grid((L.p_total + wpb - 1) / wpb), block(64 * wpb);
-  // the samples in registers when every active instance's row fits
   int max_n = 0;
   for (const Inst &I : host_inst)
     if (I.flags & kActive) max_n = std::max(max_n, I.n);
+  launch_moments_n(s, L, max_n);
+}
+
+void launch_moments_n(hipStream_t s, const Layout &L, int max_n) {
+  const int wpb = 4;
+  const dim3 grid((L.p_total + wpb - 1) / wpb), block(64 * wpb);
+  // the samples in registers when every active instance's row fits
   if (max_n <= 64 * 32)
     hipLaunchKernelGGL(moments_kernel<32>, grid, block, 0, s, L);
   else if (max_n <= 64 * 80)
@@ -568,7 +572,11 @@ void launch_corr(hipStream_t s, const Layout &L, const std::vector<Inst> &host_i
   }
   (void)hipMemcpyAsync(d_prefix, h_prefix.data(), h_prefix.size() * sizeof(long long),
                        hipMemcpyHostToDevice, s);
-  const long long tiled = h_prefix[K], masked = h_prefix[2 * K + 1];
+  launch_corr_sums(s, L, d_prefix, h_prefix[K], h_prefix[2 * K + 1]);
+}
+
+void launch_corr_sums(hipStream_t s, const Layout &L, const long long *d_prefix, long long tiled,
+                      long long masked) {
   if (tiled > 0) {
     const long long per_xcd = (tiled + kXcds - 1) / kXcds;
     hipLaunchKernelGGL(corr_tile_kernel, dim3((unsigned)(per_xcd * kXcds)), dim3(256), 0, s, L,
@@ -578,7 +586,7 @@ void launch_corr(hipStream_t s, const Layout &L, const std::vector<Inst> &host_i
     const int wpb = 4;
     const long long blocks = std::min<long long>((masked + wpb - 1) / wpb, 4096);
     hipLaunchKernelGGL(corr_canon_kernel, dim3((unsigned)blocks), dim3(64 * wpb), 0, s, L,
-                       d_prefix + K + 1);
+                       d_prefix + L.K + 1);
   }
 }
 

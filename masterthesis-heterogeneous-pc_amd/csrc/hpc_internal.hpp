@@ -136,10 +136,20 @@ struct Layout {
 
 // ---- launchers (each defined in the .hip file that owns the kernel) ------
 void launch_moments(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst);
+// The same for active instances of at most max_n samples.
+void launch_moments_n(hipStream_t s, const Layout &L, int max_n);
 // pairwise sums: tiled kernel for NaN-free instances, masked kernel for the
 // others; prefix arrays hold per-instance block counts (device and host)
 void launch_corr(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst,
                  long long *d_prefix, std::vector<long long> &h_prefix);
+// The pairwise-sum kernels of launch_corr on block prefixes already in device
+// memory: d_prefix holds the K + 1 tiled-slot prefixes, then the K + 1
+// masked-block prefixes; tiled and masked are their totals.
+void launch_corr_sums(hipStream_t s, const Layout &L, const long long *d_prefix, long long tiled,
+                      long long masked);
+// Work units of one instance in the tiled and the masked pairwise kernels.
+long long tiled_slots(int p);
+long long masked_blocks(int p);
 // Fast correlation of the packed instances with `fast` set: MFMA FP32 on the
 // float z-scores, FP64 across 64-sample chunks (instances with NaN cells are
 // left to launch_corr's masked kernel).

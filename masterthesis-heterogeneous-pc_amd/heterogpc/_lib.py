@@ -166,6 +166,7 @@ SIGNATURES = {
     "hpc_server_inject_failure": (_i, [_vp, _i64, _i]),
     "hpc_plan_tasks": (_i, [_u64, _i, _i, C.POINTER(_i64)]),
     "hpc_wait_word": (_i, [_vp, _u64, _d]),
+    "hpc_corr_noalloc": (_i, [_i, _vp, _u64, _i, _vp, _vp]),
 }
 
 _lib = None
