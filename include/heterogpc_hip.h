@@ -127,9 +127,14 @@ enum {
                                   whose staging needs more is searched by a
                                   gather kernel that reads the correlation
                                   matrix; results do not depend on it */
-  HPC_TUNE_WAIT_LIMIT = 5      /* seconds the host waits for the device's
+  HPC_TUNE_WAIT_LIMIT = 5,     /* seconds the host waits for the device's
                                   counts of a level before the call fails with
                                   HPC_EHIP (0 = no limit; default 900) */
+  HPC_TUNE_QUEUE = 6           /* 1 (default): the level >= 2 CI-test kernel
+                                  is persistent and its workgroups claim
+                                  blocks of tasks from a device queue; 0:
+                                  blocks go to workgroups by id.  Results do
+                                  not depend on it */
 };
 int hpc_set_tuning(hpc_state *st, int param, int value);
 
@@ -137,7 +142,8 @@ int hpc_set_tuning(hpc_state *st, int param, int value);
  * 1..max_level once, timing each level's CI-test kernel on that level's
  * snapshot for every candidate launch shape -- level 1: ordered edges per
  * workgroup in {8..128}; levels >= 2: conditioning sets per task in
- * {256..4096} x tasks per workgroup in {4..128} -- and keeps the fastest per
+ * {256..4096} x tasks per workgroup in {4..128} x task order (device queue
+ * or workgroup id) -- and keeps the fastest per
  * level for this state's later runs.  Results do not depend on the shape.
  * The state is left reset (adjacency and sepsets back to the start). */
 typedef struct hpc_tune_entry {
@@ -145,6 +151,10 @@ typedef struct hpc_tune_entry {
   int32_t ranks_per_task;    /* levels >= 2 */
   int32_t tasks_per_block;   /* levels >= 2 */
   int32_t edges_per_block;   /* level 1 */
+  int32_t queue;             /* levels >= 2: 1 = blocks claimed from the device
+                                task queue, 2 = blocks by workgroup id, 0 = the
+                                state's HPC_TUNE_QUEUE */
+  int32_t pad_;
   double ms_best;            /* CI-test kernel time with the chosen shape */
   double ms_default;         /* with the default shape */
 } hpc_tune_entry;
@@ -205,6 +215,22 @@ int hpc_test_level(hpc_state *st, int level, const int32_t *rows, int nrows,
 int hpc_search_level(hpc_state *st, int level, const int32_t *rows, int nrows,
                      hpc_level_stats *stats);
 int hpc_commit_level(hpc_state *st, int level, hpc_level_stats *stats);
+/* The work-stealing level of Balancer::executeWorkstealing
+ * (src/loadbalance/balancer.cpp:186-234) on one GPU: the level's search tasks
+ * (ordered edge, range of conditioning-set ranks) form one queue.  A
+ * persistent device kernel claims blocks of tasks from its front while
+ * `threads` host threads (all cores when <= 0) take blocks from its back, as
+ * the CPU executor claims rows longest first beside the GPU
+ * (src/executor/cpuExecutor.cpp:15-78, gpuWorkstealingTests.cu:21-39); each
+ * side skips blocks the other has finished and the host stops when the
+ * device has claimed everything.  The host's first passing ranks are merged
+ * with the device's (lowest wins), so the result equals a device-only
+ * search.  Levels 1..16, exact correlation, after hpc_compact;
+ * hpc_commit_level then removes the edges.  host_stats receives the host's
+ * share (edges and tests it searched; `removed` holds the blocks it
+ * finished), device_stats the level's search time. */
+int hpc_search_level_shared(hpc_state *st, int level, int threads,
+                            hpc_level_stats *device_stats, hpc_level_stats *host_stats);
 /* CPUExecutor::executeLevel (src/executor/cpuExecutor.cpp:80-119): the
  * host's share of a level >= 1 -- rows[0..nrows) searched on `threads` host
  * threads (all cores when <= 0) with the device's arithmetic, ranks written
@@ -284,6 +310,14 @@ int hpc_batch_get_sepset_list(hpc_batch *b, int id, int32_t *out, int64_t cap,
 int hpc_batch_set_tuning(hpc_batch *b, int param, int value);
 /* hpc_apply_tuning for every instance of the batch. */
 int hpc_batch_apply_tuning(hpc_batch *b, const hpc_tune_entry *entries, int count);
+/* Autotuner of a batch (no counterpart in the reference): runs the batch's
+ * unfinished instances once from their start, timing each level's CI-test
+ * kernel with every candidate shape (those of hpc_autotune) on the first step
+ * that meets the level -- the step's own mix of instances -- and keeps the
+ * fastest per level for the batch's later steps; out[0..cap) receives them
+ * and *levels their number.  Every instance is left at its start with its
+ * correlation.  Exact instances only; results do not depend on the shapes. */
+int hpc_batch_autotune(hpc_batch *b, hpc_tune_entry *out, int cap, int *levels);
 /* Correlation mode (HPC_CORR_*) of an instance, before its first step. */
 int hpc_batch_set_corr_mode(hpc_batch *b, int id, int mode);
 

@@ -216,6 +216,10 @@ extern "C" int hpc_set_tuning(hpc_state *st, int param, int value) {
       if (value < 0) return fail(HPC_EINVAL, "wait limit must be >= 0 seconds");
       st->eng.wait_limit_s = (double)value;
       return HPC_OK;
+    case HPC_TUNE_QUEUE:
+      if (value != 0 && value != 1) return fail(HPC_EINVAL, "queue must be 0 or 1");
+      st->eng.use_queue = value != 0;
+      return HPC_OK;
     default:
       return fail(HPC_EINVAL, "unknown tuning parameter %d", param);
   }

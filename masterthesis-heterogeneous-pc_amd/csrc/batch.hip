@@ -221,6 +221,9 @@ static int check_engine_tuning(int param, int value) {
     case HPC_TUNE_WAIT_LIMIT:
       if (value < 0) return hpc_fail(HPC_EINVAL, "wait limit must be >= 0 seconds");
       return HPC_OK;
+    case HPC_TUNE_QUEUE:
+      if (value != 0 && value != 1) return hpc_fail(HPC_EINVAL, "queue must be 0 or 1");
+      return HPC_OK;
     default:
       return hpc_fail(HPC_EINVAL, "tuning parameter %d does not apply here", param);
   }
@@ -231,6 +234,7 @@ static int set_engine_tuning(Engine &e, int param, int value) {
   if (rc) return rc;
   if (param == HPC_TUNE_RANKS_PER_TASK) e.ranks_per_task = value;
   else if (param == HPC_TUNE_LDS_LIMIT) e.lds_limit = (size_t)value;
+  else if (param == HPC_TUNE_QUEUE) e.use_queue = value != 0;
   else e.wait_limit_s = (double)value;
   return HPC_OK;
 }
@@ -239,6 +243,17 @@ extern "C" int hpc_batch_set_tuning(hpc_batch *b, int param, int value) {
   if (!b) return hpc_fail(HPC_EINVAL, "batch is NULL");
   const int rc = set_engine_tuning(b->eng, param, value);
   return rc;
+}
+
+extern "C" int hpc_batch_autotune(hpc_batch *b, hpc_tune_entry *out, int cap, int *levels) {
+  if (!b) return hpc_fail(HPC_EINVAL, "batch is NULL");
+  if (cap < 0 || (cap > 0 && !out)) return hpc_fail(HPC_EINVAL, "bad output buffer");
+  std::vector<hpc_tune_entry> res;
+  const int rc = engine_batch_autotune(b->eng, res);
+  if (rc) return rc;
+  for (int k = 0; k < (int)res.size() && k < cap; ++k) out[k] = res[k];
+  if (levels) *levels = (int)res.size();
+  return HPC_OK;
 }
 
 extern "C" int hpc_batch_inject_failure(hpc_batch *b, int id, int level) {

@@ -164,6 +164,10 @@ int engine_init(Engine &e, int device) {
   HPC_HIPCHK(hipHostGetDevicePointer((void **)&e.d_seq, e.h_seq, 0));
   __atomic_store_n(e.h_seq, 0ull, __ATOMIC_RELEASE);
   e.seq = 0;
+  // the task-queue head starts at 0; every queue launch leaves it at 0
+  int rc = dalloc((void **)&e.d_qhead, 64);
+  if (rc) return rc;
+  HPC_HIPCHK(hipMemset(e.d_qhead, 0, 64));
   return ensure_inst_cap(e, 1);
 }
 
@@ -189,7 +193,10 @@ void engine_free(Engine &e) {
   dfree(e.d_nbr); dfree(e.d_erow); dfree(e.d_rowmask); dfree(e.d_fp); dfree(e.d_row_tasks);
   dfree(e.d_task_off); dfree(e.d_binom); dfree(e.d_prefix); dfree(e.d_ctr);
   e.d_ictr = nullptr;
-  dfree(e.d_scatter_idx); dfree(e.d_scatter_val); dfree(e.d_rchk);
+  dfree(e.d_scatter_idx); dfree(e.d_scatter_val); dfree(e.d_rchk); dfree(e.d_qhead);
+  if (e.h_qdone) (void)hipHostFree(e.h_qdone);
+  e.h_qdone = e.d_qdone = nullptr;
+  e.cap_qdone = 0;
   dfree(e.d_blk); dfree(e.d_l0slot);
   if (e.h_ctr) (void)hipHostFree(e.h_ctr);
   if (e.h_ctr2) (void)hipHostFree(e.h_ctr2);
@@ -821,7 +828,7 @@ int engine_search(Engine &e, int level, const int32_t *rows, int nrows, float *m
                                      search_lds_limit(e.lds_limit));
     else
       launch_search_ln(e.stream, level, L, engine_rpt(e, level), e.d_row_tasks, e.d_task_off, fast,
-                       engine_tpb(e, level), search_lds_limit(e.lds_limit));
+                       engine_tpb(e, level), search_lds_limit(e.lds_limit), device_queue(e, level));
     if (fast) launch_recheck(e.stream, level, L);
   }
   HPC_HIPCHK(hipGetLastError());
@@ -849,6 +856,97 @@ int engine_search(Engine &e, int level, const int32_t *rows, int nrows, float *m
     HPC_HIPCHK(hipEventSynchronize(e.ev[6]));
     *ms = elapsed(e.ev[5], e.ev[6]);
   }
+  return HPC_OK;
+}
+
+int engine_search_shared_launch(Engine &e, int level, SharedPlan &plan) {
+  HPC_HIPCHK(hipSetDevice(e.device));
+  e.rechecks = 0;
+  plan = SharedPlan{};
+  if (e.packed.size() != 1) return hpc_fail(HPC_ESTATE, "the shared queue takes one instance");
+  const InstHost &h = e.inst[e.packed[0]];
+  if (h.fast) return hpc_fail(HPC_ESTATE, "the shared queue needs an exact correlation");
+  if (e.n_edges <= 0 || h.max_deg - 1 < level) return HPC_OK;
+  if (!engine_plannable(e, h.max_deg, level))
+    return hpc_fail(HPC_ELEVEL,
+                    "level %d: C(%d, %d) conditioning sets per edge is beyond what one search "
+                    "plans", level, h.max_deg - 1, level);
+  Layout L = layout(e);
+  plan.rpt = engine_rpt(e, level);
+  plan.tpb = engine_tpb(e, level);
+  launch_task_plan(e.stream, level, L, plan.rpt, nullptr, e.d_row_tasks, e.d_task_off);
+  HPC_HIPCHK(hipGetLastError());
+  int rc = read_counters(e);
+  if (rc) return rc;
+  plan.n_tasks = (long long)e.h_ctr->n_tasks;
+  plan.n_blocks = (plan.n_tasks + plan.tpb - 1) / plan.tpb;
+  if (plan.n_blocks == 0) return HPC_OK;
+  // host copies of the plan and the snapshot (the stream is idle here)
+  const int p = e.p_total;
+  e.h_rowptr.resize(p + 1);
+  e.h_nbr.resize(e.n_edges);
+  e.h_task_off.resize(p);
+  e.h_row_tasks.resize(p);
+  HPC_HIPCHK(hipMemcpy(e.h_rowptr.data(), e.d_rowptr, (p + 1) * sizeof(int32_t),
+                       hipMemcpyDeviceToHost));
+  HPC_HIPCHK(hipMemcpy(e.h_nbr.data(), e.d_nbr, e.n_edges * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HPC_HIPCHK(hipMemcpy(e.h_task_off.data(), e.d_task_off, p * sizeof(long long),
+                       hipMemcpyDeviceToHost));
+  HPC_HIPCHK(hipMemcpy(e.h_row_tasks.data(), e.d_row_tasks, p * sizeof(long long),
+                       hipMemcpyDeviceToHost));
+  if (!e.h_cor_valid) {
+    e.h_cor.resize((size_t)h.p * h.p);
+    HPC_HIPCHK(hipMemcpy(e.h_cor.data(), h.cor, (size_t)h.p * h.p * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    e.h_cor_valid = true;
+  }
+  // the done flags, one per block, in host-visible memory
+  if ((size_t)plan.n_blocks > e.cap_qdone) {
+    if (e.h_qdone) HPC_HIPCHK(hipHostFree(e.h_qdone));
+    e.h_qdone = nullptr;
+    e.cap_qdone = 0;
+    const size_t cap = std::max<size_t>((size_t)plan.n_blocks, 1024);
+    HPC_HIPCHK(hipHostMalloc((void **)&e.h_qdone, cap * sizeof(int),
+                             hipHostMallocMapped | hipHostMallocCoherent));
+    HPC_HIPCHK(hipHostGetDevicePointer((void **)&e.d_qdone, e.h_qdone, 0));
+    e.cap_qdone = cap;
+  }
+  for (long long b = 0; b < plan.n_blocks; ++b) __atomic_store_n(&e.h_qdone[b], 0, __ATOMIC_RELAXED);
+  __atomic_thread_fence(__ATOMIC_SEQ_CST);
+  plan.done = e.h_qdone;
+  L.max_deg = e.max_deg;
+  HPC_HIPCHK(hipEventRecord(e.ev[5], e.stream));
+  launch_search_ln(e.stream, level, L, plan.rpt, e.d_row_tasks, e.d_task_off, false, plan.tpb,
+                   search_lds_limit(e.lds_limit), TaskQueue{e.d_qhead, e.d_qdone});
+  HPC_HIPCHK(hipGetLastError());
+  HPC_HIPCHK(hipEventRecord(e.ev[6], e.stream));
+  plan.device_done = e.ev[6];
+  return HPC_OK;
+}
+
+int engine_search_shared_finish(Engine &e, const std::vector<int64_t> &idx,
+                                const std::vector<int64_t> &val, float *ms) {
+  HPC_HIPCHK(hipSetDevice(e.device));
+  const size_t cnt = idx.size();
+  if (cnt > 0) {
+    if (cnt > e.scatter_cap) {
+      dfree(e.d_scatter_idx);
+      dfree(e.d_scatter_val);
+      e.scatter_cap = 0;
+      int rc = dalloc((void **)&e.d_scatter_idx, cnt * sizeof(int64_t));
+      if (!rc) rc = dalloc((void **)&e.d_scatter_val, cnt * sizeof(int64_t));
+      if (rc) return rc;
+      e.scatter_cap = cnt;
+    }
+    HPC_HIPCHK(hipMemcpyAsync(e.d_scatter_idx, idx.data(), cnt * sizeof(int64_t),
+                              hipMemcpyHostToDevice, e.stream));
+    HPC_HIPCHK(hipMemcpyAsync(e.d_scatter_val, val.data(), cnt * sizeof(int64_t),
+                              hipMemcpyHostToDevice, e.stream));
+    launch_scatter_min_fp(e.stream, e.d_scatter_idx, e.d_scatter_val, (long long)cnt, e.d_fp);
+    HPC_HIPCHK(hipGetLastError());
+  }
+  HPC_HIPCHK(hipStreamSynchronize(e.stream));
+  if (ms) *ms = elapsed(e.ev[5], e.ev[6]);
   return HPC_OK;
 }
 
@@ -1113,7 +1211,7 @@ int engine_run_single(Engine &e, int id) {
                            search_lds_limit(e.lds_limit));
         else
           launch_search_ln(e.stream, l, L, engine_rpt(e, l), e.d_row_tasks, e.d_task_off, h.fast,
-                           engine_tpb(e, l), search_lds_limit(e.lds_limit));
+                           engine_tpb(e, l), search_lds_limit(e.lds_limit), device_queue(e, l));
         // exact re-runs of the tests near the threshold (fast correlations);
         // the removal below clears the queue with the other counters
         if (h.fast) launch_recheck(e.stream, l, L, /*clear_queue=*/false);
@@ -1160,7 +1258,8 @@ int engine_run_single(Engine &e, int id) {
         if (!h.fast && h.max_level >= 0) {
           HPC_HIPCHK(mark(e.lev_ev[l + 1][4]));
           launch_search_ln(e.stream, l + 1, L, engine_rpt(e, l + 1), e.d_row_tasks, e.d_task_off,
-                           false, engine_tpb(e, l + 1), search_lds_limit(e.lds_limit));
+                           false, engine_tpb(e, l + 1), search_lds_limit(e.lds_limit),
+                           device_queue(e, l + 1));
           HPC_HIPCHK(hipGetLastError());
           HPC_HIPCHK(mark(e.lev_ev[l + 1][5]));
           searched = true;
@@ -1202,6 +1301,133 @@ int engine_run_single(Engine &e, int id) {
     }
   }
   return rc_run;
+}
+
+// Candidate launch shapes of a level's CI-test kernel: (ranks per task,
+// tasks per workgroup, task order 1 = device queue / 2 = workgroup order),
+// or for level 1 edges per workgroup in the second field.
+struct TuneCand {
+  int rpt, tpb, queue;
+};
+static std::vector<TuneCand> tune_candidates(int level) {
+  std::vector<TuneCand> cand;
+  if (level == 1) {
+    for (int epb : {8, 16, 32, 64, 128}) cand.push_back({0, epb, 0});
+  } else {
+    for (int rpt : {256, 512, 1024, 2048, 4096})
+      for (int tpb : {4, 8, 16, 32, 64, 128})
+        for (int q : {1, 2}) cand.push_back({rpt, tpb, q});
+  }
+  return cand;
+}
+
+static void set_shape(Engine &e, int level, const TuneCand &c) {
+  if (level == 1) {
+    e.epb_level1 = c.tpb;
+  } else {
+    e.rpt_level[level] = c.rpt;
+    e.tpb_level[level] = c.tpb;
+    e.queue_level[level] = c.queue;
+  }
+}
+
+// The batch tuning pass's timing of one level on the current step: every
+// candidate twice (its own task plan first), best of the two; the fastest
+// shape stays set for the level and is recorded in e.tune_out.
+static int tune_step_level(Engine &e, Layout L, int level) {
+  L.max_deg = e.max_deg;
+  const std::vector<TuneCand> cand = tune_candidates(level);
+  hpc_tune_entry t;
+  memset(&t, 0, sizeof(t));
+  t.level = level;
+  const int q_default = e.use_queue ? 1 : 2;
+  double best = 1e300;
+  TuneCand pick = cand[0];
+  for (const TuneCand &c : cand) {
+    set_shape(e, level, c);
+    double ms_min = 1e300;
+    for (int rep = 0; rep < 2; ++rep) {
+      HPC_HIPCHK(hipMemsetAsync(e.d_fp, 0xFF, (size_t)e.n_edges * sizeof(int64_t), e.stream));
+      if (level >= 2)
+        launch_task_plan(e.stream, level, L, engine_rpt(e, level), nullptr, e.d_row_tasks,
+                         e.d_task_off);
+      HPC_HIPCHK(hipEventRecord(e.ev[6], e.stream));
+      if (level == 1)
+        launch_search_l1(e.stream, L, e.n_edges, nullptr, false, engine_epb(e),
+                         search_lds_limit(e.lds_limit));
+      else
+        launch_search_ln(e.stream, level, L, engine_rpt(e, level), e.d_row_tasks, e.d_task_off,
+                         false, engine_tpb(e, level), search_lds_limit(e.lds_limit),
+                         device_queue(e, level));
+      HPC_HIPCHK(hipGetLastError());
+      HPC_HIPCHK(hipEventRecord(e.ev[7], e.stream));
+      HPC_HIPCHK(hipEventSynchronize(e.ev[7]));
+      ms_min = std::min(ms_min, (double)elapsed(e.ev[6], e.ev[7]));
+    }
+    const bool dflt = level == 1 ? c.tpb == 32 : (c.rpt == 512 && c.tpb == 16 && c.queue == q_default);
+    if (dflt) t.ms_default = ms_min;
+    if (ms_min < best) {
+      best = ms_min;
+      pick = c;
+    }
+  }
+  set_shape(e, level, pick);
+  if (level == 1) {
+    t.edges_per_block = pick.tpb;
+  } else {
+    t.ranks_per_task = pick.rpt;
+    t.tasks_per_block = pick.tpb;
+    t.queue = pick.queue;
+  }
+  t.ms_best = best;
+  e.tune_out.push_back(t);
+  e.tuned[level] = true;
+  return HPC_OK;
+}
+
+int engine_batch_autotune(Engine &e, std::vector<hpc_tune_entry> &out) {
+  std::vector<int> ids;
+  for (size_t k = 0; k < e.inst.size(); ++k)
+    if (e.inst[k].used && !e.inst[k].finished) ids.push_back((int)k);
+  out.clear();
+  if (ids.empty()) return HPC_OK;
+  for (int id : ids)
+    if (e.inst[id].fast) return hpc_fail(HPC_ESTATE, "the batch tuner needs exact instances");
+  // from every instance's start (instances with a correlation keep it)
+  for (int id : ids) {
+    InstHost &h = e.inst[id];
+    if (h.have_cor) {
+      int rc = engine_reset(e, id);
+      if (rc) return rc;
+      h.hist.clear();
+    }
+  }
+  e.tune_pass = true;
+  e.tune_out.clear();
+  for (auto &v : e.tuned) v = false;
+  int rc = HPC_OK;
+  for (int guard = 0; guard < 4 * kMaxLevel + 8; ++guard) {
+    bool left = false;
+    for (int id : ids) left |= !e.inst[id].finished;
+    if (!left) break;
+    rc = engine_step(e, ids);
+    if (rc) break;
+  }
+  e.tune_pass = false;
+  if (rc) return rc;
+  out = e.tune_out;
+  std::sort(out.begin(), out.end(),
+            [](const hpc_tune_entry &a, const hpc_tune_entry &b) { return a.level < b.level; });
+  // the pass's results go: every instance back to its start
+  for (int id : ids) {
+    InstHost &h = e.inst[id];
+    rc = engine_reset(e, id);
+    if (rc) return rc;
+    h.hist.clear();
+    h.error = 0;
+    h.message.clear();
+  }
+  return HPC_OK;
 }
 
 int engine_step(Engine &e, const std::vector<int> &ids) {
@@ -1357,6 +1583,25 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
       levels_ge2.push_back(l);
   }
   std::sort(levels_ge2.begin(), levels_ge2.end());
+  // a tuning pass (engine_batch_autotune): each exact search level met for
+  // the first time is timed on this step's snapshot with every candidate
+  // shape and the fastest kept; the searches below then run with it (and
+  // plan their tasks again: the candidates' plans replaced the compaction's)
+  bool replan = false;
+  if (e.tune_pass && e.n_edges > 0) {
+    std::vector<int> lv;
+    if (run1 && !any_fast_at(e, 1)) lv.push_back(1);
+    for (int l : levels_ge2)
+      if (!any_fast_at(e, l)) lv.push_back(l);
+    for (int l : lv) {
+      if (e.tuned[l]) continue;
+      rc = tune_step_level(e, L, l);
+      if (rc) return rc;
+      replan = true;
+    }
+    if (replan)
+      HPC_HIPCHK(hipMemsetAsync(e.d_fp, 0xFF, (size_t)e.n_edges * sizeof(int64_t), e.stream));
+  }
   // search kernels, each bracketed by its own event pair (task plans of
   // levels >= 2 run outside the pairs)
   int npairs = 0;
@@ -1372,12 +1617,12 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
     }
     for (int l : levels_ge2) {
       // (the fused compaction planned the tasks of its level)
-      if (!fused)
+      if (!fused || replan)
         launch_task_plan(e.stream, l, L, engine_rpt(e, l), nullptr, e.d_row_tasks, e.d_task_off);
       HPC_HIPCHK(hipEventRecord(e.sev[2 * npairs], e.stream));
       const bool fl = any_fast_at(e, l);
       launch_search_ln(e.stream, l, L, engine_rpt(e, l), e.d_row_tasks, e.d_task_off, fl,
-                       engine_tpb(e, l), search_lds_limit(e.lds_limit));
+                       engine_tpb(e, l), search_lds_limit(e.lds_limit), device_queue(e, l));
       if (fl) launch_recheck(e.stream, l, L);
       HPC_HIPCHK(hipEventRecord(e.sev[2 * npairs + 1], e.stream));
       ++npairs;
@@ -1469,6 +1714,12 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
   return HPC_OK;
 }
 
+TaskQueue device_queue(const Engine &e, int level) {
+  const int q = (level >= 0 && level <= kMaxLevel) ? e.queue_level[level] : 0;
+  const bool on = q == 1 || (q == 0 && e.use_queue);
+  return TaskQueue{on ? e.d_qhead : nullptr, nullptr};
+}
+
 bool engine_plannable(const Engine &e, int max_deg, int level) {
   if (level < 2 || max_deg - 1 < level) return true;
   const size_t row = (size_t)(max_deg - 1);
@@ -1507,6 +1758,8 @@ int engine_check_tuning(const hpc_tune_entry *t, int count) {
       if (x.tasks_per_block < 1 || x.tasks_per_block > 1024)
         return hpc_fail(HPC_EINVAL, "tuning entry %d: tasks per workgroup %d outside 1..1024", k,
                         x.tasks_per_block);
+      if (x.queue < 0 || x.queue > 2)
+        return hpc_fail(HPC_EINVAL, "tuning entry %d: queue %d outside 0..2", k, x.queue);
     }
   }
   return HPC_OK;
@@ -1522,6 +1775,7 @@ int engine_apply_tuning(Engine &e, const hpc_tune_entry *t, int count) {
     } else {
       e.rpt_level[x.level] = x.ranks_per_task;
       e.tpb_level[x.level] = x.tasks_per_block;
+      e.queue_level[x.level] = x.queue;
     }
   }
   return HPC_OK;
@@ -1544,19 +1798,12 @@ int engine_autotune(Engine &e, int id, int max_level, std::vector<hpc_tune_entry
     hpc_tune_entry t;
     memset(&t, 0, sizeof(t));
     t.level = l;
-    // candidates (ranks per task, tasks per workgroup) or edges per workgroup
-    std::vector<std::pair<int, int>> cand;
-    if (l == 1) {
-      for (int epb : {8, 16, 32, 64, 128}) cand.push_back({0, epb});
-    } else {
-      for (int rpt : {256, 512, 1024, 2048, 4096})
-        for (int tpb : {4, 8, 16, 32, 64, 128}) cand.push_back({rpt, tpb});
-    }
+    const std::vector<TuneCand> cand = tune_candidates(l);
     double best = 1e300;
-    std::pair<int, int> pick = cand[0];
+    TuneCand pick = cand[0];
+    const int q_default = e.use_queue ? 1 : 2;
     for (const auto &c : cand) {
-      if (l == 1) e.epb_level1 = c.second;
-      else { e.rpt_level[l] = c.first; e.tpb_level[l] = c.second; }
+      set_shape(e, l, c);
       double ms_min = 1e300;
       for (int rep = 0; rep < 2; ++rep) {
         HPC_HIPCHK(hipMemsetAsync(e.d_fp, 0xFF, (size_t)e.n_edges * sizeof(int64_t), e.stream));
@@ -1565,18 +1812,20 @@ int engine_autotune(Engine &e, int id, int max_level, std::vector<hpc_tune_entry
         if (rc) return rc;
         ms_min = std::min(ms_min, (double)ms);
       }
-      const bool dflt = l == 1 ? c.second == 32 : (c.first == 512 && c.second == 16);
+      const bool dflt = l == 1 ? c.tpb == 32 : (c.rpt == 512 && c.tpb == 16 && c.queue == q_default);
       if (dflt) t.ms_default = ms_min;
       if (ms_min < best) { best = ms_min; pick = c; }
     }
     if (l == 1) {
-      e.epb_level1 = pick.second;
-      t.edges_per_block = pick.second;
+      e.epb_level1 = pick.tpb;
+      t.edges_per_block = pick.tpb;
     } else {
-      e.rpt_level[l] = pick.first;
-      e.tpb_level[l] = pick.second;
-      t.ranks_per_task = pick.first;
-      t.tasks_per_block = pick.second;
+      e.rpt_level[l] = pick.rpt;
+      e.tpb_level[l] = pick.tpb;
+      e.queue_level[l] = pick.queue;
+      t.ranks_per_task = pick.rpt;
+      t.tasks_per_block = pick.tpb;
+      t.queue = pick.queue;
     }
     t.ms_best = best;
     out.push_back(t);

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <atomic>
 #include <chrono>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -108,6 +109,77 @@ long long search_edge(const HostSnap &s, int i, int y, unsigned long long &tests
     ++rank;
   } while (next_comb<L>(q, nn));
   return -1;
+}
+
+// Ranks [r_begin, r_end) of ordered edge (i, N(i)[y]) in lexicographic
+// order, as one task of the device's plan: the lowest passing rank or -1;
+// `tests` counts the sets tested.
+template <int L>
+long long search_range(const HostSnap &s, int i, int y, unsigned long long r_begin,
+                       unsigned long long r_end, unsigned long long &tests,
+                       unsigned long long &fallbacks) {
+  const int base = s.rowptr[i];
+  const int nn = s.rowptr[i + 1] - base - 1;
+  const int j = s.nbr[base + y];
+  const double *cor = s.cor;
+  const size_t p = s.p;
+  const double a = cor[(size_t)i * p + j];
+  if (r_begin >= r_end) return -1;
+  if (L == 1) {
+    for (unsigned long long q = r_begin; q < r_end; ++q) {
+      const int k = slot_node(s, base, y, (int)q);
+      const double x2 = cor[(size_t)i * p + k], x3 = cor[(size_t)j * p + k];
+      ++tests;
+      if (!(isnan(a) || isnan(x2) || isnan(x3)) && removes(pcor_l1(a, x2, x3), s.r_crit))
+        return (long long)q;
+    }
+    return -1;
+  }
+  int q[L];
+  unrank<L>(s.binom, kBinomK, r_begin, nn, q);
+  for (unsigned long long rank = r_begin; rank < r_end; ++rank) {
+    int S[L];
+    for (int c = 0; c < L; ++c) S[c] = slot_node(s, base, y, q[c]);
+    double x[L], yv[L], C[L][L];
+    bool anynan = isnan(a);
+    for (int c = 0; c < L; ++c) {
+      x[c] = cor[(size_t)i * p + S[c]];
+      yv[c] = cor[(size_t)j * p + S[c]];
+      anynan |= isnan(x[c]) || isnan(yv[c]);
+    }
+    for (int c = 0; c < L; ++c)
+      for (int r = c + 1; r < L; ++r) {
+        C[r][c] = cor[(size_t)S[c] * p + S[r]];
+        anynan |= isnan(C[r][c]);
+      }
+    ++tests;
+    if (!anynan) {
+      bool fb;
+      const double r = pcor_canon<L>(a, x, yv, C, fb);
+      fallbacks += fb ? 1 : 0;
+      if (removes(r, s.r_crit)) return (long long)rank;
+    }
+    next_comb<L>(q, nn);
+  }
+  return -1;
+}
+
+using RangeFn = long long (*)(const HostSnap &, int, int, unsigned long long,
+                              unsigned long long, unsigned long long &, unsigned long long &);
+
+template <int L>
+RangeFn pick_range() { return &search_range<L>; }
+
+RangeFn range_for(int level) {
+  switch (level) {
+    case 1: return pick_range<1>();   case 2: return pick_range<2>();   case 3: return pick_range<3>();
+    case 4: return pick_range<4>();   case 5: return pick_range<5>();   case 6: return pick_range<6>();
+    case 7: return pick_range<7>();   case 8: return pick_range<8>();   case 9: return pick_range<9>();
+    case 10: return pick_range<10>(); case 11: return pick_range<11>(); case 12: return pick_range<12>();
+    case 13: return pick_range<13>(); case 14: return pick_range<14>(); case 15: return pick_range<15>();
+    case 16: return pick_range<16>();
+    default: return nullptr;
+  }
 }
 
 using SearchFn = long long (*)(const HostSnap &, int, int, unsigned long long &,
@@ -243,6 +315,119 @@ extern "C" int hpc_search_level_host(hpc_state *st, int level, const int32_t *ro
     s.edges_left = h.undirected;
     s.ms_wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *stats = s;
+  }
+  return HPC_OK;
+}
+
+extern "C" int hpc_search_level_shared(hpc_state *st, int level, int threads,
+                                       hpc_level_stats *device_stats,
+                                       hpc_level_stats *host_stats) {
+  if (!st) return hpc_fail(HPC_EINVAL, "state is NULL");
+  Engine &e = st->eng;
+  InstHost &h = e.inst[st->id];
+  if (level < 1 || level > kMaxLevel)
+    return hpc_fail(HPC_ELEVEL, "the shared queue supports levels 1..%d, not %d", kMaxLevel, level);
+  if (!h.compacted) return hpc_fail(HPC_ESTATE, "level %d needs hpc_compact first", level);
+  if (h.fast)
+    return hpc_fail(HPC_ESTATE, "the shared queue needs an exact correlation (HPC_CORR_CANON)");
+  auto t0 = std::chrono::steady_clock::now();
+  engine_note_level(e, st->id, level);
+  int rc = engine_pack(e, {st->id}, {level});
+  SharedPlan plan;
+  if (!rc) rc = engine_search_shared_launch(e, level, plan);
+  if (rc) return rc;
+  // host threads: blocks from the back, until they reach one the device has
+  // finished or the device's kernels are done (every block is then claimed
+  // there, and each claimed block that the host had not finished was run)
+  std::atomic<long long> tail{plan.n_blocks};
+  std::atomic<bool> device_over{plan.n_blocks == 0};
+  std::atomic<unsigned long long> T{0}, E{0}, F{0}, B{0};
+  std::mutex out_mu;
+  std::vector<int64_t> idx, val;
+  HostSnap snap{e.h_cor.data(), e.h_rowptr.data(), e.h_nbr.data(), e.h_binom.data(), h.p, h.r_crit};
+  RangeFn fn = range_for(level);
+  const int p = e.p_total;
+  auto device_finished = [&]() -> bool {
+    if (device_over.load(std::memory_order_relaxed)) return true;
+    if (hipEventQuery(plan.device_done) == hipSuccess) device_over.store(true);
+    return device_over.load(std::memory_order_relaxed);
+  };
+  auto worker = [&]() {
+    unsigned long long t = 0, ed = 0, f = 0, blocks = 0;
+    std::vector<int64_t> li, lv;
+    for (;;) {
+      const long long c = tail.fetch_sub(1) - 1;
+      if (c < 0 || device_finished()) break;
+      if (__atomic_load_n(&plan.done[c], __ATOMIC_ACQUIRE)) break;  // the device's side
+      const long long b0 = c * plan.tpb, b1 = std::min(plan.n_tasks, b0 + plan.tpb);
+      bool whole = true;
+      for (long long task = b0; task < b1; ++task) {
+        if (device_finished()) { whole = false; break; }
+        // the task's row: the last row whose offset is <= task
+        const long long *off = e.h_task_off.data();
+        int lo = 0, hi = p - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (off[mid] <= task) lo = mid; else hi = mid - 1;
+        }
+        const int i = lo;
+        const int deg = snap.rowptr[i + 1] - snap.rowptr[i];
+        const long long nch = e.h_row_tasks[i] / deg;
+        const long long local = task - off[i];
+        const int y = (int)(local / nch);
+        const unsigned long long total = binom(snap.binom, kBinomK, deg - 1, level);
+        const unsigned long long r_begin = (unsigned long long)(local % nch) * plan.rpt;
+        const unsigned long long r_end = std::min(total, r_begin + (unsigned long long)plan.rpt);
+        const long long r = fn(snap, i, y, r_begin, r_end, t, f);
+        if (r_begin == 0) ++ed;
+        if (r >= 0) {
+          li.push_back(snap.rowptr[i] + y);
+          lv.push_back(r);
+        }
+      }
+      if (!whole) break;
+      __atomic_store_n(&plan.done[c], 1, __ATOMIC_RELEASE);
+      ++blocks;
+    }
+    T += t; E += ed; F += f; B += blocks;
+    std::lock_guard<std::mutex> lk(out_mu);
+    idx.insert(idx.end(), li.begin(), li.end());
+    val.insert(val.end(), lv.begin(), lv.end());
+  };
+  if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+  std::vector<std::thread> pool;
+  if (plan.n_blocks > 0) {
+    for (int k = 1; k < threads; ++k) pool.emplace_back(worker);
+    worker();
+    for (auto &th : pool) th.join();
+  }
+  float ms = 0.f;
+  rc = engine_search_shared_finish(e, idx, val, plan.n_blocks > 0 ? &ms : nullptr);
+  if (rc) return rc;
+  const double wall =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (device_stats) {
+    hpc_level_stats s;
+    memset(&s, 0, sizeof(s));
+    s.level = level;
+    s.max_deg = h.max_deg;
+    s.edges_left = h.undirected;
+    s.ms_test = ms;
+    s.ms_wall = wall;
+    *device_stats = s;
+  }
+  if (host_stats) {
+    hpc_level_stats s;
+    memset(&s, 0, sizeof(s));
+    s.level = level;
+    s.max_deg = h.max_deg;
+    s.tests = (int64_t)T.load();
+    s.edges = (int64_t)E.load();
+    s.removed = (int64_t)B.load();
+    s.pinv_fallbacks = (int64_t)F.load();
+    s.edges_left = h.undirected;
+    s.ms_wall = wall;
+    *host_stats = s;
   }
   return HPC_OK;
 }

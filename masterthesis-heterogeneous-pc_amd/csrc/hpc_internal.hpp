@@ -240,14 +240,27 @@ void launch_task_plan(hipStream_t s, int level, const Layout &L, int rpt,
 // The plan's per-row task counts and exclusive offsets define the tasks; the
 // kernels read the task count from the device counters.
 // tpb: tasks per workgroup of the LDS-staged form
+// A task queue of the level >= 1 search kernels: `head` (device memory, 0
+// between launches; the kernel leaves it so) is claimed a block of tasks at a
+// time by the workgroups; with `done` (one int per block in host-visible
+// memory) host threads take blocks from the other end and each side skips
+// blocks the other has finished.  No head: blocks go to workgroups by id.
+struct TaskQueue {
+  unsigned long long *head;
+  int *done;
+};
+// level 1 here only through a shared queue (exact instances)
 void launch_search_ln(hipStream_t s, int level, const Layout &L, int rpt,
                       const long long *row_tasks, const long long *task_off, bool fast, int tpb,
-                      size_t lds_limit);
+                      size_t lds_limit, const TaskQueue &queue = TaskQueue{nullptr, nullptr});
 // With pub.flag set, the last workgroup to finish publishes the counters.
 void launch_commit(hipStream_t s, const Layout &L, int64_t n_edges, const Publish &pub,
                    const NextCount &next = NextCount{nullptr, nullptr, 0, 0, 0});
 void launch_scatter_fp(hipStream_t s, const int64_t *idx, const int64_t *val,
                        long long count, int64_t *fp);
+// fp[idx[t]] = min(fp[idx[t]], val[t]) (ranks >= 0; -1 counts as none).
+void launch_scatter_min_fp(hipStream_t s, const int64_t *idx, const int64_t *val,
+                           long long count, int64_t *fp);
 
 // ---------------------------------------------------------------------------
 // Host side
@@ -362,6 +375,11 @@ struct Engine {
   // task `ranks_per_task`, 16 tasks and 32 ordered edges per workgroup)
   int rpt_level[kMaxLevel + 1] = {};
   int tpb_level[kMaxLevel + 1] = {};
+  int queue_level[kMaxLevel + 1] = {};  // 0 = use_queue, 1 = device queue, 2 = workgroup order
+  // batch tuning pass (engine_batch_autotune): levels tuned so far and their entries
+  bool tune_pass = false;
+  bool tuned[kMaxLevel + 1] = {};
+  std::vector<hpc_tune_entry> tune_out;
   int epb_level1 = 0;
   hipEvent_t ev[8] = {};
   hipEvent_t sev[2 * (kMaxLevel + 1)] = {};  // search kernel brackets of one step
@@ -372,6 +390,13 @@ struct Engine {
   std::vector<int32_t> h_rowptr, h_nbr;
   int64_t *d_scatter_idx = nullptr, *d_scatter_val = nullptr;
   size_t scatter_cap = 0;
+  // task queue: the device head (0 between launches) and the per-block done
+  // flags a queue shared with host threads uses (host-visible memory)
+  unsigned long long *d_qhead = nullptr;
+  int *h_qdone = nullptr, *d_qdone = nullptr;
+  size_t cap_qdone = 0;
+  std::vector<long long> h_task_off, h_row_tasks;
+  bool use_queue = true;  // level >= 2 searches claim task blocks (HPC_TUNE_QUEUE)
   // recheck queues of fast instances
   Recheck *d_rchk = nullptr;
   size_t cap_rchk = 0;
@@ -444,6 +469,21 @@ int engine_exact_from_z(Engine &e, const std::vector<int> &ids);
 int engine_compact(Engine &e, float *ms);
 // Search of the packed instances at `level` (rows: global ids, NULL = all).
 int engine_search(Engine &e, int level, const int32_t *rows, int nrows, float *ms);
+// The level's search on one task queue shared by the device and host threads
+// (one exact instance packed at `level`).  _launch plans the tasks, copies the
+// plan, snapshot and correlation to the host, clears the done flags and
+// queues the persistent kernels; host threads then take blocks from the back
+// (SharedPlan), and _finish lowers the device's first passing ranks with the
+// host's results and waits for the stream.
+struct SharedPlan {
+  long long n_tasks = 0, n_blocks = 0;
+  int rpt = 0, tpb = 0;
+  int *done = nullptr;             // host view of the done flags
+  hipEvent_t device_done = nullptr;  // recorded after the device's kernels
+};
+int engine_search_shared_launch(Engine &e, int level, SharedPlan &plan);
+int engine_search_shared_finish(Engine &e, const std::vector<int64_t> &idx,
+                                const std::vector<int64_t> &val, float *ms);
 // Level 0 of the packed instances at level 0 (tests and removes).
 int engine_level0(Engine &e, float *ms);
 int engine_commit(Engine &e, float *ms);
@@ -461,6 +501,9 @@ int engine_step(Engine &e, const std::vector<int> &ids);
 // length).  Kernels give rows that cannot be planned no tasks; the host fails
 // the level with HPC_ELEVEL.
 bool engine_plannable(const Engine &e, int max_deg, int level);
+// The device task queue of a level >= 2 search: the level's tuned choice,
+// else HPC_TUNE_QUEUE (none when off).
+TaskQueue device_queue(const Engine &e, int level);
 // Launch shapes of a level's CI-test kernel.
 int engine_rpt(const Engine &e, int level);
 int engine_tpb(const Engine &e, int level);
@@ -470,6 +513,12 @@ int engine_epb(const Engine &e);
 // (results do not depend on the shape), keeps the fastest per level in the
 // engine and leaves the instance reset.
 int engine_autotune(Engine &e, int id, int max_level, std::vector<hpc_tune_entry> &out);
+// Autotuner of a batch: its unfinished (exact) instances run from their start
+// once, and each level's CI-test kernel is timed with every candidate shape on
+// the first step that meets the level (the step's own mix of instances); the
+// fastest stays set per level and goes to `out`; every instance is then back
+// at its start (correlations kept).
+int engine_batch_autotune(Engine &e, std::vector<hpc_tune_entry> &out);
 // hpc_run's level loop for one instance on exact correlations (levels from
 // h.next_level to its cap), with one host wait per level.
 int engine_run_single(Engine &e, int id);

@@ -308,6 +308,23 @@ __device__ __forceinline__ int task_row(const long long *__restrict__ task_off, 
   return lo;
 }
 
+// Next block of a task queue for the calling workgroup (every thread gets
+// it).  Each workgroup makes exactly one claim past the last block (its
+// last), so the claim that returns n_blocks + gridDim.x - 1 is the grid's
+// final one and resets the head for the next launch on the stream.
+__device__ __forceinline__ long long claim_block(const TaskQueue &Q, long long n_blocks,
+                                                 long long &s_blk) {
+  __syncthreads();  // every thread has read the previous claim
+  if (threadIdx.x == 0) {
+    const unsigned long long b = atomicAdd(Q.head, 1ull);
+    if (b == (unsigned long long)n_blocks + gridDim.x - 1)
+      __hip_atomic_store(Q.head, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_blk = (long long)b;
+  }
+  __syncthreads();
+  return s_blk;
+}
+
 // Level L >= 2 (testRowLN<>, gpuRowTests.cu:102-205; the general-level form of
 // cpuWorkstealingTests.cpp:39-112) for the rows whose neighbourhood block
 // does not fit the LDS-staged kernel below (row length + 1 > dmin), gathering
@@ -404,9 +421,11 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_kernel(
                 for (int r = c + 1; r < LV; ++r) anynan |= __builtin_isnan(C[r][c]);
             }
             if (!anynan) {
-              bool fb;
+              bool fb = false;
               double r;
-              if constexpr (FAST) {
+              if constexpr (LV == 1) {
+                r = pcor_l1(a, x[0], yv[0]);
+              } else if constexpr (FAST) {
                 double sens;
                 r = pcor_canon_sens<LV>(a, x, yv, C, fb, sens);
                 unc = fast && (fb || !fast_certain(r, r_crit, sens, In.eps));
@@ -442,10 +461,19 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_kernel(
 // the device), one staging per row a block meets.  Rows whose block is larger
 // than dmax x dmax are left to search_ln_kernel.  Same ranks, same arithmetic
 // and the same orientation of every correlation read as search_ln_kernel.
+//
+// With a task queue (Q.head set) the kernel is persistent: each workgroup
+// claims the next block of tpb tasks with one device atomic until the blocks
+// run out, instead of taking blocks blockIdx.x, + gridDim.x, ...  With a
+// shared queue (Q.done set) host threads take blocks from the other end: a
+// workgroup skips a block the host has finished and marks each block it
+// finishes.  Level 1 runs here only in that shared form (one rank = one
+// neighbour k, the closed form of pcor_l1).
 template <int LV, bool FAST>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
     Layout L, int rpt, const long long *__restrict__ task_off,
-    const long long *__restrict__ row_tasks, int tpb, int dmax) {
+    const long long *__restrict__ row_tasks, int tpb, int dmax, TaskQueue Q) {
+  static_assert(LV >= 2 || !FAST, "level 1 takes this kernel in exact mode only");
   extern __shared__ unsigned long long smem[];
   // binomial rows 0..dmax-1 (a staged row's tests reach row deg - 1 < dmax)
   unsigned long long *bt = smem;                                   // dmax x (LV+1)
@@ -459,9 +487,21 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
     bt[t] = L.binom[(size_t)m * kBinomK + k];
   }
   const long long n_tasks = (long long)L.ctr->n_tasks;
+  const long long n_blocks = (n_tasks + tpb - 1) / tpb;
   unsigned long long F = 0;
   unsigned long long *fpu = reinterpret_cast<unsigned long long *>(L.fp);
-  for (long long blk = blockIdx.x; blk * tpb < n_tasks; blk += gridDim.x) {
+  __shared__ long long s_blk;
+  __shared__ int s_done;
+  for (long long blk = Q.head ? claim_block(Q, n_blocks, s_blk) : blockIdx.x; blk < n_blocks;
+       blk = Q.head ? claim_block(Q, n_blocks, s_blk) : blk + gridDim.x) {
+    if (Q.done) {
+      // the host's end of a shared queue: skip a block it has finished (the
+      // next claim's barrier orders this read before s_done is rewritten)
+      if (tid == 0)
+        s_done = __hip_atomic_load(&Q.done[blk], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+      __syncthreads();
+      if (s_done) continue;
+    }
     long long t0 = blk * tpb;
     const long long t1 = min(n_tasks, t0 + tpb);
     while (t0 < t1) {
@@ -547,13 +587,15 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
                   for (int r = c + 1; r < LV; ++r) anynan |= __builtin_isnan(C[r][c]);
               }
               if (!anynan) {
-                bool fb, unc = false;
+                bool fb = false, unc = false;
                 double r;
                 // the fast instantiation runs one straight-line path for
                 // every instance (a branch on In.fast kept both paths'
                 // registers live and halved occupancy); exact instances of a
                 // mixed launch get the same statistic and ignore the bound
-                if constexpr (FAST) {
+                if constexpr (LV == 1) {
+                  r = pcor_l1(a, x[0], yv[0]);
+                } else if constexpr (FAST) {
                   double sens;
                   r = pcor_canon_sens<LV>(a, x, yv, C, fb, sens);
                   unc = fast && (fb || !fast_certain(r, r_crit, sens, In.eps));
@@ -587,6 +629,10 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
       }
       t0 = seg_end;
     }
+    // the block is finished on this side (other waves may still be on their
+    // last tasks; the host then skips a block the device completes anyway)
+    if (Q.done && tid == 0)
+      __hip_atomic_store(&Q.done[blk], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 #pragma unroll
   for (int s = 32; s > 0; s >>= 1) F += __shfl_xor(F, s, 64);
@@ -1022,6 +1068,14 @@ __global__ void scatter_kernel(const int64_t *__restrict__ idx, const int64_t *_
   if (t < count) fp[idx[t]] = val[t];
 }
 
+// Lowers first passing ranks (-1 = none is the largest unsigned value).
+__global__ void scatter_min_kernel(const int64_t *__restrict__ idx, const int64_t *__restrict__ val,
+                                   long long count, int64_t *__restrict__ fp) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count)
+    atomicMin(reinterpret_cast<unsigned long long *>(fp) + idx[t], (unsigned long long)val[t]);
+}
+
 // LDS bytes of search_ln_lds_kernel for blocks of side d (row length + 1):
 // binomial rows 0..d-1, the d x d neighbourhood block and its ids.
 size_t lds_bytes(int d_in, int lv) {
@@ -1048,13 +1102,13 @@ constexpr int kL1GatherEpb = 64;
 // read the same task plan; each skips the other's rows.
 template <int LV, bool FAST>
 void launch_ln_as(hipStream_t s, const Layout &L, int rpt, const long long *row_tasks,
-                  const long long *task_off, int tpb, size_t limit) {
+                  const long long *task_off, int tpb, size_t limit, const TaskQueue &Q) {
   const int dmax = std::min(L.max_deg + 1, lds_side(limit, LV));
   // a testable row has deg - 1 >= LV, so a block side of at least LV + 2
   if (dmax >= LV + 2)
     hipLaunchKernelGGL((search_ln_lds_kernel<LV, FAST>), dim3(kSearchBlocks),
                        dim3(64 * kWavesPerBlock), lds_bytes(dmax, LV), s, L, rpt, task_off,
-                       row_tasks, tpb, dmax);
+                       row_tasks, tpb, dmax, Q);
   if (L.max_deg + 1 <= dmax) return;
   const size_t lds = (size_t)(L.max_deg + 1) * (LV + 1) * sizeof(unsigned long long);
   const int use = lds <= 48 * 1024;
@@ -1066,9 +1120,16 @@ void launch_ln_as(hipStream_t s, const Layout &L, int rpt, const long long *row_
 // instantiated into launches that hold a fast instance.
 template <int LV>
 void launch_ln(hipStream_t s, const Layout &L, int rpt, const long long *row_tasks,
-               const long long *task_off, bool fast, int tpb, size_t limit) {
-  if (fast) launch_ln_as<LV, true>(s, L, rpt, row_tasks, task_off, tpb, limit);
-  else launch_ln_as<LV, false>(s, L, rpt, row_tasks, task_off, tpb, limit);
+               const long long *task_off, bool fast, int tpb, size_t limit, const TaskQueue &Q) {
+  if (fast) launch_ln_as<LV, true>(s, L, rpt, row_tasks, task_off, tpb, limit, Q);
+  else launch_ln_as<LV, false>(s, L, rpt, row_tasks, task_off, tpb, limit, Q);
+}
+
+// Level 1 through the task-plan kernels (the shared queue): exact only.
+template <>
+void launch_ln<1>(hipStream_t s, const Layout &L, int rpt, const long long *row_tasks,
+                  const long long *task_off, bool, int tpb, size_t limit, const TaskQueue &Q) {
+  launch_ln_as<1, false>(s, L, rpt, row_tasks, task_off, tpb, limit, Q);
 }
 
 }  // namespace
@@ -1116,23 +1177,24 @@ void launch_task_plan(hipStream_t s, int level, const Layout &L, int rpt,
 
 void launch_search_ln(hipStream_t s, int level, const Layout &L, int rpt,
                       const long long *row_tasks, const long long *task_off, bool fast, int tpb,
-                      size_t limit) {
+                      size_t limit, const TaskQueue &Q) {
   switch (level) {
-    case 2: launch_ln<2>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
-    case 3: launch_ln<3>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
-    case 4: launch_ln<4>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
-    case 5: launch_ln<5>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
-    case 6: launch_ln<6>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
-    case 7: launch_ln<7>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
-    case 8: launch_ln<8>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
-    case 9: launch_ln<9>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
-    case 10: launch_ln<10>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
-    case 11: launch_ln<11>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
-    case 12: launch_ln<12>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
-    case 13: launch_ln<13>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
-    case 14: launch_ln<14>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
-    case 15: launch_ln<15>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
-    case 16: launch_ln<16>(s, L, rpt, row_tasks, task_off, fast, tpb, limit); break;
+    case 1: launch_ln<1>(s, L, rpt, row_tasks, task_off, fast, tpb, limit, Q); break;
+    case 2: launch_ln<2>(s, L, rpt, row_tasks, task_off, fast, tpb, limit, Q); break;
+    case 3: launch_ln<3>(s, L, rpt, row_tasks, task_off, fast, tpb, limit, Q); break;
+    case 4: launch_ln<4>(s, L, rpt, row_tasks, task_off, fast, tpb, limit, Q); break;
+    case 5: launch_ln<5>(s, L, rpt, row_tasks, task_off, fast, tpb, limit, Q); break;
+    case 6: launch_ln<6>(s, L, rpt, row_tasks, task_off, fast, tpb, limit, Q); break;
+    case 7: launch_ln<7>(s, L, rpt, row_tasks, task_off, fast, tpb, limit, Q); break;
+    case 8: launch_ln<8>(s, L, rpt, row_tasks, task_off, fast, tpb, limit, Q); break;
+    case 9: launch_ln<9>(s, L, rpt, row_tasks, task_off, fast, tpb, limit, Q); break;
+    case 10: launch_ln<10>(s, L, rpt, row_tasks, task_off, fast, tpb, limit, Q); break;
+    case 11: launch_ln<11>(s, L, rpt, row_tasks, task_off, fast, tpb, limit, Q); break;
+    case 12: launch_ln<12>(s, L, rpt, row_tasks, task_off, fast, tpb, limit, Q); break;
+    case 13: launch_ln<13>(s, L, rpt, row_tasks, task_off, fast, tpb, limit, Q); break;
+    case 14: launch_ln<14>(s, L, rpt, row_tasks, task_off, fast, tpb, limit, Q); break;
+    case 15: launch_ln<15>(s, L, rpt, row_tasks, task_off, fast, tpb, limit, Q); break;
+    case 16: launch_ln<16>(s, L, rpt, row_tasks, task_off, fast, tpb, limit, Q); break;
     default: break;  // rejected by the caller (kMaxLevel)
   }
 }
@@ -1154,6 +1216,13 @@ void launch_scatter_fp(hipStream_t s, const int64_t *idx, const int64_t *val, lo
   if (count <= 0) return;
   hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, idx,
                      val, count, fp);
+}
+
+void launch_scatter_min_fp(hipStream_t s, const int64_t *idx, const int64_t *val, long long count,
+                           int64_t *fp) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(scatter_min_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s,
+                     idx, val, count, fp);
 }
 
 }  // namespace hpc

@@ -25,6 +25,7 @@ HPC_TUNE_RECHECK_QUEUE = 2
 HPC_TUNE_KERNEL_TIMING = 3
 HPC_TUNE_LDS_LIMIT = 4
 HPC_TUNE_WAIT_LIMIT = 5
+HPC_TUNE_QUEUE = 6
 
 
 class LevelStats(C.Structure):
@@ -78,6 +79,8 @@ class TuneEntry(C.Structure):
         ("ranks_per_task", C.c_int32),
         ("tasks_per_block", C.c_int32),
         ("edges_per_block", C.c_int32),
+        ("queue", C.c_int32),
+        ("pad_", C.c_int32),
         ("ms_best", C.c_double),
         ("ms_default", C.c_double),
     ]
@@ -167,6 +170,8 @@ SIGNATURES = {
     "hpc_plan_tasks": (_i, [_u64, _i, _i, C.POINTER(_i64)]),
     "hpc_wait_word": (_i, [_vp, _u64, _d]),
     "hpc_corr_noalloc": (_i, [_i, _vp, _u64, _i, _vp, _vp]),
+    "hpc_search_level_shared": (_i, [_vp, _i, _i, _ps, _ps]),
+    "hpc_batch_autotune": (_i, [_vp, C.POINTER(TuneEntry), _i, C.POINTER(C.c_int)]),
 }
 
 _lib = None

@@ -135,6 +135,17 @@ class Batch:
         kernels."""
         check(self.lib.hpc_batch_set_tuning(self.h, 4, int(nbytes)))
 
+    def autotune(self) -> list[dict]:
+        """Per-level launch shapes timed on the batch's own steps; every
+        instance is left at its start."""
+        from ._lib import TuneEntry
+
+        buf = (TuneEntry * 32)()
+        n = C.c_int()
+        check(self.lib.hpc_batch_autotune(self.h, buf, 32, C.byref(n)))
+        return [{name: getattr(buf[k], name) for name, _ in TuneEntry._fields_}
+                for k in range(min(n.value, 32))]
+
     def inject_failure(self, iid: int, level: int) -> None:
         """Test hook: the host side of instance iid's `level` fails."""
         check(self.lib.hpc_batch_inject_failure(self.h, int(iid), int(level)))

@@ -179,6 +179,20 @@ class DeviceSkeleton:
             check(self.lib.hpc_search_level(self.h, int(level), ptr, cnt, C.byref(s)))
         return s.as_dict()
 
+    def search_level_shared(self, level: int, threads: int = 0) -> tuple[dict, dict]:
+        """One queue of the level's tasks, the device taking blocks from the
+        front and `threads` host threads from the back; returns (device,
+        host) records (host `removed` = blocks the host finished).
+        hpc_commit_level then removes the edges."""
+        d, h = LevelStats(), LevelStats()
+        check(self.lib.hpc_search_level_shared(self.h, int(level), int(threads), C.byref(d),
+                                               C.byref(h)))
+        return d.as_dict(), h.as_dict()
+
+    def set_queue(self, on: bool) -> None:
+        """Device task queue of the level >= 2 CI-test kernel (default on)."""
+        check(self.lib.hpc_set_tuning(self.h, 6, 1 if on else 0))
+
     def commit_level(self, level: int) -> dict:
         s = LevelStats()
         check(self.lib.hpc_commit_level(self.h, int(level), C.byref(s)))

@@ -131,6 +131,20 @@ std::tuple<TestResult, TestResult> Balancer::execute(int level, int numThreads) 
 
 std::tuple<TestResult, TestResult> Balancer::executeWorkstealing(int level, int numThreads) {
   if (verbose) std::cout << "Start workstealing execution..." << std::endl;
+  if (level >= 1 && heterogeneity == All) {
+    // one task queue: the device claims from the front, host threads from
+    // the back, until they meet (balancer.cpp:186-234 on one GPU)
+    hpc_level_stats dev{}, host{};
+    hpcCheck(hpc_search_level_shared(state->dev, level, numThreads, &dev, &host),
+             "work-stealing level");
+    if (verbose)
+      std::cout << "Shared queue: the host finished " << host.removed << " blocks ("
+                << host.tests << " tests) beside the device." << std::endl;
+    std::tuple<TestResult, TestResult> res{TestResult{(float)host.ms_wall, host.tests},
+                                           TestResult{(float)dev.ms_wall, dev.tests}};
+    commit(level, res);
+    return res;
+  }
   cpuExecutor->cleanupSplitTasks();
   gpuExecutor->cleanupSplitTasks();
   if (level == 0 || heterogeneity == GPUOnly)

@@ -28,9 +28,10 @@ struct Balancer {
   // balancer.cpp:138-184: both executors concurrently, then the level's
   // removals (one commit over both shares).
   std::tuple<TestResult, TestResult> execute(int level, int numThreads);
-  // balancer.cpp:186-234: the host takes the longest rows (row-mult share of
-  // the thread count), the device the rest, concurrently; every level is
-  // tested.
+  // balancer.cpp:186-234: every level is tested.  With both sides (All) the
+  // level's tasks form one queue that the device takes from the front and
+  // host threads from the back (hpc_search_level_shared); GPU-only and
+  // CPU-only keep their one-sided shares.
   std::tuple<TestResult, TestResult> executeWorkstealing(int level, int numThreads);
 
   Balancer() {}

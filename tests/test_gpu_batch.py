@@ -238,3 +238,31 @@ def test_server_applied_tuning_keeps_results(hpc, orc):
             assert np.array_equal(sep[:, :, :mc], ref.sepsets()), f"{label}: sepsets differ"
             assert (sep[:, :, mc:] == -1).all()
             assert info["tests"] == sum(o["tests"] for o in recs), label
+
+
+def test_batch_autotune_keeps_results(hpc, orc):
+    """The batch tuner times each level's kernel on the batch's own steps and
+    leaves every instance at its start: running the same batch afterwards,
+    and another batch with the tuned shapes, gives the oracle's results."""
+    inputs = _mixed_inputs(hpc)[:8]
+    with hpc.Batch() as b:
+        ids = [b.add(x, a, ml) for _, x, a, ml in inputs]
+        tune = b.autotune()
+        print(f"[batch] tuned: {[(t['level'], t['ranks_per_task'], t['tasks_per_block'], t['queue'], t['edges_per_block']) for t in tune]}", flush=True)
+        levels = [t["level"] for t in tune]
+        assert levels == sorted(set(levels)) and 1 in levels and 2 in levels
+        for t in tune:
+            if t["level"] >= 2:
+                assert t["queue"] in (1, 2) and t["ms_best"] <= t["ms_default"] + 1e-9
+        for iid in ids:
+            finished, lv, err = b.status(iid)
+            assert not finished and lv == 0 and err == 0
+        b.run()
+        for iid, (label, x, a, ml) in zip(ids, inputs):
+            _check_instance(b, iid, f"{label}/tuned-self", orc, x, a, ml)
+    with hpc.Batch() as b2:
+        b2.apply_tuning(tune)
+        ids = [b2.add(x, a, ml) for _, x, a, ml in inputs]
+        b2.run()
+        for iid, (label, x, a, ml) in zip(ids, inputs):
+            _check_instance(b2, iid, f"{label}/tuned", orc, x, a, ml)
