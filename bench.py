@@ -1,12 +1,20 @@
 """Benchmark of the PC-stable skeleton hot path on one MI355X.
 
 A step is one full skeleton of the plain p=2000 workload on device-resident
-observations: correlation (moments + pairwise sums), then levels 0, 1, ...
-(compaction, CI tests, removal) until no edge has enough neighbours.
+observations: correlation (moments + pairwise sums), then levels 0..4
+(compaction, CI tests, removal) -- the reference's default maximum level
+(main.cpp:31), at which this workload still holds 30-neighbour rows.
 
 value = CI tests/s = sum over levels of the canonical test count T_l divided
 by the level wall time (the reference's "Execution duration" scope,
-skeleton.cpp:167-180).  Prints one JSON line.
+skeleton.cpp:167-180).  Beside it: every kernel of the step against the
+roofline that bounds it, the HBM copy bandwidth measured here, a second
+plain leg at p=1000 with no level cap, the FP32-MFMA end-to-end run of the
+same p=2000 skeleton, the batched and served forms, and the reference's
+OpenMP CPU path (median of three runs).  Prints one JSON line.
+
+`python bench.py --copy-probe` runs only device-to-device copies of a known
+size (the counter calibration of the committed rocprofv3 summaries).
 """
 from __future__ import annotations
 
@@ -26,6 +34,60 @@ for _p in (REPO, os.path.join(REPO, "masterthesis-heterogeneous-pc_amd"), os.pat
 METRIC = "CI tests/sec + skeleton wall time at p=2000; served p50/p99 vs plain path"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+
+
+def flops_per_test(level: int) -> float:
+    """FP64 work of one level >= 2 test in the LDL^T / Schur form: the
+    factorisation of the l x l conditioning block, two triangular solves,
+    the 2 x 2 Schur block and the statistic, l^3/3 + 2 l^2 + 6 l + 30."""
+    return level ** 3 / 3.0 + 2.0 * level ** 2 + 6.0 * level + 30.0
+
+
+def hbm_copy(nbytes: int = 1 << 30, reps: int = 10, probe_only: bool = False) -> dict:
+    """HBM bandwidth of a device-to-device copy (the HIP runtime's copy kernel,
+    through the runtime libheterogpc_hip.so links): 2 * nbytes moved per
+    copy, timed with HIP events on the stream the copies go to."""
+    import ctypes as C
+
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    hip = C.CDLL(os.path.join(rocm, "lib", "libamdhip64.so.7"))
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+    hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+    hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+    hip.hipEventSynchronize.argtypes = [C.c_void_p]
+    hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+    hip.hipEventDestroy.argtypes = [C.c_void_p]
+    src, dst = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), nbytes) or hip.hipMalloc(C.byref(dst), nbytes):
+        raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(src, 1, nbytes):
+            raise RuntimeError("hipMemset failed")
+        for _ in range(2):
+            hip.hipMemcpyAsync(dst, src, nbytes, 3, None)  # hipMemcpyDeviceToDevice
+        e0, e1 = C.c_void_p(), C.c_void_p()
+        hip.hipEventCreate(C.byref(e0))
+        hip.hipEventCreate(C.byref(e1))
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            if hip.hipMemcpyAsync(dst, src, nbytes, 3, None):
+                raise RuntimeError("hipMemcpyAsync failed")
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = C.c_float()
+        hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    per = ms.value / reps
+    return {"bytes_per_copy": 2 * nbytes, "ms_per_copy": per,
+            "GBs": 2 * nbytes / (per * 1e-3) / 1e9,
+            "note": "device-to-device hipMemcpyAsync of %d MiB, read + write" % (nbytes >> 20)}
 
 
 def bytes_per_test(level: int) -> int:
@@ -73,29 +135,68 @@ def run_step(dev):
     }
 
 
-def cpu_baseline(x, n, alpha, max_level, tests_total):
+def host_cpu() -> dict:
+    """Model, sockets, cores and threads of the host (lscpu)."""
+    import subprocess
+
+    info = {}
+    try:
+        txt = subprocess.run(["lscpu"], stdout=subprocess.PIPE, text=True, timeout=10).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return {"model": info.get("Model name"), "sockets": info.get("Socket(s)"),
+            "cores_per_socket": info.get("Core(s) per socket"),
+            "threads_per_core": info.get("Thread(s) per core"),
+            "cpus_online": info.get("CPU(s)"), "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def cpu_baseline(x, n, alpha, max_level, gpu_adj, runs=3):
     """The oracle's reference-faithful CPU path (per-test pseudoinverse,
-    immediate deletion with early exit, OpenMP over rows sorted by length)
-    on all host threads, on the same workload."""
+    immediate deletion with early exit, OpenMP `collapse(2)
+    schedule(dynamic, 10)` over rows sorted by length, cpuExecutor.cpp:88-109)
+    on all host threads of this box, on the same workload: median of `runs`
+    runs of the level loop (the correlation once, as the reference's timer
+    excludes it).  value = the CPU's own tests (it stops testing an edge once
+    either side removed it) / its level time.  The CPU's adjacency is
+    compared with the GPU's."""
+    host = host_cpu()  # before OpenMP binds this thread to its first place
     import oracle
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     t0 = time.perf_counter()
     cor = oracle.corr(x, threads=threads)
-    t1 = time.perf_counter()
-    sk = oracle.Skeleton(cor, n, alpha, max_level=max_level, mode=oracle.MODE_PINV,
-                         flags=oracle.FLAG_REF_EARLY_EXIT, threads=threads)
-    lv = sk.run(max_level)
-    t2 = time.perf_counter()
+    t_corr = time.perf_counter() - t0
+    times, tests, adj = [], [], None
+    for _ in range(runs):
+        sk = oracle.Skeleton(cor, n, alpha, max_level=max_level, mode=oracle.MODE_PINV,
+                             flags=oracle.FLAG_REF_EARLY_EXIT, threads=threads)
+        t1 = time.perf_counter()
+        lv = sk.run(max_level)
+        times.append(time.perf_counter() - t1)
+        tests.append(sum(l["tests"] for l in lv))
+        adj = sk.adj()
+    k = int(np.argsort(times)[len(times) // 2])
+    diff = int(np.count_nonzero(np.triu(adj != gpu_adj, 1)))
     return {
-        "value": tests_total / (t2 - t1),
+        "value": tests[k] / times[k],
         "unit": "CI tests/s",
         "cores": threads,
         "kind": "port",
-        "sample": "the whole p=2000 skeleton once (correlation %.1f s, levels %.2f s, %d levels)"
-                  % (t1 - t0, t2 - t1, len(lv)),
-        "skeleton_ms_levels": (t2 - t1) * 1e3,
-        "skeleton_ms_e2e": (t2 - t0) * 1e3,
+        "sample": "the whole p=2000 skeleton, levels 0..%d, median of %d runs (%s s); the "
+                  "correlation once (%.2f s)" % (max_level, runs,
+                                                 ", ".join("%.2f" % t for t in times), t_corr),
+        "runs_s": times,
+        "tests_per_run": tests,
+        "skeleton_ms_levels": times[k] * 1e3,
+        "skeleton_ms_e2e": (times[k] + t_corr) * 1e3,
+        "adjacency_identical_to_gpu": diff == 0,
+        "adjacency_edges_differing": diff,
+        "omp": {"OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"),
+                "OMP_PLACES": os.environ.get("OMP_PLACES")},
+        "host": host,
     }
 
 
@@ -118,19 +219,22 @@ def _plain_skeleton(heterogpc, x, alpha, max_level):
 
 def bench_batched(heterogpc, insts):
     """Batched form: the same instances one per launch sequence (plain) and all
-    in one batch; skeletons/hour of both and whether every result agrees."""
+    in one batch; skeletons/hour of both and whether every result agrees.
+    The batch's launch shapes come from the batch autotuner, run beforehand
+    on a separate batch of the same instances (untimed)."""
     for x, a, ml in insts[:4]:
         _plain_skeleton(heterogpc, x, a, ml)
     with heterogpc.Batch() as b:
-        for x, a, ml in insts[:8]:
+        for x, a, ml in insts:
             b.add(x, a, ml)
-        b.run()
+        tuned = b.autotune()
     heterogpc.load().hpc_device_synchronize(0)
     t0 = time.perf_counter()
     plain = [_plain_skeleton(heterogpc, x, a, ml) for x, a, ml in insts]
     t_plain = time.perf_counter() - t0
     t0 = time.perf_counter()
     with heterogpc.Batch() as b:
+        b.apply_tuning(tuned)
         ids = [b.add(x, a, ml) for x, a, ml in insts]
         steps = b.run()
         got = [(b.adj(i), b.sepset_list(i)) for i in ids]
@@ -154,6 +258,9 @@ def bench_batched(heterogpc, insts):
         "results_identical": bool(same),
         "scope": "host observations -> device, correlation, all levels, adjacency matrix "
                  "and sepset list back on the host",
+        "autotune": [{k: t[k] for k in ("level", "ranks_per_task", "tasks_per_block", "queue",
+                                        "edges_per_block", "ms_best", "ms_default")}
+                     for t in tuned],
     }
 
 
@@ -210,7 +317,7 @@ def _plain_trace(heterogpc, reqs, arrivals):
     return out
 
 
-def bench_served(heterogpc, trace, load):
+def bench_served(heterogpc, trace, load, label):
     """Served form.  The plain path's capacity C (requests/s, one at a time)
     is measured on the requests first; Poisson arrivals at `load` x C are
     then replayed in real time through the continuous-batching server and
@@ -235,9 +342,8 @@ def bench_served(heterogpc, trace, load):
     same = all(np.array_equal(sa, pa) and np.array_equal(ss, ps)
                for (sa, ss, _), (pa, ps, _) in zip(served + sat_served, plain + sat_plain))
     return {
-        "workload": "%d requests, p log-uniform [50,2000], n U[500,10000], degree U[2,5], alpha "
-                    "{0.01,0.05}, FP32 data, max level 4; Poisson arrivals at %.1f/s = %.2f x the "
-                    "plain path's capacity" % (n_req, rate, load),
+        "workload": "%d requests, %s; Poisson arrivals at %.1f/s = %.2f x the plain path's "
+                    "capacity" % (n_req, label, rate, load),
         "requests": n_req,
         "arrival_rate_per_s": rate,
         "served_p50_ms": _pct(s_lat, 50),
@@ -252,6 +358,37 @@ def bench_served(heterogpc, trace, load):
         "server": {k: stats[k] for k in ("steps", "peak_active", "peak_bytes", "budget", "busy_ms")},
         "results_identical": bool(same),
     }
+
+
+def bench_plain(heterogpc, x, n, alpha, max_level, mode=0, steps=5):
+    """A plain leg: one state, device-resident observations, `steps` timed
+    skeletons (correlation in `mode`, then the levels), after one untimed
+    one.  Returns CI tests/s, level and end-to-end wall times (medians) and
+    the last run's adjacency and sepset list."""
+    p = x.shape[0]
+    with heterogpc.DeviceSkeleton(p, n, alpha, max_level=max_level) as dev:
+        dev.set_kernel_timing(False)
+        dev.set_observations(x)
+        dev.corr(mode)
+        dev.run(-2)
+        lev, e2e, st = [], [], None
+        for _ in range(steps):
+            dev.reset()
+            t0 = time.perf_counter()
+            dev.corr(mode)
+            t1 = time.perf_counter()
+            st = dev.run(-2)
+            dev.sync()
+            t2 = time.perf_counter()
+            lev.append((t2 - t1) * 1e3)
+            e2e.append((t2 - t0) * 1e3)
+        tests = int(sum(l["tests"] for l in st))
+        lv_ms = float(np.median(lev))
+        return {"ci_tests_per_s": tests / (lv_ms * 1e-3), "tests": tests,
+                "levels_run": len(st), "skeleton_ms_levels": lv_ms,
+                "skeleton_ms_e2e": float(np.median(e2e)),
+                "rechecks": int(sum(l["rechecks"] for l in st)),
+                "edges_left": int(st[-1]["edges_left"]) if st else 0}, dev.adj(), dev.sepset_list()
 
 
 FP32_MFMA_PEAK_TFLOPS = 157.3
@@ -328,11 +465,21 @@ def main() -> int:
     ap.add_argument("--served-load", type=float, default=0.7,
                     help="arrival rate of the served form as a fraction of the plain "
                          "path's measured capacity")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only device-to-device copies of a known size (counter calibration)")
+    ap.add_argument("--legs", type=int, default=1,
+                    help="the p=1000 uncapped and the FP32 p=2000 plain legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(every instance runs on a single MI355X).", file=sys.stderr)
         return 2
+    if args.copy_probe:
+        print(json.dumps(hbm_copy(reps=20)))
+        return 0
+    # the CPU baseline's OpenMP placement (read when the oracle library loads)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
 
     import heterogpc
     import workloads
@@ -381,38 +528,74 @@ def main() -> int:
     k_test = avg(lambda s: sum(lv["ms_test"] for lv in s["levels"]))
     k_compact = avg(lambda s: sum(lv["ms_compact"] for lv in s["levels"]))
     k_commit = avg(lambda s: sum(lv["ms_commit"] for lv in s["levels"]))
+
+    def lvl_avg(key, level):
+        return avg(lambda s: sum(lv[key] for lv in s["levels"] if lv["level"] == level))
+
     test_bytes = sum(lv["tests"] * bytes_per_test(lv["level"]) for lv in last)
     ci_gbs = test_bytes / (k_test * 1e-3) / 1e9 if k_test > 0 else 0.0
     corr_flops = 2.0 * n * p * (p + 1) / 2.0  # distinct pairs x 2n
     corr_tflops = corr_flops / (k_corr * 1e-3) / 1e12 if k_corr > 0 else 0.0
-
-    # The metric's timed region is the level loop, so the roofline object is
-    # the CI-test kernels' (levels 0..4 together: algorithmic bytes per step
-    # over their summed HIP-event time on the engine stream); the correlation
-    # has its own entry.  HBM traffic per step comes from the committed
-    # rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload.
+    copy = hbm_copy()
     traffic = _profiled_traffic()
-    roof = {"kernel": "CI-test kernels, levels 0..4 (level0_kernel, search_l1_lds_kernel, "
-                      "search_ln_lds_kernel<2..4>)",
-            "bound": "hbm", "achieved": ci_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": ci_gbs / HBM_PEAK_GBS,
-            "traffic": traffic.get("ci_tests_traffic_bytes_per_step"),
-            "algorithmic_bytes_per_step": test_bytes,
-            "traffic_source": traffic.get("source")}
-    if roof["traffic"] and k_test > 0:
-        # what the kernels actually move through HBM, at the measured time
-        rate = roof["traffic"] / (k_test * 1e-3) / 1e9
-        roof["hbm_traffic_GBs"] = rate
-        roof["hbm_traffic_frac"] = rate / HBM_PEAK_GBS
-        roof["note"] = ("achieved counts algorithmic bytes; the level >= 2 kernels read their "
-                        "operands from an LDS-staged row block, so HBM traffic is %.0f%% of them "
-                        "and frac can exceed 1; those kernels are bound by FP64 VALU issue"
-                        % (100.0 * roof["traffic"] / max(test_bytes, 1)))
-    corr_roof = {"kernel": "corr_tile_kernel (fp64, canonical summation order)", "bound": "fp64",
-                 "achieved": corr_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                 "frac": corr_tflops / FP64_PEAK_TFLOPS,
-                 "traffic": traffic.get("corr_traffic_bytes"),
-                 "algorithmic": "%.3e flops per launch (n * p * (p+1))" % corr_flops}
+
+    # Every kernel of the step against the roofline that bounds it.  The
+    # level >= 2 searches read their operands from an LDS-staged block, so
+    # their algorithmic bytes (B(l) per test) exceed what HBM could deliver
+    # in their time: they are bound by FP64 issue, F(l) flops per canonical
+    # test; their HBM fraction at algorithmic bytes is given beside it.
+    npad = (n + 63) // 64 * 64
+    rooflines = []
+
+    def entry(kernel, bound, achieved, peak, unit, ms, work, **kw):
+        d = {"kernel": kernel, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+             "frac": achieved / peak if peak else None, "ms": ms, "work_per_launch": work}
+        d.update(kw)
+        rooflines.append(d)
+
+    mom_bytes = 8 * p * n + 9 * p * npad + 16 * p
+    entry("moments_kernel", "hbm", mom_bytes / (k_mom * 1e-3) / 1e9 if k_mom else 0.0,
+          HBM_PEAK_GBS, "GB/s", k_mom, "%d B: observations read, z-scores and NaN mask written"
+          % mom_bytes)
+    entry("corr_tile_kernel", "fp64", corr_tflops, FP64_PEAK_TFLOPS, "TFLOP/s", k_corr,
+          "%.3e flops: 2n per pair, p(p+1)/2 pairs" % corr_flops)
+    for lv in last:
+        l, T = lv["level"], lv["tests"]
+        ms = lvl_avg("ms_test", l)
+        if ms <= 0:
+            continue
+        if l == 0:
+            entry("level0_kernel + level0_fold_kernel", "hbm", 12 * T / (ms * 1e-3) / 1e9,
+                  HBM_PEAK_GBS, "GB/s", ms, "%d pairs x 12 B" % T, level=0)
+        elif l == 1:
+            entry("search_l1_lds_kernel", "hbm", 28 * T / (ms * 1e-3) / 1e9, HBM_PEAK_GBS, "GB/s",
+                  ms, "%d canonical tests x 28 B" % T, level=1)
+        else:
+            fl = flops_per_test(l) * T
+            hb = bytes_per_test(l) * T / (ms * 1e-3) / 1e9
+            entry("search_ln_lds_kernel<%d>" % l, "fp64", fl / (ms * 1e-3) / 1e12,
+                  FP64_PEAK_TFLOPS, "TFLOP/s", ms,
+                  "%d canonical tests x %.1f flops" % (T, flops_per_test(l)), level=l,
+                  hbm_algorithmic_GBs=hb, hbm_algorithmic_frac=hb / HBM_PEAK_GBS)
+        if l >= 1:
+            mc, mm = lvl_avg("ms_compact", l), lvl_avg("ms_commit", l)
+            rooflines.append({"kernel": "compaction + removal, level %d" % l, "bound": "latency",
+                              "ms_compact": mc, "ms_commit": mm, "edges": lv["edges"],
+                              "note": "dependent loads and atomics per row / edge; a few "
+                                      "microseconds of publication each"})
+
+    # The headline: the step's dominant kernel (the exact correlation, about
+    # 45% of the step) on its own launch time, HIP events on the engine
+    # stream.  Its per-launch HBM bytes come from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE summary of this command.
+    roof = {"kernel": "corr_tile_kernel", "bound": "mfma", "achieved": corr_tflops,
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": corr_tflops / FP64_PEAK_TFLOPS,
+            "traffic": traffic.get("corr_traffic_bytes"),
+            "algorithmic": "%.3e flops per launch (n p (p+1))" % corr_flops,
+            "traffic_source": traffic.get("source"),
+            "note": "FP64 FMA on the vector units (the canonical summation order takes one "
+                    "fma per product in sequence, so no MFMA); 78.6 TFLOP/s is MI355X's FP64 "
+                    "peak, the same for the vector units and the matrix cores"}
 
     out = {
         "metric": METRIC,
@@ -444,14 +627,36 @@ def main() -> int:
                                        "ms_commit", "ms_wall")} for lv in last],
         "kernel_ms": {"moments": k_mom, "corr": k_corr, "ci_tests": k_test,
                       "compact": k_compact, "commit": k_commit},
-        "ci_tests_hbm": {"achieved_GBs": ci_gbs, "frac": ci_gbs / HBM_PEAK_GBS,
-                         "algorithmic_bytes": test_bytes},
+        "ci_tests_algorithmic": {"bytes_per_step": test_bytes, "GBs": ci_gbs,
+                                 "note": "sum of B(l) x T_l over the levels' searches; above "
+                                         "HBM's rate because levels >= 2 reuse an LDS-staged "
+                                         "block (rooflines has each kernel's own bound)"},
+        "level_overhead_ms": levels_ms - k_test,
         "corr_tflops": corr_tflops,
+        "hbm_copy": copy,
         "autotune": tuned,
         "roofline": roof,
-        "corr_roofline": corr_roof,
+        "rooflines": rooflines,
     }
+    gpu_adj = dev.adj()
     dev.close()
+    if args.legs:
+        print("[bench] p=1000 uncapped and FP32 p=2000 legs", file=sys.stderr, flush=True)
+        x2, n2, a2, ml2 = workloads.sem("c2")
+        c2, _, _ = bench_plain(heterogpc, x2, n2, a2, ml2)
+        c2["workload"] = ("plain PC-stable at p=1000: SEM p=1000, n=5000, expected degree 3, "
+                          "alpha=0.05, levels until no edge has enough neighbours")
+        out["c2_uncapped"] = c2
+        del x2
+        f32, f32_adj, f32_list = bench_plain(heterogpc, x, n, alpha, max_level, mode=1)
+        ex, ex_adj, ex_list = bench_plain(heterogpc, x, n, alpha, max_level, mode=0)
+        f32["workload"] = ("the headline p=2000 skeleton with the FP32-MFMA correlation and "
+                           "exact rechecks (HPC_CORR_F32)")
+        f32["exact_same_run"] = {k: ex[k] for k in ("ci_tests_per_s", "skeleton_ms_levels",
+                                                    "skeleton_ms_e2e")}
+        f32["skeleton_identical_to_exact"] = bool(np.array_equal(f32_adj, ex_adj) and
+                                                  np.array_equal(f32_list, ex_list))
+        out["c2k_f32_mfma"] = f32
     if args.precision:
         print("[bench] precision form", file=sys.stderr, flush=True)
         xc, nc, ac, _ = workloads.sem("c3")
@@ -462,11 +667,23 @@ def main() -> int:
         out["batched"] = bench_batched(heterogpc, workloads.batch_mix(args.batch_count))
     if args.served_count > 0:
         print("[bench] served form", file=sys.stderr, flush=True)
-        out["served"] = bench_served(heterogpc, workloads.served_trace(args.served_count, 1.0),
-                                     args.served_load)
+        out["served"] = bench_served(
+            heterogpc, workloads.served_trace(args.served_count, 1.0), args.served_load,
+            "p log-uniform [50,2000], n U[500,10000], degree U[2,5], alpha {0.01,0.05}, FP32 "
+            "data, max level 4")
+    if args.batch_count > 0 and args.served_count > 0:
+        print("[bench] served form, batched mix", file=sys.stderr, flush=True)
+        mix = workloads.batch_mix(args.batch_count)
+        rng = np.random.default_rng(4)
+        t = np.cumsum(rng.exponential(1.0, len(mix)))
+        out["served_c4"] = bench_served(
+            heterogpc, [(float(ta), xx, a, ml) for ta, (xx, a, ml) in zip(t, mix)],
+            args.served_load,
+            "the batched form's %d instances as requests (p U[20,400], n U[200,5000], degree "
+            "U[2,6], alpha {0.01,0.05}, no level cap)" % len(mix))
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(x, n, alpha, max_level, tests_total)
+        out["cpu_baseline"] = cpu_baseline(x, n, alpha, max_level, gpu_adj)
     print(json.dumps(out))
     return 0
 
