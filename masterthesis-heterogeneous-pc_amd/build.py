@@ -18,7 +18,6 @@ CSRC = os.path.join(PKG_DIR, "csrc")
 BUILD = os.path.join(PKG_DIR, "_build")
 LIB = os.path.join(PKG_DIR, "libheterogpc_hip.so")
 ORACLE_DIR = os.path.join(REPO, "oracle")
-COPY_PROBE = os.path.join(PKG_DIR, "copy_probe")
 HOST_DIR = os.path.join(PKG_DIR, "host")
 CLI = os.path.join(PKG_DIR, "bin", "heterogpc")
 HOST_SOURCES = ["state.cpp", "io.cpp", "executors.cpp", "balancer.cpp", "skeleton.cpp", "main.cpp"]
@@ -61,7 +60,6 @@ def build_hip(jobs: int = 4) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp])
     os.replace(tmp, LIB)
-    _run([HIPCC, *HIP_FLAGS, os.path.join(CSRC, "copy_probe.hip"), "-o", COPY_PROBE])
     return LIB
 
 
