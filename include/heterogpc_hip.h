@@ -130,13 +130,25 @@ enum {
   HPC_TUNE_WAIT_LIMIT = 5,     /* seconds the host waits for the device's
                                   counts of a level before the call fails with
                                   HPC_EHIP (0 = no limit; default 900) */
-  HPC_TUNE_QUEUE = 6           /* 1 (default): the level >= 2 CI-test kernel
+  HPC_TUNE_QUEUE = 6,          /* 1 (default): the level >= 2 CI-test kernel
                                   is persistent and its workgroups claim
                                   blocks of tasks from a device queue; 0:
                                   blocks go to workgroups by id.  Results do
                                   not depend on it */
+  HPC_TUNE_GRAPH = 7           /* 1 (default): hpc_run on a freshly reset state
+                                  (hpc_reset) with kernel timing off and a
+                                  level cap records its launches; the next
+                                  such run captures them in a hipGraph and
+                                  later ones replay it with one host wait,
+                                  checking every level's sizes against the
+                                  recorded run (a difference recomputes the
+                                  run without the graph).  0: never */
 };
 int hpc_set_tuning(hpc_state *st, int param, int value);
+/* Level-loop graph of a state (HPC_TUNE_GRAPH): runs replayed, graphs
+ * captured, and replays whose sizes differed from the recorded run (and
+ * were recomputed). */
+int hpc_graph_stats(hpc_state *st, int64_t *replays, int64_t *captures, int64_t *mismatches);
 
 /* Autotuner (no counterpart in the reference): runs the state's levels
  * 1..max_level once, timing each level's CI-test kernel on that level's

@@ -220,9 +220,23 @@ extern "C" int hpc_set_tuning(hpc_state *st, int param, int value) {
       if (value != 0 && value != 1) return fail(HPC_EINVAL, "queue must be 0 or 1");
       st->eng.use_queue = value != 0;
       return HPC_OK;
+    case HPC_TUNE_GRAPH:
+      if (value != 0 && value != 1) return fail(HPC_EINVAL, "graph must be 0 or 1");
+      st->eng.use_graph = value != 0;
+      if (!value) graph_drop(st->eng);
+      return HPC_OK;
     default:
       return fail(HPC_EINVAL, "unknown tuning parameter %d", param);
   }
+}
+
+extern "C" int hpc_graph_stats(hpc_state *st, int64_t *replays, int64_t *captures,
+                               int64_t *mismatches) {
+  if (!st) return fail(HPC_EINVAL, "state is NULL");
+  if (replays) *replays = st->eng.graph_replays;
+  if (captures) *captures = st->eng.graph_captures;
+  if (mismatches) *mismatches = st->eng.graph_mismatches;
+  return HPC_OK;
 }
 
 extern "C" int hpc_apply_tuning(hpc_state *st, const hpc_tune_entry *entries, int count) {

@@ -193,6 +193,10 @@ void engine_free(Engine &e) {
   dfree(e.d_nbr); dfree(e.d_erow); dfree(e.d_rowmask); dfree(e.d_fp); dfree(e.d_row_tasks);
   dfree(e.d_task_off); dfree(e.d_binom); dfree(e.d_prefix); dfree(e.d_ctr);
   e.d_ictr = nullptr;
+  graph_drop(e);
+  if (e.h_slots) (void)hipHostFree(e.h_slots);
+  e.h_slots = nullptr;
+  e.d_slots = nullptr;
   dfree(e.d_scatter_idx); dfree(e.d_scatter_val); dfree(e.d_rchk); dfree(e.d_qhead);
   if (e.h_qdone) (void)hipHostFree(e.h_qdone);
   e.h_qdone = e.d_qdone = nullptr;
@@ -301,6 +305,7 @@ int engine_reset(Engine &e, int id) {
   HPC_HIPCHK(hipSetDevice(e.device));
   launch_adj_init(e.stream, h.adj, h.p, h.W);
   HPC_HIPCHK(hipGetLastError());
+  h.fresh = true;
   h.rec_used = 0;
   h.mc = initial_mc(h.max_level);
   h.next_level = 0;
@@ -549,7 +554,7 @@ int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &
     InstHost &h = e.inst[ids[k]];
     Inst &I = e.h_inst[k];
     I.x = h.x; I.z = h.z; I.nanm = h.nanm; I.mean = h.mean; I.sd = h.sd; I.cor = h.cor;
-    I.adj = h.adj; I.rec = h.rec; I.rec_base = h.rec_used;
+    I.adj = h.adj; I.rec = h.rec; I.rec_base = h.rec_used; I.rec_cap = h.rec_cap;
     I.z32 = h.z32;
     I.fast = h.fast ? 1 : 0;
     I.eps = kFastEps;
@@ -742,6 +747,7 @@ int engine_exact_from_z(Engine &e, const std::vector<int> &ids) {
 
 int engine_level0(Engine &e, float *ms) {
   HPC_HIPCHK(hipSetDevice(e.device));
+  for (int id : e.packed) e.inst[id].fresh = false;
   int rc = zero_counters(e);
   if (rc) return rc;
   Layout L = layout(e);
@@ -952,6 +958,7 @@ int engine_search_shared_finish(Engine &e, const std::vector<int64_t> &idx,
 
 int engine_commit(Engine &e, float *ms) {
   HPC_HIPCHK(hipSetDevice(e.device));
+  for (int id : e.packed) e.inst[id].fresh = false;
   // room for the level's sepset records; the table is rewritten, so wait for
   // earlier uploads of it
   if (e.host_upload_pending) {
@@ -966,6 +973,7 @@ int engine_commit(Engine &e, float *ms) {
     if (rc) return rc;
     e.h_inst[k].rec = h.rec;
     e.h_inst[k].rec_base = h.rec_used;
+    e.h_inst[k].rec_cap = h.rec_cap;
   }
   if (!e.packed.empty()) {
     int rc = upload_table(e);
@@ -1030,10 +1038,33 @@ size_t engine_footprint(int p, int n, int max_level) {
 // overflowed did nothing (the removal skips itself); the next wait finds it,
 // the correlations are made exact and the loop returns with h.next_level at
 // that level for engine_step to continue.
-int engine_run_single(Engine &e, int id) {
+// Publication slot of a level in a captured run: kind 0 = its compaction,
+// 1 = its removal (level 0: the fold kernel), device and host addresses.
+static unsigned long long *slot_dev(Engine &e, int level, int kind) {
+  return reinterpret_cast<unsigned long long *>(reinterpret_cast<char *>(e.d_slots) +
+                                                (size_t)(2 * level + kind) * kSlotBytes);
+}
+static const Counters *slot_host(const Engine &e, int level, int kind) {
+  return reinterpret_cast<const Counters *>(reinterpret_cast<const char *>(e.h_slots) +
+                                            (size_t)(2 * level + kind) * kSlotBytes);
+}
+static const InstCounters *slot_inst(const Engine &e, int level, int kind) {
+  return reinterpret_cast<const InstCounters *>(reinterpret_cast<const char *>(slot_host(e, level, kind)) +
+                                                kIctrOffset);
+}
+
+// The one-instance level loop.  Eager (cap == nullptr): it waits for each
+// level's published counts and, with `rec`, records the sizes it read.  In
+// capture mode it enqueues the same launches without waiting, taking every
+// size from `cap` (a recorded run of the same state) and sending each
+// publication to its own per-level slot; the host-side bookkeeping is left to
+// the replay.
+static int run_single_impl(Engine &e, int id, const RunShape *cap, RunShape *rec) {
   HPC_HIPCHK(hipSetDevice(e.device));
   InstHost &h = e.inst[id];
   if (!h.have_cor) return hpc_fail(HPC_ESTATE, "no correlation matrix");
+  const bool capmode = cap != nullptr;
+  h.fresh = false;
   if (!e.lev_ev_ready) {
     for (auto &row : e.lev_ev)
       for (auto &ev : row) HPC_HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableSystemFence));
@@ -1048,14 +1079,14 @@ int engine_run_single(Engine &e, int id) {
   hipEvent_t comp_start[kMaxLevel + 1] = {};
   // the level-loop events, unless kernel timing is off
   auto mark = [&](hipEvent_t ev) -> hipError_t {
-    return e.kernel_timing ? hipEventRecord(ev, e.stream) : hipSuccess;
+    return (e.kernel_timing && !capmode) ? hipEventRecord(ev, e.stream) : hipSuccess;
   };
   hipEvent_t prev_commit_end = nullptr;
   const InstCounters *i2 =
       reinterpret_cast<const InstCounters *>(reinterpret_cast<const char *>(e.h_ctr2) + kIctrOffset);
-  auto publish2 = [&]() {
+  auto publish2 = [&](int lv) {
     Publish pb = next_publish(e);
-    pb.dst = reinterpret_cast<unsigned long long *>(e.h_ctr2_dev);
+    pb.dst = capmode ? slot_dev(e, lv, 1) : reinterpret_cast<unsigned long long *>(e.h_ctr2_dev);
     return pb;
   };
   // the level whose counts sit in h_ctr2 (or will, once the next wait ends)
@@ -1113,7 +1144,7 @@ int engine_run_single(Engine &e, int id) {
       HPC_HIPCHK(mark(e.lev_ev[0][4]));
       // the fold kernel publishes level 0's counts, clears the counters and
       // sets the device table to level 1
-      Publish pb = publish2();
+      Publish pb = publish2(0);
       pb.advance = 1;
       launch_level0(e.stream, L, e.h_inst, e.d_prefix + 2 * (L.K + 1), e.h_prefix, &pb);
       HPC_HIPCHK(hipGetLastError());
@@ -1149,9 +1180,16 @@ int engine_run_single(Engine &e, int id) {
       const Clock::time_point t_lv = Clock::now();
       Layout L = layout(e);
       if (counted) {
-        e.n_edges = (int64_t)e.h_ctr2->n_edges;
-        e.max_deg = e.h_ctr2->max_deg;
-        h.max_deg = i2[0].max_deg;
+        if (capmode) {
+          e.n_edges = cap->lv[l].n_edges;
+          e.max_deg = cap->lv[l].max_deg;
+          h.max_deg = cap->lv[l].inst_max_deg;
+        } else {
+          e.n_edges = (int64_t)e.h_ctr2->n_edges;
+          e.max_deg = e.h_ctr2->max_deg;
+          h.max_deg = i2[0].max_deg;
+        }
+        if (rec) rec->lv[l] = LevelShape{e.n_edges, e.max_deg, h.max_deg, 1};
         counted = false;
       } else {
         if (prev_commit_end) {
@@ -1161,20 +1199,30 @@ int engine_run_single(Engine &e, int id) {
           comp_start[l] = e.lev_ev[l][0];
         }
         prev_commit_end = nullptr;
-        const Publish pa = next_publish(e);
+        Publish pa = next_publish(e);
+        if (capmode) pa.dst = slot_dev(e, l, 0);
         launch_compact_fused(e.stream, L, l, engine_rpt(e, l), e.d_blk, e.d_row_tasks,
                              e.d_task_off, e.cap_edges, pa);
         HPC_HIPCHK(hipGetLastError());
-        rc = wait_published(e, pa.seq);
-        if (rc) return rc;
-        if (pend >= 0) {
-          if (e.h_ctr2->overflow) return voided(pend);
-          finalize(pend, pend_max_deg, t_lv);
+        if (capmode) {
+          if (pend == 0) h.cor_nan = cap->cor_nan;  // what level 0's counts would say
           pend = -1;
+          e.n_edges = cap->lv[l].n_edges;
+          e.max_deg = cap->lv[l].max_deg;
+          h.max_deg = cap->lv[l].inst_max_deg;
+        } else {
+          rc = wait_published(e, pa.seq);
+          if (rc) return rc;
+          if (pend >= 0) {
+            if (e.h_ctr2->overflow) return voided(pend);
+            finalize(pend, pend_max_deg, t_lv);
+            pend = -1;
+          }
+          e.n_edges = (int64_t)e.h_ctr->n_edges;
+          e.max_deg = e.h_ctr->max_deg;
+          h.max_deg = e.h_ictr[0].max_deg;
         }
-        e.n_edges = (int64_t)e.h_ctr->n_edges;
-        e.max_deg = e.h_ctr->max_deg;
-        h.max_deg = e.h_ictr[0].max_deg;
+        if (rec) rec->lv[l] = LevelShape{e.n_edges, e.max_deg, h.max_deg, 0};
       }
       pend_t0 = t_lv;
       h.compacted = true;
@@ -1189,13 +1237,18 @@ int engine_run_single(Engine &e, int id) {
                         "level %d: C(%d, %d) conditioning sets per edge is beyond what one search "
                         "plans", l, h.max_deg - 1, l);
       // record space, the NaN flag of level 1 on: the table when it changed
-      rc = prepare_level(e, h, l);
-      if (rc) return rc;
+      // (a capture reuses the recorded run's record buffer, sized for every
+      // level at once)
+      if (!capmode) {
+        rc = prepare_level(e, h, l);
+        if (rc) return rc;
+      }
       Inst &I = e.h_inst[0];
       const int flags = kActive | (h.cor_nan != 0 ? kCorNaN : 0);
-      const bool retable = I.rec != h.rec || I.flags != flags;
+      const bool retable = I.rec != h.rec || I.flags != flags || I.rec_cap != h.rec_cap;
       I.level = l;
       I.rec = h.rec;
+      I.rec_cap = h.rec_cap;
       I.rec_base = h.rec_used;
       I.flags = flags;
       if (retable) {
@@ -1225,7 +1278,7 @@ int engine_run_single(Engine &e, int id) {
       // (device atomics on a few cache lines) cost more than the separate
       // count kernel
       const bool count_next = e.n_edges > 0 && !last && l + 1 <= kMaxLevel && l >= 2;
-      Publish pb = publish2();
+      Publish pb = publish2(l);
       if (e.n_edges > 0) {
         // publishes, clears the counters, advances level and record base
         pb.advance = 1;
@@ -1264,10 +1317,12 @@ int engine_run_single(Engine &e, int id) {
           HPC_HIPCHK(mark(e.lev_ev[l + 1][5]));
           searched = true;
         }
-        rc = wait_published(e, pb.seq);
-        if (rc) return rc;
-        if (e.h_ctr2->overflow) return voided(l);
-        finalize(l, h.max_deg, Clock::now());
+        if (!capmode) {
+          rc = wait_published(e, pb.seq);
+          if (rc) return rc;
+          if (e.h_ctr2->overflow) return voided(l);
+          finalize(l, h.max_deg, Clock::now());
+        }
         counted = true;
       } else {
         pend = l;
@@ -1279,7 +1334,7 @@ int engine_run_single(Engine &e, int id) {
       e.h_inst[0].level = l;  // the device table's level after the removal
       if (last) break;
     }
-    if (pend >= 0) {
+    if (pend >= 0 && !capmode) {
       rc = wait_published(e, pend_seq);
       if (rc) return rc;
       if (e.h_ctr2->overflow) return voided(pend);
@@ -1301,6 +1356,172 @@ int engine_run_single(Engine &e, int id) {
     }
   }
   return rc_run;
+}
+
+// The launch shapes a captured graph bakes in (a change of any voids it).
+static std::vector<long long> tuning_key(const Engine &e) {
+  std::vector<long long> k{e.ranks_per_task, (long long)e.lds_limit, e.use_queue ? 1 : 0,
+                           e.epb_level1};
+  for (int l = 0; l <= kMaxLevel; ++l) {
+    k.push_back(e.rpt_level[l]);
+    k.push_back(e.tpb_level[l]);
+    k.push_back(e.queue_level[l]);
+  }
+  return k;
+}
+
+void graph_drop(Engine &e) {
+  if (e.graph.exec) (void)hipGraphExecDestroy(e.graph.exec);
+  if (e.graph.graph) (void)hipGraphDestroy(e.graph.graph);
+  e.graph.exec = nullptr;
+  e.graph.graph = nullptr;
+  e.graph.id = -1;
+}
+
+// Captures the level loop of a fresh run of instance id with the sizes of
+// `shape` into a graph (the instance's host state is left as it was).
+static int graph_capture(Engine &e, int id, const RunShape &shape) {
+  graph_drop(e);
+  InstHost &h = e.inst[id];
+  if (!e.h_slots) {
+    HPC_HIPCHK(hipHostMalloc((void **)&e.h_slots, (size_t)2 * (kMaxLevel + 2) * kSlotBytes,
+                             hipHostMallocMapped | hipHostMallocCoherent));
+    HPC_HIPCHK(hipHostGetDevicePointer((void **)&e.d_slots, e.h_slots, 0));
+  }
+  HPC_HIPCHK(hipStreamSynchronize(e.stream));
+  e.host_upload_pending = false;
+  const InstHost saved = h;
+  const int64_t n_edges = e.n_edges;
+  const int max_deg = e.max_deg;
+  // a fresh run from level 0, as the recorded one
+  h.next_level = 0;
+  h.finished = false;
+  h.cor_nan = -1;
+  h.rec_used = 0;
+  h.undirected = (int64_t)h.p * (h.p - 1) / 2;
+  h.max_deg = h.p;
+  const unsigned long long seq0 = e.seq;
+  hipGraph_t graph = nullptr;
+  HPC_HIPCHK(hipStreamBeginCapture(e.stream, hipStreamCaptureModeRelaxed));
+  const int rc = run_single_impl(e, id, &shape, nullptr);
+  const hipError_t ec = hipStreamEndCapture(e.stream, &graph);
+  h = saved;
+  e.n_edges = n_edges;
+  e.max_deg = max_deg;
+  e.packed.clear();  // the device table is whatever the graph leaves
+  if (rc || ec != hipSuccess || !graph) {
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    return rc ? rc : hpc_fail(HPC_EHIP, "level-loop capture failed: %s", hipGetErrorString(ec));
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return hpc_fail(HPC_EHIP, "level-loop graph instantiation failed: %s", hipGetErrorString(ei));
+  }
+  e.graph.graph = graph;
+  e.graph.exec = exec;
+  e.graph.id = id;
+  e.graph.shape = shape;
+  e.graph.first_seq = seq0 + 1;
+  e.graph.final_seq = e.seq;
+  e.graph.tuning = tuning_key(e);
+  ++e.graph_captures;
+  return HPC_OK;
+}
+
+// Replays the captured level loop on instance id (fresh, same tuning).
+// Returns 1 when the device's sizes differ from the recorded ones: the run's
+// results are then void and the caller recomputes them.
+static int graph_replay(Engine &e, int id) {
+  using Clock = std::chrono::steady_clock;
+  InstHost &h = e.inst[id];
+  const RunShape &sh = e.graph.shape;
+  const auto t0 = Clock::now();
+  // every earlier publication has arrived (each run waits for its last)
+  __atomic_store_n(e.h_seq, 0ull, __ATOMIC_RELEASE);
+  HPC_HIPCHK(hipGraphLaunch(e.graph.exec, e.stream));
+  int rc = wait_published(e, e.graph.final_seq);
+  if (rc) return rc;
+  h.fresh = false;
+  e.packed.clear();
+  // the sizes each level ran with, as the device computed them this time
+  if ((slot_inst(e, 0, 1)->has_nan ? 1 : 0) != sh.cor_nan) return 1;
+  for (int l = 1; l <= sh.max_level; ++l) {
+    const LevelShape &x = sh.lv[l];
+    const int kind = x.src ? 1 : 0, at = x.src ? l - 1 : l;
+    const Counters *c = slot_host(e, at, kind);
+    if ((int64_t)c->n_edges != x.n_edges || c->max_deg != x.max_deg ||
+        slot_inst(e, at, kind)->max_deg != x.inst_max_deg)
+      return 1;
+  }
+  const double wall = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+  h.cor_nan = sh.cor_nan;
+  for (int l = 0; l <= sh.max_level; ++l) {
+    const Counters *c = slot_host(e, l, 1);
+    const InstCounters &ic = *slot_inst(e, l, 1);
+    hpc_level_stats st;
+    memset(&st, 0, sizeof(st));
+    st.level = l;
+    st.max_deg = l == 0 ? h.p : sh.lv[l].inst_max_deg;
+    st.tests = (int64_t)ic.tests;
+    st.tests_max = (int64_t)ic.tests_max;
+    st.edges = (int64_t)ic.edges;
+    st.removed = (int64_t)ic.removed;
+    h.undirected -= st.removed;
+    st.edges_left = h.undirected;
+    st.pinv_fallbacks = (int64_t)c->pinv_fallbacks;
+    st.rechecks = (int64_t)c->rechecks;
+    if (l >= 1) {
+      h.rec_used += (size_t)ic.rec_words;
+      engine_note_level(e, id, l);
+    }
+    st.ms_wall = wall / (sh.max_level + 1);  // the replay's wall time, split evenly
+    h.hist.push_back(st);
+  }
+  h.max_deg = sh.lv[sh.max_level].inst_max_deg;
+  e.n_edges = sh.lv[sh.max_level].n_edges;
+  e.max_deg = sh.lv[sh.max_level].max_deg;
+  h.compacted = false;
+  h.ever_compacted = true;
+  h.next_level = sh.max_level + 1;
+  h.finished = true;
+  return HPC_OK;
+}
+
+int engine_run_single(Engine &e, int id) {
+  InstHost &h = e.inst[id];
+  // graph replay: a fresh run of an exact, capped instance without per-level
+  // kernel timing, with the launch shapes and record buffer of the capture
+  const bool graphable = e.use_graph && !e.kernel_timing && !h.fast && h.fresh &&
+                         h.next_level == 0 && h.max_level >= 1 && h.max_level <= kMaxLevel &&
+                         h.have_cor && h.inject_level < 0;
+  if (graphable && e.graph.exec && e.graph.id == id && e.graph.shape.max_level == h.max_level &&
+      e.graph.shape.rec == h.rec && e.graph.tuning == tuning_key(e)) {
+    const size_t first = h.hist.size();
+    const int r = graph_replay(e, id);
+    if (r == 0) ++e.graph_replays;
+    if (r <= 0) return r;
+    // the device's sizes left the recorded ones: recompute eagerly
+    h.hist.resize(first);
+    graph_drop(e);
+    int rc = engine_reset(e, id);
+    if (rc) return rc;
+    ++e.graph_mismatches;
+  }
+  RunShape rec;
+  rec.max_level = h.max_level;
+  const bool record = graphable && h.fresh;
+  int rc = run_single_impl(e, id, nullptr, record ? &rec : nullptr);
+  if (rc || !record || !h.finished || h.next_level != h.max_level + 1) return rc;
+  rec.cor_nan = h.cor_nan;
+  rec.rec = h.rec;
+  rec.valid = true;
+  // the next fresh run of this instance replays these launches; a failed
+  // capture only means it runs eagerly again
+  if (graph_capture(e, id, rec)) graph_drop(e);
+  return HPC_OK;
 }
 
 // Candidate launch shapes of a level's CI-test kernel: (ranks per task,
@@ -1491,6 +1712,7 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
   if (act.empty()) return HPC_OK;
   rc = engine_pack(e, act, lev, /*zero_ctr=*/true);
   if (rc) return rc;
+  for (int id : act) e.inst[id].fresh = false;
   // observations and z-scores are only read by the correlation, which the
   // pack above has waited for; blocks freed here may go to an upload on
   // another stream

@@ -103,6 +103,7 @@ struct Inst {
   // kernel: i, j (i < j, local ids), level, then the level members
   int32_t *rec;
   unsigned long long rec_base;  // words written by earlier levels
+  unsigned long long rec_cap;   // words the record buffer holds (writes beyond are dropped)
   double r_crit;          // Fisher-z decision boundary of (n, alpha)
   double eps;             // bound on |cor - exact cor| when `fast`
   int start;              // first global variable id
@@ -297,6 +298,7 @@ struct InstHost {
   int error = 0;           // first error of a batched run (HPC_E*), 0 = none
   std::string message;     // its description
   int inject_level = -1;   // test hook: the host side of this level fails
+  bool fresh = false;      // adjacency complete, no level run since (replayable)
   std::vector<hpc_level_stats> hist;  // one record per level run
 };
 
@@ -318,6 +320,37 @@ struct Arena {
 
 // Byte offset of the instance counters in the counter block.
 constexpr size_t kIctrOffset = (sizeof(Counters) + 63) & ~(size_t)63;
+
+// Sizes one level of a one-instance run read from the device: ordered
+// edges and longest row of the layout and of the instance, and where they
+// were published (0: the level's compaction, 1: the previous removal).
+struct LevelShape {
+  int64_t n_edges = 0;
+  int max_deg = 0;
+  int inst_max_deg = 0;
+  int src = 0;
+};
+// The sizes of a whole fresh run, levels 0..max_level, and what else a
+// capture of its launches bakes in.
+struct RunShape {
+  bool valid = false;
+  int max_level = 0;
+  int cor_nan = -1;
+  const int32_t *rec = nullptr;  // record buffer (its address is a kernel argument)
+  LevelShape lv[kMaxLevel + 2];
+};
+// A captured one-instance level loop (engine_run_single replays it).
+struct GraphRun {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int id = -1;
+  RunShape shape;
+  unsigned long long first_seq = 0, final_seq = 0;  // publication numbers it writes
+  std::vector<long long> tuning;                     // launch shapes it bakes in
+};
+// Bytes per publication slot of a captured run (one counter block, K = 1).
+constexpr size_t kSlotBytes = 256;
+static_assert(kIctrOffset + sizeof(InstCounters) <= kSlotBytes, "slot too small");
 
 // A set of instances on one device with their shared per-level arrays and
 // one stream.
@@ -397,6 +430,13 @@ struct Engine {
   size_t cap_qdone = 0;
   std::vector<long long> h_task_off, h_row_tasks;
   bool use_queue = true;  // level >= 2 searches claim task blocks (HPC_TUNE_QUEUE)
+  // one-instance level loop captured in a hipGraph (HPC_TUNE_GRAPH), and the
+  // per-level publication slots its replays write (host-visible memory)
+  bool use_graph = true;
+  GraphRun graph;
+  Counters *h_slots = nullptr, *d_slots = nullptr;
+  int64_t graph_mismatches = 0;  // replays whose sizes left the recorded run's
+  int64_t graph_replays = 0, graph_captures = 0;
   // recheck queues of fast instances
   Recheck *d_rchk = nullptr;
   size_t cap_rchk = 0;
@@ -519,6 +559,8 @@ int engine_autotune(Engine &e, int id, int max_level, std::vector<hpc_tune_entry
 // fastest stays set per level and goes to `out`; every instance is then back
 // at its start (correlations kept).
 int engine_batch_autotune(Engine &e, std::vector<hpc_tune_entry> &out);
+// Drops a captured level loop.
+void graph_drop(Engine &e);
 // hpc_run's level loop for one instance on exact correlations (levels from
 // h.next_level to its cap), with one host wait per level.
 int engine_run_single(Engine &e, int id);

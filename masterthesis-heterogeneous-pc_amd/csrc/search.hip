@@ -119,6 +119,8 @@ template <bool FAST>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_kernel(
     Layout L, long long n_edges, const int32_t *__restrict__ row_mask, int cap, int epb) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // the snapshot's own size bounds the count a launch was sized with
+  n_edges = min(n_edges, (long long)L.rowptr[L.p_total]);
   for (long long b0 = (long long)blockIdx.x * epb; b0 < n_edges;
        b0 += (long long)gridDim.x * epb) {
     const long long b1 = min(n_edges, b0 + epb);
@@ -128,6 +130,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_kernel(
       const int base = L.rowptr[i];
       const int deg = L.rowptr[i + 1] - base;
       const long long seg_end = min(b1, (long long)base + deg);
+      if (seg_end <= e0) break;  // (a consistent snapshot always advances)
       const Inst &In = L.inst[L.row_inst[i]];
       // uniform across the workgroup
       if (deg > cap && In.level == 1 && (!row_mask || row_mask[i])) {
@@ -198,6 +201,8 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_lds_kernel(
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int cls = blockIdx.x % kL1Classes;
   const long long span = (long long)kL1Classes * epb;
+  // the snapshot's own size bounds the count a launch was sized with
+  n_edges = min(n_edges, (long long)L.rowptr[L.p_total]);
   long long e0 = (long long)(blockIdx.x / kL1Classes) * span;
   const long long e1 = min(n_edges, e0 + span);
   while (e0 < e1) {
@@ -205,6 +210,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_lds_kernel(
     const int base = L.rowptr[i];
     const int deg = L.rowptr[i + 1] - base;
     const long long seg_end = min(e1, (long long)base + deg);
+    if (seg_end <= e0) break;  // (a consistent snapshot always advances)
     const Inst &In = L.inst[L.row_inst[i]];
     // uniform across the workgroup: the row is skipped or staged by all
     if (In.level == 1 && (!row_mask || row_mask[i]) && deg >= 2 && deg <= cap) {
@@ -365,9 +371,12 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_kernel(
       const int i = task_row(task_off, L.p_total, t0);
       const long long off = task_off[i];
       const long long seg_end = min(t1, off + row_tasks[i]);
+      if (seg_end <= t0) break;  // (a consistent plan always advances)
       const int base = L.rowptr[i];
       const int deg = L.rowptr[i + 1] - base;
-      if (deg + 1 <= dmin) {  // the staged kernel's row
+      // the staged kernel's row; or one longer than the layout's longest
+      // (only in a launch sized for other data, whose results are discarded)
+      if (deg + 1 <= dmin || deg > L.max_deg) {
         t0 = seg_end;
         continue;
       }
@@ -509,6 +518,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
       const int i = task_row(task_off, L.p_total, t0);
       const long long off = task_off[i];
       const long long seg_end = min(t1, off + row_tasks[i]);
+      if (seg_end <= t0) break;  // (a consistent plan always advances)
       const int base = L.rowptr[i];
       const int deg = L.rowptr[i + 1] - base;
       const int D = deg + 1;
@@ -701,7 +711,8 @@ __device__ __forceinline__ int find_pos(const int32_t *row, int len, int node) {
 // group of lanes, summed with shuffles (the code stays small: this runs once
 // per launch, from a cold instruction cache).
 __device__ void next_count(const Layout &L, const NextCount &nc, const unsigned long long *tab,
-                           unsigned long long &te, unsigned long long &tt, int &tmax) {
+                           int tab_rows, unsigned long long &te, unsigned long long &tt,
+                           int &tmax) {
   __shared__ unsigned long long s_we[kCommitWaves], s_wt[kCommitWaves];
   __shared__ int s_wm[kCommitWaves];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nt = blockDim.x;
@@ -715,7 +726,8 @@ __device__ void next_count(const Layout &L, const NextCount &nc, const unsigned 
       c = __hip_atomic_load(&L.deg[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (nc.level >= 2 && c > 0) {
         // a row that cannot be planned gets no tasks; the host fails its level
-        tk = max(0ll, row_task_count(binom(tab, kBinomK, c - 1, nc.level), c, nc.rpt));
+        const unsigned long long *tb = c - 1 < tab_rows ? tab : L.binom;
+        tk = max(0ll, row_task_count(binom(tb, kBinomK, c - 1, nc.level), c, nc.rpt));
       }
       nc.row_tasks[g] = tk;
     }
@@ -821,6 +833,8 @@ __global__ __launch_bounds__(64 * kCommitWaves) void commit_kernel(Layout L, lon
   // an overflowed recheck queue leaves the level undecided: the host redoes
   // it (the workgroups still take part in the publication below)
   const bool live = !L.ctr->overflow;
+  // the snapshot's own size bounds the count a launch was sized with
+  n_edges = min(n_edges, (long long)L.rowptr[L.p_total]);
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   int k = -1, level = 0;
@@ -877,7 +891,9 @@ __global__ __launch_bounds__(64 * kCommitWaves) void commit_kernel(Layout L, lon
   __shared__ unsigned long long s_base;
   if (lane == 0) sk[wv] = uniform ? kk : -2;
   __syncthreads();  // also: the staged binomial rows are in place
-  const unsigned long long *tab = tab_rows > 0 ? s_tab : L.binom;
+  // the staged rows when they cover this row's sets (they cover every row of
+  // a launch sized from its own counts), else the global table
+  const unsigned long long *tab = (tab_rows > 0 && nni < tab_rows) ? s_tab : L.binom;
   if (E) {
     const unsigned long long total = binom(tab, kBinomK, nni, level);
     if (total > 0) {
@@ -931,6 +947,9 @@ __global__ __launch_bounds__(64 * kCommitWaves) void commit_kernel(Layout L, lon
   } else if (rm) {
     off = atomicAdd(&L.ictr[k].rec_words, need);
   }
+  // a record that would pass the buffer's end is dropped (only a launch sized
+  // for other data can get there: its results are discarded)
+  if (rm && L.inst[k].rec_base + off + need > L.inst[k].rec_cap) rm = false;
   if (rm) {
     const Inst &In = L.inst[k];
     const int W = In.W, s0 = In.start;
@@ -1027,7 +1046,8 @@ __global__ __launch_bounds__(64 * kCommitWaves) void commit_kernel(Layout L, lon
     __threadfence();
     unsigned long long nte = 0, ntt = 0;
     int nmax = 0;
-    if (nc.on) next_count(L, nc, tab, nte, ntt, nmax);
+    if (nc.on) next_count(L, nc, tab_rows > 0 ? s_tab : L.binom, tab_rows > 0 ? tab_rows : 1 << 30,
+                          nte, ntt, nmax);
     const unsigned long long *src = reinterpret_cast<const unsigned long long *>(L.ctr);
     unsigned long long words_rec = 0;
     if (pub.advance && L.K == 1 && threadIdx.x == 0)

@@ -26,6 +26,7 @@ HPC_TUNE_KERNEL_TIMING = 3
 HPC_TUNE_LDS_LIMIT = 4
 HPC_TUNE_WAIT_LIMIT = 5
 HPC_TUNE_QUEUE = 6
+HPC_TUNE_GRAPH = 7
 
 
 class LevelStats(C.Structure):
@@ -172,6 +173,7 @@ SIGNATURES = {
     "hpc_corr_noalloc": (_i, [_i, _vp, _u64, _i, _vp, _vp]),
     "hpc_search_level_shared": (_i, [_vp, _i, _i, _ps, _ps]),
     "hpc_batch_autotune": (_i, [_vp, C.POINTER(TuneEntry), _i, C.POINTER(C.c_int)]),
+    "hpc_graph_stats": (_i, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)]),
 }
 
 _lib = None

@@ -189,6 +189,16 @@ class DeviceSkeleton:
                                                C.byref(h)))
         return d.as_dict(), h.as_dict()
 
+    def set_graph(self, on: bool) -> None:
+        """hipGraph replay of the level loop for fresh, capped runs without
+        kernel timing (default on)."""
+        check(self.lib.hpc_set_tuning(self.h, 7, 1 if on else 0))
+
+    def graph_stats(self) -> dict:
+        r, c, m = C.c_int64(), C.c_int64(), C.c_int64()
+        check(self.lib.hpc_graph_stats(self.h, C.byref(r), C.byref(c), C.byref(m)))
+        return {"replays": r.value, "captures": c.value, "mismatches": m.value}
+
     def set_queue(self, on: bool) -> None:
         """Device task queue of the level >= 2 CI-test kernel (default on)."""
         check(self.lib.hpc_set_tuning(self.h, 6, 1 if on else 0))
