@@ -140,8 +140,9 @@ std::tuple<TestResult, TestResult> Balancer::executeWorkstealing(int level, int 
     if (verbose)
       std::cout << "Shared queue: the host finished " << host.removed << " blocks ("
                 << host.tests << " tests) beside the device." << std::endl;
-    std::tuple<TestResult, TestResult> res{TestResult{(float)host.ms_wall, host.tests},
-                                           TestResult{(float)dev.ms_wall, dev.tests}};
+    std::tuple<TestResult, TestResult> res{
+        TestResult{(unsigned long long)host.ms_wall, host.tests},
+        TestResult{(unsigned long long)dev.ms_wall, dev.tests}};
     commit(level, res);
     return res;
   }

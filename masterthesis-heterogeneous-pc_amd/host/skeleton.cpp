@@ -87,7 +87,7 @@ LevelMetrics SkeletonCalculator::calcLevel(int level) {
   return {levelDur, balanceDur, execRes};
 }
 
-int SkeletonCalculator::run(bool print_sepsets) {
+void SkeletonCalculator::run(bool print_sepsets) {
   if (verbose) {
     cout << "Using following GPUs:" << endl;
     for (auto deviceId : gpuList) cout << "\t" << deviceId << endl;
@@ -103,7 +103,6 @@ int SkeletonCalculator::run(bool print_sepsets) {
       chrono::duration_cast<chrono::milliseconds>(chrono::system_clock::now() - start).count();
   if (verbose) cout << "Execution duration: " << executionDuration << " ms." << endl;
   state.pull();
-  const int nrEdges = printSepsets(&state, print_sepsets, verbose);
+  nrEdges = printSepsets(&state, print_sepsets, verbose);
   export_metrics(levelMetrics, executionDuration, nrEdges);
-  return nrEdges;
 }

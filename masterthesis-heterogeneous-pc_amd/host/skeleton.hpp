@@ -24,6 +24,7 @@ class SkeletonCalculator {
   std::vector<int> gpuList;
   MMState state;
   Balancer balancer;
+  int nrEdges = 0;
 
  public:
   SkeletonCalculator(int maxLevel, double alpha, bool workstealing, std::string csvExportFile,
@@ -42,9 +43,10 @@ class SkeletonCalculator {
   void add_correlation_matrix(std::string input_file, int observation_count,
                               bool use_p9_ats = false);
   void initialize_balancer(std::tuple<float, float, float> balancer_thresholds);
-  // skeleton.cpp:152-190: levels 0..maxLevel, sepset printout, metrics CSV;
-  // returns the number of edges left
-  int run(bool print_sepsets);
+  // skeleton.cpp:152-190: levels 0..maxLevel, sepset printout, metrics CSV
+  void run(bool print_sepsets);
+  // edges left after the last run()
+  int edges_left() const { return nrEdges; }
   LevelMetrics calcLevel(int level);
   void export_metrics(std::vector<LevelMetrics> levelMetrics, int64_t executionDuration,
                       int nrEdges);
