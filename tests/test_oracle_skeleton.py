@@ -138,16 +138,39 @@ def test_invariants_and_order_independence(orc):
     assert np.array_equal(a.sepsets(), b.sepsets())
 
 
-def test_reference_faithful_mode_same_adjacency(orc):
+def _faithful_input(name):
+    if name == "latent":
+        # 100+-neighbour rows at levels 1 and 2
+        x = workloads.latent()
+        return x, x.shape[1], 0.05, 2
+    if name == "collinear":
+        # duplicated and collinear columns: singular blocks in both modes
+        x, n, alpha, _ = workloads.sem("small")
+        x = x[:60].copy()
+        x[7] = x[3]
+        x[11] = 2.0 * x[3] - 1.0
+        x[20] = x[21] + x[22]
+        return x, n, alpha, -1
+    if name == "small":
+        x, n, alpha, _ = workloads.sem("small")
+        return x, n, alpha, 4
+    return workloads.sem(name)
+
+
+@pytest.mark.parametrize("name", ["small", "c1", "dense", "deep", "latent", "collinear"])
+def test_reference_faithful_mode_same_adjacency(orc, name):
     """Per-test pinv with immediate deletion and early exit (the reference CPU
-    executor's behaviour) removes exactly the same edges."""
-    x, n, alpha, _ = workloads.sem("small")
+    executor's behaviour) removes exactly the same edges as the canonical
+    LDL^T decisions, at every level: sparse, dense, deep (no level cap),
+    long-row and singular inputs."""
+    x, n, alpha, ml = _faithful_input(name)
     cor = orc.corr(x)
-    a = orc.Skeleton(cor, n, alpha, max_level=4)
-    b = orc.Skeleton(cor, n, alpha, max_level=4, mode=orc.MODE_PINV, flags=orc.FLAG_REF_EARLY_EXIT)
-    a.run(4)
-    b.run(4)
-    assert np.array_equal(a.adj(), b.adj())
+    a = orc.Skeleton(cor, n, alpha, max_level=ml)
+    b = orc.Skeleton(cor, n, alpha, max_level=ml, mode=orc.MODE_PINV, flags=orc.FLAG_REF_EARLY_EXIT)
+    ra = a.run(ml)
+    rb = b.run(ml)
+    assert len(ra) == len(rb), name
+    assert np.array_equal(a.adj(), b.adj()), name
     # b's sepsets depend on scheduling but each one passes the test
     so = b.sepsets()
     p = cor.shape[0]
