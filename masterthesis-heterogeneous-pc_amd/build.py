@@ -38,6 +38,11 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
+# per-source flags: the matrix-core kernels keep their accumulators in
+# ordinary VGPRs (the default AGPR form made the compiler copy them between
+# register files around every step of the exact correlation's loop)
+FILE_FLAGS = {"corr.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
 
 def _run(cmd: list[str], cwd: str | None = None) -> None:
     proc = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE,
@@ -54,7 +59,8 @@ def build_hip(jobs: int = 4) -> str:
     for src in SOURCES:
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
         objs.append(obj)
-        cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC, *HIP_FLAGS, *FILE_FLAGS.get(src, []), "-c", os.path.join(CSRC, src),
+                     "-o", obj])
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"

@@ -70,6 +70,27 @@ __device__ __forceinline__ int tree64i(int v) {
   return v;
 }
 
+// Class-major z-scores (zc, read by the matrix-core kernel).  Lane class t
+// holds the samples t, t + 64, ... (m = 0 .. M - 1, M = npad / 64) after
+// P0 = M8 - M leading zeros, M8 = M rounded up to 8 (zc_m8); slot
+// m' = m + P0 = 8 g + r.  Row a is M8 / 8 groups of 64 classes x 8 slots
+// (group g, class t at a * 64 M8 + 512 g + 8 t), so the moments kernel's
+// lanes write adjacent 64-byte pieces.  Within a group slot 2 (r % 4) + r / 4
+// holds m', so one 16-byte load of lane class kk (0..3) at 2 kk yields
+// m' = 8 g + kk and 8 g + 4 + kk: the k-th element of two consecutive 4-step
+// MFMA operands.
+__device__ __forceinline__ size_t zc_index(int m8, int p0, int a, int k) {
+  const int t = k & 63, mp = (k >> 6) + p0;
+  return (size_t)a * 64 * m8 + 512 * (mp >> 3) + 8 * t + 2 * (mp & 3) + ((mp >> 2) & 1);
+}
+
+// z-score of row a, sample k, from whichever layout the instance holds
+__device__ __forceinline__ double zval(const Inst &I, int a, int k) {
+  if (I.z) return I.z[(size_t)a * I.npad + k];
+  const int m8 = zc_m8(I.npad);
+  return I.zc[zc_index(m8, m8 - (I.npad >> 6), a, k)];
+}
+
 // k with prefix[k] <= idx < prefix[k+1] (empty instances skipped)
 __device__ __forceinline__ int find_inst(const long long *prefix, int K, long long idx) {
   int lo = 0, hi = K - 1;
@@ -81,14 +102,23 @@ __device__ __forceinline__ int find_inst(const long long *prefix, int K, long lo
 }
 
 // One wavefront per variable: mean over non-NaN samples, sd rounded through
-// float (corHelper.cu:75), z-scores (x - mean) / sd padded with zeros to npad,
-// NaN mask.  A variable with NaN cells flags its instance (ictr.has_nan),
+// float (corHelper.cu:75), z-scores (x - mean) / sd padded with zeros to npad
+// (canonical layout, class-major layout, float copy: whichever the instance
+// holds), NaN mask.  A variable with NaN cells flags its instance (ictr.has_nan),
 // which selects the masked pairwise kernel for it.  With J > 0 (every
 // instance's n <= 64 J) lane t loads its samples t, t + 64, ... once into
 // registers and the three passes run from there; J = 0 re-reads the row for
-// each pass.  Both do the same operations in the same order.
+// each pass.  Both do the same operations in the same order.  With J > 0 a
+// row's class-major z-scores are assembled in the wave's LDS area and stored
+// as whole 1 KB pieces (written lane by lane, each store would touch 32 cache
+// lines).
 template <int J>
-__global__ __launch_bounds__(256) void moments_kernel(Layout L) {
+constexpr int moments_waves() { return J > 32 ? 2 : 4; }
+
+template <int J>
+__global__ __launch_bounds__(64 * moments_waves<J>()) void moments_kernel(Layout L) {
+  // one class-major row per wave (64 classes x up to J slots)
+  __shared__ double zrow[J > 0 ? moments_waves<J>() : 1][J > 0 ? 64 * J : 1];
   const int lane = threadIdx.x & 63;
   const int g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (g >= L.p_total) return;
@@ -149,12 +179,21 @@ __global__ __launch_bounds__(256) void moments_kernel(Layout L) {
     I.mean[a] = mu;
     I.sd[a] = sd;
   }
-  double *za = I.z + (size_t)a * npad;
+  double *za = I.z ? I.z + (size_t)a * npad : nullptr;
   uint8_t *na = I.nanm + (size_t)a * npad;
   float *z32 = I.z32 ? I.z32 + (size_t)a * npad : nullptr;
+  const int m8 = zc_m8(npad), p0 = m8 - (npad >> 6);
+  // class-major target: the wave's LDS row (J > 0), else the row in memory
+  double *zr = nullptr;
+  if (I.zc) zr = J > 0 ? zrow[threadIdx.x >> 6] : I.zc + (size_t)a * 64 * m8;
+  if (zr) {
+    // the leading zeros of this lane's class (all in group 0)
+    for (int mp = 0; mp < p0; ++mp) zr[8 * lane + 2 * (mp & 3) + ((mp >> 2) & 1)] = 0.0;
+  }
   auto put = [&](int k, double v) {
     const double zk = k < n ? (v - mu) / sd : 0.0;
-    za[k] = zk;
+    if (za) za[k] = zk;
+    if (zr) zr[zc_index(m8, p0, 0, k)] = zk;
     na[k] = (k < n && __builtin_isnan(v)) ? 1 : 0;
     if (z32) z32[k] = (float)zk;
   };
@@ -163,6 +202,16 @@ __global__ __launch_bounds__(256) void moments_kernel(Layout L) {
     for (int j = 0; j < J; ++j) {
       const int k = lane + 64 * j;
       if (k < npad) put(k, xv[j]);
+    }
+    if (zr) {
+      // every lane's pieces are in LDS (one wave: its LDS operations are
+      // ordered); 1 KB of the row per store
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      double2 *dst = reinterpret_cast<double2 *>(I.zc + (size_t)a * 64 * m8);
+      const double2 *src = reinterpret_cast<const double2 *>(zr);
+      for (int i = lane; i < 32 * m8; i += 64) dst[i] = src[i];
     }
   } else {
     for (int k = lane; k < npad; k += 64) put(k, k < n ? xa[k] : 0.0);
@@ -202,12 +251,12 @@ __device__ __forceinline__ void corr_masked_block(Layout L, int k, long long rem
     bool na[TB], nbm[TB];
 #pragma unroll
     for (int u = 0; u < TB; ++u) {
-      za[u] = I.z[(size_t)ia[u] * npad + kk];
+      za[u] = zval(I, ia[u], kk);
       na[u] = I.nanm[(size_t)ia[u] * npad + kk] != 0;
     }
 #pragma unroll
     for (int v = 0; v < TB; ++v) {
-      zb[v] = I.z[(size_t)ib[v] * npad + kk];
+      zb[v] = zval(I, ib[v], kk);
       nbm[v] = I.nanm[(size_t)ib[v] * npad + kk] != 0;
     }
 #pragma unroll
@@ -268,9 +317,9 @@ __global__ __launch_bounds__(256) void corr_tile_kernel(Layout L, const long lon
   const long long Lg = (g % kXcds) * per_xcd + g / kXcds;
   if (Lg >= prefix[L.K]) return;
   const int k = find_inst(prefix, L.K, Lg);
-  // instances with NaN cells go to the masked kernel, fast ones to the MFMA
-  // kernel
-  if (L.ictr[k].has_nan || L.inst[k].fast) return;
+  // instances with NaN cells go to the masked kernel, fast ones to the FP32
+  // MFMA kernel, those with class-major z-scores to the FP64 MFMA kernel
+  if (L.ictr[k].has_nan || L.inst[k].fast || L.inst[k].zc) return;
   const Inst I = L.inst[k];
   const int p = I.p, npad = I.npad;
   const int nt = (p + CT - 1) / CT;
@@ -384,6 +433,205 @@ __global__ __launch_bounds__(256) void corr_tile_kernel(Layout L, const long lon
     const double r = acc[0] / div;
     I.cor[(size_t)a * p + b] = r;
     I.cor[(size_t)b * p + a] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Exact form on the FP64 matrix cores, for NaN-free instances that hold the
+// class-major z-scores.  v_mfma_f64_16x16x4f64 computes every output element
+// as four fused multiply-adds in k order, each rounded (checked bit for bit
+// against fma chains on the device, ties, subnormals and signed zeros
+// included), so one instruction advances the canonical sums of 16 x 16 pairs
+// by four consecutive samples of one lane class t: the same fma chain, in the
+// same order, as lane t of the tiled kernel.  A class's P0 leading zero steps
+// are exact too: fma(0, 0, +0) = +0, and every chain starts at +0.
+//
+// A 256-thread workgroup owns a 32 x 32 block of pairs as 2 x 2 MFMA tiles
+// (loads issued XD steps ahead).  Wave c (0..3) runs the classes t = c + 4 u, which form the
+// subtree N(c, 4) of the halving tree (N(t, s) = N(t, 2 s) + N(t + s, 2 s),
+// N(t, 64) = class t's sum, the total is N(0, 1)): it visits the leaves depth
+// first (u = the 4-bit reversal of 0, 1, ..., 15) and adds finished subtrees
+// on a four-entry stack (levels 0 and 1 in registers, 2 and 3 in LDS), so
+// each addition is the tree's.  The four subtrees then meet in LDS as
+// (N0 + N2) + (N1 + N3).  Operands come straight from L2/HBM into registers;
+// no LDS staging and no barrier in the loop.  Two workgroups share a CU, so
+// one's start and end overlap the other's matrix work.
+//
+// v_mfma_f64_16x16x4f64 lane layout: lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16]; it receives D[4 r + l / 16][l % 16] in register r.
+// ---------------------------------------------------------------------------
+constexpr int XS = 32;  // pairs per block side
+constexpr int XD = 4;   // steps (8 samples of a class each) loaded ahead
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+// Valid block v (0 <= v < nb (nb + 1) / 2) of an instance's upper block
+// triangle, in super-tile order: 8 x 8 super-tiles row-major over SA <= SB,
+// blocks row-major within each.  Only valid blocks are numbered, so equal
+// ranges of v are equal work (the XCDs each take one).  P(SA) counts the
+// blocks of super-tile rows before SA, all of them full (8 block rows).
+__device__ __forceinline__ void xblock(int nb, long long v, int &BA, int &BB) {
+  auto P = [&](long long a) { return 36 * a + 8 * (a * (nb - 8) - 4 * a * (a - 1)); };
+  const int ns = (nb + 7) >> 3;
+  int lo = 0, hi = ns - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (P(mid) <= v) lo = mid; else hi = mid - 1;
+  }
+  const int SA = lo, r = min(8, nb - 8 * SA);
+  long long o = v - P(SA);
+  const int D = r * (r + 1) / 2;
+  if (o < D) {  // the diagonal super-tile's upper triangle
+    int i = 0;
+    while (o >= r - i) { o -= r - i; ++i; }
+    BA = 8 * SA + i;
+    BB = 8 * SA + i + (int)o;
+    return;
+  }
+  o -= D;
+  const long long k = o / (8 * r);
+  const int SB = SA + 1 + (int)k;
+  o -= k * 8 * r;
+  const int cols = min(8, nb - 8 * SB);
+  BA = 8 * SA + (int)(o / cols);
+  BB = 8 * SB + (int)(o % cols);
+}
+
+__global__ __launch_bounds__(256, 2) void corr_xmfma_kernel(Layout L, const long long *__restrict__ prefix,
+                                                         long long per_xcd) {
+  // merge stack levels 2 and 3: wave, level - 2, tile, register, lane (the
+  // first also takes the wave's subtree at the end)
+  __shared__ double stk[4][2][4][4][64];
+  __shared__ double fin[XS][XS + 1];
+  const int tid = threadIdx.x, lane = tid & 63, c = tid >> 6;
+  const long long g = blockIdx.x;
+  const long long Lg = (g % kXcds) * per_xcd + g / kXcds;
+  if (Lg >= prefix[L.K]) return;
+  const int k = find_inst(prefix, L.K, Lg);
+  // the masked kernel takes instances with NaN cells, the tiled one those
+  // without class-major z-scores
+  if (L.ictr[k].has_nan || L.inst[k].fast || !L.inst[k].zc) return;
+  const Inst I = L.inst[k];
+  const int p = I.p;
+  const int nb = (p + XS - 1) / XS;
+  int BA, BB;
+  xblock(nb, Lg - prefix[k], BA, BB);
+  const int a0 = BA * XS, b0 = BB * XS;
+
+  const int m8 = zc_m8(I.npad), G = m8 >> 3;
+  const size_t pitch = 64 * (size_t)m8;
+  const int li = lane & 15, kk = lane >> 4;
+  const double *r0 = I.zc + (size_t)min(a0 + li, p - 1) * pitch + 2 * kk;
+  const double *r1 = I.zc + (size_t)min(a0 + 16 + li, p - 1) * pitch + 2 * kk;
+  const double *r2 = I.zc + (size_t)min(b0 + li, p - 1) * pitch + 2 * kk;
+  const double *r3 = I.zc + (size_t)min(b0 + 16 + li, p - 1) * pitch + 2 * kk;
+  // step s: leaf ci = s / G of the subtree (depth first), group s % G of its
+  // class
+  const int nsteps = 16 * G;
+  auto offset = [&](int st) -> size_t {
+    const int ci = st / G, gg = st - ci * G;
+    const int u = ((ci & 1) << 3) | ((ci & 2) << 1) | ((ci >> 1) & 2) | ((ci >> 3) & 1);
+    return (size_t)512 * gg + 8 * (c + 4 * u);
+  };
+  double2 buf[XD][4];
+  auto load = [&](double2 (&b)[4], int st) {
+    const size_t o = offset(min(st, nsteps - 1));
+    b[0] = *reinterpret_cast<const double2 *>(r0 + o);
+    b[1] = *reinterpret_cast<const double2 *>(r1 + o);
+    b[2] = *reinterpret_cast<const double2 *>(r2 + o);
+    b[3] = *reinterpret_cast<const double2 *>(r3 + o);
+  };
+  // slots loaded in order (the barriers keep the scheduler from reordering
+  // them), so the loop's wait for slot d is for its own loads only
+#pragma unroll
+  for (int d = 0; d < XD; ++d) {
+    load(buf[d], d);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  const d4v zero4 = {0.0, 0.0, 0.0, 0.0};
+  // acc[2 u + v]: tile (rows a0 + 16 u.., columns b0 + 16 v..)
+  d4v acc[4] = {zero4, zero4, zero4, zero4};
+  d4v st0[4] = {zero4, zero4, zero4, zero4}, st1[4] = {zero4, zero4, zero4, zero4};
+  for (int s = 0; s < nsteps; s += XD) {  // nsteps is a multiple of 16
+#pragma unroll
+    for (int d = 0; d < XD; ++d) {
+      // the slot's MFMAs, then its reload (ahead by XD steps): the loaded
+      // registers are never copied, so each wait is for one slot's loads
+      acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(buf[d][0].x, buf[d][2].x, acc[0], 0, 0, 0);
+      acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(buf[d][0].x, buf[d][3].x, acc[1], 0, 0, 0);
+      acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(buf[d][1].x, buf[d][2].x, acc[2], 0, 0, 0);
+      acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(buf[d][1].x, buf[d][3].x, acc[3], 0, 0, 0);
+      acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(buf[d][0].y, buf[d][2].y, acc[0], 0, 0, 0);
+      acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(buf[d][0].y, buf[d][3].y, acc[1], 0, 0, 0);
+      acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(buf[d][1].y, buf[d][2].y, acc[2], 0, 0, 0);
+      acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(buf[d][1].y, buf[d][3].y, acc[3], 0, 0, 0);
+      load(buf[d], s + d + XD);
+      const int st = s + d;
+      if ((st + 1) % G == 0) {
+        // leaf ci done: binary-counter merge, the earlier subtree (from the
+        // stack) the left operand, as in the tree
+        const int ci = st / G;
+        if (!(ci & 1)) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            st0[q] = acc[q];
+            acc[q] = zero4;
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc[q] = st0[q] + acc[q];
+          if (!(ci & 2)) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              st1[q] = acc[q];
+              acc[q] = zero4;
+            }
+          } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] = st1[q] + acc[q];
+            int lvl = 2;
+            for (; lvl < 4 && ((ci >> lvl) & 1); ++lvl) {
+#pragma unroll
+              for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[q][r] = stk[c][lvl - 2][q][r][lane] + acc[q][r];
+            }
+            if (lvl < 4) {  // else ci == 15: acc holds N(c, 4)
+#pragma unroll
+              for (int q = 0; q < 4; ++q) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) stk[c][lvl - 2][q][r][lane] = acc[q][r];
+                acc[q] = zero4;
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) stk[c][0][q][r][lane] = acc[q][r];
+  __syncthreads();
+  const double div = (double)((float)I.n - 1.f);
+#pragma unroll
+  for (int h = 0; h < 4; ++h) {
+    const int e = tid + 256 * h;
+    const int q = e >> 8, r = (e >> 6) & 3, l = e & 63;
+    const double tot = (stk[0][0][q][r][l] + stk[2][0][q][r][l]) + (stk[1][0][q][r][l] + stk[3][0][q][r][l]);
+    fin[16 * (q >> 1) + 4 * r + (l >> 4)][16 * (q & 1) + (l & 15)] = tot / div;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int h = 0; h < 4; ++h) {
+    const int e = tid + 256 * h;
+    const int x = e >> 5, y = e & 31;
+    // row a0 + x, columns b0 + y; and row b0 + x, columns a0 + y
+    const int a = a0 + x, b = b0 + y;
+    if (a < p && b < p && (BA != BB || a <= b)) I.cor[(size_t)a * p + b] = fin[x][y];
+    const int bt = b0 + x, at = a0 + y;
+    if (at < p && bt < p && (BA != BB || at <= bt)) I.cor[(size_t)bt * p + at] = fin[y][x];
   }
 }
 
@@ -532,6 +780,11 @@ long long tiled_slots(int p) {
   return (long long)ns * (ns + 1) / 2 * ST * ST;
 }
 
+long long xmfma_slots(int p) {
+  const long long nb = (p + XS - 1) / XS;
+  return nb * (nb + 1) / 2;
+}
+
 long long masked_blocks(int p) {
   const long long nb = (p + TB - 1) / TB;
   return nb * (nb + 1) / 2;
@@ -544,16 +797,21 @@ void launch_moments(hipStream_t s, const Layout &L, const std::vector<Inst> &hos
   launch_moments_n(s, L, max_n);
 }
 
+template <int J>
+static void launch_moments_j(hipStream_t s, const Layout &L) {
+  const int wpb = moments_waves<J>();
+  hipLaunchKernelGGL(moments_kernel<J>, dim3((L.p_total + wpb - 1) / wpb), dim3(64 * wpb), 0, s,
+                     L);
+}
+
 void launch_moments_n(hipStream_t s, const Layout &L, int max_n) {
-  const int wpb = 4;
-  const dim3 grid((L.p_total + wpb - 1) / wpb), block(64 * wpb);
   // the samples in registers when every active instance's row fits
   if (max_n <= 64 * 32)
-    hipLaunchKernelGGL(moments_kernel<32>, grid, block, 0, s, L);
+    launch_moments_j<32>(s, L);
   else if (max_n <= 64 * 80)
-    hipLaunchKernelGGL(moments_kernel<80>, grid, block, 0, s, L);
+    launch_moments_j<80>(s, L);
   else
-    hipLaunchKernelGGL(moments_kernel<0>, grid, block, 0, s, L);
+    launch_moments_j<0>(s, L);
 }
 
 void launch_corr(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst,
@@ -564,23 +822,33 @@ void launch_corr(hipStream_t s, const Layout &L, const std::vector<Inst> &host_i
   // kernel's NaN flag decides which kernel computes an instance (fast ones
   // without NaN cells go to the MFMA kernel)
   h_prefix.assign(2 * (K + 1), 0);
+  bool xm = false, tile = false;
   for (int k = 0; k < K; ++k) {
     const Inst &I = host_inst[k];
     const bool on = I.flags & kActive;
-    h_prefix[k + 1] = h_prefix[k] + (on && !I.fast ? tiled_slots(I.p) : 0);
+    const bool exact = on && !I.fast;
+    // one prefix for both exact kernels, in each instance's own unit; each
+    // kernel skips the other's instances
+    h_prefix[k + 1] = h_prefix[k] + (exact ? (I.zc ? xmfma_slots(I.p) : tiled_slots(I.p)) : 0);
     h_prefix[K + 1 + k + 1] = h_prefix[K + 1 + k] + (on ? masked_blocks(I.p) : 0);
+    xm |= exact && I.zc;
+    tile |= exact && !I.zc;
   }
   (void)hipMemcpyAsync(d_prefix, h_prefix.data(), h_prefix.size() * sizeof(long long),
                        hipMemcpyHostToDevice, s);
-  launch_corr_sums(s, L, d_prefix, h_prefix[K], h_prefix[2 * K + 1]);
+  launch_corr_sums(s, L, d_prefix, h_prefix[K], h_prefix[2 * K + 1], xm, tile);
 }
 
 void launch_corr_sums(hipStream_t s, const Layout &L, const long long *d_prefix, long long tiled,
-                      long long masked) {
+                      long long masked, bool xmfma, bool tile) {
   if (tiled > 0) {
     const long long per_xcd = (tiled + kXcds - 1) / kXcds;
-    hipLaunchKernelGGL(corr_tile_kernel, dim3((unsigned)(per_xcd * kXcds)), dim3(256), 0, s, L,
-                       d_prefix, per_xcd);
+    if (xmfma)
+      hipLaunchKernelGGL(corr_xmfma_kernel, dim3((unsigned)(per_xcd * kXcds)), dim3(256), 0, s, L,
+                         d_prefix, per_xcd);
+    if (tile)
+      hipLaunchKernelGGL(corr_tile_kernel, dim3((unsigned)(per_xcd * kXcds)), dim3(256), 0, s, L,
+                         d_prefix, per_xcd);
   }
   if (masked > 0) {
     const int wpb = 4;

@@ -171,8 +171,19 @@ int engine_init(Engine &e, int device) {
   return ensure_inst_cap(e, 1);
 }
 
+// Bytes of the z-score buffers the instance holds.
+static size_t zbytes(const InstHost &h) {
+  size_t b = 0;
+  if (h.z) b += (size_t)h.p * h.npad * 8;
+  if (h.z32) b += (size_t)h.p * h.npad * 4;
+  if (h.zc) b += (size_t)h.p * 64 * zc_m8(h.npad) * 8;
+  return b;
+}
+
 static void free_obs(Engine &e, InstHost &h) {
-  ifree(e, h.x); ifree(e, h.z); ifree(e, h.nanm); ifree(e, h.z32);
+  h.bytes -= std::min(h.bytes, zbytes(h) + (h.x ? (size_t)h.p * h.n * 8 : 0) +
+                                   (h.nanm ? (size_t)h.p * h.npad : 0));
+  ifree(e, h.x); ifree(e, h.z); ifree(e, h.nanm); ifree(e, h.z32); ifree(e, h.zc);
 }
 
 static void free_blocks(Engine &e, InstHost &h) {
@@ -323,10 +334,9 @@ int engine_alloc_observations(Engine &e, int id, double **x) {
   int rc = 0;
   const bool fresh = !h.x;
   if (!h.x) rc = ialloc(e, h.x, (size_t)h.p * h.n);
-  if (!rc && !h.z) rc = ialloc(e, h.z, (size_t)h.p * h.npad);
   if (!rc && !h.nanm) rc = ialloc(e, h.nanm, (size_t)h.p * h.npad);
   if (rc) return rc;
-  if (fresh) h.bytes += (size_t)h.p * h.n * 8 + (size_t)h.p * h.npad * 9;
+  if (fresh) h.bytes += (size_t)h.p * h.n * 8 + (size_t)h.p * h.npad;
   *x = h.x;
   return HPC_OK;
 }
@@ -556,6 +566,7 @@ int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &
     I.x = h.x; I.z = h.z; I.nanm = h.nanm; I.mean = h.mean; I.sd = h.sd; I.cor = h.cor;
     I.adj = h.adj; I.rec = h.rec; I.rec_base = h.rec_used; I.rec_cap = h.rec_cap;
     I.z32 = h.z32;
+    I.zc = h.zc;
     I.fast = h.fast ? 1 : 0;
     I.eps = kFastEps;
     I.r_crit = h.r_crit;
@@ -719,11 +730,22 @@ int engine_corr(Engine &e, float *ms_moments, float *ms_corr) {
   return HPC_OK;
 }
 
+// The z-score buffers of the instance's correlation mode: the exact form's
+// class-major copy, or the fast form's float copy with the canonical FP64
+// one (its exact rechecks read it).
 int engine_prepare_corr(Engine &e, int id) {
   InstHost &h = e.inst[id];
   h.fast = h.corr_mode == HPC_CORR_F32;
-  if (h.fast && !h.z32) return ialloc(e, h.z32, (size_t)h.p * h.npad);
-  return HPC_OK;
+  const size_t before = zbytes(h);
+  int rc = HPC_OK;
+  if (h.fast) {
+    if (!h.z) rc = ialloc(e, h.z, (size_t)h.p * h.npad);
+    if (!rc && !h.z32) rc = ialloc(e, h.z32, (size_t)h.p * h.npad);
+  } else if (!h.zc) {
+    rc = ialloc(e, h.zc, (size_t)h.p * 64 * zc_m8(h.npad));
+  }
+  h.bytes += zbytes(h) - before;
+  return rc;
 }
 
 int engine_exact_from_z(Engine &e, const std::vector<int> &ids) {
@@ -1015,7 +1037,9 @@ size_t engine_footprint(int p, int n, int max_level) {
              + pp * 8               // sepset records (level 1, every edge removed)
              + (size_t)p * W * 8    // bit adjacency
              + (size_t)p * n * 8    // observations
-             + (size_t)p * npad * 13 // z-scores (FP64, and FP32 for HPC_CORR_F32) and NaN mask
+             // z-scores (class-major FP64, or canonical FP64 + FP32 for
+             // HPC_CORR_F32: this bounds both) and the NaN mask
+             + (size_t)p * 64 * zc_m8((int)npad) * 8 + (size_t)p * npad * 5
              + (size_t)p * 16;      // mean, sd
   b += pp * (4 + 4 + 8);            // worst-case share of nbr / erow / first-pass arrays
   b += (size_t)p * 40;              // row arrays
@@ -1721,11 +1745,10 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
       InstHost &h = e.inst[id];
       if (h.fast) {
         // exact rechecks read the z-scores until the instance finishes
-        ifree(e, h.x); ifree(e, h.nanm);
         h.bytes -= std::min(h.bytes, (size_t)h.p * h.n * 8 + (size_t)h.p * h.npad);
+        ifree(e, h.x); ifree(e, h.nanm);
       } else {
         free_obs(e, h);
-        h.bytes -= std::min(h.bytes, (size_t)h.p * h.n * 8 + (size_t)h.p * h.npad * 9);
       }
       h.have_x = false;
     }

@@ -90,10 +90,15 @@ enum InstFlags {
   kCorNaN = 2,   // the correlation matrix may hold NaN: tests check their inputs
 };
 
+// Slots per lane class of the class-major z-scores: npad / 64 samples rounded
+// up to a multiple of 8.
+__host__ __device__ __forceinline__ int zc_m8(int npad) { return ((npad >> 6) + 7) & ~7; }
+
 // Device-side descriptor of one instance.
 struct Inst {
   const double *x;        // [p][n] observations
-  double *z;              // [p][npad] z-scores
+  double *z;              // [p][npad] z-scores (the fast form's rechecks; or null)
+  double *zc;             // class-major z-scores (the exact form; or null)
   uint8_t *nanm;          // [p][npad] NaN mask
   double *mean, *sd;      // [p]
   double *cor;            // [p][p]
@@ -144,10 +149,12 @@ void launch_moments_n(hipStream_t s, const Layout &L, int max_n);
 void launch_corr(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst,
                  long long *d_prefix, std::vector<long long> &h_prefix);
 // The pairwise-sum kernels of launch_corr on block prefixes already in device
-// memory: d_prefix holds the K + 1 tiled-slot prefixes, then the K + 1
-// masked-block prefixes; tiled and masked are their totals.
+// memory: d_prefix holds the K + 1 exact-kernel prefixes (matrix-core blocks
+// for instances with class-major z-scores, tiled slots for the others), then
+// the K + 1 masked-block prefixes; tiled and masked are their totals, xmfma
+// and tile whether any instance takes the matrix-core or the tiled kernel.
 void launch_corr_sums(hipStream_t s, const Layout &L, const long long *d_prefix, long long tiled,
-                      long long masked);
+                      long long masked, bool xmfma = false, bool tile = true);
 // Work units of one instance in the tiled and the masked pairwise kernels.
 long long tiled_slots(int p);
 long long masked_blocks(int p);
@@ -274,6 +281,7 @@ struct InstHost {
   double alpha = 0.0, r_crit = 0.0;
   // device blocks (from the engine's arena)
   double *x = nullptr, *z = nullptr, *mean = nullptr, *sd = nullptr, *cor = nullptr;
+  double *zc = nullptr;  // class-major z-scores of the exact form
   uint8_t *nanm = nullptr;
   unsigned long long *adj = nullptr;
   int32_t *rec = nullptr;
