@@ -466,35 +466,39 @@ constexpr int XD = 4;   // steps (8 samples of a class each) loaded ahead
 typedef double d4v __attribute__((ext_vector_type(4)));
 
 // Valid block v (0 <= v < nb (nb + 1) / 2) of an instance's upper block
-// triangle, in super-tile order: 8 x 8 super-tiles row-major over SA <= SB,
+// triangle, in super-tile order: XT x XT super-tiles row-major over SA <= SB,
 // blocks row-major within each.  Only valid blocks are numbered, so equal
 // ranges of v are equal work (the XCDs each take one).  P(SA) counts the
-// blocks of super-tile rows before SA, all of them full (8 block rows).
+// blocks of super-tile rows before SA, all of them full (XT block rows).
+constexpr int XT = 8;
+
 __device__ __forceinline__ void xblock(int nb, long long v, int &BA, int &BB) {
-  auto P = [&](long long a) { return 36 * a + 8 * (a * (nb - 8) - 4 * a * (a - 1)); };
-  const int ns = (nb + 7) >> 3;
+  auto P = [&](long long a) {
+    return (long long)XT * (XT + 1) / 2 * a + (long long)XT * (a * (nb - XT) - (long long)XT * a * (a - 1) / 2);
+  };
+  const int ns = (nb + XT - 1) / XT;
   int lo = 0, hi = ns - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (P(mid) <= v) lo = mid; else hi = mid - 1;
   }
-  const int SA = lo, r = min(8, nb - 8 * SA);
+  const int SA = lo, r = min(XT, nb - XT * SA);
   long long o = v - P(SA);
   const int D = r * (r + 1) / 2;
   if (o < D) {  // the diagonal super-tile's upper triangle
     int i = 0;
     while (o >= r - i) { o -= r - i; ++i; }
-    BA = 8 * SA + i;
-    BB = 8 * SA + i + (int)o;
+    BA = XT * SA + i;
+    BB = XT * SA + i + (int)o;
     return;
   }
   o -= D;
-  const long long k = o / (8 * r);
+  const long long k = o / ((long long)XT * r);
   const int SB = SA + 1 + (int)k;
-  o -= k * 8 * r;
-  const int cols = min(8, nb - 8 * SB);
-  BA = 8 * SA + (int)(o / cols);
-  BB = 8 * SB + (int)(o % cols);
+  o -= k * XT * r;
+  const int cols = min(XT, nb - XT * SB);
+  BA = XT * SA + (int)(o / cols);
+  BB = XT * SB + (int)(o % cols);
 }
 
 __global__ __launch_bounds__(256, 2) void corr_xmfma_kernel(Layout L, const long long *__restrict__ prefix,

@@ -570,6 +570,7 @@ int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &
     I.fast = h.fast ? 1 : 0;
     I.eps = kFastEps;
     I.r_crit = h.r_crit;
+    square_bounds(h.r_crit, I.r2_lo, I.r2_hi);
     I.start = start;
     I.p = h.p; I.n = h.n; I.npad = h.npad; I.W = h.W;
     I.level = levels[k];

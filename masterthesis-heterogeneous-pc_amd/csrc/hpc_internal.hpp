@@ -110,6 +110,7 @@ struct Inst {
   unsigned long long rec_base;  // words written by earlier levels
   unsigned long long rec_cap;   // words the record buffer holds (writes beyond are dropped)
   double r_crit;          // Fisher-z decision boundary of (n, alpha)
+  double r2_lo, r2_hi;    // square_bounds(r_crit): the squared decision filter
   double eps;             // bound on |cor - exact cor| when `fast`
   int start;              // first global variable id
   int p, n, npad, W;

@@ -107,59 +107,161 @@ __host__ __device__ __noinline__ double pcor_pinv(double (&A)[M][M]) {
 // (gpuRowTests.cu:143-170).  One reciprocal per pivot and one square root per
 // test.  A pivot at or below 1e-12 or a numerically singular 2x2 Schur block
 // switches to the pinv form.
-template <int L, bool SENS>
-__host__ __device__ __forceinline__ double pcor_canon_impl(double a, const double (&x)[L],
-                                                  const double (&y)[L],
-                                                  const double (&C)[L][L], bool &used_pinv,
-                                                  double *sens) {
-  double Um[L][L], Em[L][L], inv[L];
-  double u[L], v[L];
-  used_pinv = false;
-  bool sing = false;
+// The part of pcor_canon that depends on S and x only (one row i, one
+// conditioning set): the factorisation C = U D U^T, u = U^-1 x, w = D^-1 u
+// and s00 = 1 - w.u.  A search that tests one S against many j computes it
+// once; every value is formed by the same operations as in a per-test call.
+template <int L>
+struct PcorPrep {
+  double Um[L][L], inv[L], u[L], wu[L];
+  double s00;
+  bool sing;  // a pivot at or below kPivTol
+};
+
+template <int L>
+__host__ __device__ __forceinline__ void pcor_prep(const double (&x)[L], const double (&C)[L][L],
+                                                   PcorPrep<L> &P) {
+  double Em[L][L];
+  P.sing = false;
 #pragma unroll
   for (int r = 0; r < L; ++r) {
 #pragma unroll
     for (int c = 0; c < r; ++c) {
       double e = C[r][c];
 #pragma unroll
-      for (int k = 0; k < c; ++k) e = __builtin_fma(-Em[r][k], Um[c][k], e);
+      for (int k = 0; k < c; ++k) e = __builtin_fma(-Em[r][k], P.Um[c][k], e);
       Em[r][c] = e;
-      Um[r][c] = e * inv[c];
+      P.Um[r][c] = e * P.inv[c];
     }
     double d = 1.0;
 #pragma unroll
-    for (int k = 0; k < r; ++k) d = __builtin_fma(-Em[r][k], Um[r][k], d);
-    sing |= !(d > kPivTol);
-    inv[r] = 1.0 / d;
+    for (int k = 0; k < r; ++k) d = __builtin_fma(-Em[r][k], P.Um[r][k], d);
+    P.sing |= !(d > kPivTol);
+    P.inv[r] = 1.0 / d;
   }
 #pragma unroll
   for (int c = 0; c < L; ++c) {
-    double su = x[c], sv = y[c];
+    double su = x[c];
 #pragma unroll
-    for (int k = 0; k < c; ++k) {
-      su = __builtin_fma(-Um[c][k], u[k], su);
-      sv = __builtin_fma(-Um[c][k], v[k], sv);
-    }
-    u[c] = su;
-    v[c] = sv;
+    for (int k = 0; k < c; ++k) su = __builtin_fma(-P.Um[c][k], P.u[k], su);
+    P.u[c] = su;
   }
-  double s00 = 1.0, s11 = 1.0, s01 = a;
-  double gx[L], gy[L];
+  double s00 = 1.0;
 #pragma unroll
   for (int k = 0; k < L; ++k) {
-    const double wu = u[k] * inv[k], wv = v[k] * inv[k];
-    s00 = __builtin_fma(-wu, u[k], s00);
-    s11 = __builtin_fma(-wv, v[k], s11);
-    s01 = __builtin_fma(-wu, v[k], s01);
-    gx[k] = wu;
-    gy[k] = wv;
+    const double wu = P.u[k] * P.inv[k];
+    s00 = __builtin_fma(-wu, P.u[k], s00);
+    P.wu[k] = wu;
   }
-  double q = s00 * s11;
-  sing |= !(s00 > kPivTol) || !(s11 > kPivTol) ||
-          !(__builtin_fma(-s01, s01, q) > kPivTol * q);
+  P.s00 = s00;
+}
+
+// The j part: v = U^-1 y, s11 = 1 - (D^-1 v).v, s01 = a - w.v and
+// q = s00 s11 (wv receives D^-1 v).  Returns whether S or the 2 x 2 Schur
+// block is numerically singular (then the pinv form decides); else the
+// statistic is s01 / sqrt(q).
+template <int L>
+__host__ __device__ __forceinline__ bool pcor_schur(const PcorPrep<L> &P, double a,
+                                                    const double (&y)[L], double &s11o,
+                                                    double &s01o, double &qo,
+                                                    double (&wv)[L]) {
+  double v[L];
+#pragma unroll
+  for (int c = 0; c < L; ++c) {
+    double sv = y[c];
+#pragma unroll
+    for (int k = 0; k < c; ++k) sv = __builtin_fma(-P.Um[c][k], v[k], sv);
+    v[c] = sv;
+  }
+  double s11 = 1.0, s01 = a;
+#pragma unroll
+  for (int k = 0; k < L; ++k) {
+    const double w = v[k] * P.inv[k];
+    s11 = __builtin_fma(-w, v[k], s11);
+    s01 = __builtin_fma(-P.wu[k], v[k], s01);
+    wv[k] = w;
+  }
+  const double s00 = P.s00;
+  const double q = s00 * s11;
+  const bool sing = P.sing || !(s00 > kPivTol) || !(s11 > kPivTol) ||
+                    !(__builtin_fma(-s01, s01, q) > kPivTol * q);
+  s11o = s11;
+  s01o = s01;
+  qo = q;
+  return sing;
+}
+
+// pcor_schur, then r = s01 / sqrt(q) unless singular.
+template <int L>
+__host__ __device__ __forceinline__ bool pcor_finish(const PcorPrep<L> &P, double a,
+                                                     const double (&y)[L], double &r,
+                                                     double &s11o, double &s01o,
+                                                     double (&wv)[L]) {
+  double q;
+  const bool sing = pcor_schur<L>(P, a, y, s11o, s01o, q, wv);
+  if (!sing) r = s01o / __builtin_sqrt(q);
+  return sing;
+}
+
+// removes(s01 / sqrt(q), r_crit) for finite s01 and q > 0, mostly without the
+// square root and the division.  With u = 2^-53, R = r_crit,
+// lo = fl(fl(R R) (1 - 64 u)) and hi = fl(fl(R R) (1 + 64 u)) (square_bounds,
+// for 1e-100 < R < 0.9999999, where the clamp of removes cannot matter):
+// fl(s01 s01) <= fl(lo q) gives s01^2 / q <= R^2 (1 - 64 u)(1 + u)^3 / (1 - u)
+// <= R^2 (1 - 59 u), so |s01| / sqrt(q) <= R (1 - 29 u) and the computed
+// quotient |s01| / fl(sqrt(q)) <= R (1 - 27 u) < R, which rounds to at most R:
+// removed.  fl(s01 s01) >= fl(hi q) gives likewise a quotient of at least
+// R (1 + 27 u), more than one ulp above R, which rounds above R: kept.  Only
+// squares inside that band take the exact quotient, so the decision is
+// always that of the exact statistic.  (A subnormal square only errs towards
+// zero, where the quotient is far below R.)
+__host__ __device__ __forceinline__ bool removes_sq(double s01, double q, double r_crit, double lo,
+                                                    double hi) {
+  const double s2 = s01 * s01;
+  if (s2 <= lo * q) return true;
+  if (s2 >= hi * q) return false;
+  return removes(s01 / __builtin_sqrt(q), r_crit);
+}
+
+// The bounds of removes_sq; outside 1e-100 < r_crit < 0.9999999 they never
+// decide (lo q < 0 <= s01^2 < hi q = inf), so the exact quotient does.
+__host__ __device__ inline void square_bounds(double r_crit, double &lo, double &hi) {
+  if (r_crit > 1e-100 && r_crit < kCutThr) {
+    const double r2 = r_crit * r_crit;
+    lo = r2 * (1.0 - 0x1p-47);
+    hi = r2 * (1.0 + 0x1p-47);
+  } else {
+    lo = -1.0;
+    hi = __builtin_inf();
+  }
+}
+
+// Partial correlation of i and j given S (|S| = L) from a = r_ij,
+// x[c] = r_i,S[c], y[c] = r_j,S[c] and the strictly lower triangle
+// C[r][c] = r_S[c],S[r] (r > c).  With C = U D U^T (U unit lower, E = U D),
+// u = U^-1 x, v = U^-1 y and w = D^-1 u:
+// r = (a - w.v) / sqrt((1 - w.u)(1 - (D^-1 v).v)), the same value as
+// -P01/sqrt(P00 P11) for P = inverse of the (L+2)-square submatrix [i, j, S]
+// (gpuRowTests.cu:143-170).  One reciprocal per pivot and one square root per
+// test.  A pivot at or below 1e-12 or a numerically singular 2x2 Schur block
+// switches to the pinv form.
+template <int L, bool SENS>
+__host__ __device__ __forceinline__ double pcor_canon_impl(double a, const double (&x)[L],
+                                                  const double (&y)[L],
+                                                  const double (&C)[L][L], bool &used_pinv,
+                                                  double *sens) {
+  PcorPrep<L> P;
+  pcor_prep<L>(x, C, P);
+  double gx[L], gy[L], s11, s01, r = 0.0;
+  const bool sing = pcor_finish<L>(P, a, y, r, s11, s01, gy);
+  used_pinv = false;
+  const double s00 = P.s00;
+#pragma unroll
+  for (int k = 0; k < L; ++k) gx[k] = P.wu[k];
   if (!sing) {
-    const double r = s01 / __builtin_sqrt(q);
     if (SENS) {
+      const double q = s00 * s11;
+      const auto &Um = P.Um;
       // rows 0 and 1 of P = M^-1: [S^-1, -S^-1 G^T] with S the 2x2 Schur
       // block and G = C^-1 [x y] = U^-T D^-1 [u v] (back substitution)
 #pragma unroll

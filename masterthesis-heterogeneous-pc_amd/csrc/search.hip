@@ -649,6 +649,194 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
   if (lane == 0 && F) atomicAdd(&L.ctr->pinv_fallbacks, F);
 }
 
+// Level LV >= 2 of exact instances, by conditioning set: the same tests and
+// results as search_ln_lds_kernel, organised around S instead of the edge.
+// Every test (i, j | S) of row i has S an LV-subset of N(i) not holding j, so
+// the row's C(deg, LV) sets S, each against every j of N(i) \ S, are exactly
+// its tests.  The part of a test that depends on i and S only (the LDL^T
+// factorisation of S's block, U^-1 x and s00; pcor_prep) is computed once per
+// S and shared by its deg - LV tests (pcor_finish does the rest, with the same
+// operations as a per-test call, so every statistic has the same bits).  A
+// passing (j, S) is turned into its rank among the LV-subsets of N(i) \ {j} in
+// lexicographic order of positions (the edge's own enumeration) and lowers the
+// edge's first passing rank with atomicMin; a lane walks its sets in
+// lexicographic order, in which each edge's ranks rise, so it skips an edge
+// for the rest of its run once the edge passed.  The row's tasks of the plan
+// share its sets evenly: the segment [t0, t1) of the row's U tasks takes sets
+// [t0 U^-1 NS, t1 U^-1 NS) (floors, NS = C(deg, LV)), spread over the
+// workgroup's lanes in runs of consecutive sets.  Rows longer than the LDS
+// block are left to search_ln_kernel, as for search_ln_lds_kernel.
+__device__ __forceinline__ unsigned long long mul_div(unsigned long long a, unsigned long long b,
+                                                      unsigned long long c) {
+  return (unsigned long long)(((unsigned __int128)a * b) / c);
+}
+
+template <int LV>
+__global__ __launch_bounds__(64 * kWavesPerBlock) void search_ls_kernel(
+    Layout L, const long long *__restrict__ task_off, const long long *__restrict__ row_tasks,
+    int tpb, int dmax, TaskQueue Q) {
+  static_assert(LV >= 2, "levels >= 2");
+  extern __shared__ unsigned long long smem[];
+  unsigned long long *bt = smem;  // binomial rows 0..dmax-1 x (LV+1)
+  double *M = reinterpret_cast<double *>(smem + (size_t)dmax * (LV + 1));  // dmax x dmax
+  int *ids = reinterpret_cast<int *>(M + (size_t)dmax * dmax);
+  const lds_u64 *tb = (const lds_u64 *)bt;
+  const int tid = threadIdx.x;
+  const int nthreads = 64 * kWavesPerBlock;
+  for (int t = tid; t < dmax * (LV + 1); t += nthreads) {
+    const int m = t / (LV + 1), k = t % (LV + 1);
+    bt[t] = L.binom[(size_t)m * kBinomK + k];
+  }
+  const long long n_tasks = (long long)L.ctr->n_tasks;
+  const long long n_blocks = (n_tasks + tpb - 1) / tpb;
+  unsigned long long F = 0;
+  unsigned long long *fpu = reinterpret_cast<unsigned long long *>(L.fp);
+  __shared__ long long s_blk;
+  for (long long blk = Q.head ? claim_block(Q, n_blocks, s_blk) : blockIdx.x; blk < n_blocks;
+       blk = Q.head ? claim_block(Q, n_blocks, s_blk) : blk + gridDim.x) {
+    long long t0 = blk * tpb;
+    const long long t1 = min(n_tasks, t0 + tpb);
+    while (t0 < t1) {
+      const int i = task_row(task_off, L.p_total, t0);
+      const long long off = task_off[i];
+      const long long U = row_tasks[i];
+      const long long seg_end = min(t1, off + U);
+      if (seg_end <= t0) break;  // (a consistent plan always advances)
+      const int base = L.rowptr[i];
+      const int deg = L.rowptr[i + 1] - base;
+      const int D = deg + 1;
+      if (D > dmax) {  // search_ln_kernel's row
+        t0 = seg_end;
+        continue;
+      }
+      const Inst &In = L.inst[L.row_inst[i]];
+      const int p = In.p, s0 = In.start;
+      __syncthreads();  // the previous segment is done with M (and bt is staged)
+      for (int a = tid; a < D; a += nthreads) ids[a] = (a < deg ? L.nbr[base + a] : i) - s0;
+      __syncthreads();
+      for (int idx = tid; idx < D * D; idx += nthreads) {
+        const int a = idx / D, b = idx - a * D;
+        M[idx] = In.cor[(size_t)ids[a] * p + ids[b]];
+      }
+      __syncthreads();
+      const double r_crit = In.r_crit, r2_lo = In.r2_lo, r2_hi = In.r2_hi;
+      const bool chk = In.flags & kCorNaN;
+      const double *Mi = M + (size_t)deg * D;  // row i
+      const unsigned long long NS = tb[deg * (LV + 1) + LV];
+      const unsigned long long s_lo = mul_div((unsigned long long)(t0 - off), NS, (unsigned long long)U);
+      const unsigned long long s_hi = mul_div((unsigned long long)(seg_end - off), NS, (unsigned long long)U);
+      const unsigned long long cnt = s_hi - s_lo;
+      const unsigned long long kr = (cnt + nthreads - 1) / nthreads;
+      const unsigned long long sb = s_lo + (unsigned long long)tid * kr;
+      const unsigned long long se = min(s_hi, sb + kr);
+      const int nn = deg - 1;  // an edge's enumeration: LV-subsets of N(i) \ {j}
+      if (sb < se) {
+        int q[LV];  // S as positions in N(i), ascending
+        unrank_lds<LV>(tb, LV + 1, sb, deg, q);
+        unsigned long long done0 = 0, done1 = 0;  // edges (positions) passed in this run
+        const unsigned long long all0 = deg >= 64 ? ~0ull : (1ull << deg) - 1;
+        const unsigned long long all1 = deg > 64 ? (1ull << (deg - 64)) - 1 : 0ull;
+        for (unsigned long long sr = sb; sr < se; ++sr) {
+          double x[LV], C[LV][LV];
+          unsigned long long in0 = 0, in1 = 0;  // S's positions
+#pragma unroll
+          for (int c = 0; c < LV; ++c) {
+            x[c] = Mi[q[c]];
+            if (q[c] < 64) in0 |= 1ull << q[c]; else in1 |= 1ull << (q[c] - 64);
+          }
+#pragma unroll
+          for (int c = 0; c < LV; ++c)
+#pragma unroll
+            for (int r = c + 1; r < LV; ++r) C[r][c] = M[q[c] * D + q[r]];
+          bool nanS = false;
+          if (chk) {
+#pragma unroll
+            for (int c = 0; c < LV; ++c) nanS |= __builtin_isnan(x[c]);
+#pragma unroll
+            for (int c = 0; c < LV; ++c)
+#pragma unroll
+              for (int r = c + 1; r < LV; ++r) nanS |= __builtin_isnan(C[r][c]);
+          }
+          // NaN inputs keep the edge (cpuInd.cpp:23-25,31-33): no test of S
+          if (!nanS) {
+            PcorPrep<LV> P;
+            pcor_prep<LV>(x, C, P);
+            // the edges to test: positions of N(i) outside S not yet passed,
+            // two at a time (two independent statistics in flight per lane)
+            unsigned long long c0 = ~(in0 | done0) & all0, c1 = ~(in1 | done1) & all1;
+            auto pop = [&]() -> int {
+              if (c0) {
+                const int y = __builtin_ctzll(c0);
+                c0 &= c0 - 1;
+                return y;
+              }
+              const int y = 64 + __builtin_ctzll(c1);
+              c1 &= c1 - 1;
+              return y;
+            };
+            while (c0 | c1) {
+              const int ya = pop();
+              const bool has_b = (c0 | c1) != 0;
+              const int yb = has_b ? pop() : ya;
+              const double aa = Mi[ya], ab = Mi[yb];
+              double va[LV], vb[LV];
+#pragma unroll
+              for (int c = 0; c < LV; ++c) {
+                va[c] = M[ya * D + q[c]];
+                vb[c] = M[yb * D + q[c]];
+              }
+              double s11, s01a, s01b, qa, qb, wv[LV];
+              const bool sa = pcor_schur<LV>(P, aa, va, s11, s01a, qa, wv);
+              const bool sb2 = pcor_schur<LV>(P, ab, vb, s11, s01b, qb, wv);
+#pragma unroll
+              for (int h = 0; h < 2; ++h) {
+                if (h == 1 && !has_b) break;
+                const int y = h ? yb : ya;
+                const double a = h ? ab : aa;
+                if (chk) {
+                  bool nan = __builtin_isnan(a);
+#pragma unroll
+                  for (int c = 0; c < LV; ++c) nan |= __builtin_isnan(h ? vb[c] : va[c]);
+                  if (nan) continue;
+                }
+                bool rm;
+                if (h ? sb2 : sa) {
+                  bool fb;
+                  // the pinv form, as per test
+                  rm = removes(pcor_canon<LV>(a, x, h ? vb : va, C, fb), r_crit);
+                  ++F;
+                } else {
+                  rm = removes_sq(h ? s01b : s01a, h ? qb : qa, r_crit, r2_lo, r2_hi);
+                }
+                if (!rm) continue;
+                // S's rank among the LV-subsets of N(i) \ {j}: positions past
+                // y move down by one; rank = sum over members of
+                // C(nn - prev - 1, LV - c) - C(nn - q'_c, LV - c)
+                unsigned long long rk = 0;
+                int prev = -1;
+#pragma unroll
+                for (int c = 0; c < LV; ++c) {
+                  const int qc = q[c] - (q[c] > y ? 1 : 0);
+                  rk += tb[(nn - prev - 1) * (LV + 1) + (LV - c)] - tb[(nn - qc) * (LV + 1) + (LV - c)];
+                  prev = qc;
+                }
+                const long long e = base + y;
+                if (rk < __atomic_load_n(&fpu[e], __ATOMIC_RELAXED)) atomicMin(&fpu[e], rk);
+                if (y < 64) done0 |= 1ull << y; else done1 |= 1ull << (y - 64);
+              }
+            }
+          }
+          if (sr + 1 < se) next_comb<LV>(q, deg);
+        }
+      }
+      t0 = seg_end;
+    }
+  }
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) F += __shfl_xor(F, s, 64);
+  if ((tid & 63) == 0 && F) atomicAdd(&L.ctr->pinv_fallbacks, F);
+}
+
 // Tasks per row: deg * ceil(C(deg - 1, level) / rpt) (row_task_count) for rows
 // of instances at this level (and in the row mask).
 __global__ void task_count_kernel(Layout L, int level, int rpt, const int32_t *__restrict__ row_mask,
@@ -1125,10 +1313,24 @@ void launch_ln_as(hipStream_t s, const Layout &L, int rpt, const long long *row_
                   const long long *task_off, int tpb, size_t limit, const TaskQueue &Q) {
   const int dmax = std::min(L.max_deg + 1, lds_side(limit, LV));
   // a testable row has deg - 1 >= LV, so a block side of at least LV + 2
-  if (dmax >= LV + 2)
-    hipLaunchKernelGGL((search_ln_lds_kernel<LV, FAST>), dim3(kSearchBlocks),
-                       dim3(64 * kWavesPerBlock), lds_bytes(dmax, LV), s, L, rpt, task_off,
-                       row_tasks, tpb, dmax, Q);
+  if (dmax >= LV + 2) {
+    // exact levels >= 2 by conditioning set, unless host threads share the
+    // queue (their side runs the per-edge tasks)
+    if constexpr (!FAST && LV >= 2) {
+      if (!Q.done) {
+        hipLaunchKernelGGL((search_ls_kernel<LV>), dim3(kSearchBlocks), dim3(64 * kWavesPerBlock),
+                           lds_bytes(dmax, LV), s, L, task_off, row_tasks, tpb, dmax, Q);
+      } else {
+        hipLaunchKernelGGL((search_ln_lds_kernel<LV, FAST>), dim3(kSearchBlocks),
+                           dim3(64 * kWavesPerBlock), lds_bytes(dmax, LV), s, L, rpt, task_off,
+                           row_tasks, tpb, dmax, Q);
+      }
+    } else {
+      hipLaunchKernelGGL((search_ln_lds_kernel<LV, FAST>), dim3(kSearchBlocks),
+                         dim3(64 * kWavesPerBlock), lds_bytes(dmax, LV), s, L, rpt, task_off,
+                         row_tasks, tpb, dmax, Q);
+    }
+  }
   if (L.max_deg + 1 <= dmax) return;
   const size_t lds = (size_t)(L.max_deg + 1) * (LV + 1) * sizeof(unsigned long long);
   const int use = lds <= 48 * 1024;
