@@ -557,7 +557,7 @@ def main() -> int:
     entry("moments_kernel", "hbm", mom_bytes / (k_mom * 1e-3) / 1e9 if k_mom else 0.0,
           HBM_PEAK_GBS, "GB/s", k_mom, "%d B: observations read, z-scores and NaN mask written"
           % mom_bytes)
-    entry("corr_tile_kernel", "fp64", corr_tflops, FP64_PEAK_TFLOPS, "TFLOP/s", k_corr,
+    entry("corr_xmfma_kernel", "mfma", corr_tflops, FP64_PEAK_TFLOPS, "TFLOP/s", k_corr,
           "%.3e flops: 2n per pair, p(p+1)/2 pairs" % corr_flops)
     for lv in last:
         l, T = lv["level"], lv["tests"]
@@ -573,7 +573,7 @@ def main() -> int:
         else:
             fl = flops_per_test(l) * T
             hb = bytes_per_test(l) * T / (ms * 1e-3) / 1e9
-            entry("search_ln_lds_kernel<%d>" % l, "fp64", fl / (ms * 1e-3) / 1e12,
+            entry("search_ls_kernel<%d>" % l, "fp64", fl / (ms * 1e-3) / 1e12,
                   FP64_PEAK_TFLOPS, "TFLOP/s", ms,
                   "%d canonical tests x %.1f flops" % (T, flops_per_test(l)), level=l,
                   hbm_algorithmic_GBs=hb, hbm_algorithmic_frac=hb / HBM_PEAK_GBS)
@@ -588,14 +588,14 @@ def main() -> int:
     # 45% of the step) on its own launch time, HIP events on the engine
     # stream.  Its per-launch HBM bytes come from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE summary of this command.
-    roof = {"kernel": "corr_tile_kernel", "bound": "mfma", "achieved": corr_tflops,
+    roof = {"kernel": "corr_xmfma_kernel", "bound": "mfma", "achieved": corr_tflops,
             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": corr_tflops / FP64_PEAK_TFLOPS,
             "traffic": traffic.get("corr_traffic_bytes"),
             "algorithmic": "%.3e flops per launch (n p (p+1))" % corr_flops,
             "traffic_source": traffic.get("source"),
-            "note": "FP64 FMA on the vector units (the canonical summation order takes one "
-                    "fma per product in sequence, so no MFMA); 78.6 TFLOP/s is MI355X's FP64 "
-                    "peak, the same for the vector units and the matrix cores"}
+            "note": "FP64 matrix cores (v_mfma_f64_16x16x4f64: each output is four fused "
+                    "multiply-adds in sample order, so the canonical summation order is kept "
+                    "bit for bit); 78.6 TFLOP/s is MI355X's dense FP64 MFMA peak"}
 
     out = {
         "metric": METRIC,
