@@ -72,23 +72,30 @@ __device__ __forceinline__ int tree64i(int v) {
 
 // Class-major z-scores (zc, read by the matrix-core kernel).  Lane class t
 // holds the samples t, t + 64, ... (m = 0 .. M - 1, M = npad / 64) after
-// P0 = M8 - M leading zeros, M8 = M rounded up to 8 (zc_m8); slot
-// m' = m + P0 = 8 g + r.  Row a is M8 / 8 groups of 64 classes x 8 slots
-// (group g, class t at a * 64 M8 + 512 g + 8 t), so the moments kernel's
-// lanes write adjacent 64-byte pieces.  Within a group slot 2 (r % 4) + r / 4
-// holds m', so one 16-byte load of lane class kk (0..3) at 2 kk yields
-// m' = 8 g + kk and 8 g + 4 + kk: the k-th element of two consecutive 4-step
-// MFMA operands.
-__device__ __forceinline__ size_t zc_index(int m8, int p0, int a, int k) {
+// P0 = MS - M leading zeros, MS = M rounded up to 16 (zc_slots); slot
+// m' = m + P0 = 16 g + j, j = 8 h + 4 b + kk (h, b in 0..1, kk in 0..3).
+// Rows are stored in blocks of 16: block R holds rows 16 R .. 16 R + 15 as
+// [g][t][h][kk][row % 16][b], 256 doubles per (g, t).  For one (g, t) and
+// half h, the 64 lanes of a wavefront (lane = 16 kk + row % 16) load 16 bytes
+// each from 1 KB of consecutive memory, and get m' = 16 g + 8 h + kk and
+// 16 g + 8 h + 4 + kk of their row: the k-th elements of two consecutive
+// 4-step MFMA operands of 16 rows.  (One load instruction then touches 8
+// whole cache lines; a row-major layout, 16 rows per instruction, measured at
+// half the load throughput.)
+__host__ __device__ __forceinline__ size_t zc_off(int g, int t, int j, int r) {
+  return (size_t)16384 * g + 256 * t + 32 * (4 * (j >> 3) + (j & 3)) + 2 * r + ((j >> 2) & 1);
+}
+
+__device__ __forceinline__ size_t zc_index(size_t blk, int p0, int a, int k) {
   const int t = k & 63, mp = (k >> 6) + p0;
-  return (size_t)a * 64 * m8 + 512 * (mp >> 3) + 8 * t + 2 * (mp & 3) + ((mp >> 2) & 1);
+  return (size_t)(a >> 4) * blk + zc_off(mp >> 4, t, mp & 15, a & 15);
 }
 
 // z-score of row a, sample k, from whichever layout the instance holds
 __device__ __forceinline__ double zval(const Inst &I, int a, int k) {
   if (I.z) return I.z[(size_t)a * I.npad + k];
-  const int m8 = zc_m8(I.npad);
-  return I.zc[zc_index(m8, m8 - (I.npad >> 6), a, k)];
+  const int ms = zc_slots(I.npad);
+  return I.zc[zc_index(zc_block(I.npad), ms - (I.npad >> 6), a, k)];
 }
 
 // k with prefix[k] <= idx < prefix[k+1] (empty instances skipped)
@@ -110,14 +117,16 @@ __device__ __forceinline__ int find_inst(const long long *prefix, int K, long lo
 // registers and the three passes run from there; J = 0 re-reads the row for
 // each pass.  Both do the same operations in the same order.  With J > 0 a
 // row's class-major z-scores are assembled in the wave's LDS area and stored
-// as whole 1 KB pieces (written lane by lane, each store would touch 32 cache
-// lines).
+// 16 bytes per lane (written value by value, each store would touch 64
+// cache lines).
 template <int J>
 constexpr int moments_waves() { return J > 32 ? 2 : 4; }
 
 template <int J>
 __global__ __launch_bounds__(64 * moments_waves<J>()) void moments_kernel(Layout L) {
   // one class-major row per wave (64 classes x up to J slots)
+  // (a class-major row is 64 x zc_slots(npad) <= 64 J doubles)
+  static_assert(J % 16 == 0, "whole 16-slot groups");
   __shared__ double zrow[J > 0 ? moments_waves<J>() : 1][J > 0 ? 64 * J : 1];
   const int lane = threadIdx.x & 63;
   const int g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -182,18 +191,29 @@ __global__ __launch_bounds__(64 * moments_waves<J>()) void moments_kernel(Layout
   double *za = I.z ? I.z + (size_t)a * npad : nullptr;
   uint8_t *na = I.nanm + (size_t)a * npad;
   float *z32 = I.z32 ? I.z32 + (size_t)a * npad : nullptr;
-  const int m8 = zc_m8(npad), p0 = m8 - (npad >> 6);
-  // class-major target: the wave's LDS row (J > 0), else the row in memory
-  double *zr = nullptr;
-  if (I.zc) zr = J > 0 ? zrow[threadIdx.x >> 6] : I.zc + (size_t)a * 64 * m8;
-  if (zr) {
+  const int ms = zc_slots(npad), p0 = ms - (npad >> 6);
+  // class-major target: with J > 0 the row is assembled in the wave's LDS
+  // area, ordered [g][t][h][kk][b] (zc_off of row 0 within one 16-slot
+  // group, packed), then stored piece by piece; else each value goes to its
+  // place in the row block directly
+  const size_t zblk = zc_block(npad);
+  double *zl = J > 0 && I.zc ? zrow[threadIdx.x >> 6] : nullptr;
+  auto lds_pos = [](int mp, int t) {
+    const int j = mp & 15;
+    return 1024 * (mp >> 4) + 16 * t + 2 * (4 * (j >> 3) + (j & 3)) + ((j >> 2) & 1);
+  };
+  if (I.zc) {
     // the leading zeros of this lane's class (all in group 0)
-    for (int mp = 0; mp < p0; ++mp) zr[8 * lane + 2 * (mp & 3) + ((mp >> 2) & 1)] = 0.0;
+    for (int mp = 0; mp < p0; ++mp) {
+      if (zl) zl[lds_pos(mp, lane)] = 0.0;
+      else I.zc[(size_t)(a >> 4) * zblk + zc_off(0, lane, mp, a & 15)] = 0.0;
+    }
   }
   auto put = [&](int k, double v) {
     const double zk = k < n ? (v - mu) / sd : 0.0;
     if (za) za[k] = zk;
-    if (zr) zr[zc_index(m8, p0, 0, k)] = zk;
+    if (zl) zl[lds_pos((k >> 6) + p0, k & 63)] = zk;
+    else if (I.zc) I.zc[zc_index(zblk, p0, a, k)] = zk;
     na[k] = (k < n && __builtin_isnan(v)) ? 1 : 0;
     if (z32) z32[k] = (float)zk;
   };
@@ -203,15 +223,17 @@ __global__ __launch_bounds__(64 * moments_waves<J>()) void moments_kernel(Layout
       const int k = lane + 64 * j;
       if (k < npad) put(k, xv[j]);
     }
-    if (zr) {
+    if (zl) {
       // every lane's pieces are in LDS (one wave: its LDS operations are
-      // ordered); 1 KB of the row per store
+      // ordered); piece q = (g, t, 4 h + kk) of 16 bytes goes to its row's
+      // place in the block
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      double2 *dst = reinterpret_cast<double2 *>(I.zc + (size_t)a * 64 * m8);
-      const double2 *src = reinterpret_cast<const double2 *>(zr);
-      for (int i = lane; i < 32 * m8; i += 64) dst[i] = src[i];
+      double *dst = I.zc + (size_t)(a >> 4) * zblk + 2 * (a & 15);
+      const double2 *src = reinterpret_cast<const double2 *>(zl);
+      for (int q = lane; q < 32 * ms; q += 64)
+        *reinterpret_cast<double2 *>(dst + (size_t)16384 * (q >> 9) + 256 * ((q >> 3) & 63) + 32 * (q & 7)) = src[q];
     }
   } else {
     for (int k = lane; k < npad; k += 64) put(k, k < n ? xa[k] : 0.0);
@@ -461,7 +483,10 @@ __global__ __launch_bounds__(256) void corr_tile_kernel(Layout L, const long lon
 // B[l / 16][l % 16]; it receives D[4 r + l / 16][l % 16] in register r.
 // ---------------------------------------------------------------------------
 constexpr int XS = 32;  // pairs per block side
-constexpr int XD = 4;   // steps (8 samples of a class each) loaded ahead
+#ifndef HPC_XMFMA_XD
+#define HPC_XMFMA_XD 2
+#endif
+constexpr int XD = HPC_XMFMA_XD;  // steps (16 samples of a class each) loaded ahead
 
 typedef double d4v __attribute__((ext_vector_type(4)));
 
@@ -522,28 +547,34 @@ __global__ __launch_bounds__(256, 2) void corr_xmfma_kernel(Layout L, const long
   xblock(nb, Lg - prefix[k], BA, BB);
   const int a0 = BA * XS, b0 = BB * XS;
 
-  const int m8 = zc_m8(I.npad), G = m8 >> 3;
-  const size_t pitch = 64 * (size_t)m8;
-  const int li = lane & 15, kk = lane >> 4;
-  const double *r0 = I.zc + (size_t)min(a0 + li, p - 1) * pitch + 2 * kk;
-  const double *r1 = I.zc + (size_t)min(a0 + 16 + li, p - 1) * pitch + 2 * kk;
-  const double *r2 = I.zc + (size_t)min(b0 + li, p - 1) * pitch + 2 * kk;
-  const double *r3 = I.zc + (size_t)min(b0 + 16 + li, p - 1) * pitch + 2 * kk;
+  const int G = zc_slots(I.npad) >> 4;
+  // the four 16-row blocks (rows past p are the zero pad rows)
+  const size_t zblk = zc_block(I.npad);
+  const double *r0 = I.zc + (size_t)(a0 >> 4) * zblk + 2 * lane;
+  const double *r1 = r0 + zblk;
+  const double *r2 = I.zc + (size_t)(b0 >> 4) * zblk + 2 * lane;
+  const double *r3 = r2 + zblk;
   // step s: leaf ci = s / G of the subtree (depth first), group s % G of its
-  // class
+  // class: 16 samples, four MFMA k-steps per tile
   const int nsteps = 16 * G;
   auto offset = [&](int st) -> size_t {
     const int ci = st / G, gg = st - ci * G;
     const int u = ((ci & 1) << 3) | ((ci & 2) << 1) | ((ci >> 1) & 2) | ((ci >> 3) & 1);
-    return (size_t)512 * gg + 8 * (c + 4 * u);
+    return (size_t)16384 * gg + 256 * (c + 4 * u);
   };
-  double2 buf[XD][4];
-  auto load = [&](double2 (&b)[4], int st) {
+  // slot: half h = 0 of the four row blocks (k-steps 0, 1), then h = 1
+  // (k-steps 2, 3); each load is 1 KB of consecutive memory
+  double2 buf[XD][8];
+  auto load = [&](double2 (&b)[8], int st) {
     const size_t o = offset(min(st, nsteps - 1));
     b[0] = *reinterpret_cast<const double2 *>(r0 + o);
     b[1] = *reinterpret_cast<const double2 *>(r1 + o);
     b[2] = *reinterpret_cast<const double2 *>(r2 + o);
     b[3] = *reinterpret_cast<const double2 *>(r3 + o);
+    b[4] = *reinterpret_cast<const double2 *>(r0 + o + 128);
+    b[5] = *reinterpret_cast<const double2 *>(r1 + o + 128);
+    b[6] = *reinterpret_cast<const double2 *>(r2 + o + 128);
+    b[7] = *reinterpret_cast<const double2 *>(r3 + o + 128);
   };
   // slots loaded in order (the barriers keep the scheduler from reordering
   // them), so the loop's wait for slot d is for its own loads only
@@ -556,62 +587,76 @@ __global__ __launch_bounds__(256, 2) void corr_xmfma_kernel(Layout L, const long
   // acc[2 u + v]: tile (rows a0 + 16 u.., columns b0 + 16 v..)
   d4v acc[4] = {zero4, zero4, zero4, zero4};
   d4v st0[4] = {zero4, zero4, zero4, zero4}, st1[4] = {zero4, zero4, zero4, zero4};
-  for (int s = 0; s < nsteps; s += XD) {  // nsteps is a multiple of 16
+  // step s + d from slot d: the slot's MFMAs in k order, then its reload
+  // (ahead by XD steps); the loaded registers are never copied, so each wait
+  // is for one slot's loads
+  auto step = [&](const int d, const int s) __attribute__((always_inline)) {
 #pragma unroll
-    for (int d = 0; d < XD; ++d) {
-      // the slot's MFMAs, then its reload (ahead by XD steps): the loaded
-      // registers are never copied, so each wait is for one slot's loads
-      acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(buf[d][0].x, buf[d][2].x, acc[0], 0, 0, 0);
-      acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(buf[d][0].x, buf[d][3].x, acc[1], 0, 0, 0);
-      acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(buf[d][1].x, buf[d][2].x, acc[2], 0, 0, 0);
-      acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(buf[d][1].x, buf[d][3].x, acc[3], 0, 0, 0);
-      acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(buf[d][0].y, buf[d][2].y, acc[0], 0, 0, 0);
-      acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(buf[d][0].y, buf[d][3].y, acc[1], 0, 0, 0);
-      acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(buf[d][1].y, buf[d][2].y, acc[2], 0, 0, 0);
-      acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(buf[d][1].y, buf[d][3].y, acc[3], 0, 0, 0);
-      load(buf[d], s + d + XD);
-      const int st = s + d;
-      if ((st + 1) % G == 0) {
-        // leaf ci done: binary-counter merge, the earlier subtree (from the
-        // stack) the left operand, as in the tree
-        const int ci = st / G;
-        if (!(ci & 1)) {
+    for (int h = 0; h < 2; ++h) {
+      const double2 *b = buf[d] + 4 * h;
+      acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[0].x, b[2].x, acc[0], 0, 0, 0);
+      acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[0].x, b[3].x, acc[1], 0, 0, 0);
+      acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[1].x, b[2].x, acc[2], 0, 0, 0);
+      acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[1].x, b[3].x, acc[3], 0, 0, 0);
+      acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[0].y, b[2].y, acc[0], 0, 0, 0);
+      acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[0].y, b[3].y, acc[1], 0, 0, 0);
+      acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[1].y, b[2].y, acc[2], 0, 0, 0);
+      acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[1].y, b[3].y, acc[3], 0, 0, 0);
+    }
+    load(buf[d], s + d + XD);
+    const int st = s + d;
+    if ((st + 1) % G == 0) {
+      // leaf ci done: binary-counter merge, the earlier subtree (from the
+      // stack) the left operand, as in the tree
+      const int ci = st / G;
+      if (!(ci & 1)) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          st0[q] = acc[q];
+          acc[q] = zero4;
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] = st0[q] + acc[q];
+        if (!(ci & 2)) {
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
-            st0[q] = acc[q];
+            st1[q] = acc[q];
             acc[q] = zero4;
           }
         } else {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) acc[q] = st0[q] + acc[q];
-          if (!(ci & 2)) {
+          for (int q = 0; q < 4; ++q) acc[q] = st1[q] + acc[q];
+          int lvl = 2;
+          for (; lvl < 4 && ((ci >> lvl) & 1); ++lvl) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+              for (int r = 0; r < 4; ++r) acc[q][r] = stk[c][lvl - 2][q][r][lane] + acc[q][r];
+          }
+          if (lvl < 4) {  // else ci == 15: acc holds N(c, 4)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-              st1[q] = acc[q];
+#pragma unroll
+              for (int r = 0; r < 4; ++r) stk[c][lvl - 2][q][r][lane] = acc[q][r];
               acc[q] = zero4;
-            }
-          } else {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) acc[q] = st1[q] + acc[q];
-            int lvl = 2;
-            for (; lvl < 4 && ((ci >> lvl) & 1); ++lvl) {
-#pragma unroll
-              for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) acc[q][r] = stk[c][lvl - 2][q][r][lane] + acc[q][r];
-            }
-            if (lvl < 4) {  // else ci == 15: acc holds N(c, 4)
-#pragma unroll
-              for (int q = 0; q < 4; ++q) {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) stk[c][lvl - 2][q][r][lane] = acc[q][r];
-                acc[q] = zero4;
-              }
             }
           }
         }
       }
     }
+  };
+  // whole rounds of XD steps, then the last nsteps % XD (nsteps is a
+  // multiple of 16) outside the loop, so the loop's waits stay per slot
+  const int nmain = nsteps - nsteps % XD;
+  for (int s = 0; s < nmain; s += XD) {
+#pragma unroll
+    for (int d = 0; d < XD; ++d) step(d, s);
+  }
+  if (nmain < nsteps) {
+#pragma unroll
+    for (int d = 0; d < XD; ++d)
+      if (nmain + d < nsteps) step(d, nmain);
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q)

@@ -176,7 +176,7 @@ static size_t zbytes(const InstHost &h) {
   size_t b = 0;
   if (h.z) b += (size_t)h.p * h.npad * 8;
   if (h.z32) b += (size_t)h.p * h.npad * 4;
-  if (h.zc) b += (size_t)h.p * 64 * zc_m8(h.npad) * 8;
+  if (h.zc) b += zc_size(h.p, h.npad) * 8;
   return b;
 }
 
@@ -743,7 +743,12 @@ int engine_prepare_corr(Engine &e, int id) {
     if (!h.z) rc = ialloc(e, h.z, (size_t)h.p * h.npad);
     if (!rc && !h.z32) rc = ialloc(e, h.z32, (size_t)h.p * h.npad);
   } else if (!h.zc) {
-    rc = ialloc(e, h.zc, (size_t)h.p * 64 * zc_m8(h.npad));
+    rc = ialloc(e, h.zc, zc_size(h.p, h.npad));
+    // the rows past p (read, never stored, by the matrix-core kernel) are
+    // zeros: the last pair of row blocks, cleared once here
+    const size_t tail = 2 * zc_block(h.npad);
+    if (!rc && hipMemsetAsync(h.zc + zc_size(h.p, h.npad) - tail, 0, tail * 8, e.stream) != hipSuccess)
+      rc = hpc_fail(HPC_EHIP, "clearing the z-score pad rows failed");
   }
   h.bytes += zbytes(h) - before;
   return rc;
@@ -1040,7 +1045,7 @@ size_t engine_footprint(int p, int n, int max_level) {
              + (size_t)p * n * 8    // observations
              // z-scores (class-major FP64, or canonical FP64 + FP32 for
              // HPC_CORR_F32: this bounds both) and the NaN mask
-             + (size_t)p * 64 * zc_m8((int)npad) * 8 + (size_t)p * npad * 5
+             + std::max(zc_size(p, (int)npad) * 8, (size_t)p * npad * 12) + (size_t)p * npad
              + (size_t)p * 16;      // mean, sd
   b += pp * (4 + 4 + 8);            // worst-case share of nbr / erow / first-pass arrays
   b += (size_t)p * 40;              // row arrays

@@ -92,7 +92,18 @@ enum InstFlags {
 
 // Slots per lane class of the class-major z-scores: npad / 64 samples rounded
 // up to a multiple of 8.
-__host__ __device__ __forceinline__ int zc_m8(int npad) { return ((npad >> 6) + 7) & ~7; }
+// Class-major z-scores: slots per lane class, npad / 64 rounded up to a
+// whole group of 16.
+__host__ __device__ __forceinline__ int zc_slots(int npad) { return ((npad >> 6) + 15) & ~15; }
+// The class-major z-scores are stored in blocks of 16 rows (layout in
+// corr.hip): zc_block doubles per block, rows allocated in whole pairs of
+// blocks (the matrix-core kernel reads 32 rows at a time; the rows past p
+// are zeros).
+__host__ __device__ __forceinline__ int zc_rows(int p) { return (p + 31) & ~31; }
+__host__ __device__ __forceinline__ size_t zc_block(int npad) { return 1024 * (size_t)zc_slots(npad); }
+__host__ __device__ __forceinline__ size_t zc_size(int p, int npad) {
+  return (size_t)(zc_rows(p) / 16) * zc_block(npad);
+}
 
 // Device-side descriptor of one instance.
 struct Inst {
