@@ -671,6 +671,41 @@ __device__ __forceinline__ unsigned long long mul_div(unsigned long long a, unsi
   return (unsigned long long)(((unsigned __int128)a * b) / c);
 }
 
+// The pinv form of one test of the by-set search (S or the 2 x 2 Schur
+// block numerically singular), its operands read again from the staged
+// block: called rarely, and out of line, so that none of the per-set values
+// it needs stays live through the fast path.  M is the staged block of side
+// D with row i at Mi, s the members of S as positions, y the tested one.
+#ifndef HPC_LS_PAIR
+#define HPC_LS_PAIR 1
+#endif
+// two tests in flight per lane in the by-set search
+constexpr bool kLsPair = HPC_LS_PAIR != 0;
+// the highest level whose pinv form runs out of line
+constexpr int kLsOutOfLine = 6;
+
+template <int LV>
+struct Members {
+  int v[LV];
+};
+
+template <int LV>
+__device__ __noinline__ bool ls_pinv_removes(const double *M, int D, const double *Mi, Members<LV> s,
+                                             int y, double r_crit) {
+  double x[LV], yv[LV], C[LV][LV];
+#pragma unroll
+  for (int c = 0; c < LV; ++c) {
+    x[c] = Mi[s.v[c]];
+    yv[c] = M[y * D + s.v[c]];
+  }
+#pragma unroll
+  for (int c = 0; c < LV; ++c)
+#pragma unroll
+    for (int r = c + 1; r < LV; ++r) C[r][c] = M[s.v[c] * D + s.v[r]];
+  bool fb;
+  return removes(pcor_canon<LV>(Mi[y], x, yv, C, fb), r_crit);
+}
+
 template <int LV>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ls_kernel(
     Layout L, const long long *__restrict__ task_off, const long long *__restrict__ row_tasks,
@@ -776,7 +811,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ls_kernel(
             };
             while (c0 | c1) {
               const int ya = pop();
-              const bool has_b = (c0 | c1) != 0;
+              const bool has_b = kLsPair && (c0 | c1) != 0;
               const int yb = has_b ? pop() : ya;
               const double aa = Mi[ya], ab = Mi[yb];
               double va[LV], vb[LV];
@@ -801,9 +836,18 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ls_kernel(
                 }
                 bool rm;
                 if (h ? sb2 : sa) {
-                  bool fb;
-                  // the pinv form, as per test
-                  rm = removes(pcor_canon<LV>(a, x, h ? vb : va, C, fb), r_crit);
+                  // the pinv form, as per test: out of line at the levels
+                  // where that frees registers of the fast path (at higher
+                  // levels the call's argument passing costs more)
+                  if constexpr (LV <= kLsOutOfLine) {
+                    Members<LV> S;
+#pragma unroll
+                    for (int c = 0; c < LV; ++c) S.v[c] = q[c];
+                    rm = ls_pinv_removes<LV>(M, D, Mi, S, y, r_crit);
+                  } else {
+                    bool fb;
+                    rm = removes(pcor_canon<LV>(a, x, h ? vb : va, C, fb), r_crit);
+                  }
                   ++F;
                 } else {
                   rm = removes_sq(h ? s01b : s01a, h ? qb : qa, r_crit, r2_lo, r2_hi);
