@@ -208,6 +208,8 @@ void engine_free(Engine &e) {
   if (e.h_slots) (void)hipHostFree(e.h_slots);
   e.h_slots = nullptr;
   e.d_slots = nullptr;
+  if (e.d_slots_local) (void)hipFree(e.d_slots_local);
+  e.d_slots_local = nullptr;
   dfree(e.d_scatter_idx); dfree(e.d_scatter_val); dfree(e.d_rchk); dfree(e.d_qhead);
   if (e.h_qdone) (void)hipHostFree(e.h_qdone);
   e.h_qdone = e.d_qdone = nullptr;
@@ -665,6 +667,7 @@ static Publish next_publish(Engine &e) {
   pub.seq = ++e.seq;
   pub.words = (int)counter_words(e.packed.size());
   pub.advance = 0;
+  pub.sys = 1;
   return pub;
 }
 
@@ -1070,8 +1073,12 @@ size_t engine_footprint(int p, int n, int max_level) {
 // that level for engine_step to continue.
 // Publication slot of a level in a captured run: kind 0 = its compaction,
 // 1 = its removal (level 0: the fold kernel), device and host addresses.
+// A captured level writes its slot in device memory (no system-scope fence
+// per level); the capture's last publication copies all of them to the host
+// slots.
+constexpr size_t kSlotsBytes = (size_t)2 * (kMaxLevel + 2) * kSlotBytes;
 static unsigned long long *slot_dev(Engine &e, int level, int kind) {
-  return reinterpret_cast<unsigned long long *>(reinterpret_cast<char *>(e.d_slots) +
+  return reinterpret_cast<unsigned long long *>(reinterpret_cast<char *>(e.d_slots_local) +
                                                 (size_t)(2 * level + kind) * kSlotBytes);
 }
 static const Counters *slot_host(const Engine &e, int level, int kind) {
@@ -1117,6 +1124,7 @@ static int run_single_impl(Engine &e, int id, const RunShape *cap, RunShape *rec
   auto publish2 = [&](int lv) {
     Publish pb = next_publish(e);
     pb.dst = capmode ? slot_dev(e, lv, 1) : reinterpret_cast<unsigned long long *>(e.h_ctr2_dev);
+    if (capmode) pb.sys = 0;
     return pb;
   };
   // the level whose counts sit in h_ctr2 (or will, once the next wait ends)
@@ -1230,7 +1238,10 @@ static int run_single_impl(Engine &e, int id, const RunShape *cap, RunShape *rec
         }
         prev_commit_end = nullptr;
         Publish pa = next_publish(e);
-        if (capmode) pa.dst = slot_dev(e, l, 0);
+        if (capmode) {
+          pa.dst = slot_dev(e, l, 0);
+          pa.sys = 0;
+        }
         launch_compact_fused(e.stream, L, l, engine_rpt(e, l), e.d_blk, e.d_row_tasks,
                              e.d_task_off, e.cap_edges, pa);
         HPC_HIPCHK(hipGetLastError());
@@ -1373,7 +1384,14 @@ static int run_single_impl(Engine &e, int id, const RunShape *cap, RunShape *rec
     if (h.max_level >= 0 && h.next_level > h.max_level) h.finished = true;
     return HPC_OK;
   };
-  const int rc_run = levels();
+  int rc_run = levels();
+  if (capmode && rc_run == HPC_OK) {
+    // every level's slot to host memory, then the run's last flag
+    const Publish fin = next_publish(e);
+    launch_publish(e.stream, e.d_slots_local, kSlotsBytes / 8,
+                   reinterpret_cast<unsigned long long *>(e.d_slots), fin.flag, fin.seq);
+    if (hipGetLastError() != hipSuccess) rc_run = hpc_fail(HPC_EHIP, "slot publication launch failed");
+  }
   if (!e.kernel_timing) timed.clear();
   for (const auto &t : timed) {
     hpc_level_stats &st = h.hist[t.first];
@@ -1413,8 +1431,9 @@ void graph_drop(Engine &e) {
 static int graph_capture(Engine &e, int id, const RunShape &shape) {
   graph_drop(e);
   InstHost &h = e.inst[id];
+  if (!e.d_slots_local) HPC_HIPCHK(hipMalloc((void **)&e.d_slots_local, kSlotsBytes));
   if (!e.h_slots) {
-    HPC_HIPCHK(hipHostMalloc((void **)&e.h_slots, (size_t)2 * (kMaxLevel + 2) * kSlotBytes,
+    HPC_HIPCHK(hipHostMalloc((void **)&e.h_slots, kSlotsBytes,
                              hipHostMallocMapped | hipHostMallocCoherent));
     HPC_HIPCHK(hipHostGetDevicePointer((void **)&e.d_slots, e.h_slots, 0));
   }

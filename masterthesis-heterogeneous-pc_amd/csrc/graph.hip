@@ -182,8 +182,12 @@ __global__ __launch_bounds__(64) void level0_fold_kernel(Layout L, Publish pub) 
   if (!pub.flag || lane != 0) return;
   unsigned long long *blk = reinterpret_cast<unsigned long long *>(L.ctr);
   for (int w = 0; w < pub.words; ++w) pub.dst[w] = blk[w];
-  __threadfence_system();
-  __hip_atomic_store(pub.flag, pub.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  if (pub.sys) {
+    __threadfence_system();
+    __hip_atomic_store(pub.flag, pub.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  } else {
+    __threadfence();
+  }
   if (pub.advance) {
     for (int w = 0; w < pub.words; ++w) blk[w] = 0ull;
     const_cast<Inst *>(L.inst)[0].level = 1;
@@ -461,6 +465,7 @@ __global__ __launch_bounds__(64 * kCompWaves) void compact_count_kernel(
   if (!pub.flag) return;
   const unsigned long long *src = reinterpret_cast<const unsigned long long *>(L.ctr);
   for (int w = tid; w < pub.words; w += nt) pub.dst[w] = ld_agent(&src[w]);
+  if (!pub.sys) return;  // device-memory slot: the kernel's end publishes it
   __threadfence_system();
   __syncthreads();
   if (tid == 0)

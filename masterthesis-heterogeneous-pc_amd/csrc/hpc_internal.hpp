@@ -206,6 +206,10 @@ struct Publish {
   // removal kernel of a one-instance run: after publishing, clear the counter
   // block and advance the instance's level and record base on the device
   int advance;
+  // 1: dst is host memory, written behind a system-scope fence, then the
+  // flag is set; 0: dst is device memory and no flag is set (the levels of a
+  // captured loop, whose last publication copies every level's words out)
+  int sys;
 };
 // Level 0 of the packed instances at level 0.  With `pub` (one instance),
 // the kernel that folds the counts also publishes them, clears the counter
@@ -455,6 +459,7 @@ struct Engine {
   bool use_graph = true;
   GraphRun graph;
   Counters *h_slots = nullptr, *d_slots = nullptr;
+  unsigned long long *d_slots_local = nullptr;  // the captured levels' slots in device memory
   int64_t graph_mismatches = 0;  // replays whose sizes left the recorded run's
   int64_t graph_replays = 0, graph_captures = 0;
   // recheck queues of fast instances

@@ -1289,10 +1289,15 @@ __global__ __launch_bounds__(64 * kCommitWaves) void commit_kernel(Layout L, lon
       if (nc.on) v = with_next(w, v, nte, ntt, nmax);
       pub.dst[w] = v;
     }
-    __threadfence_system();
-    __syncthreads();
-    if (threadIdx.x == 0)
-      __hip_atomic_store(pub.flag, pub.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (pub.sys) {
+      __threadfence_system();
+      __syncthreads();
+      if (threadIdx.x == 0)
+        __hip_atomic_store(pub.flag, pub.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    } else {
+      // (the words reach device memory before the kernel's end)
+      __syncthreads();
+    }
     if (pub.advance && L.K == 1) {
       // the next level of the (single) instance starts from clear counters
       unsigned long long *blk = reinterpret_cast<unsigned long long *>(L.ctr);
