@@ -1445,7 +1445,10 @@ static int graph_capture(Engine &e, int id, const RunShape &shape) {
   // a fresh run from level 0, as the recorded one
   h.next_level = 0;
   h.finished = false;
-  h.cor_nan = -1;
+  // the recorded NaN state from the start (only the searches read the flag,
+  // and a replay whose level 0 finds otherwise is void), so the table the
+  // levels start from needs no rewrite before level 1
+  h.cor_nan = shape.cor_nan;
   h.rec_used = 0;
   h.undirected = (int64_t)h.p * (h.p - 1) / 2;
   h.max_deg = h.p;
