@@ -3,7 +3,10 @@
 A step is one full skeleton of the plain p=2000 workload on device-resident
 observations: correlation (moments + pairwise sums), then levels 0..4
 (compaction, CI tests, removal) -- the reference's default maximum level
-(main.cpp:31), at which this workload still holds 30-neighbour rows.
+(main.cpp:31), at which this workload still holds 30-neighbour rows.  The
+level loop of each timed step is a replay of the hipGraph captured from the
+first warm-up run (the graph's launch sizes are checked against what the
+device computed; a mismatch would re-run the levels eagerly).
 
 value = CI tests/s = sum over levels of the canonical test count T_l divided
 by the level wall time (the reference's "Execution duration" scope,
@@ -503,6 +506,7 @@ def main() -> int:
     steps = [run_step(dev) for _ in range(args.steps)]
     heterogpc.load().hpc_device_synchronize(0)
     wall = time.perf_counter() - t0
+    graph = dev.graph_stats()
 
     tests_total = sum(lv["tests"] for lv in steps[-1]["levels"])
     levels_ms = sum(s["levels_ms"] for s in steps) / len(steps)
@@ -621,6 +625,13 @@ def main() -> int:
                     "without per-level HIP events; kernel_ms, levels[].ms_* and roofline from "
                     "a second pass of the same steps with the events on",
             "skeleton_ms_levels_with_events": levels_ms_events},
+        "level_loop": {
+            "graph": graph,
+            "note": "each timed step resets the state and runs correlation then levels 0..4; the "
+                    "first warm-up run records the level loop's launch sizes and captures it "
+                    "into a hipGraph, later runs replay it (one launch, one host wait) and check "
+                    "every level's sizes against the recorded ones (a mismatch re-runs eagerly); "
+                    "the kernel-timing pass runs the loop eagerly"},
         "tests_total": tests_total,
         "levels": [{k: lv[k] for k in ("level", "max_deg", "tests", "tests_max", "edges",
                                        "removed", "edges_left", "ms_compact", "ms_test",
