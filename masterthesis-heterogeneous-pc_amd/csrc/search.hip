@@ -303,13 +303,31 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_lds_kernel(
 // Task t of a level's plan: row g with task_off[g] <= t < task_off[g] +
 // row_tasks[g] (the last row whose exclusive offset is <= t), then ordered
 // edge rowptr[g] + local / nch and rank chunk local % nch, where
-// nch = row_tasks[g] / deg.
-__device__ __forceinline__ int task_row(const long long *__restrict__ task_off, int rows,
-                                        long long t) {
-  int lo = 0, hi = rows - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (task_off[mid] <= t) lo = mid; else hi = mid - 1;
+// nch = row_tasks[g] / deg.  Found by a whole wavefront (every lane takes
+// part and gets the same row), given task_off[lo] <= t: with lo > 0 (the row
+// of the block's previous segment) a window of the 64 rows from lo first,
+// which holds the next row unless 63 or more rows in between have no tasks;
+// then 64-ary rounds, one dependent load each (two for a few thousand rows,
+// where a binary search takes a dozen).
+__device__ __forceinline__ int task_row_wave(const long long *__restrict__ task_off, int rows,
+                                             int lo, long long t) {
+  const int lane = threadIdx.x & 63;
+  if (lo > 0) {
+    const int c = lo + lane;
+    const bool ok = c < rows && task_off[c] <= t;
+    const unsigned long long m = __ballot(ok) | 1ull;
+    const int last = 63 - __clzll(m);
+    if (last < 63 || lo + 64 >= rows) return lo + last;
+    lo += 63;
+  }
+  int n = rows - lo;  // the row is in [lo, lo + n)
+  while (n > 1) {
+    const int step = (n + 63) >> 6;
+    const bool ok = lane * step < n && task_off[lo + lane * step] <= t;
+    const unsigned long long m = __ballot(ok) | 1ull;
+    const int nlo = lo + (63 - __clzll(m)) * step;
+    n = min(step, lo + n - nlo);
+    lo = nlo;
   }
   return lo;
 }
@@ -366,9 +384,11 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_kernel(
   for (long long blk = blockIdx.x; blk * tpb < n_tasks; blk += gridDim.x) {
     long long t0 = blk * tpb;
     const long long t1 = min(n_tasks, t0 + tpb);
+    int prev = 0;  // the row of the block's previous segment
     while (t0 < t1) {
       // the segment of this block's tasks on row i (uniform across the block)
-      const int i = task_row(task_off, L.p_total, t0);
+      const int i = task_row_wave(task_off, L.p_total, prev, t0);
+      prev = i;
       const long long off = task_off[i];
       const long long seg_end = min(t1, off + row_tasks[i]);
       if (seg_end <= t0) break;  // (a consistent plan always advances)
@@ -513,9 +533,11 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ln_lds_kernel(
     }
     long long t0 = blk * tpb;
     const long long t1 = min(n_tasks, t0 + tpb);
+    int prev = 0;  // the row of the block's previous segment
     while (t0 < t1) {
       // the segment of this block's tasks on row i (uniform across the block)
-      const int i = task_row(task_off, L.p_total, t0);
+      const int i = task_row_wave(task_off, L.p_total, prev, t0);
+      prev = i;
       const long long off = task_off[i];
       const long long seg_end = min(t1, off + row_tasks[i]);
       if (seg_end <= t0) break;  // (a consistent plan always advances)
@@ -731,8 +753,10 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ls_kernel(
        blk = Q.head ? claim_block(Q, n_blocks, s_blk) : blk + gridDim.x) {
     long long t0 = blk * tpb;
     const long long t1 = min(n_tasks, t0 + tpb);
+    int prev = 0;  // the row of the block's previous segment
     while (t0 < t1) {
-      const int i = task_row(task_off, L.p_total, t0);
+      const int i = task_row_wave(task_off, L.p_total, prev, t0);
+      prev = i;
       const long long off = task_off[i];
       const long long U = row_tasks[i];
       const long long seg_end = min(t1, off + U);
