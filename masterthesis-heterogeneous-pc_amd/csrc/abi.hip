@@ -3,6 +3,8 @@
 // synthetic SEM generator and result export live here too.
 #pragma clang fp contract(off)
 
+#include <atomic>
+#include <thread>
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -486,16 +488,153 @@ int export_cor(Engine &e, int id, double *cor) {
 
 int fetch_adj_bits(Engine &e, int id, std::vector<unsigned long long> &bits) {
   InstHost &h = e.inst[id];
+  if (h.res_bits_valid) {
+    bits = h.res_bits;
+    return HPC_OK;
+  }
   HIPCHK(hipSetDevice(e.device));
   bits.resize((size_t)h.p * h.W);
   HIPCHK(hipMemcpyAsync(bits.data(), h.adj, bits.size() * sizeof(bits[0]), hipMemcpyDeviceToHost,
                         e.stream));
   HIPCHK(hipStreamSynchronize(e.stream));
+  if (h.finished) {  // final until the instance is packed or reset again
+    h.res_bits = bits;
+    h.res_bits_valid = true;
+  }
+  return HPC_OK;
+}
+
+// Records are appended in scheduling order: sorted by (i, j) here, by a
+// counting sort on i (each pair has at most one record) and a sort on j
+// within each i (insertion sort for short buckets).
+static int sort_records(const InstHost &h, const int32_t *raw, size_t words,
+                        std::vector<int32_t> &rec) {
+  const int p = h.p;
+  std::vector<uint32_t> first(p + 1, 0);
+  size_t n = 0;
+  for (size_t a = 0; a + 3 <= words; ++n) {
+    const int i = raw[a], j = raw[a + 1], l = raw[a + 2];
+    if (l < 1 || a + 3 + (size_t)l > words || i < 0 || i >= p || j < 0 || j >= p)
+      return fail(HPC_EHIP, "corrupt sepset record at word %zu", a);
+    ++first[i + 1];
+    a += 3 + (size_t)l;
+  }
+  for (int i = 0; i < p; ++i) first[i + 1] += first[i];
+  std::vector<size_t> at(n);  // record start words, bucketed by i
+  {
+    std::vector<uint32_t> fill(first.begin(), first.end() - 1);
+    for (size_t a = 0; a + 3 <= words; a += 3 + (size_t)raw[a + 2]) at[fill[raw[a]]++] = a;
+  }
+  for (int i = 0; i < p; ++i) {
+    if (first[i + 1] - first[i] > 32) {  // a long bucket: O(k log k)
+      std::sort(at.begin() + first[i], at.begin() + first[i + 1],
+                [&](size_t u, size_t v) { return raw[u + 1] < raw[v + 1]; });
+      continue;
+    }
+    for (uint32_t x = first[i] + 1; x < first[i + 1]; ++x) {
+      const size_t v = at[x];
+      uint32_t y = x;
+      while (y > first[i] && raw[at[y - 1] + 1] > raw[v + 1]) {
+        at[y] = at[y - 1];
+        --y;
+      }
+      at[y] = v;
+    }
+  }
+  rec.clear();
+  rec.reserve(words);
+  for (size_t a : at) rec.insert(rec.end(), raw + a, raw + a + 3 + raw[a + 2]);
+  return HPC_OK;
+}
+
+int engine_prefetch_results(Engine &e, const std::vector<int> &ids) {
+  HIPCHK(hipSetDevice(e.device));
+  struct Item {
+    int id;
+    size_t bits_at, rec_at;
+  };
+  std::vector<Item> items;
+  std::vector<CopyOut> copies;
+  size_t bytes = 0;
+  for (int id : ids) {
+    if (id < 0 || id >= (int)e.inst.size()) continue;
+    const InstHost &h = e.inst[id];
+    if (!h.used || !h.finished || h.error || !h.adj) continue;
+    if (h.res_bits_valid && h.res_rec_valid) continue;
+    Item it{id, bytes, 0};
+    copies.push_back(CopyOut{h.adj, (unsigned long long)h.p * h.W * 8, (unsigned long long)bytes});
+    bytes += (size_t)h.p * h.W * 8;
+    it.rec_at = bytes;
+    if (h.rec_used) copies.push_back(CopyOut{h.rec, (unsigned long long)h.rec_used * 4, (unsigned long long)bytes});
+    bytes += (h.rec_used * 4 + 7) & ~(size_t)7;
+    items.push_back(it);
+  }
+  if (items.empty()) return HPC_OK;
+  if (bytes > e.d_stage_cap) {
+    if (e.d_stage) HIPCHK(hipFree(e.d_stage));
+    e.d_stage = nullptr;
+    e.d_stage_cap = 0;
+    const size_t cap = bytes + bytes / 4;
+    HIPCHK(hipMalloc((void **)&e.d_stage, cap));
+    e.d_stage_cap = cap;
+  }
+  // (ordinary host memory: pinning a fresh buffer per batch costs more than
+  // the one copy saves)
+  e.h_stage.resize(bytes);
+  if (copies.size() > e.cap_copies) {
+    if (e.d_copies) HIPCHK(hipFree(e.d_copies));
+    e.d_copies = nullptr;
+    e.cap_copies = 0;
+    const size_t cap = copies.size() + copies.size() / 4 + 16;
+    HIPCHK(hipMalloc((void **)&e.d_copies, cap * sizeof(CopyOut)));
+    e.cap_copies = cap;
+  }
+  // one kernel gathers every block into one device buffer, one copy brings
+  // it to host memory: one upload of the table, one launch, one copy, one
+  // wait (a device-to-host copy per block measured about 80 us each)
+  HIPCHK(hipMemcpyAsync(e.d_copies, copies.data(), copies.size() * sizeof(CopyOut),
+                        hipMemcpyHostToDevice, e.stream));
+  launch_copy_out(e.stream, e.d_copies, (int)copies.size(), e.d_stage);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(e.h_stage.data(), e.d_stage, bytes, hipMemcpyDeviceToHost, e.stream));
+  HIPCHK(hipStreamSynchronize(e.stream));
+  const char *stage = e.h_stage.data();
+  // the instances' unpacking (the record sort is most of it) on a few host
+  // threads; each touches only its own instances
+  std::atomic<size_t> next{0};
+  std::atomic<int> bad{HPC_OK};
+  auto unpack = [&]() {
+    for (size_t k; (k = next.fetch_add(1)) < items.size();) {
+      const Item &it = items[k];
+      InstHost &h = e.inst[it.id];
+      const unsigned long long *bw = reinterpret_cast<const unsigned long long *>(stage + it.bits_at);
+      h.res_bits.assign(bw, bw + (size_t)h.p * h.W);
+      h.res_bits_valid = true;
+      std::vector<int32_t> rec;
+      if (sort_records(h, reinterpret_cast<const int32_t *>(stage + it.rec_at), h.rec_used, rec)) {
+        bad = HPC_EHIP;
+        continue;
+      }
+      h.res_rec.swap(rec);
+      h.res_rec_valid = true;
+    }
+  };
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t nthr = std::min<size_t>(std::min<size_t>(8, hw), (items.size() + 15) / 16);
+  std::vector<std::thread> pool;
+  for (size_t t = 1; t < nthr; ++t) pool.emplace_back(unpack);
+  unpack();
+  for (auto &t : pool) t.join();
+  if (bad != HPC_OK) return fail(HPC_EHIP, "corrupt sepset records in a batch result");
   return HPC_OK;
 }
 
 int fetch_records(Engine &e, int id, std::vector<int32_t> &rec) {
   InstHost &h = e.inst[id];
+  if (h.res_rec_valid) {
+    rec = h.res_rec;
+    return HPC_OK;
+  }
   HIPCHK(hipSetDevice(e.device));
   std::vector<int32_t> raw(h.rec_used);
   if (h.rec_used) {
@@ -503,24 +642,15 @@ int fetch_records(Engine &e, int id, std::vector<int32_t> &rec) {
                           e.stream));
     HIPCHK(hipStreamSynchronize(e.stream));
   }
-  // records are appended in scheduling order: sort by (i, j)
-  std::vector<std::pair<int64_t, size_t>> order;
-  for (size_t a = 0; a + 3 <= raw.size();) {
-    const int l = raw[a + 2];
-    if (l < 1 || a + 3 + (size_t)l > raw.size())
-      return fail(HPC_EHIP, "corrupt sepset record at word %zu", a);
-    order.push_back({(int64_t)raw[a] * h.p + raw[a + 1], a});
-    a += 3 + (size_t)l;
-  }
-  std::sort(order.begin(), order.end());
-  rec.clear();
-  rec.reserve(raw.size());
-  for (auto &o : order) {
-    const size_t a = o.second;
-    rec.insert(rec.end(), raw.begin() + a, raw.begin() + a + 3 + raw[a + 2]);
+  const int rc = sort_records(h, raw.data(), raw.size(), rec);
+  if (rc) return rc;
+  if (h.finished) {
+    h.res_rec = rec;
+    h.res_rec_valid = true;
   }
   return HPC_OK;
 }
+
 
 void dense_adj(const std::vector<unsigned long long> &bits, int p, int32_t *adj) {
   const int W = (p + 63) / 64;
@@ -549,6 +679,11 @@ void dense_sepsets(const std::vector<unsigned long long> &bits, const std::vecto
 }
 
 int export_adj(Engine &e, int id, int32_t *adj) {
+  const InstHost &h = e.inst[id];
+  if (h.res_bits_valid) {
+    dense_adj(h.res_bits, h.p, adj);
+    return HPC_OK;
+  }
   std::vector<unsigned long long> bits;
   int rc = fetch_adj_bits(e, id, bits);
   if (rc) return rc;
@@ -567,6 +702,20 @@ int export_sepsets(Engine &e, int id, int32_t *sep, int mc) {
 }
 
 int export_sepset_list(Engine &e, int id, int32_t *out, int64_t cap, int64_t *words) {
+  if (!out) {
+    // the size query: the sorted list has the records' word count
+    if (words) *words = (int64_t)e.inst[id].rec_used;
+    return HPC_OK;
+  }
+  const InstHost &h = e.inst[id];
+  if (h.res_rec_valid) {
+    if (words) *words = (int64_t)h.res_rec.size();
+    if ((int64_t)h.res_rec.size() > cap)
+      return fail(HPC_EINVAL, "sepset list needs %zu words, cap is %lld", h.res_rec.size(),
+                  (long long)cap);
+    std::copy(h.res_rec.begin(), h.res_rec.end(), out);
+    return HPC_OK;
+  }
   std::vector<int32_t> rec;
   int rc = fetch_records(e, id, rec);
   if (rc) return rc;

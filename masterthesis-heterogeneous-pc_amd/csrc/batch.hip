@@ -141,7 +141,12 @@ extern "C" int hpc_batch_run(hpc_batch *b, int *steps) {
     ++n;
   } while (left > 0);
   if (steps) *steps = n;
-  return HPC_OK;
+  // every instance's results to host memory in one round (the exports then
+  // read them without a device wait each)
+  std::vector<int> all;
+  for (size_t k = 0; k < b->eng.inst.size(); ++k)
+    if (b->eng.inst[k].used) all.push_back((int)k);
+  return engine_prefetch_results(b->eng, all);
 }
 
 extern "C" int hpc_batch_status(hpc_batch *b, int id, int *finished, int *levels, int *error) {
@@ -516,6 +521,12 @@ static void server_loop(hpc_server *s) {
       (rc || h.finished ? done : keep).push_back(r);
     }
     std::vector<int> codes;
+    if (!rc && done.size() > 1) {
+      // the finished requests' results in one round of copies and one wait
+      std::vector<int> slots;
+      for (auto &r : done) slots.push_back(r->slot);
+      (void)engine_prefetch_results(s->eng, slots);  // (a failure leaves the per-request fetches)
+    }
     for (auto &r : done) codes.push_back(server_collect(s, *r));
     const auto now = Clock::now();
     std::lock_guard<std::mutex> lk(s->mu);

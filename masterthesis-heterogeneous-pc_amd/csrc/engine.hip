@@ -210,6 +210,13 @@ void engine_free(Engine &e) {
   e.d_slots = nullptr;
   if (e.d_slots_local) (void)hipFree(e.d_slots_local);
   e.d_slots_local = nullptr;
+  if (e.d_stage) (void)hipFree(e.d_stage);
+  e.d_stage = nullptr;
+  e.d_stage_cap = 0;
+  e.h_stage = std::vector<char>();
+  if (e.d_copies) (void)hipFree(e.d_copies);
+  e.d_copies = nullptr;
+  e.cap_copies = 0;
   dfree(e.d_scatter_idx); dfree(e.d_scatter_val); dfree(e.d_rchk); dfree(e.d_qhead);
   if (e.h_qdone) (void)hipHostFree(e.h_qdone);
   e.h_qdone = e.d_qdone = nullptr;
@@ -313,8 +320,15 @@ void engine_remove(Engine &e, int id) {
   e.inst[id] = InstHost();
 }
 
+void engine_drop_results(InstHost &h) {
+  h.res_bits_valid = h.res_rec_valid = false;
+  h.res_bits.clear();
+  h.res_rec.clear();
+}
+
 int engine_reset(Engine &e, int id) {
   InstHost &h = e.inst[id];
+  engine_drop_results(h);
   HPC_HIPCHK(hipSetDevice(e.device));
   launch_adj_init(e.stream, h.adj, h.p, h.W);
   HPC_HIPCHK(hipGetLastError());
@@ -564,6 +578,7 @@ int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &
   int start = 0;
   for (int k = 0; k < K; ++k) {
     InstHost &h = e.inst[ids[k]];
+    engine_drop_results(h);  // a packed instance's device state may change
     Inst &I = e.h_inst[k];
     I.x = h.x; I.z = h.z; I.nanm = h.nanm; I.mean = h.mean; I.sd = h.sd; I.cor = h.cor;
     I.adj = h.adj; I.rec = h.rec; I.rec_base = h.rec_used; I.rec_cap = h.rec_cap;
@@ -778,7 +793,10 @@ int engine_exact_from_z(Engine &e, const std::vector<int> &ids) {
 
 int engine_level0(Engine &e, float *ms) {
   HPC_HIPCHK(hipSetDevice(e.device));
-  for (int id : e.packed) e.inst[id].fresh = false;
+  for (int id : e.packed) {
+    e.inst[id].fresh = false;
+    engine_drop_results(e.inst[id]);
+  }
   int rc = zero_counters(e);
   if (rc) return rc;
   Layout L = layout(e);
@@ -989,7 +1007,10 @@ int engine_search_shared_finish(Engine &e, const std::vector<int64_t> &idx,
 
 int engine_commit(Engine &e, float *ms) {
   HPC_HIPCHK(hipSetDevice(e.device));
-  for (int id : e.packed) e.inst[id].fresh = false;
+  for (int id : e.packed) {
+    e.inst[id].fresh = false;
+    engine_drop_results(e.inst[id]);
+  }
   // room for the level's sepset records; the table is rewritten, so wait for
   // earlier uploads of it
   if (e.host_upload_pending) {
@@ -1544,6 +1565,7 @@ static int graph_replay(Engine &e, int id) {
 
 int engine_run_single(Engine &e, int id) {
   InstHost &h = e.inst[id];
+  engine_drop_results(h);
   // graph replay: a fresh run of an exact, capped instance without per-level
   // kernel timing, with the launch shapes and record buffer of the capture
   const bool graphable = e.use_graph && !e.kernel_timing && !h.fast && h.fresh &&

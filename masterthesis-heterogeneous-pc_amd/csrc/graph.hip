@@ -541,7 +541,28 @@ __global__ __launch_bounds__(256) void publish_kernel(const unsigned long long *
   if (threadIdx.x == 0) __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+__global__ __launch_bounds__(256) void copy_out_kernel(const CopyOut *__restrict__ entries,
+                                                      char *__restrict__ dst) {
+  const CopyOut c = entries[blockIdx.x];
+  const unsigned long long w8 = c.bytes / 8;
+  const unsigned long long *s8 = static_cast<const unsigned long long *>(c.src);
+  unsigned long long *d8 = reinterpret_cast<unsigned long long *>(dst + c.dst);
+  if ((reinterpret_cast<uintptr_t>(c.src) & 7) == 0) {
+    for (unsigned long long w = threadIdx.x; w < w8; w += blockDim.x) d8[w] = s8[w];
+    if (threadIdx.x == 0 && (c.bytes & 4))
+      reinterpret_cast<uint32_t *>(dst + c.dst)[2 * w8] = static_cast<const uint32_t *>(c.src)[2 * w8];
+  } else {
+    const uint32_t *s4 = static_cast<const uint32_t *>(c.src);
+    uint32_t *d4 = reinterpret_cast<uint32_t *>(dst + c.dst);
+    for (unsigned long long w = threadIdx.x; w < c.bytes / 4; w += blockDim.x) d4[w] = s4[w];
+  }
+}
+
 }  // namespace
+
+void launch_copy_out(hipStream_t s, const CopyOut *entries, int count, char *dst) {
+  if (count > 0) hipLaunchKernelGGL(copy_out_kernel, dim3((unsigned)count), dim3(256), 0, s, entries, dst);
+}
 
 void launch_publish(hipStream_t s, const unsigned long long *src, size_t words,
                     unsigned long long *dst, unsigned long long *flag, unsigned long long seq) {

@@ -192,6 +192,14 @@ void launch_table(hipStream_t s, Inst *dst, const TableArg &arg);
 // Copies `words` 8-byte words from device memory to fine-grained host memory
 // (device-visible pointers), then stores `seq` to *flag: the host sees the
 // words once it sees the flag.
+// One copy of a gather of device blocks into one (host-mapped) buffer.
+struct CopyOut {
+  const void *src;
+  unsigned long long bytes;  // a multiple of 4
+  unsigned long long dst;    // byte offset, a multiple of 8
+};
+// Copies every entry's bytes to dst + entry.dst, one workgroup per entry.
+void launch_copy_out(hipStream_t s, const CopyOut *entries, int count, char *dst);
 void launch_publish(hipStream_t s, const unsigned long long *src, size_t words,
                     unsigned long long *dst, unsigned long long *flag, unsigned long long seq);
 void launch_compact(hipStream_t s, const Layout &L);
@@ -324,6 +332,11 @@ struct InstHost {
   int inject_level = -1;   // test hook: the host side of this level fails
   bool fresh = false;      // adjacency complete, no level run since (replayable)
   std::vector<hpc_level_stats> hist;  // one record per level run
+  // host copies of the adjacency bits and the sorted sepset records, read by
+  // the exports; dropped whenever device work may change the instance
+  bool res_bits_valid = false, res_rec_valid = false;
+  std::vector<unsigned long long> res_bits;
+  std::vector<int32_t> res_rec;
 };
 
 // Device memory of instance blocks: slabs from hipMalloc, sub-allocated
@@ -460,6 +473,11 @@ struct Engine {
   GraphRun graph;
   Counters *h_slots = nullptr, *d_slots = nullptr;
   unsigned long long *d_slots_local = nullptr;  // the captured levels' slots in device memory
+  std::vector<char> h_stage;  // host staging of engine_prefetch_results
+  char *d_stage = nullptr;    // and its device-side gather buffer
+  size_t d_stage_cap = 0;
+  CopyOut *d_copies = nullptr;  // the copy table of engine_prefetch_results
+  size_t cap_copies = 0;
   int64_t graph_mismatches = 0;  // replays whose sizes left the recorded run's
   int64_t graph_replays = 0, graph_captures = 0;
   // recheck queues of fast instances
@@ -604,6 +622,10 @@ int export_sepset_list(Engine &e, int id, int32_t *out, int64_t cap, int64_t *wo
 // records sorted by (i, j).
 int fetch_adj_bits(Engine &e, int id, std::vector<unsigned long long> &bits);
 int fetch_records(Engine &e, int id, std::vector<int32_t> &rec);
+// Host copies of the results of every finished instance of `ids` that has
+// none yet: all device-to-host copies in flight together, one wait.
+int engine_prefetch_results(Engine &e, const std::vector<int> &ids);
+void engine_drop_results(InstHost &h);
 // Reference layouts from those host copies (sepSets: -1 empty, -2 removed
 // at level 0, cpuUtil.cpp:3-48).
 void dense_adj(const std::vector<unsigned long long> &bits, int p, int32_t *adj);
