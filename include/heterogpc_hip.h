@@ -155,7 +155,8 @@ int hpc_graph_stats(hpc_state *st, int64_t *replays, int64_t *captures, int64_t 
  * snapshot for every candidate launch shape -- level 1: ordered edges per
  * workgroup in {8..128}; levels >= 2: conditioning sets per task in
  * {256..4096} x tasks per workgroup in {4..128} x task order (device queue
- * or workgroup id) -- and keeps the fastest per
+ * or workgroup id), and with each conditioning-set size one contiguous
+ * equal share of the tasks per resident workgroup -- and keeps the fastest per
  * level for this state's later runs.  Results do not depend on the shape.
  * The state is left reset (adjacency and sepsets back to the start). */
 typedef struct hpc_tune_entry {
@@ -164,8 +165,11 @@ typedef struct hpc_tune_entry {
   int32_t tasks_per_block;   /* levels >= 2 */
   int32_t edges_per_block;   /* level 1 */
   int32_t queue;             /* levels >= 2: 1 = blocks claimed from the device
-                                task queue, 2 = blocks by workgroup id, 0 = the
-                                state's HPC_TUNE_QUEUE */
+                                task queue, 2 = blocks by workgroup id, 3 = one
+                                contiguous share of the tasks per resident
+                                workgroup (exact instances; the others take
+                                blocks by workgroup id), 0 = the state's
+                                HPC_TUNE_QUEUE */
   int32_t pad_;
   double ms_best;            /* CI-test kernel time with the chosen shape */
   double ms_default;         /* with the default shape */

@@ -1609,9 +1609,11 @@ static std::vector<TuneCand> tune_candidates(int level) {
   if (level == 1) {
     for (int epb : {8, 16, 32, 64, 128}) cand.push_back({0, epb, 0});
   } else {
-    for (int rpt : {256, 512, 1024, 2048, 4096})
+    for (int rpt : {256, 512, 1024, 2048, 4096}) {
       for (int tpb : {4, 8, 16, 32, 64, 128})
         for (int q : {1, 2}) cand.push_back({rpt, tpb, q});
+      cand.push_back({rpt, 64, 3});  // one share per workgroup (tpb unused)
+    }
   }
   return cand;
 }
@@ -2012,7 +2014,7 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
 TaskQueue device_queue(const Engine &e, int level) {
   const int q = (level >= 0 && level <= kMaxLevel) ? e.queue_level[level] : 0;
   const bool on = q == 1 || (q == 0 && e.use_queue);
-  return TaskQueue{on ? e.d_qhead : nullptr, nullptr};
+  return TaskQueue{on ? e.d_qhead : nullptr, nullptr, q == 3 ? 1 : 0};
 }
 
 bool engine_plannable(const Engine &e, int max_deg, int level) {
@@ -2053,8 +2055,8 @@ int engine_check_tuning(const hpc_tune_entry *t, int count) {
       if (x.tasks_per_block < 1 || x.tasks_per_block > 1024)
         return hpc_fail(HPC_EINVAL, "tuning entry %d: tasks per workgroup %d outside 1..1024", k,
                         x.tasks_per_block);
-      if (x.queue < 0 || x.queue > 2)
-        return hpc_fail(HPC_EINVAL, "tuning entry %d: queue %d outside 0..2", k, x.queue);
+      if (x.queue < 0 || x.queue > 3)
+        return hpc_fail(HPC_EINVAL, "tuning entry %d: queue %d outside 0..3", k, x.queue);
     }
   }
   return HPC_OK;

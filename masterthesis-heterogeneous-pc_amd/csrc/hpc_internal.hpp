@@ -280,6 +280,9 @@ void launch_task_plan(hipStream_t s, int level, const Layout &L, int rpt,
 struct TaskQueue {
   unsigned long long *head;
   int *done;
+  // the by-set search: each workgroup of a grid of resident size takes one
+  // contiguous share of the level's tasks (no blocks, no claims)
+  int balanced = 0;
 };
 // level 1 here only through a shared queue (exact instances)
 void launch_search_ln(hipStream_t s, int level, const Layout &L, int rpt,

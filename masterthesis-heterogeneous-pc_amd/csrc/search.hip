@@ -745,14 +745,21 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_ls_kernel(
     bt[t] = L.binom[(size_t)m * kBinomK + k];
   }
   const long long n_tasks = (long long)L.ctr->n_tasks;
-  const long long n_blocks = (n_tasks + tpb - 1) / tpb;
+  // balanced: one block per workgroup, its equal contiguous share of tasks
+  const long long n_blocks = Q.balanced ? gridDim.x : (n_tasks + tpb - 1) / tpb;
   unsigned long long F = 0;
   unsigned long long *fpu = reinterpret_cast<unsigned long long *>(L.fp);
   __shared__ long long s_blk;
   for (long long blk = Q.head ? claim_block(Q, n_blocks, s_blk) : blockIdx.x; blk < n_blocks;
        blk = Q.head ? claim_block(Q, n_blocks, s_blk) : blk + gridDim.x) {
-    long long t0 = blk * tpb;
-    const long long t1 = min(n_tasks, t0 + tpb);
+    long long t0, t1;
+    if (Q.balanced) {
+      t0 = (long long)mul_div((unsigned long long)n_tasks, (unsigned long long)blk, gridDim.x);
+      t1 = (long long)mul_div((unsigned long long)n_tasks, (unsigned long long)blk + 1, gridDim.x);
+    } else {
+      t0 = blk * tpb;
+      t1 = min(n_tasks, t0 + tpb);
+    }
     int prev = 0;  // the row of the block's previous segment
     while (t0 < t1) {
       const int i = task_row_wave(task_off, L.p_total, prev, t0);
@@ -1378,6 +1385,19 @@ constexpr int kSearchBlocks = 256 * 8;
 // Ordered edges per workgroup block of the level-1 gather kernel.
 constexpr int kL1GatherEpb = 64;
 
+// Workgroups of 64 kWavesPerBlock threads and `lds` bytes of dynamic LDS that
+// the device holds at once (the grid of a balanced by-set search).
+int resident_blocks(const void *kernel, size_t lds) {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 64 * kWavesPerBlock, lds) !=
+          hipSuccess ||
+      cus <= 0 || per <= 0)
+    return kSearchBlocks;
+  return cus * per;
+}
+
 // Rows whose block fits `limit` bytes of LDS go to the staged kernel, longer
 // ones to the gather kernel (launched only when such a row exists).  Both
 // read the same task plan; each skips the other's rows.
@@ -1391,7 +1411,19 @@ void launch_ln_as(hipStream_t s, const Layout &L, int rpt, const long long *row_
     // queue (their side runs the per-edge tasks)
     if constexpr (!FAST && LV >= 2) {
       if (!Q.done) {
-        hipLaunchKernelGGL((search_ls_kernel<LV>), dim3(kSearchBlocks), dim3(64 * kWavesPerBlock),
+        int grid = kSearchBlocks;
+        if (Q.balanced) {
+          // the occupancy query once per LDS size and host thread
+          thread_local size_t cached_lds = 0;
+          thread_local int cached_grid = 0;
+          const size_t lds = lds_bytes(dmax, LV);
+          if (lds != cached_lds || cached_grid <= 0) {
+            cached_grid = resident_blocks((const void *)search_ls_kernel<LV>, lds);
+            cached_lds = lds;
+          }
+          grid = cached_grid;
+        }
+        hipLaunchKernelGGL((search_ls_kernel<LV>), dim3(grid), dim3(64 * kWavesPerBlock),
                            lds_bytes(dmax, LV), s, L, task_off, row_tasks, tpb, dmax, Q);
       } else {
         hipLaunchKernelGGL((search_ln_lds_kernel<LV, FAST>), dim3(kSearchBlocks),
