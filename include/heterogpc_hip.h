@@ -153,7 +153,7 @@ int hpc_graph_stats(hpc_state *st, int64_t *replays, int64_t *captures, int64_t 
 /* Autotuner (no counterpart in the reference): runs the state's levels
  * 1..max_level once, timing each level's CI-test kernel on that level's
  * snapshot for every candidate launch shape -- level 1: ordered edges per
- * workgroup in {8..128}; levels >= 2: conditioning sets per task in
+ * workgroup in {8..256}; levels >= 2: conditioning sets per task in
  * {256..4096} x tasks per workgroup in {4..128} x task order (device queue
  * or workgroup id), and with each conditioning-set size one contiguous
  * equal share of the tasks per resident workgroup -- and keeps the fastest per

@@ -201,7 +201,8 @@ void engine_free(Engine &e) {
     if (sl.base) (void)hipFree(sl.base);
   e.arena = Arena();
   dfree(e.d_inst); dfree(e.d_row_inst); dfree(e.d_deg); dfree(e.d_rowptr);
-  dfree(e.d_nbr); dfree(e.d_erow); dfree(e.d_rowmask); dfree(e.d_fp); dfree(e.d_row_tasks);
+  dfree(e.d_nbr); dfree(e.d_erow); dfree(e.d_rowmask); dfree(e.d_fp); dfree(e.d_ccor);
+  dfree(e.d_row_tasks);
   dfree(e.d_task_off); dfree(e.d_binom); dfree(e.d_prefix); dfree(e.d_ctr);
   e.d_ictr = nullptr;
   graph_drop(e);
@@ -526,11 +527,12 @@ static int ensure_row_cap(Engine &e, size_t rows) {
 static int ensure_edge_cap(Engine &e, size_t edges) {
   if (edges <= e.cap_edges && e.d_nbr) return HPC_OK;
   const size_t c = std::max<size_t>(edges + edges / 4, 1024);
-  dfree(e.d_nbr); dfree(e.d_erow); dfree(e.d_fp);
+  dfree(e.d_nbr); dfree(e.d_erow); dfree(e.d_fp); dfree(e.d_ccor);
   e.cap_edges = 0;
   int rc = dalloc((void **)&e.d_nbr, c * sizeof(int32_t));
   if (!rc) rc = dalloc((void **)&e.d_erow, c * sizeof(int32_t));
   if (!rc) rc = dalloc((void **)&e.d_fp, c * sizeof(int64_t));
+  if (!rc) rc = dalloc((void **)&e.d_ccor, c * sizeof(double));
   if (!rc) e.cap_edges = c;
   return rc;
 }
@@ -598,6 +600,7 @@ int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &
   }
   e.packed = ids;
   e.p_total = p_total;
+  e.pmax = pmax;
   if (K > 0) {
     rc = upload_table(e, zero_ctr);
     if (rc) return rc;
@@ -627,12 +630,13 @@ static Layout layout(Engine &e) {
   L.K = (int)e.packed.size();
   L.p_total = e.p_total;
   L.max_deg = e.max_deg;
-  L.pad_ = 0;
+  L.pmax = e.pmax;
   L.deg = e.d_deg;
   L.rowptr = e.d_rowptr;
   L.nbr = e.d_nbr;
   L.erow = e.d_erow;
   L.fp = e.d_fp;
+  L.ccor = e.d_ccor;
   L.binom = e.d_binom;
   L.ctr = e.d_ctr;
   L.rchk = e.d_rchk;
@@ -1071,7 +1075,7 @@ size_t engine_footprint(int p, int n, int max_level) {
              // HPC_CORR_F32: this bounds both) and the NaN mask
              + std::max(zc_size(p, (int)npad) * 8, (size_t)p * npad * 12) + (size_t)p * npad
              + (size_t)p * 16;      // mean, sd
-  b += pp * (4 + 4 + 8);            // worst-case share of nbr / erow / first-pass arrays
+  b += pp * (4 + 4 + 8 + 8);        // worst-case share of nbr / erow / first-pass / ccor arrays
   b += (size_t)p * 40;              // row arrays
   return b;
 }
@@ -1607,7 +1611,7 @@ struct TuneCand {
 static std::vector<TuneCand> tune_candidates(int level) {
   std::vector<TuneCand> cand;
   if (level == 1) {
-    for (int epb : {8, 16, 32, 64, 128}) cand.push_back({0, epb, 0});
+    for (int epb : {8, 16, 32, 64, 128, 256}) cand.push_back({0, epb, 0});
   } else {
     for (int rpt : {256, 512, 1024, 2048, 4096}) {
       for (int tpb : {4, 8, 16, 32, 64, 128})
@@ -2035,7 +2039,7 @@ int engine_tpb(const Engine &e, int level) {
   return v > 0 ? v : 16;
 }
 
-int engine_epb(const Engine &e) { return e.epb_level1 > 0 ? e.epb_level1 : 32; }
+int engine_epb(const Engine &e) { return e.epb_level1 > 0 ? e.epb_level1 : 64; }
 
 int engine_check_tuning(const hpc_tune_entry *t, int count) {
   if (count < 0 || (count > 0 && !t)) return hpc_fail(HPC_EINVAL, "no tuning entries");

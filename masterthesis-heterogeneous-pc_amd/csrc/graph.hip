@@ -265,6 +265,42 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(Layout L) {
   }
 }
 
+// The entries of one lane's adjacency word w of row g, written from CSR
+// offset `off` on: neighbour ids, the row, first passing rank -1 and, for a
+// row at level 1, its correlations cor[g][nbr] (four at a time, their loads
+// in flight together).
+__device__ __forceinline__ void fill_word(const Layout &L, const Inst &In, int g, int w,
+                                          unsigned long long word, int off) {
+  const bool wcor = L.ccor && In.level == 1;
+  const double *crow = In.cor + (size_t)(g - In.start) * In.p;
+  while (word) {
+    int col[4];
+    int cnt = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      col[u] = -1;
+      if (word) {
+        col[u] = w * 64 + __ffsll((long long)word) - 1;
+        word &= word - 1ull;
+        cnt = u + 1;
+      }
+    }
+    double cv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) cv[u] = (wcor && col[u] >= 0) ? crow[col[u]] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (u < cnt) {
+        L.nbr[off + u] = In.start + col[u];
+        L.erow[off + u] = g;
+        L.fp[off + u] = -1;  // no passing set found yet
+        if (wcor) L.ccor[off + u] = cv[u];
+      }
+    }
+    off += cnt;
+  }
+}
+
 // compactCUTiled (compact.cu:4-89): the ascending neighbour ids of row g
 // (global ids), written at rowptr[g] (CSR instead of a p-wide padded row),
 // plus the row of every ordered edge, whose first passing rank starts at -1.
@@ -287,15 +323,7 @@ __global__ __launch_bounds__(256) void fill_kernel(Layout L) {
       int v = __shfl_up(incl, s, 64);
       if (lane >= s) incl += v;
     }
-    int off = base + incl - c;
-    while (word) {
-      int b = __ffsll((long long)word) - 1;
-      word &= word - 1ull;
-      L.nbr[off] = In.start + w * 64 + b;
-      L.erow[off] = g;
-      L.fp[off] = -1;  // no passing set found yet
-      ++off;
-    }
+    fill_word(L, In, g, w, word, base + incl - c);
     base += __shfl(incl, 63, 64);
   }
 }
@@ -472,7 +500,9 @@ __global__ __launch_bounds__(64 * kCompWaves) void compact_count_kernel(
     __hip_atomic_store(pub.flag, pub.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-__global__ __launch_bounds__(64 * kCompWaves) void compact_fill_kernel(
+// One wavefront per row of the workgroup's kCompRows, so the rows' dependent
+// loads (row pointer, adjacency words, correlations) overlap.
+__global__ __launch_bounds__(64 * kCompRows) void compact_fill_kernel(
     Layout L, const unsigned long long *__restrict__ blk, const long long *__restrict__ row_tasks,
     long long *__restrict__ task_off, unsigned long long edge_cap) {
   __shared__ int s_deg[kCompRows];
@@ -494,14 +524,13 @@ __global__ __launch_bounds__(64 * kCompWaves) void compact_fill_kernel(
       task_off[g] = (long long)(ext + pt);
     }
   }
-#pragma unroll 1
-  for (int rr = 0; rr < kCompRowsPerWave; ++rr) {
-    const int r = wv * kCompRowsPerWave + rr;
+  {
+    const int r = wv;
     const int g = g0 + r;
-    if (g >= L.p_total || s_deg[r] == 0) continue;
+    if (g >= L.p_total || s_deg[r] == 0) return;
     unsigned long long base = exe;
     for (int q = 0; q < r; ++q) base += (unsigned long long)s_deg[q];
-    if (base + (unsigned long long)s_deg[r] > edge_cap) continue;  // never with a true prefix
+    if (base + (unsigned long long)s_deg[r] > edge_cap) return;  // never with a true prefix
     const Inst &In = L.inst[L.row_inst[g]];
     const unsigned long long *row = In.adj + (size_t)(g - In.start) * In.W;
     int wbase = (int)base;
@@ -515,15 +544,7 @@ __global__ __launch_bounds__(64 * kCompWaves) void compact_fill_kernel(
         const int v = __shfl_up(incl, sh, 64);
         if (lane >= sh) incl += v;
       }
-      int off = wbase + incl - c;
-      while (word) {
-        const int bit = __ffsll((long long)word) - 1;
-        word &= word - 1ull;
-        L.nbr[off] = In.start + w * 64 + bit;
-        L.erow[off] = g;
-        L.fp[off] = -1;
-        ++off;
-      }
+      fill_word(L, In, g, w, word, wbase + incl - c);
       wbase += __shfl(incl, 63, 64);
     }
   }
@@ -577,7 +598,7 @@ void launch_compact_fused(hipStream_t s, const Layout &L, int level, int rpt,
   const int nblk = compact_fused_blocks(L.p_total);
   hipLaunchKernelGGL(compact_count_kernel, dim3(nblk), dim3(64 * kCompWaves), 0, s, L, level, rpt,
                      nblk, blk, row_tasks, pub);
-  hipLaunchKernelGGL(compact_fill_kernel, dim3(nblk), dim3(64 * kCompWaves), 0, s, L,
+  hipLaunchKernelGGL(compact_fill_kernel, dim3(nblk), dim3(64 * kCompRows), 0, s, L,
                      (const unsigned long long *)blk, (const long long *)row_tasks, task_off,
                      (unsigned long long)edge_cap);
 }
@@ -585,7 +606,7 @@ void launch_compact_fused(hipStream_t s, const Layout &L, int level, int rpt,
 void launch_compact_fill(hipStream_t s, const Layout &L, const unsigned long long *blk,
                          const long long *row_tasks, long long *task_off, size_t edge_cap) {
   const int nblk = compact_fused_blocks(L.p_total);
-  hipLaunchKernelGGL(compact_fill_kernel, dim3(nblk), dim3(64 * kCompWaves), 0, s, L, blk,
+  hipLaunchKernelGGL(compact_fill_kernel, dim3(nblk), dim3(64 * kCompRows), 0, s, L, blk,
                      row_tasks, task_off, (unsigned long long)edge_cap);
 }
 

@@ -139,12 +139,15 @@ struct Layout {
   int K;
   int p_total;
   int max_deg;                  // largest row length over all instances
-  int pad_;
+  int pmax;                     // largest instance p
   int32_t *deg;                 // [p_total]
   int32_t *rowptr;              // [p_total + 1]
   int32_t *nbr;                 // [E] global neighbour ids
   int32_t *erow;                // [E] row of each ordered edge
   int64_t *fp;                  // [E] first passing rank, -1 = none
+  // [E] cor[i][nbr] of each ordered edge (i, nbr) of a row at level 1 (the
+  // fill kernels write it for those rows; the level-1 search reads it)
+  double *ccor;
   const unsigned long long *binom;  // [(pmax + 1) x kBinomK], saturating
   Counters *ctr;
   Recheck *rchk;                // recheck queue of a search
@@ -407,6 +410,7 @@ struct Engine {
   int32_t *d_deg = nullptr, *d_rowptr = nullptr, *d_nbr = nullptr, *d_erow = nullptr;
   int32_t *d_rowmask = nullptr;
   int64_t *d_fp = nullptr;
+  double *d_ccor = nullptr;
   long long *d_row_tasks = nullptr, *d_task_off = nullptr;
   // per-workgroup sums of the two-kernel compaction
   unsigned long long *d_blk = nullptr;
@@ -441,11 +445,12 @@ struct Engine {
   std::vector<int32_t> h_row_inst_dev;  // what d_row_inst holds (when row_inst_valid)
   bool row_inst_valid = false;
   int p_total = 0;
+  int pmax = 0;  // largest p of the packed instances
   int64_t n_edges = 0;
   int max_deg = 0;
   int ranks_per_task = 512;
   // launch shapes per level (set by the autotuner; 0 = defaults: ranks per
-  // task `ranks_per_task`, 16 tasks and 32 ordered edges per workgroup)
+  // task `ranks_per_task`, 16 tasks and 64 ordered edges per workgroup)
   int rpt_level[kMaxLevel + 1] = {};
   int tpb_level[kMaxLevel + 1] = {};
   int queue_level[kMaxLevel + 1] = {};  // 0 = use_queue, 1 = device queue, 2 = workgroup order

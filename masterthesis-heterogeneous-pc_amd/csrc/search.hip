@@ -300,6 +300,160 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_lds_kernel(
   }
 }
 
+// The level-1 decision of (x1, x2, x3) = (r_ij, r_ik, r_jk), as removes(
+// pcor_l1(...)) decides it: exact instances through the squared filter
+// (removes_sq, the same quotient when it cannot decide), fast ones through r
+// and the certainty test of their recheck.
+template <bool FAST>
+__device__ __forceinline__ bool l1_decide(const Inst &In, bool fast, bool chk, double r_crit,
+                                          double lo, double hi, double x1, double x2, double x3,
+                                          bool &unc) {
+  unc = false;
+  if (chk && (__builtin_isnan(x1) || __builtin_isnan(x2) || __builtin_isnan(x3))) return false;
+  if (!FAST || !fast) {
+    // pcor_l1's numerator and radicand, in its order of operations
+    const double s01 = x1 - x2 * x3;
+    const double q = (1.0 - x3 * x3) * (1.0 - x2 * x2);
+    if (q > 0.0 && __builtin_isfinite(s01)) return removes_sq(s01, q, r_crit, lo, hi);
+    return removes(s01 / __builtin_sqrt(q), r_crit);
+  }
+  const double r = pcor_l1(x1, x2, x3);
+  const double xa[1] = {x2}, ya[1] = {x3}, Cz[1][1] = {{1.0}};
+  unc = !fast_certain(r, r_crit, pcor_sensitivity<1>(x1, xa, ya, Cz), In.eps);
+  return removes(r, r_crit) && !unc;
+}
+
+// Level 1 by pivot row.  The tests of ordered edge (i, j) read cor[j][k] for
+// every k of N(i)\{j}, scattered over row j, and cor[i][k], contiguous in the
+// compacted correlations (ccor, written with the neighbour lists).  A
+// workgroup stages whole correlation rows j in LDS and runs the tests of the
+// edges (i, j) of the neighbours i of j: workgroup block b takes the CSR
+// edges (j, i) [b epb, (b + 1) epb), grouped by pivot row j, and the XCDs each
+// take a contiguous eighth of the blocks, so a pivot row's workgroups share an
+// L2.  N(i) and ccor (a few MB in all) stay in L2; HBM sees about one read of
+// each correlation row.  Same tests, ranks and decisions as search_l1_kernel.
+//
+// Ranks skip j: position pos of N(i) has rank pos - (pos > y), y the position
+// of j, which is the number of neighbours of i below j in the bit adjacency
+// (the level-start snapshot while the search runs).  A wave takes four edges,
+// positions 0..15 of each first (16 lanes per edge), then 64-position chunks
+// of each edge that has not passed yet.
+template <bool FAST>
+__global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_pivot_kernel(
+    Layout L, long long n_edges, const int32_t *__restrict__ row_mask, int epb, long long per_xcd) {
+  extern __shared__ unsigned long long smem[];
+  double *Rj = reinterpret_cast<double *>(smem);  // [p]: row j of its instance
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int grp = lane >> 4, q = lane & 15;
+  const long long g = blockIdx.x;
+  const long long blk = (g % kL1Classes) * per_xcd + g / kL1Classes;
+  // the snapshot's own size bounds the count a launch was sized with
+  n_edges = min(n_edges, (long long)L.rowptr[L.p_total]);
+  long long e0 = blk * epb;
+  const long long e1 = min(n_edges, e0 + epb);
+  while (e0 < e1) {
+    const int j = L.erow[e0];
+    const int bj = L.rowptr[j];
+    const long long seg_end = min(e1, (long long)L.rowptr[j + 1]);
+    if (seg_end <= e0) break;  // (a consistent snapshot always advances)
+    const Inst &In = L.inst[L.row_inst[j]];
+    // uniform across the workgroup: the pivot row is staged or skipped by all
+    if (In.level == 1) {
+      const int p = In.p, s0 = In.start, lj = j - s0, W = In.W;
+      const double *crow = In.cor + (size_t)lj * p;
+      __syncthreads();  // the previous pivot's tests are done with Rj
+      for (int a0 = 0; a0 < p; a0 += 8 * 64 * kWavesPerBlock) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int a = a0 + u * 64 * kWavesPerBlock + tid;
+          v[u] = a < p ? crow[a] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int a = a0 + u * 64 * kWavesPerBlock + tid;
+          if (a < p) Rj[a] = v[u];
+        }
+      }
+      __syncthreads();
+      const double r_crit = In.r_crit, lo = In.r2_lo, hi = In.r2_hi;
+      const bool fast = FAST && In.fast;
+      const bool chk = In.flags & kCorNaN;  // inputs may be NaN
+      const int wj = lj >> 6;
+      const unsigned long long below = (1ull << (lj & 63)) - 1ull;
+      const int t_lo = (int)(e0 - bj), t_hi = (int)(seg_end - bj);
+      for (int tb = t_lo + 4 * wave; tb < t_hi; tb += 4 * kWavesPerBlock) {
+        const int t = tb + grp;
+        bool live = t < t_hi;
+        const int i = live ? L.nbr[bj + t] : j;
+        if (live && row_mask && !row_mask[i]) live = false;
+        int bi = 0, degi = 0, c = 0;
+        double x1 = 0.0;
+        if (live) {
+          const int li = i - s0;
+          bi = L.rowptr[i];
+          degi = L.rowptr[i + 1] - bi;
+          x1 = In.cor[(size_t)li * p + lj];
+          const unsigned long long *arow = In.adj + (size_t)li * W;
+          for (int w = q; w <= wj; w += 16) c += __popcll(w < wj ? arow[w] : arow[w] & below);
+        }
+#pragma unroll
+        for (int sh = 8; sh > 0; sh >>= 1) c += __shfl_xor(c, sh, 64);
+        const int y = c;  // j's position in N(i)
+        live = live && degi >= 2 && y < degi;
+        const long long eg = (long long)bi + y;
+        bool pass = false, unc = false;
+        if (live && q < degi && q != y) {
+          const int k = L.nbr[bi + q];
+          pass = l1_decide<FAST>(In, fast, chk, r_crit, lo, hi, x1, L.ccor[bi + q], Rj[k - s0], unc);
+        }
+        const unsigned long long m = __ballot(pass);
+        const unsigned gm = (unsigned)(m >> (16 * grp)) & 0xFFFFu;
+        // this group's uncertain tests below its first certain pass
+        if (FAST && fast)
+          queue_rank(L, unc && (gm == 0u || q < __ffs(gm) - 1), eg,
+                     (unsigned long long)(q - (q > y ? 1 : 0)), lane);
+        long long found = -1;
+        if (gm) {
+          const int pf = __ffs(gm) - 1;
+          found = pf - (pf > y ? 1 : 0);
+        }
+        if (live && q == 0 && (found >= 0 || degi <= 16)) L.fp[eg] = found;
+        // the edges still open, one after the other with 64-position chunks
+        const unsigned long long open = __ballot(live && q == 0 && found < 0 && degi > 16);
+#pragma unroll 1
+        for (int gg = 0; gg < 4; ++gg) {
+          if (!((open >> (16 * gg)) & 1ull)) continue;
+          const int b = __shfl(bi, 16 * gg, 64), d = __shfl(degi, 16 * gg, 64);
+          const int yy = __shfl(y, 16 * gg, 64);
+          const double a1 = __shfl(x1, 16 * gg, 64);
+          const long long e = (long long)b + yy;
+          long long f = -1;
+          for (int q0 = 16; q0 < d; q0 += 64) {
+            const int ps = q0 + lane;
+            bool ok = false, un = false;
+            if (ps < d && ps != yy) {
+              const int k = L.nbr[b + ps];
+              ok = l1_decide<FAST>(In, fast, chk, r_crit, lo, hi, a1, L.ccor[b + ps], Rj[k - s0], un);
+            }
+            const unsigned long long mm = __ballot(ok);
+            if (FAST && fast)
+              queue_rank(L, un && (mm == 0ull || lane < __ffsll((long long)mm) - 1), e,
+                         (unsigned long long)(ps - (ps > yy ? 1 : 0)), lane);
+            if (mm) {
+              const int pf = q0 + __ffsll((long long)mm) - 1;
+              f = pf - (pf > yy ? 1 : 0);
+              break;
+            }
+          }
+          if (lane == 0) L.fp[e] = f;
+        }
+      }
+    }
+    e0 = seg_end;
+  }
+}
+
 // Task t of a level's plan: row g with task_off[g] <= t < task_off[g] +
 // row_tasks[g] (the last row whose exclusive offset is <= t), then ordered
 // edge rowptr[g] + local / nch and rank chunk local % nch, where
@@ -1468,6 +1622,20 @@ size_t search_lds_limit(size_t requested) {
 void launch_search_l1(hipStream_t s, const Layout &L, int64_t n_edges, const int32_t *row_mask,
                       bool fast, int epb, size_t limit) {
   if (n_edges <= 0) return;
+  // a correlation row of every instance fits the LDS budget: by pivot row
+  const size_t row_bytes = (size_t)L.pmax * sizeof(double);
+  if (L.ccor && L.pmax > 0 && row_bytes <= limit) {
+    const long long blocks = (n_edges + epb - 1) / epb;
+    const long long per_xcd = (blocks + kL1Classes - 1) / kL1Classes;
+    const dim3 grid((unsigned)(per_xcd * kL1Classes));
+    if (fast)
+      hipLaunchKernelGGL(search_l1_pivot_kernel<true>, grid, dim3(64 * kWavesPerBlock), row_bytes,
+                         s, L, (long long)n_edges, row_mask, epb, per_xcd);
+    else
+      hipLaunchKernelGGL(search_l1_pivot_kernel<false>, grid, dim3(64 * kWavesPerBlock), row_bytes,
+                         s, L, (long long)n_edges, row_mask, epb, per_xcd);
+    return;
+  }
   // rows of up to `cap` neighbours are staged (16 bytes per neighbour)
   const int cap = (int)std::min<size_t>((size_t)L.max_deg, limit / 16);
   if (cap >= 2) {

@@ -1,5 +1,7 @@
 """Rows too long for LDS staging: the gather kernels (search_l1_kernel,
-search_ln_kernel) against the oracle.
+search_ln_kernel) against the oracle.  Level 1 takes the pivot-row kernel
+whenever a whole correlation row fits the budget, so budgets below 8 p bytes
+drive its staged and gather kernels.
 
 Each level routes row by row: a row whose neighbourhood fits the workgroup's
 LDS budget is staged, a longer one is searched by the gather kernel of the
@@ -34,9 +36,13 @@ def lds_side(limit, lv):
         d = e
 
 
-def routes(limit, level, max_deg):
-    """(staged rows possible, gathered rows present) at a level."""
+def routes(limit, level, max_deg, p):
+    """(staged rows possible, gathered rows present) at a level.  Level 1
+    stages whole correlation rows (the pivot kernel) when one fits the
+    budget, else rows of up to limit / 16 neighbours."""
     if level == 1:
+        if p * 8 <= limit:
+            return True, False
         cap = min(max_deg, limit // 16)
         return cap >= 2, max_deg > cap
     dmax = min(max_deg + 1, lds_side(limit, level))
@@ -63,7 +69,7 @@ def _run_vs_oracle(hpc, orc, x, n, alpha, ml, limit, mode=0, label=""):
             assert np.array_equal(dev.adj_compact(), ref.adj_compact()), (label, level)
             if ml < 0 and o["max_deg"] - 1 < level:
                 break
-            s, g = routes(limit if limit else KIB64, level, o["max_deg"])
+            s, g = routes(limit if limit else KIB64, level, o["max_deg"], p)
             staged += s
             gathered += g
             fp_d, rp_d = dev.first_pass()

@@ -208,7 +208,7 @@ def test_autotune_picks_shapes_and_keeps_results(hpc, orc):
         entries = dev.autotune(4)
         print(f"[autotune] {entries}", flush=True)
         assert [t["level"] for t in entries] == list(range(1, len(entries) + 1))
-        assert entries[0]["edges_per_block"] in (8, 16, 32, 64, 128)
+        assert entries[0]["edges_per_block"] in (8, 16, 32, 64, 128, 256)
         for t in entries[1:]:
             assert t["ranks_per_task"] in (256, 512, 1024, 2048, 4096)
             assert t["tasks_per_block"] in (4, 8, 16, 32, 64, 128)
