@@ -557,10 +557,11 @@ def main() -> int:
         d.update(kw)
         rooflines.append(d)
 
-    mom_bytes = 8 * p * n + 9 * p * npad + 16 * p
-    entry("moments_kernel", "hbm", mom_bytes / (k_mom * 1e-3) / 1e9 if k_mom else 0.0,
-          HBM_PEAK_GBS, "GB/s", k_mom, "%d B: observations read, z-scores and NaN mask written"
-          % mom_bytes)
+    # NaN-free data: no NaN mask is written (only the masked kernel reads it)
+    mom_bytes = 8 * p * n + 8 * p * npad + 16 * p
+    entry("stats_kernel + zscore_kernel", "hbm", mom_bytes / (k_mom * 1e-3) / 1e9 if k_mom else 0.0,
+          HBM_PEAK_GBS, "GB/s", k_mom, "%d B: observations read once, z-scores, means and standard "
+          "deviations written (the two kernels read the observations twice)" % mom_bytes)
     entry("corr_xmfma_kernel", "mfma", corr_tflops, FP64_PEAK_TFLOPS, "TFLOP/s", k_corr,
           "%.3e flops: 2n per pair, p(p+1)/2 pairs" % corr_flops)
     for lv in last:

@@ -108,33 +108,26 @@ __device__ __forceinline__ int find_inst(const long long *prefix, int K, long lo
   return lo;
 }
 
-// One wavefront per variable: mean over non-NaN samples, sd rounded through
-// float (corHelper.cu:75), z-scores (x - mean) / sd padded with zeros to npad
-// (canonical layout, class-major layout, float copy: whichever the instance
-// holds), NaN mask.  A variable with NaN cells flags its instance (ictr.has_nan),
-// which selects the masked pairwise kernel for it.  With J > 0 (every
-// instance's n <= 64 J) lane t loads its samples t, t + 64, ... once into
-// registers and the three passes run from there; J = 0 re-reads the row for
-// each pass.  Both do the same operations in the same order.  With J > 0 a
-// row's class-major z-scores are assembled in the wave's LDS area and stored
-// 16 bytes per lane (written value by value, each store would touch 64
-// cache lines).
+// Means and standard deviations, one wavefront per variable: mean over
+// non-NaN samples, sd rounded through float (corHelper.cu:75); a variable
+// with NaN cells flags its instance (ictr.has_nan), which selects the masked
+// pairwise kernel for it.  With J > 0 (every instance's n <= 64 J) lane t
+// loads its samples t, t + 64, ... once into registers and both passes run
+// from there; J = 0 re-reads the row for each pass.  Both do the same
+// operations in the same order.
 template <int J>
-constexpr int moments_waves() { return J > 32 ? 2 : 4; }
+constexpr int stats_waves() { return J > 32 ? 2 : 4; }
 
 template <int J>
-__global__ __launch_bounds__(64 * moments_waves<J>()) void moments_kernel(Layout L) {
-  // one class-major row per wave (64 classes x up to J slots)
-  // (a class-major row is 64 x zc_slots(npad) <= 64 J doubles)
-  static_assert(J % 16 == 0, "whole 16-slot groups");
-  __shared__ double zrow[J > 0 ? moments_waves<J>() : 1][J > 0 ? 64 * J : 1];
+__global__ __launch_bounds__(64 * stats_waves<J>()) void stats_kernel(Layout L) {
   const int lane = threadIdx.x & 63;
   const int g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (g >= L.p_total) return;
   const int kin = L.row_inst[g];
+  // a copy: the stores below could alias the table for the compiler
   const Inst I = L.inst[kin];
   if (!(I.flags & kActive)) return;
-  const int a = g - I.start, n = I.n, npad = I.npad;
+  const int a = g - I.start, n = I.n;
   const double *xa = I.x + (size_t)a * n;
   double xv[J > 0 ? J : 1];
   if constexpr (J > 0) {
@@ -188,55 +181,88 @@ __global__ __launch_bounds__(64 * moments_waves<J>()) void moments_kernel(Layout
     I.mean[a] = mu;
     I.sd[a] = sd;
   }
-  double *za = I.z ? I.z + (size_t)a * npad : nullptr;
-  uint8_t *na = I.nanm + (size_t)a * npad;
-  float *z32 = I.z32 ? I.z32 + (size_t)a * npad : nullptr;
+}
+
+// 16-byte unit of piece (t, kk, r8) in the z-score kernel's LDS tile: the
+// low four bits of (kk & 1, r8) are XORed with t, so the 16 lanes of a
+// quarter wave hit distinct bank groups both when the pieces are written (t
+// varies) and when a line's pieces are read for the store (kk, r8 vary).
+__device__ __forceinline__ int zpiece(int t, int kk, int r8) {
+  return t * 32 + (kk >> 1) * 16 + ((((kk & 1) << 3) | r8) ^ (t & 15));
+}
+
+// z-scores (x - mean) / sd padded with zeros to npad, in every layout the
+// instance holds (canonical FP64, class-major FP64, FP32 copy), and the NaN
+// mask (instances with NaN cells only: nothing reads it otherwise).
+// Workgroup (b, gh) takes the eight rows 8 (b - grp0) .. + 7 of the instance
+// whose 8-row groups start at grp0 <= b, and the class-major slots
+// 8 gh .. 8 gh + 7: half h = gh % 2 of slot group gg = gh / 2 (slot m of
+// lane class t is sample t + 64 (m - P0), the first P0 slots zero).  Each
+// (row, slot) is a 512-byte run of samples, one coalesced load; the
+// class-major values go through a 32 KB LDS tile so that the stores write
+// whole 128-byte lines (eight rows' 16-byte pieces of one (gg, t, h, kk)),
+// where a row on its own writes 16 bytes per line.
+__global__ __launch_bounds__(256) void zscore_kernel(Layout L) {
+  __shared__ double2 tile[2048];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int b = blockIdx.x, gh = blockIdx.y, gg = gh >> 1, h = gh & 1;
+  int k = 0;
+  if (L.K > 1) {
+    int lo = 0, hi = L.K - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (L.inst[mid].grp0 <= b) lo = mid; else hi = mid - 1;
+    }
+    k = lo;
+  }
+  // a copy: the stores below could alias the table for the compiler
+  const Inst I = L.inst[k];
+  const int p = I.p, n = I.n, npad = I.npad;
+  const int a0 = 8 * (b - I.grp0);
   const int ms = zc_slots(npad), p0 = ms - (npad >> 6);
-  // class-major target: with J > 0 the row is assembled in the wave's LDS
-  // area, ordered [g][t][h][kk][b] (zc_off of row 0 within one 16-slot
-  // group, packed), then stored piece by piece; else each value goes to its
-  // place in the row block directly
-  const size_t zblk = zc_block(npad);
-  double *zl = J > 0 && I.zc ? zrow[threadIdx.x >> 6] : nullptr;
-  auto lds_pos = [](int mp, int t) {
-    const int j = mp & 15;
-    return 1024 * (mp >> 4) + 16 * t + 2 * (4 * (j >> 3) + (j & 3)) + ((j >> 2) & 1);
-  };
-  if (I.zc) {
-    // the leading zeros of this lane's class (all in group 0)
-    for (int mp = 0; mp < p0; ++mp) {
-      if (zl) zl[lds_pos(mp, lane)] = 0.0;
-      else I.zc[(size_t)(a >> 4) * zblk + zc_off(0, lane, mp, a & 15)] = 0.0;
+  // (uniform) an inactive instance, rows past the instance, slots past the row
+  if (!(I.flags & kActive) || a0 >= p || 8 * gh >= ms) return;
+  const bool mask = L.ictr[k].has_nan != 0;  // raised by the statistics kernel
+  // wave wv: rows a0 + 2 wv and a0 + 2 wv + 1, the half group's 8 slots each
+#pragma unroll
+  for (int rr = 0; rr < 2; ++rr) {
+    const int r8 = 2 * wv + rr, a = a0 + r8;
+    if (a >= p) continue;
+    const double *xa = I.x + (size_t)a * n;
+    const double mu = I.mean[a], sd = I.sd[a];
+    double xs[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int m = 8 * gh + i, kx = lane + 64 * (m - p0);
+      xs[i] = (m >= p0 && kx < n) ? xa[kx] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int m = 8 * gh + i, kx = lane + 64 * (m - p0);
+      const bool in = m >= p0 && kx < n;
+      const double zk = in ? (xs[i] - mu) / sd : 0.0;
+      if (m >= p0) {  // a sample slot: kx < npad
+        if (mask) I.nanm[(size_t)a * npad + kx] = (in && __builtin_isnan(xs[i])) ? 1 : 0;
+        if (I.z) I.z[(size_t)a * npad + kx] = zk;
+        if (I.z32) I.z32[(size_t)a * npad + kx] = (float)zk;
+      }
+      // slot 8 h + 4 bb + kk of the group: piece (t, kk), element bb
+      double *pc = reinterpret_cast<double *>(&tile[zpiece(lane, i & 3, r8)]);
+      pc[i >> 2] = zk;
     }
   }
-  auto put = [&](int k, double v) {
-    const double zk = k < n ? (v - mu) / sd : 0.0;
-    if (za) za[k] = zk;
-    if (zl) zl[lds_pos((k >> 6) + p0, k & 63)] = zk;
-    else if (I.zc) I.zc[zc_index(zblk, p0, a, k)] = zk;
-    na[k] = (k < n && __builtin_isnan(v)) ? 1 : 0;
-    if (z32) z32[k] = (float)zk;
-  };
-  if constexpr (J > 0) {
+  if (!I.zc) return;  // (uniform)
+  __syncthreads();
+  // piece (t, kk, rr) of row a0 + rr goes to zc_off(gg, t, 8 h + kk, a % 16);
+  // a wave stores 8 whole lines
+  double *blk = I.zc + (size_t)(a0 >> 4) * zc_block(npad) + (size_t)16384 * gg + 128 * h +
+                16 * ((a0 >> 3) & 1);
 #pragma unroll
-    for (int j = 0; j < J; ++j) {
-      const int k = lane + 64 * j;
-      if (k < npad) put(k, xv[j]);
-    }
-    if (zl) {
-      // every lane's pieces are in LDS (one wave: its LDS operations are
-      // ordered); piece q = (g, t, 4 h + kk) of 16 bytes goes to its row's
-      // place in the block
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      double *dst = I.zc + (size_t)(a >> 4) * zblk + 2 * (a & 15);
-      const double2 *src = reinterpret_cast<const double2 *>(zl);
-      for (int q = lane; q < 32 * ms; q += 64)
-        *reinterpret_cast<double2 *>(dst + (size_t)16384 * (q >> 9) + 256 * ((q >> 3) & 63) + 32 * (q & 7)) = src[q];
-    }
-  } else {
-    for (int k = lane; k < npad; k += 64) put(k, k < n ? xa[k] : 0.0);
+  for (int it = 0; it < 8; ++it) {
+    const int P = it * 256 + tid;
+    const int t = P >> 5, kk = (P >> 3) & 3, rr = P & 7;
+    if (a0 + rr < p)
+      *reinterpret_cast<double2 *>(blk + 256 * t + 32 * kk + 2 * rr) = tile[zpiece(t, kk, rr)];
   }
 }
 
@@ -847,20 +873,26 @@ void launch_moments(hipStream_t s, const Layout &L, const std::vector<Inst> &hos
 }
 
 template <int J>
-static void launch_moments_j(hipStream_t s, const Layout &L) {
-  const int wpb = moments_waves<J>();
-  hipLaunchKernelGGL(moments_kernel<J>, dim3((L.p_total + wpb - 1) / wpb), dim3(64 * wpb), 0, s,
-                     L);
+static void launch_stats_j(hipStream_t s, const Layout &L) {
+  const int wpb = stats_waves<J>();
+  hipLaunchKernelGGL(stats_kernel<J>, dim3((L.p_total + wpb - 1) / wpb), dim3(64 * wpb), 0, s, L);
 }
 
 void launch_moments_n(hipStream_t s, const Layout &L, int max_n) {
+  if (max_n <= 0 || L.p_total <= 0) return;
   // the samples in registers when every active instance's row fits
   if (max_n <= 64 * 32)
-    launch_moments_j<32>(s, L);
+    launch_stats_j<32>(s, L);
   else if (max_n <= 64 * 80)
-    launch_moments_j<80>(s, L);
+    launch_stats_j<80>(s, L);
   else
-    launch_moments_j<0>(s, L);
+    launch_stats_j<0>(s, L);
+  // every instance has at most ceil(p / 8) <= p / 8 + 1 row groups, and at
+  // most zc_slots(npad) / 8 half slot groups; workgroups past either exit
+  const long long groups = (long long)L.p_total / 8 + L.K;
+  const int npad = (max_n + 63) & ~63;
+  const int halves = zc_slots(npad) / 8;
+  hipLaunchKernelGGL(zscore_kernel, dim3((unsigned)groups, (unsigned)halves), dim3(256), 0, s, L);
 }
 
 void launch_corr(hipStream_t s, const Layout &L, const std::vector<Inst> &host_inst,

@@ -577,7 +577,7 @@ int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &
   if (rc) return rc;
   e.h_inst.assign(K, Inst());
   e.h_row_inst.resize(p_total);
-  int start = 0;
+  int start = 0, grp0 = 0;
   for (int k = 0; k < K; ++k) {
     InstHost &h = e.inst[ids[k]];
     engine_drop_results(h);  // a packed instance's device state may change
@@ -591,6 +591,8 @@ int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &
     I.r_crit = h.r_crit;
     square_bounds(h.r_crit, I.r2_lo, I.r2_hi);
     I.start = start;
+    I.grp0 = grp0;
+    grp0 += (h.p + 7) / 8;
     I.p = h.p; I.n = h.n; I.npad = h.npad; I.W = h.W;
     I.level = levels[k];
     I.flags = kActive | (h.cor_nan != 0 ? kCorNaN : 0);

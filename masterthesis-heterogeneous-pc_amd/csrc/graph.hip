@@ -271,8 +271,10 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(Layout L) {
 // in flight together).
 __device__ __forceinline__ void fill_word(const Layout &L, const Inst &In, int g, int w,
                                           unsigned long long word, int off) {
+  // the fields in registers: the int stores below may alias the table
+  const int start = In.start;
   const bool wcor = L.ccor && In.level == 1;
-  const double *crow = In.cor + (size_t)(g - In.start) * In.p;
+  const double *crow = In.cor + (size_t)(g - start) * In.p;
   while (word) {
     int col[4];
     int cnt = 0;
@@ -291,7 +293,7 @@ __device__ __forceinline__ void fill_word(const Layout &L, const Inst &In, int g
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       if (u < cnt) {
-        L.nbr[off + u] = In.start + col[u];
+        L.nbr[off + u] = start + col[u];
         L.erow[off + u] = g;
         L.fp[off + u] = -1;  // no passing set found yet
         if (wcor) L.ccor[off + u] = cv[u];

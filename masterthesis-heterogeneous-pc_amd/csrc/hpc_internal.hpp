@@ -128,7 +128,7 @@ struct Inst {
   int level;              // level of this phase
   int flags;
   int fast;               // cor holds fast (MFMA) values: decisions need rechecks
-  int pad_;
+  int grp0;               // first 8-row group of this instance in the pack (z-score kernel)
 };
 
 // What every kernel sees: the instance table and the global arrays.
