@@ -573,8 +573,9 @@ def main() -> int:
             entry("level0_kernel + level0_fold_kernel", "hbm", 12 * T / (ms * 1e-3) / 1e9,
                   HBM_PEAK_GBS, "GB/s", ms, "%d pairs x 12 B" % T, level=0)
         elif l == 1:
-            entry("search_l1_lds_kernel", "hbm", 28 * T / (ms * 1e-3) / 1e9, HBM_PEAK_GBS, "GB/s",
-                  ms, "%d canonical tests x 28 B" % T, level=1)
+            entry("search_l1_pivot_kernel", "hbm", 28 * T / (ms * 1e-3) / 1e9, HBM_PEAK_GBS,
+                  "GB/s", ms, "%d canonical tests x 28 B (operands from an LDS-staged "
+                  "correlation row and L2-resident compacted rows)" % T, level=1)
         else:
             fl = flops_per_test(l) * T
             hb = bytes_per_test(l) * T / (ms * 1e-3) / 1e9
