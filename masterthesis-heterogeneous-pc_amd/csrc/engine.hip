@@ -202,6 +202,7 @@ void engine_free(Engine &e) {
   e.arena = Arena();
   dfree(e.d_inst); dfree(e.d_row_inst); dfree(e.d_deg); dfree(e.d_rowptr);
   dfree(e.d_nbr); dfree(e.d_erow); dfree(e.d_rowmask); dfree(e.d_fp); dfree(e.d_ccor);
+  dfree(e.d_erev);
   dfree(e.d_row_tasks);
   dfree(e.d_task_off); dfree(e.d_binom); dfree(e.d_prefix); dfree(e.d_ctr);
   e.d_ictr = nullptr;
@@ -527,12 +528,13 @@ static int ensure_row_cap(Engine &e, size_t rows) {
 static int ensure_edge_cap(Engine &e, size_t edges) {
   if (edges <= e.cap_edges && e.d_nbr) return HPC_OK;
   const size_t c = std::max<size_t>(edges + edges / 4, 1024);
-  dfree(e.d_nbr); dfree(e.d_erow); dfree(e.d_fp); dfree(e.d_ccor);
+  dfree(e.d_nbr); dfree(e.d_erow); dfree(e.d_fp); dfree(e.d_ccor); dfree(e.d_erev);
   e.cap_edges = 0;
   int rc = dalloc((void **)&e.d_nbr, c * sizeof(int32_t));
   if (!rc) rc = dalloc((void **)&e.d_erow, c * sizeof(int32_t));
   if (!rc) rc = dalloc((void **)&e.d_fp, c * sizeof(int64_t));
   if (!rc) rc = dalloc((void **)&e.d_ccor, c * sizeof(double));
+  if (!rc) rc = dalloc((void **)&e.d_erev, c * sizeof(int32_t));
   if (!rc) e.cap_edges = c;
   return rc;
 }
@@ -639,6 +641,7 @@ static Layout layout(Engine &e) {
   L.erow = e.d_erow;
   L.fp = e.d_fp;
   L.ccor = e.d_ccor;
+  L.erev = e.d_erev;
   L.binom = e.d_binom;
   L.ctr = e.d_ctr;
   L.rchk = e.d_rchk;
@@ -1077,7 +1080,7 @@ size_t engine_footprint(int p, int n, int max_level) {
              // HPC_CORR_F32: this bounds both) and the NaN mask
              + std::max(zc_size(p, (int)npad) * 8, (size_t)p * npad * 12) + (size_t)p * npad
              + (size_t)p * 16;      // mean, sd
-  b += pp * (4 + 4 + 8 + 8);        // worst-case share of nbr / erow / first-pass / ccor arrays
+  b += pp * (4 + 4 + 8 + 8 + 4);    // worst-case share of nbr / erow / first-pass / ccor / erev arrays
   b += (size_t)p * 40;              // row arrays
   return b;
 }

@@ -296,6 +296,7 @@ __device__ __forceinline__ void fill_word(const Layout &L, const Inst &In, int g
         L.nbr[off + u] = start + col[u];
         L.erow[off + u] = g;
         L.fp[off + u] = -1;  // no passing set found yet
+        if (L.erev) L.erev[off + u] = -1;
         if (wcor) L.ccor[off + u] = cv[u];
       }
     }

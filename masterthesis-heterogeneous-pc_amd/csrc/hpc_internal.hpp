@@ -148,6 +148,10 @@ struct Layout {
   // [E] cor[i][nbr] of each ordered edge (i, nbr) of a row at level 1 (the
   // fill kernels write it for those rows; the level-1 search reads it)
   double *ccor;
+  // [E] CSR index of the reverse edge (nbr, i), or -1: the fill kernels clear
+  // it, the level-1 pivot search sets it for the edges it tests, the removal
+  // reads it in place of a search of row nbr
+  int32_t *erev;
   const unsigned long long *binom;  // [(pmax + 1) x kBinomK], saturating
   Counters *ctr;
   Recheck *rchk;                // recheck queue of a search
@@ -411,6 +415,7 @@ struct Engine {
   int32_t *d_rowmask = nullptr;
   int64_t *d_fp = nullptr;
   double *d_ccor = nullptr;
+  int32_t *d_erev = nullptr;
   long long *d_row_tasks = nullptr, *d_task_off = nullptr;
   // per-workgroup sums of the two-kernel compaction
   unsigned long long *d_blk = nullptr;

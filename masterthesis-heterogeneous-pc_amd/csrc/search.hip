@@ -402,6 +402,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void search_l1_pivot_kernel(
         const int y = c;  // j's position in N(i)
         live = live && degi >= 2 && y < degi;
         const long long eg = (long long)bi + y;
+        if (live && q == 0) L.erev[eg] = bj + t;  // (i, j) and its reverse (j, i)
         bool pass = false, unc = false;
         if (live && q < degi && q != y) {
           const int k = L.nbr[bi + q];
@@ -1277,11 +1278,18 @@ __global__ __launch_bounds__(64 * kCommitWaves) void commit_kernel(Layout L, lon
       if (fi >= 0) {
         bool mine = true;
         if (j < i) {
-          const int bj = L.rowptr[j];
-          const int lenj = L.rowptr[j + 1] - bj;
-          const int yj = find_pos(L.nbr + bj, lenj, i);
-          // the adjacency is symmetric, so i is in N(j); guard anyway
-          const long long fj = (yj < lenj && L.nbr[bj + yj] == i) ? L.fp[bj + yj] : -1;
+          // the reverse edge from the level-1 search, else a search of row j
+          const int rv = L.erev ? L.erev[e] : -1;
+          long long fj;
+          if (rv >= 0) {
+            fj = L.fp[rv];
+          } else {
+            const int bj = L.rowptr[j];
+            const int lenj = L.rowptr[j + 1] - bj;
+            const int yj = find_pos(L.nbr + bj, lenj, i);
+            // the adjacency is symmetric, so i is in N(j); guard anyway
+            fj = (yj < lenj && L.nbr[bj + yj] == i) ? L.fp[bj + yj] : -1;
+          }
           mine = fj < 0;
         }
         if (mine) {
