@@ -384,9 +384,8 @@ static void uploader_loop(hpc_server *s) {
       r = s->up_queue.front();
       s->up_queue.pop_front();
     }
-    hipError_t err = hipMemcpyAsync(r->dst, r->src, r->count * sizeof(double),
-                                    hipMemcpyHostToDevice, s->up_stream);
-    if (err == hipSuccess) err = hipStreamSynchronize(s->up_stream);
+    const hipError_t err =
+        upload_h2d(s->eng.device, r->dst, r->src, r->count * sizeof(double), s->up_stream);
     {
       std::lock_guard<std::mutex> lk(s->mu);
       r->up_err = err;

@@ -542,6 +542,10 @@ int engine_add(Engine &e, int p, int n, double alpha, int max_level, int *id);
 void engine_remove(Engine &e, int id);
 // Uploads [p][n] observations (NaN cells are detected on the device).
 int engine_set_observations(Engine &e, int id, const double *h_x);
+// A host-to-device copy on stream s that has completed when it returns:
+// large copies through the device's pinned staging pair, filled by several
+// host threads (copies on one device take turns), small ones directly.
+hipError_t upload_h2d(int device, void *dst, const void *src, size_t bytes, hipStream_t s);
 // The two halves of an upload done elsewhere (another stream or thread):
 // device blocks for the observations (*x receives the [p][n] block), then,
 // once the copy has completed, the instance is marked ready.
