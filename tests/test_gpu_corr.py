@@ -82,3 +82,15 @@ def test_fast_mode_then_exact_mode_same_state(hpc, orc):
         dev.pmcc(x, 0)
         got = dev.cor()
     assert _same_bits(got, orc.corr(x))
+
+
+@pytest.mark.parametrize("p,n", [(131, 1001), (513, 1025), (97, 40961)])
+def test_staged_upload_odd_sizes(hpc, orc, p, n):
+    """Observations of 1 MiB or more go through the pinned staging pair in
+    2 MiB chunks split over four host threads; sizes that leave a short last
+    chunk (and empty thread parts in it) still arrive whole: the correlation
+    equals the oracle's bit for bit."""
+    rng = np.random.default_rng(p * 7 + n)
+    x = rng.standard_normal((p, n))
+    assert x.nbytes >= 1 << 20
+    _check(hpc, orc, x, f"staged upload p={p} n={n}")
