@@ -136,13 +136,13 @@ enum {
                                   blocks go to workgroups by id.  Results do
                                   not depend on it */
   HPC_TUNE_GRAPH = 7           /* 1 (default): hpc_run on a freshly reset state
-                                  (hpc_reset) with kernel timing off and a
-                                  level cap records its launches; the next
-                                  such run captures them in a hipGraph and
-                                  later ones replay it with one host wait,
-                                  checking every level's sizes against the
-                                  recorded run (a difference recomputes the
-                                  run without the graph).  0: never */
+                                  (hpc_reset) with kernel timing off, with a
+                                  level cap or none, records its launches and
+                                  captures them in a hipGraph; later such runs
+                                  replay it with one host wait, checking every
+                                  level's sizes against the recorded run (a
+                                  difference recomputes the run without the
+                                  graph).  0: never */
 };
 int hpc_set_tuning(hpc_state *st, int param, int value);
 /* Level-loop graph of a state (HPC_TUNE_GRAPH): runs replayed, graphs

@@ -1644,7 +1644,8 @@ static int graph_replay(Engine &e, int id) {
   e.packed.clear();
   // the sizes each level ran with, as the device computed them this time
   if ((slot_inst(e, 0, 1)->has_nan ? 1 : 0) != sh.cor_nan) return 1;
-  for (int l = 1; l <= sh.max_level; ++l) {
+  const int checked = sh.stop >= 0 ? sh.stop : sh.last;
+  for (int l = 1; l <= checked; ++l) {
     const LevelShape &x = sh.lv[l];
     const int kind = x.src ? 1 : 0, at = x.src ? l - 1 : l;
     const Counters *c = slot_host(e, at, kind);
@@ -1654,7 +1655,7 @@ static int graph_replay(Engine &e, int id) {
   }
   const double wall = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
   h.cor_nan = sh.cor_nan;
-  for (int l = 0; l <= sh.max_level; ++l) {
+  for (int l = 0; l <= sh.last; ++l) {
     const Counters *c = slot_host(e, l, 1);
     const InstCounters &ic = *slot_inst(e, l, 1);
     hpc_level_stats st;
@@ -1673,15 +1674,16 @@ static int graph_replay(Engine &e, int id) {
       h.rec_used += (size_t)ic.rec_words;
       engine_note_level(e, id, l);
     }
-    st.ms_wall = wall / (sh.max_level + 1);  // the replay's wall time, split evenly
+    st.ms_wall = wall / (sh.last + 1);  // the replay's wall time, split evenly
     h.hist.push_back(st);
   }
-  h.max_deg = sh.lv[sh.max_level].inst_max_deg;
-  e.n_edges = sh.lv[sh.max_level].n_edges;
-  e.max_deg = sh.lv[sh.max_level].max_deg;
-  h.compacted = false;
+  h.max_deg = sh.lv[checked].inst_max_deg;
+  e.n_edges = sh.lv[checked].n_edges;
+  e.max_deg = sh.lv[checked].max_deg;
+  // (a run with no cap ends on a compaction, a capped one on a removal)
+  h.compacted = sh.stop >= 0;
   h.ever_compacted = true;
-  h.next_level = sh.max_level + 1;
+  h.next_level = sh.last + 1;
   h.finished = true;
   return HPC_OK;
 }
@@ -1689,10 +1691,12 @@ static int graph_replay(Engine &e, int id) {
 int engine_run_single(Engine &e, int id) {
   InstHost &h = e.inst[id];
   engine_drop_results(h);
-  // graph replay: a fresh run of an exact, capped instance without per-level
-  // kernel timing, with the launch shapes and record buffer of the capture
+  // graph replay: a fresh run of an exact instance (a level cap of 1..16, or
+  // none) without per-level kernel timing, with the launch shapes and record
+  // buffer of the capture
   const bool graphable = e.use_graph && !e.kernel_timing && !h.fast && h.fresh &&
-                         h.next_level == 0 && h.max_level >= 1 && h.max_level <= kMaxLevel &&
+                         h.next_level == 0 &&
+                         (h.max_level < 0 || (h.max_level >= 1 && h.max_level <= kMaxLevel)) &&
                          h.have_cor && h.inject_level < 0;
   if (graphable && e.graph.exec && e.graph.id == id && e.graph.shape.max_level == h.max_level &&
       e.graph.shape.rec == h.rec && e.graph.tuning == tuning_key(e)) {
@@ -1711,7 +1715,18 @@ int engine_run_single(Engine &e, int id) {
   rec.max_level = h.max_level;
   const bool record = graphable && h.fresh;
   int rc = run_single_impl(e, id, nullptr, record ? &rec : nullptr);
-  if (rc || !record || !h.finished || h.next_level != h.max_level + 1) return rc;
+  if (rc || !record || !h.finished) return rc;
+  if (h.max_level >= 0) {
+    if (h.next_level != h.max_level + 1) return rc;
+    rec.last = h.max_level;
+    rec.stop = -1;
+  } else {
+    // no cap: levels 0 .. next_level - 1 ran, and next_level's compaction
+    // found no row with enough neighbours
+    if (h.next_level < 1 || h.next_level > kMaxLevel) return rc;
+    rec.last = h.next_level - 1;
+    rec.stop = h.next_level;
+  }
   rec.cor_nan = h.cor_nan;
   rec.rec = h.rec;
   rec.valid = true;

@@ -382,6 +382,9 @@ struct LevelShape {
 struct RunShape {
   bool valid = false;
   int max_level = 0;
+  // the last level searched, and with no level cap the level whose
+  // compaction found nothing left to test (-1 with a cap)
+  int last = 0, stop = -1;
   int cor_nan = -1;
   const int32_t *rec = nullptr;  // record buffer (its address is a kernel argument)
   LevelShape lv[kMaxLevel + 2];

@@ -16,6 +16,11 @@ pytestmark = pytest.mark.gpu
 def _oracle_run(orc, cor, n, alpha, ml):
     ref = orc.Skeleton(cor, n, alpha, max_level=ml)
     recs = ref.run(ml)
+    if ml < 0:
+        # the oracle records the level that finds no testable row; hpc_run
+        # stops at its compaction without a record
+        assert recs[-1]["tests"] == 0
+        recs = recs[:-1]
     return ref.adj(), ref.sepsets(), [r["tests"] for r in recs], ref.mc()
 
 
@@ -119,3 +124,33 @@ def test_graph_then_tuning_change(hpc, orc):
             _check(dev, want, "after new shapes")
         g = dev.graph_stats()
         assert g["captures"] == 2 and g["replays"] == 3, g
+
+
+@pytest.mark.parametrize("name", ["small", "c2"])
+def test_graph_replays_uncapped(hpc, orc, name):
+    """With no level cap the recorded run ends on the compaction that finds
+    no row with enough neighbours; replays check that compaction's sizes too
+    and equal the oracle, and other data is detected and recomputed."""
+    x, n, alpha, _ = workloads.sem(name)
+    p = x.shape[0]
+    want = _oracle_run(orc, orc.corr(x), n, alpha, -1)
+    with hpc.DeviceSkeleton(p, n, alpha, max_level=-1) as dev:
+        dev.set_kernel_timing(False)
+        dev.pmcc(x)
+        for k in range(4):
+            print(f"[graph] {name} uncapped: run {k}", flush=True)
+            dev.reset()
+            st = dev.run(-1)
+            assert [s["tests"] for s in st] == want[2], (name, k)
+            _check(dev, want, f"{name} uncapped run {k}")
+        g = dev.graph_stats()
+        assert g == {"replays": 3, "captures": 1, "mismatches": 0}, g
+        # other data of the same shape: the replay's sizes differ
+        x2 = x[:, np.random.default_rng(5).permutation(n)][:, : n - 7].copy()
+        x2 = np.concatenate([x2, x[:, :7] * 0.5], axis=1)
+        want2 = _oracle_run(orc, orc.corr(x2), n, alpha, -1)
+        dev.pmcc(x2)
+        dev.reset()
+        st = dev.run(-1)
+        assert [s["tests"] for s in st] == want2[2]
+        _check(dev, want2, f"{name} uncapped other data")
