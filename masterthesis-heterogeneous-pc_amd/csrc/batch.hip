@@ -385,7 +385,7 @@ static void uploader_loop(hpc_server *s) {
       s->up_queue.pop_front();
     }
     const hipError_t err =
-        upload_h2d(s->eng.device, r->dst, r->src, r->count * sizeof(double), s->up_stream);
+        upload_f64_h2d(s->eng.device, r->dst, r->src, r->count, s->up_stream, true);
     {
       std::lock_guard<std::mutex> lk(s->mu);
       r->up_err = err;

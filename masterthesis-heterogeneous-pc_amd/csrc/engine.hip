@@ -371,115 +371,211 @@ void engine_observations_ready(Engine &e, int id) {
 
 // Host-to-device copies of observations.  A copy from pageable memory goes
 // through the runtime's own staging at about 27 GB/s on the MI355X host; here
-// a large one goes through two pinned buffers per device, each filled by
+// a large one goes through a ring of pinned buffers per device, each filled by
 // kStageThreads host threads (the caller and a pool of helpers) while the
-// other's DMA runs (about 48 GB/s).  Buffers and helpers are created on first
-// use and kept for the process (one set per device, shared by every engine
-// under a mutex; never destroyed, so no helper outlives its synchronisation
-// objects at exit).
+// others' DMAs run.  The ring carries over from one call to the next: a
+// buffer is refilled once its last use has ended, whichever call and stream
+// issued it, so a caller that does not wait for its copy (a later kernel on
+// the same stream orders itself after it) fills the next while the DMA runs.
+// Arrays of doubles that FP32 holds exactly (float data widened to double,
+// as sensor and benchmark files often are) cross the bus at half width: the
+// fill narrows each slice and checks every value round-trips, the DMA moves
+// the floats into a device buffer and widen_kernel writes the doubles, bit
+// for bit the caller's (a slice with any other value, NaN included, is sent
+// as doubles and so is the rest of that copy).  Buffers and helpers are
+// created on first use and kept for the process (one set per device, shared
+// by every engine under a mutex; never destroyed, so no helper outlives its
+// synchronisation objects at exit).
 namespace {
-constexpr size_t kStageChunk = (size_t)2 << 20;
+constexpr size_t kStageChunk = (size_t)4 << 20;
 constexpr size_t kStageMin = (size_t)1 << 20;
-constexpr int kStageThreads = 4;
+constexpr size_t kPoolMin = (size_t)256 << 10;  // smaller slices: the caller alone
+constexpr int kStageBufs = 4;
+constexpr int kStageThreads = 8;
 constexpr int kStageDevices = 64;
 
-// kStageThreads - 1 helper threads that each copy one part of a chunk.
+// narrows n doubles to floats; true when every one round-trips exactly
+// ((double)(float)v == v keeps the bits too: the one pair of equal doubles
+// with different bits, +0 and -0, narrows to +0f and -0f)
+bool narrow_exact(float *to, const double *from, size_t n) {
+  bool ok = true;
+  for (size_t i = 0; i < n; ++i) {
+    const float f = (float)from[i];
+    to[i] = f;
+    ok &= (double)f == from[i];
+  }
+  return ok;
+}
+
+// kStageThreads - 1 helper threads that each take one part of a slice.
 struct CopyPool {
   std::mutex m;
   std::condition_variable cv_work, cv_done;
   unsigned long long gen = 0;
   int pending = 0;
-  const char *src = nullptr;
-  char *dst = nullptr;
-  size_t len = 0, per = 0;
+  const void *src = nullptr;
+  void *dst = nullptr;
+  size_t count = 0, per = 0;  // bytes, or doubles when narrowing
+  bool narrow = false, exact = true;
   CopyPool() {
     for (int t = 1; t < kStageThreads; ++t) std::thread([this, t] { run(t); }).detach();
+  }
+  bool part(int t, const void *s, void *d, size_t n, size_t each, bool nar) {
+    const size_t a = std::min(n, (size_t)t * each), b = std::min(n, a + each);
+    if (a >= b) return true;
+    if (nar) return narrow_exact(static_cast<float *>(d) + a, static_cast<const double *>(s) + a, b - a);
+    memcpy(static_cast<char *>(d) + a, static_cast<const char *>(s) + a, b - a);
+    return true;
   }
   void run(int t) {
     unsigned long long seen = 0;
     for (;;) {
-      const char *s;
-      char *d;
-      size_t a, b;
+      const void *s;
+      void *d;
+      size_t n, each;
+      bool nar;
       {
         std::unique_lock<std::mutex> lk(m);
         cv_work.wait(lk, [&] { return gen != seen; });
         seen = gen;
-        s = src;
-        d = dst;
-        a = std::min(len, (size_t)t * per);
-        b = std::min(len, a + per);
+        s = src; d = dst; n = count; each = per; nar = narrow;
       }
-      if (a < b) memcpy(d + a, s + a, b - a);
+      const bool ok = part(t, s, d, n, each, nar);
       std::lock_guard<std::mutex> lk(m);
+      exact = exact && ok;
       if (--pending == 0) cv_done.notify_one();
     }
   }
-  // copies len bytes with every helper; returns when all parts are done
-  void copy(char *to, const char *from, size_t n) {
-    const size_t part = (n + kStageThreads - 1) / kStageThreads;
+  // one slice with every helper; returns when all parts are done (and, when
+  // narrowing, whether every value round-tripped)
+  bool job(void *to, const void *from, size_t n, bool nar) {
+    const size_t each = (n + kStageThreads - 1) / kStageThreads;
     {
       std::lock_guard<std::mutex> lk(m);
-      src = from;
-      dst = to;
-      len = n;
-      per = part;
+      src = from; dst = to; count = n; per = each; narrow = nar; exact = true;
       pending = kStageThreads - 1;
       ++gen;
     }
     cv_work.notify_all();
-    memcpy(to, from, std::min(part, n));
+    const bool ok = part(0, from, to, n, each, nar);
     std::unique_lock<std::mutex> lk(m);
     cv_done.wait(lk, [&] { return pending == 0; });
+    return ok && exact;
   }
 };
 
 struct Staging {
   std::mutex mu;
-  char *buf[2] = {nullptr, nullptr};
-  hipEvent_t ev[2] = {nullptr, nullptr};
+  char *buf[kStageBufs] = {};    // pinned host slices
+  float *wide[kStageBufs] = {};  // device copies of narrowed slices
+  hipEvent_t ev[kStageBufs] = {};
+  bool busy[kStageBufs] = {};  // the buffer's last use was issued (ev marks its end)
+  unsigned next = 0;           // the ring's next buffer
   CopyPool *pool = nullptr;
   bool ready = false, failed = false;
 };
 Staging g_staging[kStageDevices];
+
+__global__ void widen_kernel(const float *__restrict__ src, double *__restrict__ dst, long long n) {
+  const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (i + 4 <= n) {
+    const float4 v = *reinterpret_cast<const float4 *>(src + i);
+    double2 *d = reinterpret_cast<double2 *>(dst + i);
+    d[0] = make_double2((double)v.x, (double)v.y);
+    d[1] = make_double2((double)v.z, (double)v.w);
+  } else {
+    for (long long k = i; k < n; ++k) dst[k] = (double)src[k];
+  }
+}
+
+bool staging_init(Staging &g, int device) {
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess || hipSetDevice(device) != hipSuccess) return false;
+  bool ok = true;
+  for (int k = 0; k < kStageBufs && ok; ++k)
+    ok = hipHostMalloc((void **)&g.buf[k], kStageChunk, hipHostMallocDefault) == hipSuccess &&
+         hipMalloc((void **)&g.wide[k], kStageChunk) == hipSuccess &&
+         hipEventCreateWithFlags(&g.ev[k], hipEventDisableTiming) == hipSuccess;
+  if (ok) g.pool = new CopyPool();
+  (void)hipSetDevice(cur);
+  return ok;
+}
 }  // namespace
 
-hipError_t upload_h2d(int device, void *dst, const void *src, size_t bytes, hipStream_t s) {
+static hipError_t upload_staged(int device, void *dst, const void *src, size_t bytes,
+                                hipStream_t s, bool complete, bool doubles) {
   auto direct = [&]() {
     hipError_t err = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
     return err == hipSuccess ? hipStreamSynchronize(s) : err;
   };
-  if (bytes < kStageMin || device < 0 || device >= kStageDevices) return direct();
+  if (bytes == 0) return hipSuccess;
+  if ((complete && bytes < kStageMin) || device < 0 || device >= kStageDevices) return direct();
   Staging &g = g_staging[device];
   std::lock_guard<std::mutex> lk(g.mu);
   if (!g.ready && !g.failed) {
-    bool ok = true;
-    for (int k = 0; k < 2 && ok; ++k) {
-      ok = hipHostMalloc((void **)&g.buf[k], kStageChunk, hipHostMallocDefault) == hipSuccess &&
-           hipEventCreateWithFlags(&g.ev[k], hipEventDisableTiming) == hipSuccess;
-    }
-    if (ok) g.pool = new CopyPool();
-    g.ready = ok;
-    g.failed = !ok;  // (no pinned memory: the runtime's own path from now on)
+    g.ready = staging_init(g, device);
+    g.failed = !g.ready;  // (no pinned memory: the runtime's own path from now on)
   }
   if (!g.ready) return direct();
   const char *from = static_cast<const char *>(src);
   char *to = static_cast<char *>(dst);
-  size_t c = 0;
-  for (size_t off = 0; off < bytes; off += kStageChunk, ++c) {
-    const size_t len = std::min(kStageChunk, bytes - off);
-    char *pb = g.buf[c & 1];
-    // the buffer's previous copy (two chunks back) has left it
-    if (c >= 2) {
-      hipError_t err = hipEventSynchronize(g.ev[c & 1]);
+  // narrowing needs whole doubles and 16-byte aligned stores of them
+  bool narrow = doubles && bytes % 8 == 0 && ((uintptr_t)dst & 15) == 0;
+  const size_t slice_dbl = kStageChunk / sizeof(float);
+  for (size_t off = 0; off < bytes;) {
+    const unsigned b = g.next;
+    g.next = (g.next + 1) % kStageBufs;
+    // the buffer's (and its device copy's) last use has ended
+    if (g.busy[b]) {
+      hipError_t err = hipEventSynchronize(g.ev[b]);
       if (err != hipSuccess) return err;
+      g.busy[b] = false;
     }
-    g.pool->copy(pb, from + off, len);
-    hipError_t err = hipMemcpyAsync(to + off, pb, len, hipMemcpyHostToDevice, s);
-    if (err == hipSuccess) err = hipEventRecord(g.ev[c & 1], s);
+    hipError_t err = hipSuccess;
+    if (narrow) {
+      const size_t cnt = std::min(slice_dbl, (bytes - off) / 8);
+      const double *sd = reinterpret_cast<const double *>(from + off);
+      float *hf = reinterpret_cast<float *>(g.buf[b]);
+      const bool exact = cnt * 8 >= kPoolMin ? g.pool->job(hf, sd, cnt, true)
+                                             : narrow_exact(hf, sd, cnt);
+      if (exact) {
+        err = hipMemcpyAsync(g.wide[b], hf, cnt * sizeof(float), hipMemcpyHostToDevice, s);
+        if (err == hipSuccess) {
+          const long long quads = ((long long)cnt + 3) / 4;
+          hipLaunchKernelGGL(widen_kernel, dim3((unsigned)((quads + 255) / 256)), dim3(256), 0,
+                             s, g.wide[b], reinterpret_cast<double *>(to + off), (long long)cnt);
+          err = hipGetLastError();
+        }
+        if (err == hipSuccess) err = hipEventRecord(g.ev[b], s);
+        if (err != hipSuccess) return err;
+        g.busy[b] = true;
+        off += cnt * 8;
+        continue;
+      }
+      narrow = false;  // this slice and the rest as doubles (the buffer is idle)
+    }
+    const size_t len = std::min(kStageChunk, bytes - off);
+    if (len >= kPoolMin)
+      (void)g.pool->job(g.buf[b], from + off, len, false);
+    else
+      memcpy(g.buf[b], from + off, len);
+    err = hipMemcpyAsync(to + off, g.buf[b], len, hipMemcpyHostToDevice, s);
+    if (err == hipSuccess) err = hipEventRecord(g.ev[b], s);
     if (err != hipSuccess) return err;
+    g.busy[b] = true;
+    off += len;
   }
-  return hipStreamSynchronize(s);
+  return complete ? hipStreamSynchronize(s) : hipSuccess;
+}
+
+hipError_t upload_h2d(int device, void *dst, const void *src, size_t bytes, hipStream_t s,
+                      bool complete) {
+  return upload_staged(device, dst, src, bytes, s, complete, false);
+}
+
+hipError_t upload_f64_h2d(int device, double *dst, const double *src, size_t count,
+                          hipStream_t s, bool complete) {
+  return upload_staged(device, dst, src, count * sizeof(double), s, complete, true);
 }
 
 int engine_set_observations(Engine &e, int id, const double *h_x) {
@@ -488,7 +584,8 @@ int engine_set_observations(Engine &e, int id, const double *h_x) {
   int rc = engine_alloc_observations(e, id, &dst);
   if (rc) return rc;
   const size_t cnt = (size_t)h.p * h.n;
-  HPC_HIPCHK(upload_h2d(e.device, dst, h_x, cnt * sizeof(double), e.stream));
+  // only kernels on e.stream read the observations: no wait for the copy
+  HPC_HIPCHK(upload_f64_h2d(e.device, dst, h_x, cnt, e.stream, false));
   engine_observations_ready(e, id);
   return HPC_OK;
 }
@@ -496,7 +593,8 @@ int engine_set_observations(Engine &e, int id, const double *h_x) {
 int engine_set_corr(Engine &e, int id, const double *h_cor) {
   InstHost &h = e.inst[id];
   HPC_HIPCHK(hipSetDevice(e.device));
-  HPC_HIPCHK(upload_h2d(e.device, h.cor, h_cor, (size_t)h.p * h.p * sizeof(double), e.stream));
+  HPC_HIPCHK(
+      upload_h2d(e.device, h.cor, h_cor, (size_t)h.p * h.p * sizeof(double), e.stream, true));
   h.have_cor = true;
   h.fast = false;
   e.h_cor_valid = false;
@@ -524,14 +622,19 @@ static int prepare_level(Engine &e, InstHost &h, int level) {
   return level >= 1 ? ensure_rec(e, h, level) : HPC_OK;
 }
 
-// Room for the worst case of one level: every remaining edge removed, each
-// with a record of 3 + level words; contents kept.  With a level cap the
-// room covers every remaining level at once (an edge is removed once, its
-// record at most 3 + cap words), so the buffer grows once per run rather
-// than between a level's kernels.
+// Room for the worst case of every level the run can still reach at once:
+// each remaining edge removed, its record at most 3 + deepest words (an edge
+// is removed once); contents kept.  The deepest level is the cap, or without
+// one the last level whose tests an endpoint's degree still allows (an edge
+// is tested at level l only when an endpoint keeps l neighbours besides the
+// other; degrees only fall, so the last counted maximum bounds them).  The
+// buffer then grows once per run, not between levels: a growth is a device
+// copy per instance.
 static int ensure_rec(Engine &e, InstHost &h, int level) {
-  const int deepest =
-      h.max_level >= 0 ? std::min(std::max(h.max_level, level), kMaxLevel) : level;
+  const int reach = std::min(h.max_deg - 1, kMaxLevel);
+  if (h.rec && level > reach) return HPC_OK;  // no test, so no record, at this level
+  const int deepest = h.max_level >= 0 ? std::min(std::max(h.max_level, level), kMaxLevel)
+                                       : std::max(level, reach);
   const size_t need = h.rec_used + (size_t)std::max<int64_t>(h.undirected, 0) * (3 + deepest);
   if (need <= h.rec_cap && h.rec) return HPC_OK;
   const size_t cap = std::max(need, 2 * h.rec_cap);

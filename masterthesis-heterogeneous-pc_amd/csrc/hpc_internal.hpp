@@ -545,10 +545,17 @@ int engine_add(Engine &e, int p, int n, double alpha, int max_level, int *id);
 void engine_remove(Engine &e, int id);
 // Uploads [p][n] observations (NaN cells are detected on the device).
 int engine_set_observations(Engine &e, int id, const double *h_x);
-// A host-to-device copy on stream s that has completed when it returns:
-// large copies through the device's pinned staging pair, filled by several
-// host threads (copies on one device take turns), small ones directly.
-hipError_t upload_h2d(int device, void *dst, const void *src, size_t bytes, hipStream_t s);
+// A host-to-device copy on stream s: large copies through the device's ring
+// of pinned staging buffers, filled by several host threads (copies on one
+// device take turns), small ones directly.  With complete, the copy has ended
+// when it returns; without, the source has been read (work later on s sees
+// the data) and small copies are staged too.
+hipError_t upload_h2d(int device, void *dst, const void *src, size_t bytes, hipStream_t s,
+                      bool complete);
+// The same for an array of doubles, sent at half width when FP32 holds every
+// value exactly (the device receives the same bits either way).
+hipError_t upload_f64_h2d(int device, double *dst, const double *src, size_t count,
+                          hipStream_t s, bool complete);
 // The two halves of an upload done elsewhere (another stream or thread):
 // device blocks for the observations (*x receives the [p][n] block), then,
 // once the copy has completed, the instance is marked ready.

@@ -84,13 +84,23 @@ def test_fast_mode_then_exact_mode_same_state(hpc, orc):
     assert _same_bits(got, orc.corr(x))
 
 
-@pytest.mark.parametrize("p,n", [(131, 1001), (513, 1025), (97, 40961)])
-def test_staged_upload_odd_sizes(hpc, orc, p, n):
-    """Observations of 1 MiB or more go through the pinned staging pair in
-    2 MiB chunks split over four host threads; sizes that leave a short last
-    chunk (and empty thread parts in it) still arrive whole: the correlation
-    equals the oracle's bit for bit."""
+@pytest.mark.parametrize("kind", ["f64", "f32", "mixed"])
+@pytest.mark.parametrize("p,n", [(20, 1000), (131, 1001), (513, 1025), (97, 40961)])
+def test_staged_upload_odd_sizes(hpc, orc, p, n, kind):
+    """Observations go through the pinned staging ring in slices split over
+    the host threads; sizes that leave a short last slice (and empty thread
+    parts in it) still arrive whole.  FP32-exact data ("f32", with -0 and
+    FP32 subnormals in it) crosses at half width and is widened on the
+    device; "mixed" has one value FP32 cannot hold in its last slice, so the
+    slices before it go narrowed and the rest as doubles.  Either way the
+    correlation equals the oracle's bit for bit."""
     rng = np.random.default_rng(p * 7 + n)
     x = rng.standard_normal((p, n))
-    assert x.nbytes >= 1 << 20
-    _check(hpc, orc, x, f"staged upload p={p} n={n}")
+    if kind != "f64":
+        x = x.astype(np.float32).astype(np.float64)
+        x[0, 1] = -0.0
+        x[1, 2] = float(np.float32(1e-40))
+        x[2, 3] = float(np.float32(-3e-42))
+    if kind == "mixed":
+        x[-1, -1] = 1.0 + 2.0 ** -40
+    _check(hpc, orc, x, f"staged upload {kind} p={p} n={n}")
