@@ -4,6 +4,7 @@
 #pragma clang fp contract(off)
 
 #include <atomic>
+#include <mutex>
 #include <thread>
 #include <math.h>
 #include <stdarg.h>
@@ -547,6 +548,21 @@ static int sort_records(const InstHost &h, const int32_t *raw, size_t words,
   return HPC_OK;
 }
 
+// Pinned host memory that batch results land in, one per device, grown on
+// demand and kept for the process (shared by every engine under a mutex): a
+// copy into pageable memory, above all freshly allocated memory, took
+// milliseconds for a few MB; a pinned buffer per engine costs its pinning on
+// every new batch.
+namespace {
+struct ResultStage {
+  std::mutex mu;
+  char *buf = nullptr;
+  size_t cap = 0;
+};
+constexpr int kResultDevices = 64;
+ResultStage g_result_stage[kResultDevices];
+}  // namespace
+
 int engine_prefetch_results(Engine &e, const std::vector<int> &ids) {
   HIPCHK(hipSetDevice(e.device));
   struct Item {
@@ -578,9 +594,6 @@ int engine_prefetch_results(Engine &e, const std::vector<int> &ids) {
     HIPCHK(hipMalloc((void **)&e.d_stage, cap));
     e.d_stage_cap = cap;
   }
-  // (ordinary host memory: pinning a fresh buffer per batch costs more than
-  // the one copy saves)
-  e.h_stage.resize(bytes);
   if (copies.size() > e.cap_copies) {
     if (e.d_copies) HIPCHK(hipFree(e.d_copies));
     e.d_copies = nullptr;
@@ -596,9 +609,29 @@ int engine_prefetch_results(Engine &e, const std::vector<int> &ids) {
                         hipMemcpyHostToDevice, e.stream));
   launch_copy_out(e.stream, e.d_copies, (int)copies.size(), e.d_stage);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(e.h_stage.data(), e.d_stage, bytes, hipMemcpyDeviceToHost, e.stream));
+  std::unique_lock<std::mutex> pin_lock;
+  char *host = nullptr;
+  if (e.device >= 0 && e.device < kResultDevices) {
+    ResultStage &rs = g_result_stage[e.device];
+    pin_lock = std::unique_lock<std::mutex>(rs.mu);
+    if (bytes > rs.cap) {
+      if (rs.buf) (void)hipHostFree(rs.buf);
+      rs.buf = nullptr;
+      rs.cap = 0;
+      const size_t cap = ((bytes + bytes / 4) + ((size_t)1 << 20) - 1) & ~(((size_t)1 << 20) - 1);
+      if (hipHostMalloc((void **)&rs.buf, cap, hipHostMallocDefault) == hipSuccess) rs.cap = cap;
+      else rs.buf = nullptr;  // (then ordinary host memory below)
+    }
+    host = rs.buf;
+  }
+  if (!host) {
+    pin_lock = std::unique_lock<std::mutex>();
+    e.h_stage.resize(bytes);
+    host = e.h_stage.data();
+  }
+  HIPCHK(hipMemcpyAsync(host, e.d_stage, bytes, hipMemcpyDeviceToHost, e.stream));
   HIPCHK(hipStreamSynchronize(e.stream));
-  const char *stage = e.h_stage.data();
+  const char *stage = host;
   // the instances' unpacking (the record sort is most of it) on a few host
   // threads; each touches only its own instances
   std::atomic<size_t> next{0};

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <map>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -49,9 +50,76 @@ int spin_wait_word(const unsigned long long *word, unsigned long long value, dou
   return spin_wait(word, value, limit_s, []() -> int { return 0; });
 }
 
+// ---- slab cache ------------------------------------------------------------
+// Arena slabs of engines that have ended, kept per device for the next engine
+// (batches and servers often come one after another): a hipMalloc / hipFree
+// pair of a 512 MiB slab costs about a millisecond.  At most kSlabCacheBytes
+// per device are kept; an allocation that fails empties the device's cache
+// and tries once more.
+namespace {
+constexpr size_t kSlabCacheBytes = (size_t)8 << 30;
+constexpr int kSlabCacheDevices = 64;
+struct SlabCache {
+  std::mutex mu;
+  std::multimap<size_t, void *> free;  // size -> base
+  size_t bytes = 0;
+};
+SlabCache g_slabs[kSlabCacheDevices];
+
+// a cached slab of at least size bytes (and at most twice that); size
+// becomes its own
+void *slab_take(int device, size_t &size) {
+  if (device < 0 || device >= kSlabCacheDevices) return nullptr;
+  SlabCache &c = g_slabs[device];
+  std::lock_guard<std::mutex> lk(c.mu);
+  auto it = c.free.lower_bound(size);
+  if (it == c.free.end() || it->first > 2 * size) return nullptr;
+  void *p = it->second;
+  size = it->first;
+  c.bytes -= it->first;
+  c.free.erase(it);
+  return p;
+}
+
+// true when the cache keeps the slab (its last user has finished with it)
+bool slab_put(int device, void *base, size_t size) {
+  if (device < 0 || device >= kSlabCacheDevices) return false;
+  SlabCache &c = g_slabs[device];
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (c.bytes + size > kSlabCacheBytes) return false;
+  c.free.emplace(size, base);
+  c.bytes += size;
+  return true;
+}
+
+// frees the current device's cached slabs
+void slab_flush() {
+  int device = -1;
+  if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= kSlabCacheDevices) return;
+  SlabCache &c = g_slabs[device];
+  std::lock_guard<std::mutex> lk(c.mu);
+  for (auto &kv : c.free) (void)hipFree(kv.second);
+  c.free.clear();
+  c.bytes = 0;
+}
+
+// hipMalloc on the current device; after a failure, once more with the
+// device's slab cache emptied (the failure's error is cleared, so a later
+// hipGetLastError does not report it)
+hipError_t cached_malloc(void **ptr, size_t bytes) {
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e == hipSuccess) return e;
+  (void)hipGetLastError();
+  slab_flush();
+  e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess) (void)hipGetLastError();
+  return e;
+}
+}  // namespace
+
 static int dalloc(void **ptr, size_t bytes) {
   *ptr = nullptr;
-  hipError_t e = hipMalloc(ptr, bytes ? bytes : 8);
+  hipError_t e = cached_malloc(ptr, bytes ? bytes : 8);
   if (e != hipSuccess) {
     *ptr = nullptr;
     return hpc_fail(HPC_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
@@ -86,10 +154,12 @@ static int arena_alloc(Engine &e, size_t bytes, void **out) {
   }
   Arena::Slab sl;
   sl.size = std::max(need, A.slab_min);
-  void *base = nullptr;
-  hipError_t err = hipMalloc(&base, sl.size);
-  if (err != hipSuccess)
-    return hpc_fail(HPC_ENOMEM, "hipMalloc(%zu) failed: %s", sl.size, hipGetErrorString(err));
+  void *base = slab_take(e.device, sl.size);
+  if (!base) {
+    hipError_t err = cached_malloc(&base, sl.size);
+    if (err != hipSuccess)
+      return hpc_fail(HPC_ENOMEM, "hipMalloc(%zu) failed: %s", sl.size, hipGetErrorString(err));
+  }
   sl.base = (char *)base;
   if (sl.size > need) sl.free.emplace(need, sl.size - need);
   A.slabs.push_back(std::move(sl));
@@ -201,8 +271,9 @@ void engine_free(Engine &e) {
   if (e.stream) (void)hipStreamSynchronize(e.stream);
   if (e.copy_stream) (void)hipStreamSynchronize(e.copy_stream);
   e.inst.clear();
+  // (the streams have drained: the slabs are idle)
   for (auto &sl : e.arena.slabs)
-    if (sl.base) (void)hipFree(sl.base);
+    if (sl.base && !slab_put(e.device, sl.base, sl.size)) (void)hipFree(sl.base);
   e.arena = Arena();
   dfree(e.d_inst); dfree(e.d_row_inst); dfree(e.d_deg); dfree(e.d_rowptr);
   dfree(e.d_nbr); dfree(e.d_erow); dfree(e.d_rowmask); dfree(e.d_fp); dfree(e.d_ccor);
@@ -975,10 +1046,40 @@ int engine_corr(Engine &e, float *ms_moments, float *ms_corr) {
   return HPC_OK;
 }
 
+namespace {
+constexpr int kZeroSpans = 32;
+struct ZeroSpanArg {
+  double *ptr[kZeroSpans];
+  unsigned long long words[kZeroSpans];
+};
+// blockIdx.y picks the span; the x blocks stride over it
+__global__ void zero_spans_kernel(ZeroSpanArg a) {
+  double *p = a.ptr[blockIdx.y];
+  const unsigned long long w = a.words[blockIdx.y];
+  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < w;
+       i += (unsigned long long)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+}  // namespace
+
+int engine_zero_spans(Engine &e, const std::vector<ZeroSpan> &spans) {
+  for (size_t a = 0; a < spans.size(); a += kZeroSpans) {
+    const int n = (int)std::min<size_t>(kZeroSpans, spans.size() - a);
+    ZeroSpanArg arg{};
+    for (int k = 0; k < n; ++k) {
+      arg.ptr[k] = spans[a + k].ptr;
+      arg.words[k] = spans[a + k].words;
+    }
+    hipLaunchKernelGGL(zero_spans_kernel, dim3(64, (unsigned)n), dim3(256), 0, e.stream, arg);
+    HPC_HIPCHK(hipGetLastError());
+  }
+  return HPC_OK;
+}
+
 // The z-score buffers of the instance's correlation mode: the exact form's
 // class-major copy, or the fast form's float copy with the canonical FP64
 // one (its exact rechecks read it).
-int engine_prepare_corr(Engine &e, int id) {
+int engine_prepare_corr(Engine &e, int id, std::vector<ZeroSpan> *clears) {
   InstHost &h = e.inst[id];
   h.fast = h.corr_mode == HPC_CORR_F32;
   const size_t before = zbytes(h);
@@ -989,10 +1090,13 @@ int engine_prepare_corr(Engine &e, int id) {
   } else if (!h.zc) {
     rc = ialloc(e, h.zc, zc_size(h.p, h.npad));
     // the rows past p (read, never stored, by the matrix-core kernel) are
-    // zeros: the last pair of row blocks, cleared once here
+    // zeros: the last pair of row blocks, cleared once
     const size_t tail = 2 * zc_block(h.npad);
-    if (!rc && hipMemsetAsync(h.zc + zc_size(h.p, h.npad) - tail, 0, tail * 8, e.stream) != hipSuccess)
-      rc = hpc_fail(HPC_EHIP, "clearing the z-score pad rows failed");
+    if (!rc) {
+      const ZeroSpan z{h.zc + zc_size(h.p, h.npad) - tail, tail};
+      if (clears) clears->push_back(z);
+      else rc = engine_zero_spans(e, {z});
+    }
   }
   h.bytes += zbytes(h) - before;
   return rc;
@@ -1990,11 +2094,14 @@ int engine_step(Engine &e, const std::vector<int> &ids) {
   };
   if (!need.empty()) {
     std::vector<int> ok;
+    std::vector<ZeroSpan> clears;
     for (int id : need) {
-      rc = engine_prepare_corr(e, id);
+      rc = engine_prepare_corr(e, id, &clears);
       if (rc) drop(id, rc);
       else ok.push_back(id);
     }
+    rc = engine_zero_spans(e, clears);
+    if (rc) return rc;
     need.swap(ok);
     zero.assign(need.size(), 0);
   }

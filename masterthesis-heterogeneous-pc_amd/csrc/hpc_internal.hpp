@@ -90,8 +90,6 @@ enum InstFlags {
   kCorNaN = 2,   // the correlation matrix may hold NaN: tests check their inputs
 };
 
-// Slots per lane class of the class-major z-scores: npad / 64 samples rounded
-// up to a multiple of 8.
 // Class-major z-scores: slots per lane class, npad / 64 rounded up to a
 // whole group of 16.
 __host__ __device__ __forceinline__ int zc_slots(int npad) { return ((npad >> 6) + 15) & ~15; }
@@ -571,7 +569,16 @@ int engine_pack(Engine &e, const std::vector<int> &ids, const std::vector<int> &
 int engine_corr(Engine &e, float *ms_moments, float *ms_corr);
 // Before packing an instance for its correlation: the fast flag from its
 // correlation mode, and its float z-score block.
-int engine_prepare_corr(Engine &e, int id);
+// A device span of doubles to clear (pad rows gathered over instances).
+struct ZeroSpan {
+  double *ptr;
+  size_t words;
+};
+// Clears the spans with one launch per 32 of them, on e.stream.
+int engine_zero_spans(Engine &e, const std::vector<ZeroSpan> &spans);
+// With clears, the instance's pad-row clear is appended there (the caller
+// clears them all at once) instead of run here.
+int engine_prepare_corr(Engine &e, int id, std::vector<ZeroSpan> *clears = nullptr);
 // Exact correlations (from the device-resident z-scores) for the fast
 // instances among ids, which then count as exact.
 int engine_exact_from_z(Engine &e, const std::vector<int> &ids);
