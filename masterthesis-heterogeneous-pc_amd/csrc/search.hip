@@ -40,8 +40,12 @@ constexpr int kWavesPerBlock = 4;
 template <int LV>
 constexpr int run_len() { return LV >= 4 ? 8 : 4; }
 // wavefronts per workgroup of the removal kernel (one record reservation and
-// one set of counter atomics per workgroup)
-constexpr int kCommitWaves = 16;
+// one set of counter atomics per workgroup).  Eight spread a short level's
+// few thousand edges over twice the CUs of sixteen: the p = 1000 run without
+// a level cap (levels 5..15 remove from a few thousand edges each) took
+// 1.09 ms with sixteen, 0.99 ms with eight; four cost the p = 2000 levels
+// 0..4 5% (more workgroups for the last one to wait on).
+constexpr int kCommitWaves = 8;
 
 // Binomial table staged in LDS, addressed as LDS (32-bit addresses, ds_read)
 // rather than through generic pointers.
