@@ -188,7 +188,10 @@ int hpc_apply_tuning(hpc_state *st, const hpc_tune_entry *entries, int count);
  * observations and computes the correlation matrix on the device. */
 int hpc_pmcc(hpc_state *st, const double *h_mat, int mode);
 /* The two halves of hpc_pmcc: upload once, recompute from device-resident
- * observations. */
+ * observations.  hpc_set_observations returns once h_mat has been read (the
+ * caller may reuse it); the copy to the device ends, in stream order, before
+ * any later work of the state.  Values that FP32 holds exactly cross the bus
+ * at half width; the device receives the same doubles either way. */
 int hpc_set_observations(hpc_state *st, const double *h_mat);
 int hpc_corr(hpc_state *st, int mode);
 /* Device time of the last hpc_corr: moments (mean, sd, z-scores) and the
